@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark: SU(3) link-updates/s of the temporal-gauge leapfrog on a 128^3
+periodic lattice, FP32 links/E with FP64 reunitarization every 10 steps versus
+plain FP64, both on one MI355X.
+
+A step is one leapfrog step over the whole lattice (storeFields + the fused
+E-field/link-update kernel; every 10th step of the FP32 form also reprojects
+the links onto SU(3) in double precision). Inputs are synthetic and resident
+in HBM before timing: links exp(i A_a t_a) with A_a ~ Gaussian(0, 0.1), E
+coefficients ~ Gaussian(0, 0.05) (seeded Philox), a = g = 1, dt = 0.25.
+
+Prints one JSON line. `value` = link-updates/s of the FP32+reunit form.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+DIMS = (128, 128, 128)
+NC = 3
+DT = 0.25
+REUNIT_EVERY = 10
+# algorithmic HBM bytes per link update of the fused kernel (read U, read E, write E, write Unext)
+BYTES_PER_LINK = {"fp32": 72 + 32 + 32 + 72, "fp64": 144 + 64 + 64 + 144}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--kernel-reps", type=int, default=50, help="launches of the fused kernel for its own timing")
+    p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--dims", type=int, nargs=3, default=list(DIMS))
+    p.add_argument("--forms", default="fp32,fp64")
+    return p.parse_args()
+
+
+def make_grid(precision, dims, seed=0):
+    from openpixi_amd import Grid, inputs
+
+    n = int(np.prod(dims))
+    g = Grid(NC, dims, 1.0, DT, 1.0, precision)
+    A = inputs.random_links_coefficients(n, NC, 0.1, seed)
+    g.setLinksFromCoefficients(A)
+    del A
+    Ec = inputs.random_efield_coefficients(n, NC, 0.05, seed)
+    g.setEFromCoefficients(Ec)
+    del Ec
+    return g
+
+
+def time_form(precision, dims, steps, warmup, kernel_reps, stream):
+    import torch
+    from openpixi_amd import FieldMeasurements, _lib
+
+    L = _lib.load()
+    g = make_grid(precision, dims)
+    g._check(L.opx_set_stream(g.handle, ctypes_void(stream.cuda_stream)))
+    reunit = REUNIT_EVERY if precision == "fp32" else 0
+    # initial half step U(-dt/2) -> U(+dt/2), as Simulation.initialize
+    g._check(L.opx_step_links(g.handle, DT))
+    m0 = FieldMeasurements().measure(g)
+    g._check(L.opx_leapfrog(g.handle, warmup, reunit))
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    g._check(L.opx_leapfrog(g.handle, steps, reunit))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1)
+    m1 = FieldMeasurements().measure(g)
+    # the fused update kernel alone, back to back on the same stream
+    k0 = torch.cuda.Event(enable_timing=True)
+    k1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    k0.record(stream)
+    for _ in range(kernel_reps):
+        g._check(L.opx_store_fields(g.handle))
+        g._check(L.opx_step(g.handle, DT))
+    k1.record(stream)
+    torch.cuda.synchronize()
+    kms = k0.elapsed_time(k1) / kernel_reps
+    n = int(np.prod(dims))
+    links = 3 * n
+    energy0 = float(m0[0:6].sum())
+    energy1 = float(m1[0:6].sum())
+    res = {
+        "ms_per_step": ms / steps,
+        "wall_ms_per_step": 1e3 * wall / steps,
+        "link_updates_per_s": links * steps / (ms * 1e-3),
+        "kernel_ms": kms,
+        "kernel_gbs": links * BYTES_PER_LINK[precision] / (kms * 1e-3) / 1e9,
+        "gauss_residual_start": float(m0[6]),
+        "gauss_residual_end": float(m1[6]),
+        "energy_start": energy0,
+        "energy_end": energy1,
+        "energy_rel_drift": abs(energy1 - energy0) / abs(energy0),
+    }
+    g.close()
+    return res
+
+
+def ctypes_void(p):
+    import ctypes
+
+    return ctypes.c_void_p(int(p))
+
+
+def cpu_baseline(budget_s):
+    """The FP64 oracle (C restatement of FastTYMSolver, OpenMP over cells) on a
+    bounded SU(3) sample: a 32^3 lattice, as many steps as fit the budget."""
+    import oracle as O
+    from openpixi_amd import inputs
+
+    dims = (32, 32, 32)
+    n = int(np.prod(dims))
+    cores = os.cpu_count() or 1
+    og = O.OracleGrid(NC, dims, (1.0, 1.0, 1.0), DT, 1.0, threads=cores)
+    A = inputs.random_links_coefficients(n, NC, 0.1, 0)
+    og.U[:] = O.links_from_coeffs(NC, A)
+    og.E[:] = O.algs_from_coeffs(NC, inputs.random_efield_coefficients(n, NC, 0.05, 0))
+    t1 = og.time_steps(1, 1)
+    steps = max(1, int(budget_s / max(t1, 1e-6)))
+    t = og.time_steps(steps, 0)
+    return {
+        "value": 3 * n * steps / t,
+        "unit": "link-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"FP64 oracle (C, -O2, OpenMP {cores} threads), SU(3) 32^3, {steps} steps, {t:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this workload has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    dims = tuple(args.dims)
+    stream = torch.cuda.Stream()
+    forms = args.forms.split(",")
+    results = {}
+    for f in forms:
+        results[f] = time_form(f, dims, args.steps, args.warmup, args.kernel_reps, stream)
+        print(f"# {f}: {json.dumps(results[f])}", file=sys.stderr, flush=True)
+    main_form = "fp32" if "fp32" in results else forms[0]
+    r = results[main_form]
+    out = {
+        "metric": "SU(3) link-updates/s at 128^3: FP32+FP64-reunit vs FP64 at equal Gauss residual",
+        "value": r["link_updates_per_s"],
+        "unit": "link-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if main_form == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"SU(3) pure Yang-Mills leapfrog, periodic {dims[0]}x{dims[1]}x{dims[2]}, dt/a=0.25, "
+                        f"{'FP32 links/E + FP64 reunitarization every 10 steps' if main_form == 'fp32' else 'FP64'}",
+            "lattice": list(dims),
+            "colors": NC,
+            "reunitarize_every": REUNIT_EVERY if main_form == "fp32" else 0,
+        },
+        "forms": results,
+    }
+    if "fp32" in results and "fp64" in results:
+        out["fp32_over_fp64"] = results["fp32"]["link_updates_per_s"] / results["fp64"]["link_updates_per_s"]
+    achieved = r["kernel_gbs"]
+    out["roofline"] = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "kernel": "k_combined_update<3,%s>" % ("float" if main_form == "fp32" else "double"),
+        "kernel_ms": r["kernel_ms"],
+        "bytes_per_link": BYTES_PER_LINK[main_form],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    out["host"] = platform.node()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,163 @@
+/*
+ * opx.h -- C ABI of the MI355X (gfx950) real-time lattice Yang-Mills leapfrog
+ * and its colored-particle (CPIC) coupling.
+ *
+ * The ABI replaces OpenPixi's FieldSolver / Interpolation / FieldMeasurements
+ * seams for the FastTYMSolver path. Every entry point is a plain C function on
+ * pointers and sizes; nothing throws across it. Reference citations are to
+ * pixi/src/main/java/org/openpixi/pixi/ in the reference repository.
+ *
+ * Conventions
+ *   - Return value: OPX_OK (0) or a negative OPX_E_* code; opx_last_error()
+ *     returns the message of the last failure on a context.
+ *   - Host buffers use the reference's per-cell element order (AoS, cell-major,
+ *     direction next, then element components):
+ *       links   [cell][dir][G]   G = 4 (SU(2): e0..e3) or 18 (SU(3): 9 real, 9 imaginary parts, row-major)
+ *       algebra [cell][dir][A]   A = 3 (SU(2): v0..v2) or 9 (SU(3): Hermitian storage
+ *                                 of math/SU3AlgebraElement.java:3-11)
+ *       charge  [cell][A]
+ *     cell = ((x0*N1)+x1)*N2+x2 (physics/grid/Grid.java:757-770).
+ *     The device keeps structure-of-arrays [dir][component][cell] in the
+ *     context's precision; SU(3) algebra is kept as 8 Gell-Mann coefficients.
+ *   - One host thread per context; calls are ordered on the context's HIP
+ *     stream. Entry points that return host data synchronize that stream.
+ */
+#ifndef OPX_H
+#define OPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct opx_ctx opx_ctx;
+
+enum {
+    OPX_OK = 0,
+    OPX_E_ARG = -1,          /* invalid argument (null pointer, size mismatch, bad index) */
+    OPX_E_UNSUPPORTED = -2,  /* unsupported number of colours / dimensions / precision */
+    OPX_E_DEVICE = -3,       /* HIP runtime error (allocation, launch, copy) */
+    OPX_E_STATE = -4,        /* call not valid in the current state (e.g. no particles) */
+    OPX_E_NUMERIC = -5       /* health monitor tripped (NaN / unitarity defect) */
+};
+
+enum { OPX_FP64 = 0, OPX_FP32 = 1 };
+
+enum {
+    OPX_FIELD_U = 0,      /* links at t - dt/2            (Cell.U,     physics/grid/Cell.java:26) */
+    OPX_FIELD_UNEXT = 1,  /* links at t + dt/2            (Cell.Unext, physics/grid/Cell.java:29) */
+    OPX_FIELD_E = 2,      /* electric field at t          (Cell.E,     physics/grid/Cell.java:20) */
+    OPX_FIELD_J = 3,      /* current at t + dt/2          (Cell.J,     physics/grid/Cell.java:14) */
+    OPX_FIELD_RHO = 4     /* charge density at t          (Cell.rho,   physics/grid/Cell.java:17) */
+};
+
+/* indices into opx_measure's output */
+enum {
+    OPX_M_E2_0 = 0, OPX_M_E2_1 = 1, OPX_M_E2_2 = 2,   /* FieldMeasurements.calculateEsquared(grid, i) */
+    OPX_M_B2_0 = 3, OPX_M_B2_1 = 4, OPX_M_B2_2 = 5,   /* FieldMeasurements.calculateBsquared(grid, i) */
+    OPX_M_GAUSS = 6,                                  /* FieldMeasurements.calculateGaussConstraint  */
+    OPX_M_RHO2 = 7,                                   /* FieldMeasurements.calculateTotalChargeSquared */
+    OPX_M_COUNT = 8
+};
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the state held by Grid(Settings) (physics/grid/Grid.java:443-473):
+ * colours, cells per dimension, lattice spacings a_i, time step a_t and
+ * coupling g. Links start as identities, fields at zero, all cells active
+ * and evaluatable (physics/grid/Cell.java:49-74). num_dims must be 3.
+ * precision: OPX_FP64, or OPX_FP32 (links, E, J, rho in float; measurements,
+ * reunitarization in double). */
+int opx_create(int num_colors, int num_dims, const int *num_cells, const double *spacing,
+               double time_step, double coupling, int precision, opx_ctx **out);
+void opx_destroy(opx_ctx *ctx);
+const char *opx_last_error(const opx_ctx *ctx);
+int opx_get_info(const opx_ctx *ctx, int64_t *num_cells_total, int *num_colors, int *precision);
+/* Launch on a caller-owned hipStream_t (NULL restores the context's own stream). */
+int opx_set_stream(opx_ctx *ctx, void *hip_stream);
+int opx_sync(opx_ctx *ctx);
+
+/* ---- data at the seam ---------------------------------------------------
+ * Grid.setU/setUnext/setE/addJ/setRho and getters, whole lattice at once
+ * (physics/grid/Grid.java:126-265). n_values must equal the full buffer size. */
+int opx_upload_field(opx_ctx *ctx, int field, const double *host, int64_t n_values);
+int opx_download_field(opx_ctx *ctx, int field, double *host, int64_t n_values);
+/* Links U = exp(i sum_a c_a t_a) from Gell-Mann/Pauli coefficients [cell][dir][nc*nc-1]
+ * (AlgebraElement.getLink, math/SU2AlgebraElement.java:140-152,
+ * math/SU3AlgebraElement.java:407-417), computed on the device. */
+int opx_set_links_from_coefficients(opx_ctx *ctx, int field, const double *coeffs, int64_t n_values);
+/* E from generator coefficients [cell][dir][nc*nc-1] (AlgebraElement.set(j, v)). */
+int opx_set_efield_from_coefficients(opx_ctx *ctx, const double *coeffs, int64_t n_values);
+
+/* Grid.setActiveRegion / setEvaluationRegion (physics/grid/Grid.java:992-1020):
+ * inclusive boxes in cell coordinates. */
+int opx_set_active_region(opx_ctx *ctx, const int *lo, const int *hi);
+int opx_set_evaluation_region(opx_ctx *ctx, const int *lo, const int *hi);
+
+/* ---- FieldSolver seam (FastTYMSolver) -----------------------------------
+ * opx_step:        FastTYMSolver.step -> CombinedUpdate for every active cell
+ *                  (physics/fields/FastTYMSolver.java:25-32,40-62):
+ *                  E_i += dt proj(U_i S_i) - g a_i a_t J_i;  Unext_i = exp(-i dt E_i) U_i
+ * opx_step_links:  FastTYMSolver.stepLinks -> UpdateLinks (:35-38,65-84)
+ * opx_store_fields: Grid.storeFields, U <-> Unext (physics/grid/Grid.java:591-593)
+ * opx_reset_current / opx_reset_charge: Grid.resetCurrent / resetCharge (:570-586) */
+int opx_step(opx_ctx *ctx, double dt);
+int opx_step_links(opx_ctx *ctx, double dt);
+int opx_store_fields(opx_ctx *ctx);
+int opx_reset_current(opx_ctx *ctx);
+int opx_reset_charge(opx_ctx *ctx);
+
+/* Reprojects every link of U and Unext onto the group in double precision
+ * (SU(2): quaternion normalisation; SU(3): Gram-Schmidt on rows 1-2, row 3 =
+ * conj(row1 x row2)). No counterpart in the reference (which never
+ * reunitarizes); used by the FP32 form every few steps. */
+int opx_reunitarize(opx_ctx *ctx);
+
+/* `steps` leapfrog steps of pure Yang-Mills: for each, storeFields then step
+ * (physics/Simulation.java:389-395 with no particles), reunitarizing U and
+ * Unext after every `reunit_every`-th step (0 = never). */
+int opx_leapfrog(opx_ctx *ctx, int steps, int reunit_every);
+
+/* ---- observables seam (FieldMeasurements, double precision) -------------
+ * out[OPX_M_COUNT]: mean E_i^2/(g a_i)^2, mean 1/2(B_i^2(U)+B_i^2(Unext))/(g a_i)^2,
+ * mean |G|^2, sqrt(sum rho^2)/N over evaluatable cells
+ * (physics/measurements/FieldMeasurements.java:68-230). Sums are compensated
+ * (Kahan/two-sum) and reduced in a fixed order. */
+int opx_measure(opx_ctx *ctx, double *out);
+/* ProjectedEnergyDensity along `direction` (diagnostics/methods/ProjectedEnergyDensity.java:130-170):
+ * four arrays of num_cells[direction] doubles: transverse electric/magnetic,
+ * longitudinal electric/magnetic energy per slice. */
+int opx_projected_energy_density(opx_ctx *ctx, int direction, double *eT_el, double *eT_mag,
+                                 double *eL_el, double *eL_mag);
+
+/* ---- particle seam: CGC super particles (light-cone NGP) ----------------
+ * Replaces CGCSuperParticleInterpolationNGP (physics/grid/CGCSuperParticleInterpolationNGP.java:19-93)
+ * over a list of CGCSuperParticle (physics/particles/CGCSuperParticle.java:16-46).
+ * Arrays have `count` entries; charges holds sum(num) algebra elements in
+ * reference storage, particle after particle. The list order is kept. */
+int opx_superparticles_upload(opx_ctx *ctx, int count, const int *orientation, const int *num,
+                              const int *index_offset, const int *per_plane, const int *sub_lattice_shift,
+                              const int *particles_per_cell, const double *charges, int64_t n_values);
+int opx_superparticles_download_charges(opx_ctx *ctx, double *charges, int64_t n_values);
+/* interpolateToGrid at integer clock t: J_0 += +-Q a/dt and parallel transport by Unext_0 */
+int opx_interpolate_to_grid(opx_ctx *ctx, int t);
+/* interpolateChargedensity at integer clock t: rho += Q at the NGP offset */
+int opx_interpolate_chargedensity(opx_ctx *ctx, int t);
+
+/* ---- Simulation driver (physics/Simulation.java:318-452) -----------------
+ * opx_simulation_initialize: stepLinks, rho, J deposit at t = 0.
+ * opx_simulation_step: n times { clock++, storeFields, step, resetCharge,
+ * interpolateChargedensity, resetCurrent, interpolateToGrid }.
+ * opx_simulation_steps_done returns the integer clock. */
+int opx_simulation_initialize(opx_ctx *ctx);
+int opx_simulation_step(opx_ctx *ctx, int n);
+int opx_simulation_steps_done(const opx_ctx *ctx);
+
+/* Health monitor: counts non-finite values and links with |U U^dagger - 1|
+ * above tol (UnitarityTester, diagnostics/methods/UnitarityTester.java:31-46). */
+int opx_check_health(opx_ctx *ctx, double tol, int64_t *non_finite, int64_t *non_unitary);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

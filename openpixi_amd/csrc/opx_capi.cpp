@@ -1,0 +1,746 @@
+// opx_capi.cpp -- C ABI over the gfx950 leapfrog kernels (declared in include/opx.h).
+//
+// The context owns all device memory. Host buffers cross the ABI in the
+// reference's per-cell element order and are converted here to the device's
+// structure-of-arrays layout (and to the context precision).
+#include "opx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "opx_kernels.h"
+
+struct opx_ctx {
+    int nc = 2;
+    int prec = OPX_FP64;
+    int dims[3] = {0, 0, 0};
+    double as[3] = {1, 1, 1};
+    double at = 0.0;
+    double g = 1.0;
+    int64_t n = 0;
+    int G = 4;     // reals per link
+    int AD = 3;    // algebra coefficients on the device
+    int AS = 3;    // algebra reals in reference storage
+    size_t tsz = 8;
+    void* U = nullptr;
+    void* Unext = nullptr;
+    void* E = nullptr;
+    void* J = nullptr;
+    void* rho = nullptr;
+    bool j_nonzero = false;
+    opx::Box active{{0, 0, 0}, {0, 0, 0}, 1};
+    opx::Box eval{{0, 0, 0}, {0, 0, 0}, 1};
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    double* partials = nullptr;
+    double* dout = nullptr;
+    unsigned long long* counters = nullptr;
+    // super particles
+    int sp_count = 0;
+    int sp_max_num = 0;
+    int64_t sp_total = 0;
+    int* sp_desc = nullptr;         // 6 arrays of sp_count ints, back to back
+    int64_t* sp_qoff = nullptr;
+    void* sp_q = nullptr;           // [coef][total]
+    bool rho_pending = false;       // charge density deposit deferred until rho is read
+    int rho_pending_t = 0;
+    int steps = 0;                  // Simulation.totalSimulationSteps
+    std::string err;
+};
+
+namespace {
+
+int fail(opx_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_check(opx_ctx* c, hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return OPX_OK;
+    return fail(c, OPX_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define OPX_TRY(expr, what)                               \
+    do {                                                  \
+        int _rc = hip_check(ctx, (expr), (what));         \
+        if (_rc != OPX_OK) return _rc;                    \
+    } while (0)
+
+opx::Lattice lattice(const opx_ctx* c, double at)
+{
+    opx::Lattice L;
+    L.n = (uint32_t)c->n;
+    L.d0 = (uint32_t)c->dims[0];
+    L.d1 = (uint32_t)c->dims[1];
+    L.d2 = (uint32_t)c->dims[2];
+    L.s0 = L.d1 * L.d2;
+    L.s1 = L.d2;
+    L.at = at;
+    for (int i = 0; i < 3; i++) {
+        L.uf[i] = -(c->g * c->as[i]) * c->at;
+        L.area[i] = 1.0 / std::pow(c->as[i], 2);
+        L.as[i] = c->as[i];
+    }
+    L.g = c->g;
+    return L;
+}
+
+size_t link_count(const opx_ctx* c) { return (size_t)c->n * 3 * c->G; }
+size_t alg_count(const opx_ctx* c) { return (size_t)c->n * 3 * c->AD; }
+size_t rho_count(const opx_ctx* c) { return (size_t)c->n * c->AD; }
+
+// SU(3) Hermitian storage <-> Gell-Mann coefficients
+// (math/SU3AlgebraElement.java:93-129: get(j) and the incremental set(j))
+void su3_storage_to_coeffs(const double* v, double* c)
+{
+    const double s3 = std::sqrt(3.0);
+    c[0] = 2 * v[1];
+    c[1] = 2 * v[3];
+    c[2] = v[0] - v[4];
+    c[3] = 2 * v[2];
+    c[4] = 2 * v[6];
+    c[5] = 2 * v[5];
+    c[6] = 2 * v[7];
+    c[7] = (v[0] + v[4] - 2 * v[8]) / s3;
+}
+
+void su3_coeffs_to_storage(const double* c, double* v)
+{
+    for (int k = 0; k < 9; k++) v[k] = 0.0;
+    double cur[8];
+    const double s3 = std::sqrt(3.0);
+    for (int j = 0; j < 8; j++) {
+        su3_storage_to_coeffs(v, cur);
+        double diff = (c[j] - cur[j]) / 2;
+        switch (j) {
+        case 0: v[1] += diff; break;
+        case 1: v[3] += diff; break;
+        case 2: v[0] += diff; v[4] -= diff; break;
+        case 3: v[2] += diff; break;
+        case 4: v[6] += diff; break;
+        case 5: v[5] += diff; break;
+        case 6: v[7] += diff; break;
+        case 7: v[0] += diff / s3; v[4] += diff / s3; v[8] -= 2 * diff / s3; break;
+        }
+    }
+}
+
+template <typename T>
+int copy_to_device(opx_ctx* ctx, void* dst, const std::vector<double>& src)
+{
+    std::vector<T> tmp(src.size());
+    for (size_t i = 0; i < src.size(); i++) tmp[i] = (T)src[i];
+    OPX_TRY(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+    OPX_TRY(hipStreamSynchronize(ctx->stream), "upload sync");
+    return OPX_OK;
+}
+
+template <typename T>
+int copy_from_device(opx_ctx* ctx, const void* src, std::vector<double>& dst)
+{
+    std::vector<T> tmp(dst.size());
+    OPX_TRY(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+    OPX_TRY(hipStreamSynchronize(ctx->stream), "download sync");
+    for (size_t i = 0; i < dst.size(); i++) dst[i] = (double)tmp[i];
+    return OPX_OK;
+}
+
+int to_device(opx_ctx* ctx, void* dst, const std::vector<double>& soa)
+{
+    return ctx->prec == OPX_FP64 ? copy_to_device<double>(ctx, dst, soa) : copy_to_device<float>(ctx, dst, soa);
+}
+
+int from_device(opx_ctx* ctx, const void* src, std::vector<double>& soa)
+{
+    return ctx->prec == OPX_FP64 ? copy_from_device<double>(ctx, src, soa) : copy_from_device<float>(ctx, src, soa);
+}
+
+void* field_ptr(opx_ctx* c, int field)
+{
+    switch (field) {
+    case OPX_FIELD_U: return c->U;
+    case OPX_FIELD_UNEXT: return c->Unext;
+    case OPX_FIELD_E: return c->E;
+    case OPX_FIELD_J: return c->J;
+    case OPX_FIELD_RHO: return c->rho;
+    default: return nullptr;
+    }
+}
+
+int check_box(opx_ctx* ctx, const int* lo, const int* hi)
+{
+    if (!lo || !hi) return fail(ctx, OPX_E_ARG, "region bounds are null");
+    return OPX_OK;
+}
+
+int materialize_rho(opx_ctx* ctx);
+
+}  // namespace
+
+extern "C" {
+
+int opx_create(int num_colors, int num_dims, const int* num_cells, const double* spacing, double time_step,
+               double coupling, int precision, opx_ctx** out)
+{
+    if (!out || !num_cells || !spacing) return OPX_E_ARG;
+    *out = nullptr;
+    if (num_colors != 2 && num_colors != 3) return OPX_E_UNSUPPORTED;
+    if (num_dims != 3) return OPX_E_UNSUPPORTED;
+    if (precision != OPX_FP64 && precision != OPX_FP32) return OPX_E_UNSUPPORTED;
+    int64_t n = 1;
+    for (int i = 0; i < 3; i++) {
+        if (num_cells[i] < 1) return OPX_E_ARG;
+        if (!(spacing[i] > 0)) return OPX_E_ARG;
+        n *= num_cells[i];
+    }
+    if (n > (int64_t)0x7fffffff) return OPX_E_UNSUPPORTED;
+    opx_ctx* ctx = new opx_ctx();
+    ctx->nc = num_colors;
+    ctx->prec = precision;
+    for (int i = 0; i < 3; i++) {
+        ctx->dims[i] = num_cells[i];
+        ctx->as[i] = spacing[i];
+    }
+    ctx->at = time_step;
+    ctx->g = coupling;
+    ctx->n = n;
+    ctx->G = num_colors == 2 ? 4 : 18;
+    ctx->AD = num_colors == 2 ? 3 : 8;
+    ctx->AS = num_colors == 2 ? 3 : 9;
+    ctx->tsz = precision == OPX_FP64 ? 8 : 4;
+    int rc = hip_check(ctx, hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking), "stream");
+    if (rc == OPX_OK) {
+        ctx->stream = ctx->own_stream;
+        rc = hip_check(ctx, hipMalloc(&ctx->U, link_count(ctx) * ctx->tsz), "alloc U");
+    }
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&ctx->Unext, link_count(ctx) * ctx->tsz), "alloc Unext");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&ctx->E, alg_count(ctx) * ctx->tsz), "alloc E");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&ctx->J, alg_count(ctx) * ctx->tsz), "alloc J");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&ctx->rho, rho_count(ctx) * ctx->tsz), "alloc rho");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMalloc(&ctx->partials, opx::measure_partials_size((uint32_t)n) * sizeof(double)),
+                       "alloc partials");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&ctx->dout, 64 * sizeof(double)), "alloc out");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&ctx->counters, 4 * sizeof(unsigned long long)), "alloc counters");
+    if (rc == OPX_OK) {
+        // identities (physics/grid/Cell.java:63-68) and zero fields
+        std::vector<double> soa(link_count(ctx), 0.0);
+        for (int d = 0; d < 3; d++) {
+            for (int k : (num_colors == 2 ? std::vector<int>{0} : std::vector<int>{0, 4, 8})) {
+                double* p = soa.data() + ((size_t)d * ctx->G + k) * n;
+                for (int64_t s = 0; s < n; s++) p[s] = 1.0;
+            }
+        }
+        rc = to_device(ctx, ctx->U, soa);
+        if (rc == OPX_OK) rc = to_device(ctx, ctx->Unext, soa);
+    }
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMemsetAsync(ctx->E, 0, alg_count(ctx) * ctx->tsz, ctx->stream), "zero E");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMemsetAsync(ctx->J, 0, alg_count(ctx) * ctx->tsz, ctx->stream), "zero J");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemsetAsync(ctx->rho, 0, rho_count(ctx) * ctx->tsz, ctx->stream), "zero rho");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "create sync");
+    if (rc != OPX_OK) {
+        std::fprintf(stderr, "opx_create: %s\n", ctx->err.c_str());
+        opx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return OPX_OK;
+}
+
+void opx_destroy(opx_ctx* ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->partials, ctx->dout, ctx->counters,
+                    ctx->sp_desc, ctx->sp_qoff, ctx->sp_q};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+const char* opx_last_error(const opx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int opx_get_info(const opx_ctx* ctx, int64_t* num_cells_total, int* num_colors, int* precision)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (num_cells_total) *num_cells_total = ctx->n;
+    if (num_colors) *num_colors = ctx->nc;
+    if (precision) *precision = ctx->prec;
+    return OPX_OK;
+}
+
+int opx_set_stream(opx_ctx* ctx, void* hip_stream)
+{
+    if (!ctx) return OPX_E_ARG;
+    OPX_TRY(hipStreamSynchronize(ctx->stream), "set_stream sync");
+    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    return OPX_OK;
+}
+
+int opx_sync(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    OPX_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return OPX_OK;
+}
+
+int opx_upload_field(opx_ctx* ctx, int field, const double* host, int64_t n_values)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!host) return fail(ctx, OPX_E_ARG, "upload: null buffer");
+    void* dst = field_ptr(ctx, field);
+    if (!dst) return fail(ctx, OPX_E_ARG, "upload: unknown field");
+    const int64_t n = ctx->n;
+    std::vector<double> soa;
+    if (field == OPX_FIELD_U || field == OPX_FIELD_UNEXT) {
+        if (n_values != n * 3 * ctx->G) return fail(ctx, OPX_E_ARG, "upload: link buffer size mismatch");
+        soa.resize(link_count(ctx));
+        for (int64_t s = 0; s < n; s++)
+            for (int d = 0; d < 3; d++)
+                for (int k = 0; k < ctx->G; k++)
+                    soa[((size_t)d * ctx->G + k) * n + s] = host[(s * 3 + d) * ctx->G + k];
+    } else {
+        const int ndir = field == OPX_FIELD_RHO ? 1 : 3;
+        if (n_values != n * ndir * ctx->AS) return fail(ctx, OPX_E_ARG, "upload: algebra buffer size mismatch");
+        soa.resize((size_t)n * ndir * ctx->AD);
+        for (int64_t s = 0; s < n; s++)
+            for (int d = 0; d < ndir; d++) {
+                const double* v = host + (s * ndir + d) * ctx->AS;
+                double c[8];
+                if (ctx->nc == 2) {
+                    c[0] = v[0]; c[1] = v[1]; c[2] = v[2];
+                } else {
+                    su3_storage_to_coeffs(v, c);
+                }
+                for (int k = 0; k < ctx->AD; k++) soa[((size_t)d * ctx->AD + k) * n + s] = c[k];
+            }
+        if (field == OPX_FIELD_J) ctx->j_nonzero = true;
+        if (field == OPX_FIELD_RHO) ctx->rho_pending = false;
+    }
+    return to_device(ctx, dst, soa);
+}
+
+int opx_download_field(opx_ctx* ctx, int field, double* host, int64_t n_values)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!host) return fail(ctx, OPX_E_ARG, "download: null buffer");
+    const void* src = field_ptr(ctx, field);
+    if (!src) return fail(ctx, OPX_E_ARG, "download: unknown field");
+    const int64_t n = ctx->n;
+    if (field == OPX_FIELD_RHO) {
+        int rc = materialize_rho(ctx);
+        if (rc != OPX_OK) return rc;
+    }
+    if (field == OPX_FIELD_U || field == OPX_FIELD_UNEXT) {
+        if (n_values != n * 3 * ctx->G) return fail(ctx, OPX_E_ARG, "download: link buffer size mismatch");
+        std::vector<double> soa(link_count(ctx));
+        int rc = from_device(ctx, src, soa);
+        if (rc != OPX_OK) return rc;
+        for (int64_t s = 0; s < n; s++)
+            for (int d = 0; d < 3; d++)
+                for (int k = 0; k < ctx->G; k++) host[(s * 3 + d) * ctx->G + k] = soa[((size_t)d * ctx->G + k) * n + s];
+        return OPX_OK;
+    }
+    const int ndir = field == OPX_FIELD_RHO ? 1 : 3;
+    if (n_values != n * ndir * ctx->AS) return fail(ctx, OPX_E_ARG, "download: algebra buffer size mismatch");
+    std::vector<double> soa((size_t)n * ndir * ctx->AD);
+    int rc = from_device(ctx, src, soa);
+    if (rc != OPX_OK) return rc;
+    for (int64_t s = 0; s < n; s++)
+        for (int d = 0; d < ndir; d++) {
+            double c[8];
+            for (int k = 0; k < ctx->AD; k++) c[k] = soa[((size_t)d * ctx->AD + k) * n + s];
+            double* v = host + (s * ndir + d) * ctx->AS;
+            if (ctx->nc == 2) {
+                v[0] = c[0]; v[1] = c[1]; v[2] = c[2];
+            } else {
+                su3_coeffs_to_storage(c, v);
+            }
+        }
+    return OPX_OK;
+}
+
+int opx_set_links_from_coefficients(opx_ctx* ctx, int field, const double* coeffs, int64_t n_values)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!coeffs) return fail(ctx, OPX_E_ARG, "coefficients: null buffer");
+    if (field != OPX_FIELD_U && field != OPX_FIELD_UNEXT) return fail(ctx, OPX_E_ARG, "coefficients: not a link field");
+    const int64_t n = ctx->n;
+    if (n_values != n * 3 * ctx->AD) return fail(ctx, OPX_E_ARG, "coefficients: size mismatch");
+    std::vector<double> soa((size_t)n_values);
+    for (int64_t s = 0; s < n; s++)
+        for (int d = 0; d < 3; d++)
+            for (int k = 0; k < ctx->AD; k++) soa[((size_t)d * ctx->AD + k) * n + s] = coeffs[(s * 3 + d) * ctx->AD + k];
+    double* dc = nullptr;
+    OPX_TRY(hipMalloc(&dc, soa.size() * sizeof(double)), "alloc coefficients");
+    int rc = hip_check(ctx, hipMemcpyAsync(dc, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+                       "upload coefficients");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, opx::launch_links_from_coeffs(ctx->nc, ctx->prec, (uint32_t)n, dc, field_ptr(ctx, field),
+                                                          ctx->stream),
+                       "links_from_coefficients");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "coefficients sync");
+    (void)hipFree(dc);
+    return rc;
+}
+
+int opx_set_efield_from_coefficients(opx_ctx* ctx, const double* coeffs, int64_t n_values)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!coeffs) return fail(ctx, OPX_E_ARG, "coefficients: null buffer");
+    const int64_t n = ctx->n;
+    if (n_values != n * 3 * ctx->AD) return fail(ctx, OPX_E_ARG, "coefficients: size mismatch");
+    std::vector<double> soa((size_t)n_values);
+    for (int64_t s = 0; s < n; s++)
+        for (int d = 0; d < 3; d++)
+            for (int k = 0; k < ctx->AD; k++) soa[((size_t)d * ctx->AD + k) * n + s] = coeffs[(s * 3 + d) * ctx->AD + k];
+    return to_device(ctx, ctx->E, soa);
+}
+
+int opx_set_active_region(opx_ctx* ctx, const int* lo, const int* hi)
+{
+    if (!ctx) return OPX_E_ARG;
+    int rc = check_box(ctx, lo, hi);
+    if (rc != OPX_OK) return rc;
+    for (int i = 0; i < 3; i++) {
+        ctx->active.lo[i] = lo[i];
+        ctx->active.hi[i] = hi[i];
+    }
+    ctx->active.all = 0;
+    return OPX_OK;
+}
+
+int opx_set_evaluation_region(opx_ctx* ctx, const int* lo, const int* hi)
+{
+    if (!ctx) return OPX_E_ARG;
+    int rc = check_box(ctx, lo, hi);
+    if (rc != OPX_OK) return rc;
+    for (int i = 0; i < 3; i++) {
+        ctx->eval.lo[i] = lo[i];
+        ctx->eval.hi[i] = hi[i];
+    }
+    ctx->eval.all = 0;
+    return OPX_OK;
+}
+
+int opx_step(opx_ctx* ctx, double dt)
+{
+    if (!ctx) return OPX_E_ARG;
+    opx::Lattice L = lattice(ctx, dt);
+    OPX_TRY(opx::launch_combined_update(ctx->nc, ctx->prec, L, ctx->active, ctx->U, ctx->Unext, ctx->E, ctx->J,
+                                        ctx->j_nonzero, ctx->stream),
+            "step");
+    return OPX_OK;
+}
+
+int opx_step_links(opx_ctx* ctx, double dt)
+{
+    if (!ctx) return OPX_E_ARG;
+    opx::Lattice L = lattice(ctx, dt);
+    OPX_TRY(opx::launch_step_links(ctx->nc, ctx->prec, L, ctx->active, ctx->U, ctx->Unext, ctx->E, ctx->stream),
+            "step_links");
+    return OPX_OK;
+}
+
+int opx_store_fields(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    std::swap(ctx->U, ctx->Unext);
+    return OPX_OK;
+}
+
+int opx_reset_current(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    OPX_TRY(hipMemsetAsync(ctx->J, 0, alg_count(ctx) * ctx->tsz, ctx->stream), "reset current");
+    ctx->j_nonzero = false;
+    return OPX_OK;
+}
+
+int opx_reset_charge(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    OPX_TRY(hipMemsetAsync(ctx->rho, 0, rho_count(ctx) * ctx->tsz, ctx->stream), "reset charge");
+    ctx->rho_pending = false;
+    return OPX_OK;
+}
+
+int opx_reunitarize(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    OPX_TRY(opx::launch_reunitarize(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->U, ctx->stream), "reunitarize U");
+    OPX_TRY(opx::launch_reunitarize(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->Unext, ctx->stream),
+            "reunitarize Unext");
+    return OPX_OK;
+}
+
+int opx_leapfrog(opx_ctx* ctx, int steps, int reunit_every)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (steps < 0 || reunit_every < 0) return fail(ctx, OPX_E_ARG, "leapfrog: negative count");
+    opx::Lattice L = lattice(ctx, ctx->at);
+    for (int s = 0; s < steps; s++) {
+        std::swap(ctx->U, ctx->Unext);
+        OPX_TRY(opx::launch_combined_update(ctx->nc, ctx->prec, L, ctx->active, ctx->U, ctx->Unext, ctx->E, ctx->J,
+                                            ctx->j_nonzero, ctx->stream),
+                "leapfrog step");
+        ctx->steps++;
+        if (reunit_every > 0 && ctx->steps % reunit_every == 0)
+            OPX_TRY(opx::launch_reunitarize(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->Unext, ctx->stream),
+                    "leapfrog reunitarize");
+    }
+    return OPX_OK;
+}
+
+int opx_measure(opx_ctx* ctx, double* out)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!out) return fail(ctx, OPX_E_ARG, "measure: null output");
+    int rc = materialize_rho(ctx);
+    if (rc != OPX_OK) return rc;
+    opx::Lattice L = lattice(ctx, ctx->at);
+    OPX_TRY(opx::launch_measure(ctx->nc, ctx->prec, L, ctx->eval, ctx->U, ctx->Unext, ctx->E, ctx->rho, ctx->partials,
+                                ctx->dout, ctx->stream),
+            "measure");
+    OPX_TRY(hipMemcpyAsync(out, ctx->dout, OPX_M_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+            "measure copy");
+    OPX_TRY(hipStreamSynchronize(ctx->stream), "measure sync");
+    return OPX_OK;
+}
+
+int opx_projected_energy_density(opx_ctx* ctx, int direction, double* eT_el, double* eT_mag, double* eL_el,
+                                 double* eL_mag)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (direction < 0 || direction > 2) return fail(ctx, OPX_E_ARG, "projected energy: bad direction");
+    if (!eT_el || !eT_mag || !eL_el || !eL_mag) return fail(ctx, OPX_E_ARG, "projected energy: null output");
+    const int nd = ctx->dims[direction];
+    double* dbuf = nullptr;
+    OPX_TRY(hipMalloc(&dbuf, (size_t)4 * nd * sizeof(double)), "alloc projected energy");
+    opx::Lattice L = lattice(ctx, ctx->at);
+    int rc = hip_check(ctx, opx::launch_projected_energy(ctx->nc, ctx->prec, L, ctx->eval, direction, ctx->U,
+                                                         ctx->Unext, ctx->E, dbuf, ctx->stream),
+                       "projected energy");
+    std::vector<double> h((size_t)4 * nd);
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemcpyAsync(h.data(), dbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                       "projected energy copy");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "projected energy sync");
+    (void)hipFree(dbuf);
+    if (rc != OPX_OK) return rc;
+    std::memcpy(eT_el, h.data(), nd * sizeof(double));
+    std::memcpy(eT_mag, h.data() + nd, nd * sizeof(double));
+    std::memcpy(eL_el, h.data() + 2 * (size_t)nd, nd * sizeof(double));
+    std::memcpy(eL_mag, h.data() + 3 * (size_t)nd, nd * sizeof(double));
+    return OPX_OK;
+}
+
+int opx_superparticles_upload(opx_ctx* ctx, int count, const int* orientation, const int* num,
+                              const int* index_offset, const int* per_plane, const int* sub_lattice_shift,
+                              const int* particles_per_cell, const double* charges, int64_t n_values)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (count < 0) return fail(ctx, OPX_E_ARG, "superparticles: negative count");
+    if (count > 0 && (!orientation || !num || !index_offset || !per_plane || !sub_lattice_shift ||
+                      !particles_per_cell || !charges))
+        return fail(ctx, OPX_E_ARG, "superparticles: null array");
+    int64_t total = 0;
+    int maxn = 0;
+    std::vector<int> desc((size_t)6 * count);
+    std::vector<int64_t> qoff((size_t)count);
+    for (int p = 0; p < count; p++) {
+        if (num[p] < 0) return fail(ctx, OPX_E_ARG, "superparticles: negative particle number");
+        if (particles_per_cell[p] < 1) return fail(ctx, OPX_E_ARG, "superparticles: particlePerCell must be >= 1");
+        if (orientation[p] == 0) return fail(ctx, OPX_E_ARG, "superparticles: orientation must be +1 or -1");
+        qoff[p] = total;
+        total += num[p];
+        if (num[p] > maxn) maxn = num[p];
+        desc[0 * (size_t)count + p] = orientation[p];
+        desc[1 * (size_t)count + p] = num[p];
+        desc[2 * (size_t)count + p] = index_offset[p];
+        desc[3 * (size_t)count + p] = per_plane[p];
+        desc[4 * (size_t)count + p] = sub_lattice_shift[p];
+        desc[5 * (size_t)count + p] = particles_per_cell[p];
+    }
+    if (n_values != total * ctx->AS) return fail(ctx, OPX_E_ARG, "superparticles: charge buffer size mismatch");
+    if (ctx->sp_desc) (void)hipFree(ctx->sp_desc);
+    if (ctx->sp_qoff) (void)hipFree(ctx->sp_qoff);
+    if (ctx->sp_q) (void)hipFree(ctx->sp_q);
+    ctx->sp_desc = nullptr;
+    ctx->sp_qoff = nullptr;
+    ctx->sp_q = nullptr;
+    ctx->sp_count = 0;
+    ctx->sp_total = 0;
+    ctx->sp_max_num = 0;
+    if (count == 0) return OPX_OK;
+    OPX_TRY(hipMalloc(&ctx->sp_desc, desc.size() * sizeof(int)), "alloc sp desc");
+    OPX_TRY(hipMalloc(&ctx->sp_qoff, qoff.size() * sizeof(int64_t)), "alloc sp qoff");
+    OPX_TRY(hipMalloc(&ctx->sp_q, (size_t)(total > 0 ? total : 1) * ctx->AD * ctx->tsz), "alloc sp charges");
+    OPX_TRY(hipMemcpyAsync(ctx->sp_desc, desc.data(), desc.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream),
+            "upload sp desc");
+    OPX_TRY(hipMemcpyAsync(ctx->sp_qoff, qoff.data(), qoff.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream),
+            "upload sp qoff");
+    std::vector<double> soa((size_t)total * ctx->AD);
+    for (int64_t q = 0; q < total; q++) {
+        double c[8];
+        const double* v = charges + q * ctx->AS;
+        if (ctx->nc == 2) {
+            c[0] = v[0]; c[1] = v[1]; c[2] = v[2];
+        } else {
+            su3_storage_to_coeffs(v, c);
+        }
+        for (int k = 0; k < ctx->AD; k++) soa[(size_t)k * total + q] = c[k];
+    }
+    ctx->sp_count = count;
+    ctx->sp_total = total;
+    ctx->sp_max_num = maxn;
+    if (total > 0) return to_device(ctx, ctx->sp_q, soa);
+    return OPX_OK;
+}
+
+int opx_superparticles_download_charges(opx_ctx* ctx, double* charges, int64_t n_values)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!charges) return fail(ctx, OPX_E_ARG, "superparticles: null buffer");
+    if (n_values != ctx->sp_total * ctx->AS) return fail(ctx, OPX_E_ARG, "superparticles: charge buffer size mismatch");
+    if (ctx->sp_total == 0) return OPX_OK;
+    std::vector<double> soa((size_t)ctx->sp_total * ctx->AD);
+    int rc = from_device(ctx, ctx->sp_q, soa);
+    if (rc != OPX_OK) return rc;
+    for (int64_t q = 0; q < ctx->sp_total; q++) {
+        double c[8];
+        for (int k = 0; k < ctx->AD; k++) c[k] = soa[(size_t)k * ctx->sp_total + q];
+        double* v = charges + q * ctx->AS;
+        if (ctx->nc == 2) {
+            v[0] = c[0]; v[1] = c[1]; v[2] = c[2];
+        } else {
+            su3_coeffs_to_storage(c, v);
+        }
+    }
+    return OPX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+opx::SuperParticles sp_view(const opx_ctx* c)
+{
+    opx::SuperParticles sp;
+    sp.count = c->sp_count;
+    sp.max_num = c->sp_max_num;
+    const int* d = c->sp_desc;
+    const size_t k = (size_t)c->sp_count;
+    sp.orientation = d;
+    sp.num = d ? d + k : nullptr;
+    sp.index_offset = d ? d + 2 * k : nullptr;
+    sp.per_plane = d ? d + 3 * k : nullptr;
+    sp.shift = d ? d + 4 * k : nullptr;
+    sp.ppc = d ? d + 5 * k : nullptr;
+    sp.q_offset = c->sp_qoff;
+    sp.total = c->sp_total;
+    return sp;
+}
+
+int materialize_rho(opx_ctx* ctx)
+{
+    if (!ctx->rho_pending) return OPX_OK;
+    opx::Lattice L = lattice(ctx, ctx->at);
+    OPX_TRY(opx::launch_sp_chargedensity(ctx->nc, ctx->prec, L, sp_view(ctx), ctx->rho_pending_t, ctx->sp_q, ctx->rho,
+                                         ctx->stream),
+            "charge density");
+    ctx->rho_pending = false;
+    return OPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int opx_interpolate_to_grid(opx_ctx* ctx, int t)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (ctx->sp_count == 0) return OPX_OK;
+    opx::Lattice L = lattice(ctx, ctx->at);
+    OPX_TRY(opx::launch_sp_to_grid(ctx->nc, ctx->prec, L, sp_view(ctx), t, ctx->sp_q, ctx->Unext, ctx->J, ctx->stream),
+            "interpolate to grid");
+    ctx->j_nonzero = true;
+    return OPX_OK;
+}
+
+int opx_interpolate_chargedensity(opx_ctx* ctx, int t)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (ctx->sp_count == 0) return OPX_OK;
+    // rho is only read by measurements and downloads; the gather runs then,
+    // with the charges as they are now (they change again in interpolateToGrid).
+    // Deferral is exact only if nothing modifies the charges first, so it is
+    // resolved immediately here.
+    ctx->rho_pending = true;
+    ctx->rho_pending_t = t;
+    return materialize_rho(ctx);
+}
+
+int opx_simulation_initialize(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    int rc;
+    if ((rc = opx_step_links(ctx, ctx->at)) != OPX_OK) return rc;
+    if ((rc = opx_reset_charge(ctx)) != OPX_OK) return rc;
+    if ((rc = opx_interpolate_chargedensity(ctx, 0)) != OPX_OK) return rc;
+    if ((rc = opx_reset_current(ctx)) != OPX_OK) return rc;
+    if ((rc = opx_interpolate_to_grid(ctx, 0)) != OPX_OK) return rc;
+    return OPX_OK;
+}
+
+int opx_simulation_step(opx_ctx* ctx, int n)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (n < 0) return fail(ctx, OPX_E_ARG, "simulation_step: negative count");
+    int rc;
+    for (int s = 0; s < n; s++) {
+        ctx->steps++;
+        const int t = ctx->steps;
+        if ((rc = opx_store_fields(ctx)) != OPX_OK) return rc;
+        if ((rc = opx_step(ctx, ctx->at)) != OPX_OK) return rc;
+        if (ctx->sp_count > 0) {
+            if ((rc = opx_reset_charge(ctx)) != OPX_OK) return rc;
+            if ((rc = opx_interpolate_chargedensity(ctx, t)) != OPX_OK) return rc;
+            if ((rc = opx_reset_current(ctx)) != OPX_OK) return rc;
+            if ((rc = opx_interpolate_to_grid(ctx, t)) != OPX_OK) return rc;
+        }
+    }
+    return OPX_OK;
+}
+
+int opx_simulation_steps_done(const opx_ctx* ctx) { return ctx ? ctx->steps : -1; }
+
+int opx_check_health(opx_ctx* ctx, double tol, int64_t* non_finite, int64_t* non_unitary)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!non_finite || !non_unitary) return fail(ctx, OPX_E_ARG, "health: null output");
+    OPX_TRY(hipMemsetAsync(ctx->counters, 0, 4 * sizeof(unsigned long long), ctx->stream), "health reset");
+    OPX_TRY(opx::launch_health(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->U, tol, ctx->counters, ctx->stream),
+            "health U");
+    OPX_TRY(opx::launch_health(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->Unext, tol, ctx->counters, ctx->stream),
+            "health Unext");
+    OPX_TRY(opx::launch_health_alg(ctx->nc, ctx->prec, (uint32_t)alg_count(ctx), ctx->E, ctx->counters, ctx->stream),
+            "health E");
+    unsigned long long h[4];
+    OPX_TRY(hipMemcpyAsync(h, ctx->counters, sizeof h, hipMemcpyDeviceToHost, ctx->stream), "health copy");
+    OPX_TRY(hipStreamSynchronize(ctx->stream), "health sync");
+    *non_finite = (int64_t)h[0];
+    *non_unitary = (int64_t)h[1];
+    return (h[0] || h[1]) ? fail(ctx, OPX_E_NUMERIC, "health monitor: non-finite or non-unitary links") : OPX_OK;
+}
+
+}  // extern "C"

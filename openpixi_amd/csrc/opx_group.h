@@ -1,0 +1,433 @@
+// opx_group.h -- SU(2)/SU(3) group and algebra arithmetic for gfx950 kernels.
+//
+// Conventions (identical to OpenPixi's math package):
+//   SU(2): U = e0 + i e_k sigma_k (4 reals), algebra X = sum_a v_a sigma_a/2 (3 coefficients)
+//          (math/SU2GroupElement.java:14, math/SU2AlgebraElement.java:5)
+//   SU(3): U = 3x3 complex matrix, algebra X = sum_a c_a lambda_a/2 with the
+//          standard Gell-Mann matrices (8 coefficients; the reference stores the
+//          9-real Hermitian matrix, math/SU3AlgebraElement.java:3-11, and its
+//          get(j) gives these coefficients, :117-129)
+//   getLink(X) = exp(+i X)      (SU2AlgebraElement.java:140-152, SU3AlgebraElement.java:407-417)
+//   proj(M)    = coefficients of the traceless part of (M - M^dagger)/(2i)
+//                (SU2GroupElement.java:254-263, SU3GroupElement.java:494-505)
+//   act(X, g)  = g X g^dagger   (AlgebraElement.java:99-110)
+//
+// All functions are templated on the storage/arithmetic type T (float or double).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OPX_HD __host__ __device__ __forceinline__
+
+namespace opx {
+
+OPX_HD float fsin(float x) { return sinf(x); }
+OPX_HD double fsin(double x) { return sin(x); }
+OPX_HD float fcos(float x) { return cosf(x); }
+OPX_HD double fcos(double x) { return cos(x); }
+OPX_HD float fsqrt(float x) { return sqrtf(x); }
+OPX_HD double fsqrt(double x) { return sqrt(x); }
+
+// ------------------------------------------------------------------ SU(2)
+template <typename T>
+struct Su2 {
+    T e0, e1, e2, e3;
+};
+
+template <typename T>
+struct Alg2 {
+    T v[3];
+};
+
+template <typename T>
+OPX_HD Su2<T> mul(const Su2<T>& a, const Su2<T>& b)
+{
+    Su2<T> r;
+    r.e0 = a.e0 * b.e0 - a.e1 * b.e1 - a.e2 * b.e2 - a.e3 * b.e3;
+    r.e1 = a.e0 * b.e1 + a.e1 * b.e0 - a.e2 * b.e3 + a.e3 * b.e2;
+    r.e2 = a.e0 * b.e2 + a.e2 * b.e0 - a.e3 * b.e1 + a.e1 * b.e3;
+    r.e3 = a.e0 * b.e3 + a.e3 * b.e0 - a.e1 * b.e2 + a.e2 * b.e1;
+    return r;
+}
+
+template <typename T>
+OPX_HD Su2<T> adj(const Su2<T>& a)
+{
+    return Su2<T>{a.e0, -a.e1, -a.e2, -a.e3};
+}
+
+// a * b^dagger
+template <typename T>
+OPX_HD Su2<T> mul_adj(const Su2<T>& a, const Su2<T>& b)
+{
+    return mul(a, adj(b));
+}
+
+// a^dagger * b
+template <typename T>
+OPX_HD Su2<T> adj_mul(const Su2<T>& a, const Su2<T>& b)
+{
+    return mul(adj(a), b);
+}
+
+template <typename T>
+OPX_HD void acc(Su2<T>& s, const Su2<T>& a)
+{
+    s.e0 += a.e0; s.e1 += a.e1; s.e2 += a.e2; s.e3 += a.e3;
+}
+
+template <typename T>
+OPX_HD Su2<T> scaled(const Su2<T>& a, T f)
+{
+    return Su2<T>{a.e0 * f, a.e1 * f, a.e2 * f, a.e3 * f};
+}
+
+template <typename T>
+OPX_HD Alg2<T> proj(const Su2<T>& a)
+{
+    return Alg2<T>{{2 * a.e1, 2 * a.e2, 2 * a.e3}};
+}
+
+// exp(i * f * sum_a v_a sigma_a / 2) (SU2AlgebraElement.getLink applied to v*f)
+template <typename T>
+OPX_HD Su2<T> exp_link(const Alg2<T>& v, T f)
+{
+    T w0 = v.v[0] * f, w1 = v.v[1] * f, w2 = v.v[2] * f;
+    T sum = w0 * w0 + w1 * w1 + w2 * w2;
+    T mod = fsqrt(sum) / 2;
+    T sinfakt = (mod < T(1e-20)) ? T(0) : T(0.5) / mod * fsin(mod);
+    return Su2<T>{fcos(mod), w0 * sinfakt, w1 * sinfakt, w2 * sinfakt};
+}
+
+// g X g^dagger
+template <typename T>
+OPX_HD Alg2<T> act(const Alg2<T>& x, const Su2<T>& g)
+{
+    Su2<T> xm{T(0), x.v[0] / 2, x.v[1] / 2, x.v[2] / 2};
+    return proj(mul(g, mul(xm, adj(g))));
+}
+
+template <typename T>
+OPX_HD T square(const Alg2<T>& x)
+{
+    return x.v[0] * x.v[0] + x.v[1] * x.v[1] + x.v[2] * x.v[2];
+}
+
+// ------------------------------------------------------------------ SU(3)
+template <typename T>
+struct Su3 {
+    T r[9];   // real parts, row-major
+    T i[9];   // imaginary parts, row-major
+};
+
+template <typename T>
+struct Alg3 {
+    T c[8];
+};
+
+template <typename T>
+OPX_HD Su3<T> mul(const Su3<T>& a, const Su3<T>& b)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            T re = a.r[3 * r] * b.r[c] - a.i[3 * r] * b.i[c];
+            T im = a.r[3 * r] * b.i[c] + a.i[3 * r] * b.r[c];
+#pragma unroll
+            for (int k = 1; k < 3; k++) {
+                re += a.r[3 * r + k] * b.r[3 * k + c] - a.i[3 * r + k] * b.i[3 * k + c];
+                im += a.r[3 * r + k] * b.i[3 * k + c] + a.i[3 * r + k] * b.r[3 * k + c];
+            }
+            o.r[3 * r + c] = re;
+            o.i[3 * r + c] = im;
+        }
+    }
+    return o;
+}
+
+template <typename T>
+OPX_HD Su3<T> adj(const Su3<T>& a)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            o.r[3 * r + c] = a.r[3 * c + r];
+            o.i[3 * r + c] = -a.i[3 * c + r];
+        }
+    return o;
+}
+
+// a * b^dagger
+template <typename T>
+OPX_HD Su3<T> mul_adj(const Su3<T>& a, const Su3<T>& b)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            // sum_k a[r][k] * conj(b[c][k])
+            T re = a.r[3 * r] * b.r[3 * c] + a.i[3 * r] * b.i[3 * c];
+            T im = a.i[3 * r] * b.r[3 * c] - a.r[3 * r] * b.i[3 * c];
+#pragma unroll
+            for (int k = 1; k < 3; k++) {
+                re += a.r[3 * r + k] * b.r[3 * c + k] + a.i[3 * r + k] * b.i[3 * c + k];
+                im += a.i[3 * r + k] * b.r[3 * c + k] - a.r[3 * r + k] * b.i[3 * c + k];
+            }
+            o.r[3 * r + c] = re;
+            o.i[3 * r + c] = im;
+        }
+    }
+    return o;
+}
+
+// a^dagger * b
+template <typename T>
+OPX_HD Su3<T> adj_mul(const Su3<T>& a, const Su3<T>& b)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            // sum_k conj(a[k][r]) * b[k][c]
+            T re = a.r[r] * b.r[c] + a.i[r] * b.i[c];
+            T im = a.r[r] * b.i[c] - a.i[r] * b.r[c];
+#pragma unroll
+            for (int k = 1; k < 3; k++) {
+                re += a.r[3 * k + r] * b.r[3 * k + c] + a.i[3 * k + r] * b.i[3 * k + c];
+                im += a.r[3 * k + r] * b.i[3 * k + c] - a.i[3 * k + r] * b.r[3 * k + c];
+            }
+            o.r[3 * r + c] = re;
+            o.i[3 * r + c] = im;
+        }
+    }
+    return o;
+}
+
+template <typename T>
+OPX_HD void acc(Su3<T>& s, const Su3<T>& a)
+{
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        s.r[k] += a.r[k];
+        s.i[k] += a.i[k];
+    }
+}
+
+template <typename T>
+OPX_HD Su3<T> scaled(const Su3<T>& a, T f)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        o.r[k] = a.r[k] * f;
+        o.i[k] = a.i[k] * f;
+    }
+    return o;
+}
+
+template <typename T>
+OPX_HD T inv_sqrt3()
+{
+    return T(0.57735026918962576450914878050195746);
+}
+
+// Gell-Mann coefficients of the traceless part of (M - M^dagger)/(2i).
+template <typename T>
+OPX_HD Alg3<T> proj(const Su3<T>& m)
+{
+    Alg3<T> a;
+    a.c[0] = m.i[1] + m.i[3];
+    a.c[1] = m.r[1] - m.r[3];
+    a.c[2] = m.i[0] - m.i[4];
+    a.c[3] = m.i[2] + m.i[6];
+    a.c[4] = m.r[2] - m.r[6];
+    a.c[5] = m.i[5] + m.i[7];
+    a.c[6] = m.r[5] - m.r[7];
+    a.c[7] = (m.i[0] + m.i[4] - 2 * m.i[8]) * inv_sqrt3<T>();
+    return a;
+}
+
+// Hermitian matrix H = sum_a c_a lambda_a / 2 (scaled by f)
+template <typename T>
+OPX_HD Su3<T> herm(const Alg3<T>& x, T f)
+{
+    const T h = T(0.5) * f;
+    const T d8 = x.c[7] * inv_sqrt3<T>() * h;
+    Su3<T> m;
+    m.r[0] = x.c[2] * h + d8;  m.i[0] = 0;
+    m.r[4] = -x.c[2] * h + d8; m.i[4] = 0;
+    m.r[8] = -2 * d8;          m.i[8] = 0;
+    m.r[1] = x.c[0] * h; m.i[1] = -x.c[1] * h;
+    m.r[3] = x.c[0] * h; m.i[3] = x.c[1] * h;
+    m.r[2] = x.c[3] * h; m.i[2] = -x.c[4] * h;
+    m.r[6] = x.c[3] * h; m.i[6] = x.c[4] * h;
+    m.r[5] = x.c[5] * h; m.i[5] = -x.c[6] * h;
+    m.r[7] = x.c[5] * h; m.i[7] = x.c[6] * h;
+    return m;
+}
+
+// Coefficients of a Hermitian matrix (inverse of herm with f = 1); uses the
+// Hermitian part of the off-diagonal entries.
+template <typename T>
+OPX_HD Alg3<T> herm_coeffs(const Su3<T>& r)
+{
+    Alg3<T> a;
+    a.c[0] = r.r[1] + r.r[3];
+    a.c[1] = r.i[3] - r.i[1];
+    a.c[2] = r.r[0] - r.r[4];
+    a.c[3] = r.r[2] + r.r[6];
+    a.c[4] = r.i[6] - r.i[2];
+    a.c[5] = r.r[5] + r.r[7];
+    a.c[6] = r.i[7] - r.i[5];
+    a.c[7] = (r.r[0] + r.r[4] - 2 * r.r[8]) * inv_sqrt3<T>();
+    return a;
+}
+
+// g X g^dagger
+template <typename T>
+OPX_HD Alg3<T> act(const Alg3<T>& x, const Su3<T>& g)
+{
+    Su3<T> h = herm(x, T(1));
+    return herm_coeffs(mul(g, mul_adj(h, g)));
+}
+
+template <typename T>
+OPX_HD T square(const Alg3<T>& x)
+{
+    T s = x.c[0] * x.c[0];
+#pragma unroll
+    for (int k = 1; k < 8; k++) s += x.c[k] * x.c[k];
+    return s;
+}
+
+template <typename T> struct ExpTerms;
+template <> struct ExpTerms<double> { static constexpr int n = 18; };
+template <> struct ExpTerms<float> { static constexpr int n = 10; };
+
+// exp(i Q) for Q = H(x) * f traceless Hermitian, via the Cayley-Hamilton reduced
+// power series: (iQ)^n / n! = a_n + b_n Q + c_n Q^2 with
+//   Q^3 = k1 Q + k0,  k1 = tr(Q^2)/2, k0 = det Q = tr(Q^3)/3,
+//   (a,b,c)_{n+1} = i (c_n k0, a_n + c_n k1, b_n) / (n+1).
+// When tr(Q^2) > 1 the argument is halved s times and the result squared s times,
+// so the truncated series always runs with spectral radius <= 1.
+template <typename T>
+OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
+{
+    Su3<T> q = herm(x, f);
+    // Q^2 (Hermitian)
+    Su3<T> q2 = mul(q, q);
+    T trq2 = q2.r[0] + q2.r[4] + q2.r[8];
+    int s = 0;
+    T sc = T(1);
+    while (trq2 * sc * sc > T(1) && s < 30) {
+        sc *= T(0.5);
+        s++;
+    }
+    if (s > 0) {
+        q = scaled(q, sc);
+        q2 = scaled(q2, sc * sc);
+        trq2 = trq2 * sc * sc;
+    }
+    T k1 = trq2 / 2;
+    // tr(Q^3) = sum_ij (Q^2)_ij Q_ji  (real)
+    T trq3 = 0;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            trq3 += q2.r[3 * r + c] * q.r[3 * c + r] - q2.i[3 * r + c] * q.i[3 * c + r];
+    T k0 = trq3 / 3;
+    // series
+    T ar = 1, ai = 0, br = 0, bi = 0, cr = 0, ci = 0;
+    T f0r = 1, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
+#pragma unroll
+    for (int n = 0; n < ExpTerms<T>::n; n++) {
+        T inv = T(1) / T(n + 1);
+        // new a = i c k0; new b = i (a + c k1); new c = i b
+        T nar = -ci * k0, nai = cr * k0;
+        T tbr = ar + cr * k1, tbi = ai + ci * k1;
+        T nbr = -tbi, nbi = tbr;
+        T ncr = -bi, nci = br;
+        ar = nar * inv; ai = nai * inv;
+        br = nbr * inv; bi = nbi * inv;
+        cr = ncr * inv; ci = nci * inv;
+        f0r += ar; f0i += ai;
+        f1r += br; f1i += bi;
+        f2r += cr; f2i += ci;
+    }
+    Su3<T> e;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        e.r[k] = f1r * q.r[k] - f1i * q.i[k] + f2r * q2.r[k] - f2i * q2.i[k];
+        e.i[k] = f1r * q.i[k] + f1i * q.r[k] + f2r * q2.i[k] + f2i * q2.r[k];
+    }
+    e.r[0] += f0r; e.i[0] += f0i;
+    e.r[4] += f0r; e.i[4] += f0i;
+    e.r[8] += f0r; e.i[8] += f0i;
+    for (int k = 0; k < s; k++) e = mul(e, e);
+    return e;
+}
+
+// --------------------------------------------------- group-generic traits
+template <int NC, typename T> struct Group;
+
+template <typename T>
+struct Group<2, T> {
+    using M = Su2<T>;
+    using A = Alg2<T>;
+    static constexpr int G = 4;   // reals per group element
+    static constexpr int AD = 3;  // algebra coefficients
+    OPX_HD static M zero() { return M{0, 0, 0, 0}; }
+    OPX_HD static M identity() { return M{1, 0, 0, 0}; }
+    OPX_HD static A azero() { return A{{0, 0, 0}}; }
+    OPX_HD static T& gref(M& m, int k) { return k == 0 ? m.e0 : (k == 1 ? m.e1 : (k == 2 ? m.e2 : m.e3)); }
+    OPX_HD static T gget(const M& m, int k) { return k == 0 ? m.e0 : (k == 1 ? m.e1 : (k == 2 ? m.e2 : m.e3)); }
+};
+
+template <typename T>
+struct Group<3, T> {
+    using M = Su3<T>;
+    using A = Alg3<T>;
+    static constexpr int G = 18;
+    static constexpr int AD = 8;
+    OPX_HD static M zero()
+    {
+        M m;
+#pragma unroll
+        for (int k = 0; k < 9; k++) m.r[k] = m.i[k] = 0;
+        return m;
+    }
+    OPX_HD static M identity()
+    {
+        M m = zero();
+        m.r[0] = m.r[4] = m.r[8] = 1;
+        return m;
+    }
+    OPX_HD static A azero()
+    {
+        A a;
+#pragma unroll
+        for (int k = 0; k < 8; k++) a.c[k] = 0;
+        return a;
+    }
+    OPX_HD static T& gref(M& m, int k) { return k < 9 ? m.r[k] : m.i[k - 9]; }
+    OPX_HD static T gget(const M& m, int k) { return k < 9 ? m.r[k] : m.i[k - 9]; }
+};
+
+template <typename T>
+OPX_HD T& aref(Alg2<T>& a, int k) { return a.v[k]; }
+template <typename T>
+OPX_HD T& aref(Alg3<T>& a, int k) { return a.c[k]; }
+template <typename T>
+OPX_HD T aget(const Alg2<T>& a, int k) { return a.v[k]; }
+template <typename T>
+OPX_HD T aget(const Alg3<T>& a, int k) { return a.c[k]; }
+
+}  // namespace opx

@@ -1,0 +1,65 @@
+// opx_kernels.h -- launch interfaces of the gfx950 kernels (shared by the C-ABI layer).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace opx {
+
+// Inclusive box of cells; all = 1 means every cell.
+struct Box {
+    int lo[3];
+    int hi[3];
+    int all;
+};
+
+// Lattice geometry and time-step constants for the field kernels.
+struct Lattice {
+    uint32_t n;        // cells
+    uint32_t d0, d1, d2;
+    uint32_t s0, s1;   // strides of x0 and x1 (x2 stride is 1)
+    double at;         // time step argument of step/stepLinks
+    double uf[3];      // -g a_i a_t (FastTYMSolver.step unitFactor)
+    double area[3];    // 1/a_i^2 (Grid.getStapleSum areaFactor)
+    double as[3];      // lattice spacings
+    double g;          // coupling
+};
+
+// Super-particle descriptors (device arrays of `count` entries).
+struct SuperParticles {
+    int count;
+    int max_num;
+    const int* orientation;
+    const int* num;
+    const int* index_offset;
+    const int* per_plane;
+    const int* shift;
+    const int* ppc;
+    const int64_t* q_offset;
+    int64_t total;     // charges in total
+};
+
+// All launchers enqueue on `st` and return hipGetLastError().
+hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U,
+                                  void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
+hipError_t launch_step_links(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* Unext,
+                             const void* E, hipStream_t st);
+hipError_t launch_reunitarize(int nc, int prec, uint32_t n, void* U, hipStream_t st);
+hipError_t launch_links_from_coeffs(int nc, int prec, uint32_t n, const double* coeffs_soa, void* U,
+                                    hipStream_t st);
+// partials: device buffer of at least measure_partials_size() doubles; out: 8 device doubles
+size_t measure_partials_size(uint32_t n);
+hipError_t launch_measure(int nc, int prec, const Lattice& L, const Box& eval, const void* U, const void* Unext,
+                          const void* E, const void* rho, double* partials, double* out, hipStream_t st);
+hipError_t launch_projected_energy(int nc, int prec, const Lattice& L, const Box& eval, int dir, const void* U,
+                                   const void* Unext, const void* E, double* out4, hipStream_t st);
+hipError_t launch_health(int nc, int prec, uint32_t n, const void* U, double tol,
+                         unsigned long long* counters, hipStream_t st);
+hipError_t launch_health_alg(int nc, int prec, uint32_t n_values, const void* X,
+                             unsigned long long* counters, hipStream_t st);
+hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t, void* q,
+                             const void* Unext, void* J, hipStream_t st);
+hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
+                                   const void* q, void* rho, hipStream_t st);
+
+}  // namespace opx

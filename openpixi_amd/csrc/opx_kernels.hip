@@ -1,0 +1,747 @@
+// opx_kernels.hip -- gfx950 kernels of the temporal-gauge Yang-Mills leapfrog.
+//
+// Device layout: structure of arrays. A link field is [dir][component][cell]
+// (component k of link (cell, dir) at ((dir*G)+k)*n + cell), an algebra field is
+// [dir][coefficient][cell], the charge density [coefficient][cell]. Consecutive
+// lanes of a wavefront touch consecutive cells, so every load and store is a
+// fully coalesced 256-byte (fp32) or 512-byte (fp64) wavefront access.
+#include "opx_group.h"
+#include "opx_kernels.h"
+
+namespace opx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ bool in_box(const Box& b, uint32_t x0, uint32_t x1, uint32_t x2)
+{
+    if (b.all) return true;
+    return (int)x0 >= b.lo[0] && (int)x0 <= b.hi[0] && (int)x1 >= b.lo[1] && (int)x1 <= b.hi[1] &&
+           (int)x2 >= b.lo[2] && (int)x2 <= b.hi[2];
+}
+
+struct Coord {
+    uint32_t x0, x1, x2;
+};
+
+__device__ __forceinline__ Coord coords(const Lattice& L, uint32_t site)
+{
+    Coord c;
+    uint32_t q = site / L.d2;
+    c.x2 = site - q * L.d2;
+    c.x0 = q / L.d1;
+    c.x1 = q - c.x0 * L.d1;
+    return c;
+}
+
+__device__ __forceinline__ uint32_t dim_of(const Lattice& L, int d) { return d == 0 ? L.d0 : (d == 1 ? L.d1 : L.d2); }
+__device__ __forceinline__ uint32_t stride_of(const Lattice& L, int d) { return d == 0 ? L.s0 : (d == 1 ? L.s1 : 1u); }
+__device__ __forceinline__ uint32_t coord_of(const Coord& c, int d) { return d == 0 ? c.x0 : (d == 1 ? c.x1 : c.x2); }
+
+// periodic neighbour of `site` along d (Grid.shift, physics/grid/Grid.java:802-823)
+__device__ __forceinline__ uint32_t fwd(const Lattice& L, uint32_t site, uint32_t c, int d)
+{
+    uint32_t n = dim_of(L, d), s = stride_of(L, d);
+    return (c == n - 1) ? site - (n - 1) * s : site + s;
+}
+__device__ __forceinline__ uint32_t bwd(const Lattice& L, uint32_t site, uint32_t c, int d)
+{
+    uint32_t n = dim_of(L, d), s = stride_of(L, d);
+    return (c == 0) ? site + (n - 1) * s : site - s;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, T>::M load_link(const T* __restrict__ U, uint32_t n, uint32_t site, int dir)
+{
+    using GR = Group<NC, T>;
+    typename GR::M m;
+    const T* p = U + (size_t)dir * GR::G * n + site;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) GR::gref(m, k) = p[(size_t)k * n];
+    return m;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void store_link(T* __restrict__ U, uint32_t n, uint32_t site, int dir,
+                                           const typename Group<NC, T>::M& m)
+{
+    using GR = Group<NC, T>;
+    T* p = U + (size_t)dir * GR::G * n + site;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) p[(size_t)k * n] = GR::gget(m, k);
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, T>::A load_alg(const T* __restrict__ X, uint32_t n, uint32_t site, int dir)
+{
+    using GR = Group<NC, T>;
+    typename GR::A a;
+    const T* p = X + (size_t)dir * GR::AD * n + site;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[(size_t)k * n];
+    return a;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t n, uint32_t site, int dir,
+                                          typename Group<NC, T>::A a)
+{
+    using GR = Group<NC, T>;
+    T* p = X + (size_t)dir * GR::AD * n + site;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) p[(size_t)k * n] = aref(a, k);
+}
+
+// ---------------------------------------------------------------- kernel 1
+// FastTYMSolver.CombinedUpdate.execute (physics/fields/FastTYMSolver.java:40-62)
+// for one link (cell, dir = blockIdx.y):
+//   S   = sum_{j != dir} a_j^-2 [U_j(x+d) U_d(x+j)^+ U_j(x)^+ + (U_d(x-j) U_j(x+d-j))^+ U_j(x-j)]
+//   E  += at proj(U_d S) ;  E += uf_d J   (when a current is present)
+//   Unext = exp(-i at E) U_d
+template <int NC, typename T, bool HAS_J>
+__global__ __launch_bounds__(kBlock) void k_combined_update(Lattice L, Box active, const T* __restrict__ U,
+                                                            T* __restrict__ Unext, T* __restrict__ E,
+                                                            const T* __restrict__ J)
+{
+    using GR = Group<NC, T>;
+    using M = typename GR::M;
+    using A = typename GR::A;
+    const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
+    const int d = blockIdx.y;
+    if (site >= L.n) return;
+    const Coord c = coords(L, site);
+    if (!in_box(active, c.x0, c.x1, c.x2)) return;
+    const uint32_t n = L.n;
+    const uint32_t xpd = fwd(L, site, coord_of(c, d), d);
+    M S = GR::zero();
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (j == d) continue;
+        const uint32_t cj = coord_of(c, j);
+        const uint32_t xpj = fwd(L, site, cj, j);
+        const uint32_t xmj = bwd(L, site, cj, j);
+        const uint32_t xpd_mj = bwd(L, xpd, cj, j);
+        M u1 = mul_adj(mul_adj(load_link<NC, T>(U, n, xpd, j), load_link<NC, T>(U, n, xpj, d)),
+                       load_link<NC, T>(U, n, site, j));
+        M u2 = adj_mul(mul(load_link<NC, T>(U, n, xmj, d), load_link<NC, T>(U, n, xpd_mj, j)),
+                       load_link<NC, T>(U, n, xmj, j));
+        acc(u1, u2);
+        acc(S, scaled(u1, (T)L.area[j]));
+    }
+    const M Ud = load_link<NC, T>(U, n, site, d);
+    const A pr = proj(mul(Ud, S));
+    A e = load_alg<NC, T>(E, n, site, d);
+    const T at = (T)L.at;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * at;
+    if (HAS_J) {
+        const A jj = load_alg<NC, T>(J, n, site, d);
+        const T uf = (T)L.uf[d];
+#pragma unroll
+        for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(jj, k) * uf;
+    }
+    store_alg<NC, T>(E, n, site, d, e);
+    store_link<NC, T>(Unext, n, site, d, mul(exp_link(e, -at), Ud));
+}
+
+// ---------------------------------------------------------------- kernel 2
+// FastTYMSolver.UpdateLinks.execute (physics/fields/FastTYMSolver.java:65-84)
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_step_links(Lattice L, Box active, const T* __restrict__ U,
+                                                       T* __restrict__ Unext, const T* __restrict__ E)
+{
+    const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
+    const int d = blockIdx.y;
+    if (site >= L.n) return;
+    const Coord c = coords(L, site);
+    if (!in_box(active, c.x0, c.x1, c.x2)) return;
+    const auto e = load_alg<NC, T>(E, L.n, site, d);
+    const auto u = load_link<NC, T>(U, L.n, site, d);
+    store_link<NC, T>(Unext, L.n, site, d, mul(exp_link(e, -(T)L.at), u));
+}
+
+// ---------------------------------------------------------------- kernel 3
+// Reprojection onto the group in double precision.
+__device__ __forceinline__ Su2<double> reunit(Su2<double> u)
+{
+    double nrm = sqrt(u.e0 * u.e0 + u.e1 * u.e1 + u.e2 * u.e2 + u.e3 * u.e3);
+    double f = 1.0 / nrm;
+    return Su2<double>{u.e0 * f, u.e1 * f, u.e2 * f, u.e3 * f};
+}
+
+__device__ __forceinline__ Su3<double> reunit(Su3<double> u)
+{
+    // row 0
+    double n0 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) n0 += u.r[k] * u.r[k] + u.i[k] * u.i[k];
+    double f0 = 1.0 / sqrt(n0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        u.r[k] *= f0;
+        u.i[k] *= f0;
+    }
+    // row 1 -= <row0, row1> row0
+    double pr = 0, pi = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        // conj(r0_k) * r1_k
+        pr += u.r[k] * u.r[3 + k] + u.i[k] * u.i[3 + k];
+        pi += u.r[k] * u.i[3 + k] - u.i[k] * u.r[3 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        u.r[3 + k] -= pr * u.r[k] - pi * u.i[k];
+        u.i[3 + k] -= pr * u.i[k] + pi * u.r[k];
+    }
+    double n1 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) n1 += u.r[3 + k] * u.r[3 + k] + u.i[3 + k] * u.i[3 + k];
+    double f1 = 1.0 / sqrt(n1);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        u.r[3 + k] *= f1;
+        u.i[3 + k] *= f1;
+    }
+    // row 2 = conj(row0 x row1)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        int a = (k + 1) % 3, b = (k + 2) % 3;
+        double cr = (u.r[a] * u.r[3 + b] - u.i[a] * u.i[3 + b]) - (u.r[b] * u.r[3 + a] - u.i[b] * u.i[3 + a]);
+        double ci = (u.r[a] * u.i[3 + b] + u.i[a] * u.r[3 + b]) - (u.r[b] * u.i[3 + a] + u.i[b] * u.r[3 + a]);
+        u.r[6 + k] = cr;
+        u.i[6 + k] = -ci;
+    }
+    return u;
+}
+
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_reunitarize(uint32_t n, T* __restrict__ U)
+{
+    using GR = Group<NC, T>;
+    using GD = Group<NC, double>;
+    const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
+    const int d = blockIdx.y;
+    if (site >= n) return;
+    typename GR::M u = load_link<NC, T>(U, n, site, d);
+    typename GD::M ud;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) GD::gref(ud, k) = (double)GR::gget(u, k);
+    ud = reunit(ud);
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) GR::gref(u, k) = (T)GD::gget(ud, k);
+    store_link<NC, T>(U, n, site, d, u);
+}
+
+// ---------------------------------------------------------------- init helper
+// U = exp(i sum_a c_a t_a) from double coefficients in [dir][coef][cell] layout.
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_links_from_coeffs(uint32_t n, const double* __restrict__ C,
+                                                              T* __restrict__ U)
+{
+    using GD = Group<NC, double>;
+    using GR = Group<NC, T>;
+    const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
+    const int d = blockIdx.y;
+    if (site >= n) return;
+    typename GD::A a;
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) aref(a, k) = C[((size_t)d * GD::AD + k) * n + site];
+    typename GD::M u = exp_link(a, 1.0);
+    typename GR::M o;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) GR::gref(o, k) = (T)GD::gget(u, k);
+    store_link<NC, T>(U, n, site, d, o);
+}
+
+// ---------------------------------------------------------------- measurements
+// Compensated accumulation in double.
+struct Kahan {
+    double s, c;
+    __device__ void add(double x)
+    {
+        double y = x - c;
+        double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// (s1,c1) + (s2,c2) with an error-free transformation of the head sums
+__device__ __forceinline__ void merge(double& s1, double& c1, double s2, double c2)
+{
+    double s = s1 + s2;
+    double bp = s - s1;
+    double err = (s1 - (s - bp)) + (s2 - bp);
+    s1 = s;
+    c1 = c1 + c2 - err;
+}
+
+constexpr int kMeasureBlocks = 1024;
+constexpr int kNQ = 8;
+
+// plaquette P_{jk}(x) = U_j(x) U_k(x+j) U_j(x+k)^+ U_k(x)^+ (Grid.getPlaquette with o1 = o2 = +1)
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, double>::M plaquette_d(const Lattice& L, const T* __restrict__ U,
+                                                                     uint32_t site, const Coord& c, int j, int k)
+{
+    using GD = Group<NC, double>;
+    using GR = Group<NC, T>;
+    const uint32_t n = L.n;
+    uint32_t xpj = fwd(L, site, coord_of(c, j), j);
+    uint32_t xpk = fwd(L, site, coord_of(c, k), k);
+    typename GR::M l[4] = {load_link<NC, T>(U, n, site, j), load_link<NC, T>(U, n, xpj, k),
+                           load_link<NC, T>(U, n, xpk, j), load_link<NC, T>(U, n, site, k)};
+    typename GD::M ld[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int q = 0; q < GR::G; q++) GD::gref(ld[m], q) = (double)GR::gget(l[m], q);
+    return mul_adj(mul_adj(mul(ld[0], ld[1]), ld[2]), ld[3]);
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, double>::A load_alg_d(const T* __restrict__ X, uint32_t n, uint32_t site,
+                                                                    int dir)
+{
+    using GD = Group<NC, double>;
+    typename GD::A a;
+    const T* p = X + (size_t)dir * GD::AD * n + site;
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[(size_t)k * n];
+    return a;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, double>::M load_link_d(const T* __restrict__ U, uint32_t n, uint32_t site,
+                                                                     int dir)
+{
+    using GD = Group<NC, double>;
+    typename GD::M m;
+    const T* p = U + (size_t)dir * GD::G * n + site;
+#pragma unroll
+    for (int k = 0; k < GD::G; k++) GD::gref(m, k) = (double)p[(size_t)k * n];
+    return m;
+}
+
+// B_i^2 at one time level: proj(P_jk) a_i/(a_j a_k), (j,k) cyclic (Grid.getB, Grid.java:877-900)
+template <int NC, typename T>
+__device__ __forceinline__ double b2(const Lattice& L, const T* __restrict__ U, uint32_t site, const Coord& c, int i)
+{
+    int j = (i + 1) % 3, k = (i + 2) % 3;
+    auto p = proj(plaquette_d<NC, T>(L, U, site, c, j, k));
+    double f = L.as[i] / (L.as[j] * L.as[k]);
+    return square(p) * f * f;
+}
+
+// Gauss constraint G(x) (Grid.getGaussConstraint, Grid.java:944-968)
+template <int NC, typename T>
+__device__ __forceinline__ double gauss2(const Lattice& L, const T* __restrict__ U, const T* __restrict__ E,
+                                         const T* __restrict__ rho, uint32_t site, const Coord& c)
+{
+    using GD = Group<NC, double>;
+    typename GD::A g = GD::azero();
+    for (int i = 0; i < 3; i++) {
+        uint32_t xm = bwd(L, site, coord_of(c, i), i);
+        auto em = load_alg_d<NC, T>(E, L.n, xm, i);
+        auto u = load_link_d<NC, T>(U, L.n, xm, i);
+        auto t = act(em, adj(u));
+        auto ex = load_alg_d<NC, T>(E, L.n, site, i);
+        double f = 1.0 / (L.as[i] * (L.g * L.as[i]));
+#pragma unroll
+        for (int k = 0; k < GD::AD; k++) aref(g, k) += (aref(ex, k) - aref(t, k)) * f;
+    }
+    if (rho) {
+        auto r = load_alg_d<NC, T>(rho, L.n, site, 0);
+#pragma unroll
+        for (int k = 0; k < GD::AD; k++) aref(g, k) -= aref(r, k);
+    }
+    return square(g);
+}
+
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_measure(Lattice L, Box eval, const T* __restrict__ U,
+                                                    const T* __restrict__ Unext, const T* __restrict__ E,
+                                                    const T* __restrict__ rho, double* __restrict__ partials)
+{
+    Kahan acc[kNQ];
+#pragma unroll
+    for (int q = 0; q < kNQ; q++) acc[q] = Kahan{0.0, 0.0};
+    double uf[3];
+    for (int i = 0; i < 3; i++) uf[i] = 1.0 / ((L.g * L.as[i]) * (L.g * L.as[i]));
+    for (uint32_t site = blockIdx.x * kBlock + threadIdx.x; site < L.n; site += gridDim.x * kBlock) {
+        const Coord c = coords(L, site);
+        if (!in_box(eval, c.x0, c.x1, c.x2)) continue;
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc[i].add(square(load_alg_d<NC, T>(E, L.n, site, i)) * uf[i]);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            acc[3 + i].add(0.5 * (b2<NC, T>(L, U, site, c, i) + b2<NC, T>(L, Unext, site, c, i)) * uf[i]);
+        acc[6].add(gauss2<NC, T>(L, U, E, rho, site, c));
+        if (rho) acc[7].add(square(load_alg_d<NC, T>(rho, L.n, site, 0)));
+    }
+    // wavefront reduction (64 lanes), then across the 4 waves of the block
+    __shared__ double sh[kBlock / 64][kNQ][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < kNQ; q++) {
+        double s = acc[q].s, cc = -acc[q].c;
+        for (int off = 32; off > 0; off >>= 1) {
+            double s2 = __shfl_down(s, off, 64);
+            double c2 = __shfl_down(cc, off, 64);
+            merge(s, cc, s2, c2);
+        }
+        if (lane == 0) {
+            sh[wave][q][0] = s;
+            sh[wave][q][1] = cc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kNQ) {
+        int q = threadIdx.x;
+        double s = sh[0][q][0], cc = sh[0][q][1];
+        for (int w = 1; w < kBlock / 64; w++) merge(s, cc, sh[w][q][0], sh[w][q][1]);
+        partials[((size_t)blockIdx.x * kNQ + q) * 2 + 0] = s;
+        partials[((size_t)blockIdx.x * kNQ + q) * 2 + 1] = cc;
+    }
+}
+
+// Fixed-order reduction of the per-block partials; writes normalised results.
+__global__ __launch_bounds__(64) void k_measure_final(int nblocks, double inv_n, const double* __restrict__ partials,
+                                                      double* __restrict__ out)
+{
+    const int q = threadIdx.x;
+    if (q >= kNQ) return;
+    double s = 0.0, cc = 0.0;
+    for (int b = 0; b < nblocks; b++) merge(s, cc, partials[((size_t)b * kNQ + q) * 2], partials[((size_t)b * kNQ + q) * 2 + 1]);
+    double tot = s + cc;
+    if (q == 7)
+        out[q] = sqrt(tot) * inv_n;
+    else
+        out[q] = tot * inv_n;
+}
+
+// ProjectedEnergyDensity: one block per slice along `dir`, fixed-order reduction.
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_projected_energy(Lattice L, Box eval, int dir, const T* __restrict__ U,
+                                                             const T* __restrict__ Unext, const T* __restrict__ E,
+                                                             double* __restrict__ out)
+{
+    const uint32_t slice = blockIdx.x;
+    const uint32_t nd = dim_of(L, dir);
+    const uint32_t per = L.n / nd;
+    double area = (double)L.n / (double)nd;
+    double uf[3];
+    for (int i = 0; i < 3; i++) uf[i] = 1.0 / ((L.g * L.as[i]) * (L.g * L.as[i])) / area;
+    Kahan a[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    // the other two dimensions enumerate the slice
+    const int da = dir == 0 ? 1 : 0, db = dir == 2 ? 1 : 2;
+    const uint32_t nb = dim_of(L, db);
+    for (uint32_t m = threadIdx.x; m < per; m += kBlock) {
+        uint32_t xa = m / nb, xb = m - (m / nb) * nb;
+        uint32_t x[3];
+        x[dir] = slice;
+        x[da] = xa;
+        x[db] = xb;
+        uint32_t site = x[0] * L.s0 + x[1] * L.s1 + x[2];
+        Coord c{x[0], x[1], x[2]};
+        if (!in_box(eval, c.x0, c.x1, c.x2)) continue;
+        for (int j = 0; j < 3; j++) {
+            double el = 0.5 * square(load_alg_d<NC, T>(E, L.n, site, j)) * uf[j];
+            double mag = 0.25 * (b2<NC, T>(L, U, site, c, j) + b2<NC, T>(L, Unext, site, c, j)) * uf[j];
+            if (j == dir) {
+                a[2].add(el);
+                a[3].add(mag);
+            } else {
+                a[0].add(el);
+                a[1].add(mag);
+            }
+        }
+    }
+    __shared__ double sh[kBlock / 64][4][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q < 4; q++) {
+        double s = a[q].s, cc = -a[q].c;
+        for (int off = 32; off > 0; off >>= 1) {
+            double s2 = __shfl_down(s, off, 64);
+            double c2 = __shfl_down(cc, off, 64);
+            merge(s, cc, s2, c2);
+        }
+        if (lane == 0) {
+            sh[wave][q][0] = s;
+            sh[wave][q][1] = cc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        int q = threadIdx.x;
+        double s = sh[0][q][0], cc = sh[0][q][1];
+        for (int w = 1; w < kBlock / 64; w++) merge(s, cc, sh[w][q][0], sh[w][q][1]);
+        out[(size_t)q * nd + slice] = s + cc;
+    }
+}
+
+// ---------------------------------------------------------------- health monitor
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_health(uint32_t n, const T* __restrict__ U, double tol,
+                                                   unsigned long long* __restrict__ counters)
+{
+    using GD = Group<NC, double>;
+    const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
+    const int d = blockIdx.y;
+    if (site >= n) return;
+    auto u = load_link_d<NC, T>(U, n, site, d);
+    bool finite = true;
+#pragma unroll
+    for (int k = 0; k < GD::G; k++) finite = finite && isfinite(GD::gget(u, k));
+    auto p = mul_adj(u, u);
+    auto id = GD::identity();
+    double dev = 0;
+#pragma unroll
+    for (int k = 0; k < GD::G; k++) {
+        double x = GD::gget(p, k) - GD::gget(id, k);
+        dev = fmax(dev, fabs(x));
+    }
+    // SU(2) parametrisation: U U^+ = (sum e^2) * 1 with e1..e3 components zero
+    if (!finite) atomicAdd(&counters[0], 1ull);
+    if (!(dev <= tol)) atomicAdd(&counters[1], 1ull);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_health_alg(uint32_t nv, const T* __restrict__ X,
+                                                       unsigned long long* __restrict__ counters)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nv) return;
+    if (!isfinite((double)X[i])) atomicAdd(&counters[0], 1ull);
+}
+
+// ---------------------------------------------------------------- CPIC: super particles
+// CGCSuperParticle.getCurrentOffset / getCurrentNGPOffset / needsUpdate
+// (physics/particles/CGCSuperParticle.java:71-105)
+__device__ __forceinline__ int floor_div(int a, int b)
+{
+    int q = a / b;
+    if ((a % b != 0) && ((a < 0) != (b < 0))) q--;
+    return q;
+}
+__device__ __forceinline__ int sp_current_offset(int o, int off, int pp, int k, int ppc, int t)
+{
+    if (o > 0) return off + ((t + k) / ppc) * pp;
+    return off + floor_div(-t + k, ppc) * pp;
+}
+__device__ __forceinline__ int sp_ngp_offset(int o, int off, int pp, int k, int ppc, int t)
+{
+    int cur = sp_current_offset(o, off, pp, k, ppc, t);
+    if (o > 0) return cur + ((((t + k) % ppc) < ppc / 2) ? 0 : pp);
+    return cur + ((((t - k + ppc + ppc / 2 - 1) % ppc) < ppc / 2) ? 0 : pp);
+}
+__device__ __forceinline__ bool sp_needs_update(int o, int k, int ppc, int t)
+{
+    if (o > 0) return 0 == (t + k + ppc / 2 + 1) % ppc;
+    return 0 == (t - k + ppc / 2) % ppc;
+}
+
+// interpolateToGrid (physics/grid/CGCSuperParticleInterpolationNGP.java:19-73):
+// one block row per super particle (blockIdx.y), one thread per charge.
+// Charges are [coef][total] in T. Within one beam direction the active super
+// particles of a step cover disjoint cells; opposite beams may meet in one cell,
+// where the two additions onto the reset current commute exactly.
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_sp_to_grid(Lattice L, SuperParticles sp, int t, T* __restrict__ q,
+                                                       const T* __restrict__ Unext, T* __restrict__ J)
+{
+    using GR = Group<NC, T>;
+    const int p = blockIdx.y;
+    const int o = sp.orientation[p], k = sp.shift[p], ppc = sp.ppc[p];
+    if (!sp_needs_update(o, k, ppc, t)) return;
+    const int off = sp_current_offset(o, sp.index_offset[p], sp.per_plane[p], k, ppc, t);
+    const int quickfix = (ppc == 1) ? sp.per_plane[p] : 0;
+    const int imin = (off < 0) ? -off : 0;
+    long lim = (long)off + sp.num[p];
+    if (lim > (long)L.n - quickfix) lim = (long)L.n - quickfix;
+    long imax = lim - off;
+    if (imax < 0) imax = 0;
+    const long ireg = imax - imin;
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ireg) return;
+    const long j = i + imin;
+    const uint32_t index = (uint32_t)(off + j);
+    const size_t qi = (size_t)sp.q_offset[p] + j;
+    typename GR::A Q;
+#pragma unroll
+    for (int m = 0; m < GR::AD; m++) aref(Q, m) = q[(size_t)m * sp.total + qi];
+    const auto U = load_link<NC, T>(Unext, L.n, index, 0);
+    const T f = (T)(L.as[0] / L.at);
+    if (o > 0) {
+        T* Jc = J + index + quickfix;
+#pragma unroll
+        for (int m = 0; m < GR::AD; m++) atomicAdd(Jc + (size_t)m * L.n, aref(Q, m) * f);
+        Q = act(Q, adj(U));
+    } else {
+        Q = act(Q, U);
+        T* Jc = J + index;
+        const T fm = (T)(-L.as[0] / L.at);
+#pragma unroll
+        for (int m = 0; m < GR::AD; m++) atomicAdd(Jc + (size_t)m * L.n, aref(Q, m) * fm);
+    }
+#pragma unroll
+    for (int m = 0; m < GR::AD; m++) q[(size_t)m * sp.total + qi] = aref(Q, m);
+}
+
+// interpolateChargedensity (physics/grid/CGCSuperParticleInterpolationNGP.java:75-93)
+// as a gather: each cell sums the charges that sit on it in particle-list order,
+// the order of the sequential reference loop, with no atomics.
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, SuperParticles sp, int t,
+                                                             const T* __restrict__ q, T* __restrict__ rho)
+{
+    using GR = Group<NC, T>;
+    const uint32_t cell = blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= L.n) return;
+    typename GR::A r;
+#pragma unroll
+    for (int m = 0; m < GR::AD; m++) aref(r, m) = rho[(size_t)m * L.n + cell];
+    bool touched = false;
+    for (int p = 0; p < sp.count; p++) {
+        const int off = sp_ngp_offset(sp.orientation[p], sp.index_offset[p], sp.per_plane[p], sp.shift[p], sp.ppc[p], t);
+        const long j = (long)cell - off;
+        if (j < 0 || j >= sp.num[p]) continue;
+        const size_t qi = (size_t)sp.q_offset[p] + j;
+#pragma unroll
+        for (int m = 0; m < GR::AD; m++) aref(r, m) += q[(size_t)m * sp.total + qi];
+        touched = true;
+    }
+    if (touched) {
+#pragma unroll
+        for (int m = 0; m < GR::AD; m++) rho[(size_t)m * L.n + cell] = aref(r, m);
+    }
+}
+
+dim3 grid_links(uint32_t n) { return dim3((n + kBlock - 1) / kBlock, 3, 1); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- dispatch
+#define OPX_DISPATCH(nc, prec, ...)                                  \
+    do {                                                             \
+        if ((nc) == 2 && (prec) == 0) { constexpr int NC = 2; using T = double; __VA_ARGS__; } \
+        else if ((nc) == 2) { constexpr int NC = 2; using T = float; __VA_ARGS__; }           \
+        else if ((prec) == 0) { constexpr int NC = 3; using T = double; __VA_ARGS__; }        \
+        else { constexpr int NC = 3; using T = float; __VA_ARGS__; }                          \
+    } while (0)
+
+hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* Unext,
+                                  void* E, const void* J, bool has_j, hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        if (has_j)
+            hipLaunchKernelGGL((k_combined_update<NC, T, true>), grid_links(L.n), dim3(kBlock), 0, st, L, active,
+                               (const T*)U, (T*)Unext, (T*)E, (const T*)J);
+        else
+            hipLaunchKernelGGL((k_combined_update<NC, T, false>), grid_links(L.n), dim3(kBlock), 0, st, L, active,
+                               (const T*)U, (T*)Unext, (T*)E, (const T*)nullptr);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_step_links(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* Unext,
+                             const void* E, hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_step_links<NC, T>), grid_links(L.n), dim3(kBlock), 0, st, L, active, (const T*)U,
+                           (T*)Unext, (const T*)E);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_reunitarize(int nc, int prec, uint32_t n, void* U, hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_reunitarize<NC, T>), grid_links(n), dim3(kBlock), 0, st, n, (T*)U);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_links_from_coeffs(int nc, int prec, uint32_t n, const double* C, void* U, hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_links_from_coeffs<NC, T>), grid_links(n), dim3(kBlock), 0, st, n, C, (T*)U);
+    });
+    return hipGetLastError();
+}
+
+size_t measure_partials_size(uint32_t) { return (size_t)kMeasureBlocks * kNQ * 2; }
+
+hipError_t launch_measure(int nc, int prec, const Lattice& L, const Box& eval, const void* U, const void* Unext,
+                          const void* E, const void* rho, double* partials, double* out, hipStream_t st)
+{
+    int nb = (int)((L.n + kBlock - 1) / kBlock);
+    if (nb > kMeasureBlocks) nb = kMeasureBlocks;
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_measure<NC, T>), dim3(nb), dim3(kBlock), 0, st, L, eval, (const T*)U,
+                           (const T*)Unext, (const T*)E, (const T*)rho, partials);
+    });
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_measure_final, dim3(1), dim3(64), 0, st, nb, 1.0 / (double)L.n, partials, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_projected_energy(int nc, int prec, const Lattice& L, const Box& eval, int dir, const void* U,
+                                   const void* Unext, const void* E, double* out4, hipStream_t st)
+{
+    uint32_t nd = dir == 0 ? L.d0 : (dir == 1 ? L.d1 : L.d2);
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_projected_energy<NC, T>), dim3(nd), dim3(kBlock), 0, st, L, eval, dir, (const T*)U,
+                           (const T*)Unext, (const T*)E, out4);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_health(int nc, int prec, uint32_t n, const void* U, double tol, unsigned long long* counters,
+                         hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_health<NC, T>), grid_links(n), dim3(kBlock), 0, st, n, (const T*)U, tol, counters);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_health_alg(int nc, int prec, uint32_t nv, const void* X, unsigned long long* counters,
+                             hipStream_t st)
+{
+    dim3 g((nv + kBlock - 1) / kBlock);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_health_alg<double>, g, dim3(kBlock), 0, st, nv, (const double*)X, counters);
+    else
+        hipLaunchKernelGGL(k_health_alg<float>, g, dim3(kBlock), 0, st, nv, (const float*)X, counters);
+    (void)nc;
+    return hipGetLastError();
+}
+
+hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t, void* q,
+                             const void* Unext, void* J, hipStream_t st)
+{
+    if (sp.count == 0 || sp.max_num == 0) return hipSuccess;
+    dim3 g((sp.max_num + kBlock - 1) / kBlock, sp.count, 1);
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_sp_to_grid<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (T*)q, (const T*)Unext,
+                           (T*)J);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
+                                   const void* q, void* rho, hipStream_t st)
+{
+    if (sp.count == 0) return hipSuccess;
+    dim3 g((L.n + kBlock - 1) / kBlock);
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_sp_chargedensity<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (const T*)q, (T*)rho);
+    });
+    return hipGetLastError();
+}
+
+}  // namespace opx

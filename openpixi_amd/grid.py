@@ -1,0 +1,251 @@
+"""Host-side mirror of OpenPixi's Grid / FieldSolver / FieldMeasurements seams.
+
+The lattice lives on the GPU inside an ``opx_ctx``; these classes keep the
+reference's method names and argument meaning so a caller written against
+``org.openpixi.pixi.physics.grid.Grid`` and ``FastTYMSolver`` reads the same.
+Whole-lattice getters/setters exchange numpy arrays in the reference's per-cell
+element order (see include/opx.h).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELD_E, FIELD_J, FIELD_RHO, FIELD_U, FIELD_UNEXT, OpxError
+
+
+def group_size(nc: int) -> int:
+    return 4 if nc == 2 else 18
+
+
+def algebra_size(nc: int) -> int:
+    """Reals per algebra element in the reference storage (3 or 9)."""
+    return 3 if nc == 2 else 9
+
+
+def algebra_dim(nc: int) -> int:
+    return nc * nc - 1
+
+
+class Grid:
+    """Device-resident lattice (physics/grid/Grid.java:443-473).
+
+    numCells = [N0, N1, N2] with x0 slowest; spacing a_i; time step a_t; coupling g.
+    precision: "fp64" or "fp32".
+    """
+
+    def __init__(self, num_colors, num_cells, spacing=1.0, time_step=0.25, coupling=1.0, precision="fp64"):
+        L = _lib.load()
+        if np.isscalar(spacing):
+            spacing = [float(spacing)] * 3
+        self.num_colors = int(num_colors)
+        self.num_cells = [int(x) for x in num_cells]
+        self.spacing = [float(x) for x in spacing]
+        self.time_step = float(time_step)
+        self.coupling = float(coupling)
+        self.precision = precision
+        prec = {"fp64": _lib.OPX_FP64, "fp32": _lib.OPX_FP32}[precision]
+        h = ctypes.c_void_p()
+        rc = L.opx_create(self.num_colors, len(self.num_cells), _lib.ivec(self.num_cells), _lib.dvec(self.spacing),
+                          self.time_step, self.coupling, prec, ctypes.byref(h))
+        if rc != 0:
+            raise OpxError(rc, "opx_create failed (colours, dimensions or precision unsupported, or device error)")
+        self._h = h
+        self.total_cells = int(np.prod(self.num_cells))
+        self.simulation_steps = 0
+
+    # ---- lifetime ----
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.load().opx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = _lib.load().opx_last_error(self._h)
+            raise OpxError(rc, msg.decode() if msg else "")
+
+    # ---- reference accessors ----
+    def getNumberOfColors(self):
+        return self.num_colors
+
+    def getNumCells(self, i=None):
+        return self.num_cells if i is None else self.num_cells[i]
+
+    def getTotalNumberOfCells(self):
+        return self.total_cells
+
+    def getLatticeSpacing(self, i=0):
+        return self.spacing[i]
+
+    def getTemporalSpacing(self):
+        return self.time_step
+
+    def getGaugeCoupling(self):
+        return self.coupling
+
+    def getLatticeUnitFactor(self, i):
+        return self.coupling * self.spacing[i]
+
+    # ---- whole-lattice data exchange ----
+    def _shape(self, field):
+        nc = self.num_colors
+        if field in (FIELD_U, FIELD_UNEXT):
+            return (self.total_cells, 3, group_size(nc))
+        if field == FIELD_RHO:
+            return (self.total_cells, algebra_size(nc))
+        return (self.total_cells, 3, algebra_size(nc))
+
+    def upload(self, field, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != int(np.prod(self._shape(field))):
+            raise ValueError(f"field {field}: expected {self._shape(field)} values, got {a.size}")
+        self._check(_lib.load().opx_upload_field(self._h, field, _lib.dptr(a), a.size))
+
+    def download(self, field):
+        a = np.zeros(self._shape(field), dtype=np.float64)
+        self._check(_lib.load().opx_download_field(self._h, field, _lib.dptr(a.reshape(-1)), a.size))
+        return a
+
+    def setU(self, values):
+        self.upload(FIELD_U, values)
+
+    def setUnext(self, values):
+        self.upload(FIELD_UNEXT, values)
+
+    def setE(self, values):
+        self.upload(FIELD_E, values)
+
+    def setJ(self, values):
+        self.upload(FIELD_J, values)
+
+    def setRho(self, values):
+        self.upload(FIELD_RHO, values)
+
+    def getU(self):
+        return self.download(FIELD_U)
+
+    def getUnext(self):
+        return self.download(FIELD_UNEXT)
+
+    def getE(self):
+        return self.download(FIELD_E)
+
+    def getJ(self):
+        return self.download(FIELD_J)
+
+    def getRho(self):
+        return self.download(FIELD_RHO)
+
+    def setLinksFromCoefficients(self, coeffs, field=FIELD_U):
+        """U = exp(i sum_a c_a t_a), coefficients [cell][dir][nc^2-1], computed on the device."""
+        a = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1)
+        self._check(_lib.load().opx_set_links_from_coefficients(self._h, field, _lib.dptr(a), a.size))
+
+    def setEFromCoefficients(self, coeffs):
+        a = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1)
+        self._check(_lib.load().opx_set_efield_from_coefficients(self._h, _lib.dptr(a), a.size))
+
+    # ---- regions (physics/grid/Grid.java:992-1020) ----
+    def setActiveRegion(self, p1, p2):
+        self._check(_lib.load().opx_set_active_region(self._h, _lib.ivec(p1), _lib.ivec(p2)))
+
+    def setEvaluationRegion(self, p1, p2):
+        self._check(_lib.load().opx_set_evaluation_region(self._h, _lib.ivec(p1), _lib.ivec(p2)))
+
+    # ---- Grid actions ----
+    def storeFields(self):
+        self._check(_lib.load().opx_store_fields(self._h))
+
+    def resetCurrent(self):
+        self._check(_lib.load().opx_reset_current(self._h))
+
+    def resetCharge(self):
+        self._check(_lib.load().opx_reset_charge(self._h))
+
+    def updateGrid(self, tstep, solver=None):
+        (solver or FastTYMSolver()).step(self, tstep)
+
+    def updateLinks(self, tstep, solver=None):
+        (solver or FastTYMSolver()).stepLinks(self, tstep)
+
+    def reunitarize(self):
+        self._check(_lib.load().opx_reunitarize(self._h))
+
+    def sync(self):
+        self._check(_lib.load().opx_sync(self._h))
+
+    def checkHealth(self, tol=1e-6):
+        nf, nu = ctypes.c_int64(), ctypes.c_int64()
+        rc = _lib.load().opx_check_health(self._h, tol, ctypes.byref(nf), ctypes.byref(nu))
+        if rc not in (0, _lib.OPX_E_NUMERIC):
+            self._check(rc)
+        return nf.value, nu.value
+
+
+class FieldSolver:
+    """physics/fields/FieldSolver.java:114-156: step and stepLinks on a Grid."""
+
+    def clone(self):
+        return type(self)()
+
+    def step(self, grid: Grid, timeStep: float):
+        pass
+
+    def stepLinks(self, grid: Grid, timeStep: float):
+        pass
+
+
+class FastTYMSolver(FieldSolver):
+    """physics/fields/FastTYMSolver.java:7-85 on the gfx950 kernels."""
+
+    def step(self, grid: Grid, timeStep: float):
+        grid._check(_lib.load().opx_step(grid.handle, float(timeStep)))
+
+    def stepLinks(self, grid: Grid, timeStep: float):
+        grid._check(_lib.load().opx_step_links(grid.handle, float(timeStep)))
+
+
+class FieldMeasurements:
+    """physics/measurements/FieldMeasurements.java:26-66 computed on the device in double precision."""
+
+    def _all(self, grid: Grid):
+        out = np.zeros(8)
+        grid._check(_lib.load().opx_measure(grid.handle, _lib.dptr(out)))
+        return out
+
+    def measure(self, grid: Grid):
+        """[E2_0,E2_1,E2_2,B2_0,B2_1,B2_2,gauss_mean_sq,total_charge_squared]."""
+        return self._all(grid)
+
+    def calculateEsquared(self, grid: Grid, direction=None):
+        m = self._all(grid)
+        return float(m[direction]) if direction is not None else float(m[0:3].sum() * grid.total_cells)
+
+    def calculateBsquared(self, grid: Grid, direction=None):
+        m = self._all(grid)
+        return float(m[3 + direction]) if direction is not None else float(m[3:6].sum() * grid.total_cells)
+
+    def calculateGaussConstraint(self, grid: Grid):
+        return float(self._all(grid)[6])
+
+    def calculateTotalChargeSquared(self, grid: Grid):
+        return float(self._all(grid)[7])
+
+    def projectedEnergyDensity(self, grid: Grid, direction: int):
+        n = grid.num_cells[direction]
+        arrs = [np.zeros(n) for _ in range(4)]
+        grid._check(_lib.load().opx_projected_energy_density(grid.handle, direction, *[_lib.dptr(a) for a in arrs]))
+        return arrs

@@ -1,0 +1,186 @@
+"""Host-side mirror of OpenPixi's Settings (YAML) and Simulation driver for the
+FastTYMSolver path.
+
+``Settings.from_yaml`` accepts the reference's YAML keys
+(ui/util/yaml/YamlSettings.java:14-191) that concern this path and ignores
+output/panel sections. ``Simulation`` keeps the reference's step ordering
+(physics/Simulation.java:318-452): ``initialize()`` once, then ``step()`` while
+``continues()``, which runs ceil(duration/timeStep) + 1 steps like the reference.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib
+from .grid import FieldMeasurements, FastTYMSolver, Grid
+
+SIMULATION_TYPES = {
+    "temporal yang-mills": "TemporalYangMills",
+    "temporal cgc": "TemporalCGC",
+    "temporal cgc ngp": "TemporalCGCNGP",
+    "temporal implicit cgc ngp": "TemporalImplicitCGCNGP",
+    "temporal optimized cgc ngp": "TemporalOptimizedCGCNGP",
+    "temporal optimized implicit cgc ngp": "TemporalOptimizedImplicitCGCNGP",
+    "boost-invariant cgc": "BoostInvariantCGC",
+}
+
+# simulation types whose field solver is FastTYMSolver (physics/Settings.java:540-581)
+FAST_TYM_TYPES = {"TemporalYangMills", "TemporalCGC", "TemporalCGCNGP", "TemporalOptimizedCGCNGP"}
+
+
+@dataclass
+class Region:
+    enabled: bool = False
+    point1: Optional[List[int]] = None
+    point2: Optional[List[int]] = None
+
+    def resolved(self, grid_cells):
+        """YamlRegion.getPoint1/2: negative coordinates count from the end."""
+        p1 = [p + n if p < 0 else p for p, n in zip(self.point1, grid_cells)]
+        p2 = [p + n if p < 0 else p for p, n in zip(self.point2, grid_cells)]
+        return p1, p2
+
+
+@dataclass
+class Settings:
+    """Subset of physics/Settings.java used by the FastTYMSolver path, with the
+    YamlSettings defaults (3 dimensions, 1 colour, coupling 1, YamlSettings.java:49-56)."""
+    simulation_type: str = "TemporalYangMills"
+    time_step: float = 0.1
+    duration: float = 0.0
+    number_of_dimensions: int = 3
+    number_of_colors: int = 1
+    coupling_constant: float = 1.0
+    grid_steps: List[float] = field(default_factory=lambda: [1.0, 1.0, 1.0])
+    grid_cells: List[int] = field(default_factory=lambda: [1, 1, 1])
+    evaluation_region: Region = field(default_factory=Region)
+    active_region: Region = field(default_factory=Region)
+    precision: str = "fp64"          # device precision (not a reference key)
+    reunitarize_every: int = 0       # FP64 reprojection period (not a reference key)
+
+    @property
+    def iterations(self) -> int:
+        """physics/Settings.java:374-379: ceil(tMax / timeStep)."""
+        return int(math.ceil(self.duration / self.time_step))
+
+    @classmethod
+    def from_yaml(cls, text: str) -> "Settings":
+        import yaml
+
+        y = yaml.safe_load(text) or {}
+        s = cls()
+        if "simulationType" in y:
+            key = y["simulationType"]
+            if key not in SIMULATION_TYPES:
+                raise RuntimeError("Unknown simulation type specified in YAML file.")
+            s.simulation_type = SIMULATION_TYPES[key]
+        if "timeStep" in y:
+            s.time_step = float(y["timeStep"])
+        if "duration" in y:
+            s.duration = float(y["duration"])
+        if "numberOfDimensions" in y:
+            s.number_of_dimensions = int(y["numberOfDimensions"])
+        if "numberOfColors" in y:
+            s.number_of_colors = int(y["numberOfColors"])
+        if "couplingConstant" in y:
+            s.coupling_constant = float(y["couplingConstant"])
+        cells = y.get("gridCells")
+        if "gridSteps" in y:
+            if len(y["gridSteps"]) != s.number_of_dimensions:
+                raise RuntimeError("Size of gridSteps does not match numberOfDimensions.")
+            s.grid_steps = [float(v) for v in y["gridSteps"]]
+        elif "gridStep" in y:
+            s.grid_steps = [float(y["gridStep"])] * s.number_of_dimensions
+        if cells is not None:
+            if len(cells) != s.number_of_dimensions:
+                raise RuntimeError("Size of gridCells does not match numberOfDimensions.")
+            s.grid_cells = [int(v) for v in cells]
+        for key, attr in (("evaluationRegion", "evaluation_region"), ("activeRegion", "active_region")):
+            r = y.get(key)
+            if r and r.get("point1") is not None and r.get("point2") is not None and \
+                    len(r["point1"]) == s.number_of_dimensions and len(r["point2"]) == s.number_of_dimensions:
+                setattr(s, attr, Region(bool(r.get("enabled", False)), list(r["point1"]), list(r["point2"])))
+        dev = y.get("device") or {}
+        if "precision" in dev:
+            s.precision = str(dev["precision"])
+        if "reunitarizeEvery" in dev:
+            s.reunitarize_every = int(dev["reunitarizeEvery"])
+        return s
+
+
+@dataclass
+class SuperParticleSet:
+    """A list of CGCSuperParticle (physics/particles/CGCSuperParticle.java:16-46)."""
+    orientation: np.ndarray
+    number: np.ndarray
+    index_offset: np.ndarray
+    per_plane: np.ndarray
+    sub_lattice_shift: np.ndarray
+    particles_per_cell: np.ndarray
+    charges: np.ndarray          # (sum(number), A) reference algebra storage
+
+
+class Simulation:
+    """physics/Simulation.java driver over the device grid (FastTYMSolver types only)."""
+
+    def __init__(self, settings: Settings, superparticles: Optional[SuperParticleSet] = None):
+        if settings.simulation_type not in FAST_TYM_TYPES:
+            raise NotImplementedError(f"{settings.simulation_type} does not use FastTYMSolver")
+        if settings.number_of_dimensions != 3:
+            raise NotImplementedError("only 3 spatial dimensions are supported")
+        self.settings = settings
+        self.tstep = settings.time_step
+        self.iterations = settings.iterations
+        self.totalSimulationSteps = 0
+        self.grid = Grid(settings.number_of_colors, settings.grid_cells, settings.grid_steps, settings.time_step,
+                         settings.coupling_constant, settings.precision)
+        self.fsolver = FastTYMSolver()
+        if settings.evaluation_region.enabled:
+            self.grid.setEvaluationRegion(*settings.evaluation_region.resolved(settings.grid_cells))
+        if settings.active_region.enabled:
+            self.grid.setActiveRegion(*settings.active_region.resolved(settings.grid_cells))
+        self.superparticles = None
+        if superparticles is not None:
+            self.set_superparticles(superparticles)
+
+    def set_superparticles(self, sp: SuperParticleSet):
+        L = _lib.load()
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (sp.orientation, sp.number, sp.index_offset,
+                                                             sp.per_plane, sp.sub_lattice_shift,
+                                                             sp.particles_per_cell)]
+        q = np.ascontiguousarray(sp.charges, np.float64).reshape(-1)
+        self.grid._check(L.opx_superparticles_upload(self.grid.handle, len(arrs[0]), *[_lib.iptr(a) for a in arrs],
+                                                     _lib.dptr(q), q.size))
+        self.superparticles = sp
+
+    def superparticle_charges(self):
+        sp = self.superparticles
+        out = np.zeros(int(np.sum(sp.number)) * sp.charges.shape[-1])
+        self.grid._check(_lib.load().opx_superparticles_download_charges(self.grid.handle, _lib.dptr(out), out.size))
+        return out.reshape(-1, sp.charges.shape[-1])
+
+    def initialize(self):
+        """physics/Simulation.java:318-352."""
+        self.grid._check(_lib.load().opx_simulation_initialize(self.grid.handle))
+
+    def step(self, n: int = 1):
+        """physics/Simulation.java:373-435 (diagnostics and external currents excluded)."""
+        self.grid._check(_lib.load().opx_simulation_step(self.grid.handle, int(n)))
+        self.totalSimulationSteps = _lib.load().opx_simulation_steps_done(self.grid.handle)
+        if self.settings.reunitarize_every and self.totalSimulationSteps % self.settings.reunitarize_every == 0:
+            self.grid._check(_lib.load().opx_reunitarize(self.grid.handle))
+
+    def continues(self) -> bool:
+        """physics/Simulation.java:441-443: steps <= iterations (iterations + 1 steps in total)."""
+        return self.totalSimulationSteps <= self.iterations
+
+    def run(self):
+        while self.continues():
+            self.step()
+
+    def measure(self):
+        return FieldMeasurements().measure(self.grid)

@@ -1,0 +1,440 @@
+"""FP64 CPU oracle for OpenPixi's FastTYMSolver path (ctypes wrapper).
+
+TEST INFRASTRUCTURE ONLY: the checker against which the HIP product path is
+compared. Only tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this package. The product package ``openpixi_amd`` never
+imports it.
+
+The C sources (``opx_oracle.c``) restate the reference's Java algorithm; each
+function cites the reference file:line it follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libopx_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_ubyte)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return _LIB_PATH
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [
+        ("nc", ctypes.c_int),
+        ("dims", ctypes.c_int * 3),
+        ("as_", ctypes.c_double * 3),
+        ("at", ctypes.c_double),
+        ("g", ctypes.c_double),
+        ("n", ctypes.c_int64),
+        ("cum", ctypes.c_int64 * 4),
+        ("gsz", ctypes.c_int),
+        ("asz", ctypes.c_int),
+        ("U", _dp),
+        ("Unext", _dp),
+        ("E", _dp),
+        ("J", _dp),
+        ("rho", _dp),
+        ("active", _u8p),
+        ("evaluatable", _u8p),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+class _SP(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_int),
+        ("orientation", _ip),
+        ("num", _ip),
+        ("index_offset", _ip),
+        ("per_plane", _ip),
+        ("shift", _ip),
+        ("ppc", _ip),
+        ("q_offset", _i64p),
+        ("q", _dp),
+        ("total_charges", ctypes.c_int64),
+    ]
+
+
+class _P(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_int),
+        ("direction", _ip),
+        ("pos0", _dp),
+        ("pos1", _dp),
+        ("vel", _dp),
+        ("Q0", _dp),
+        ("Q1", _dp),
+        ("Ulink", _dp),
+        ("update_charge", _u8p),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        gp = ctypes.POINTER(_Grid)
+        spp = ctypes.POINTER(_SP)
+        pp = ctypes.POINTER(_P)
+        sig = {
+            "orc_group_mult": (None, [ctypes.c_int, _dp, _dp, _dp]),
+            "orc_group_adj": (None, [ctypes.c_int, _dp, _dp]),
+            "orc_group_proj": (None, [ctypes.c_int, _dp, _dp]),
+            "orc_alg_get_link": (None, [ctypes.c_int, _dp, _dp]),
+            "orc_alg_act": (None, [ctypes.c_int, _dp, _dp, _dp]),
+            "orc_alg_square": (ctypes.c_double, [ctypes.c_int, _dp]),
+            "orc_alg_get": (ctypes.c_double, [ctypes.c_int, _dp, ctypes.c_int]),
+            "orc_alg_from_coeffs": (None, [ctypes.c_int, _dp, _dp]),
+            "orc_links_from_coeffs": (None, [ctypes.c_int, ctypes.c_int64, _dp, _dp]),
+            "orc_algs_from_coeffs": (None, [ctypes.c_int, ctypes.c_int64, _dp, _dp]),
+            "orc_grid_create": (gp, [ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_double]),
+            "orc_grid_destroy": (None, [gp]),
+            "orc_grid_set_threads": (None, [gp, ctypes.c_int]),
+            "orc_shift": (ctypes.c_int64, [gp, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
+            "orc_set_active_region": (None, [gp, _ip, _ip]),
+            "orc_set_evaluation_region": (None, [gp, _ip, _ip]),
+            "orc_staple_sum": (None, [gp, ctypes.c_int64, ctypes.c_int, _dp]),
+            "orc_step": (None, [gp, ctypes.c_double]),
+            "orc_step_links": (None, [gp, ctypes.c_double]),
+            "orc_store_fields": (None, [gp]),
+            "orc_reset_current": (None, [gp]),
+            "orc_reset_charge": (None, [gp]),
+            "orc_plaquette": (None, [gp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, _dp]),
+            "orc_get_B": (None, [gp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp]),
+            "orc_gauss_constraint": (None, [gp, ctypes.c_int64, _dp]),
+            "orc_gauss_constraint_squared": (ctypes.c_double, [gp, ctypes.c_int64]),
+            "orc_measure": (None, [gp, _dp]),
+            "orc_projected_energy_density": (None, [gp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+            "orc_gauge_transform": (None, [gp, _dp]),
+            "orc_sp_current_offset": (ctypes.c_int, [ctypes.c_int] * 6),
+            "orc_sp_ngp_offset": (ctypes.c_int, [ctypes.c_int] * 6),
+            "orc_sp_needs_update": (ctypes.c_int, [ctypes.c_int] * 4),
+            "orc_sp_interpolate_to_grid": (None, [gp, spp, ctypes.c_int]),
+            "orc_sp_interpolate_chargedensity": (None, [gp, spp, ctypes.c_int]),
+            "orc_nearest_grid_point": (ctypes.c_int, [ctypes.c_double, ctypes.c_double]),
+            "orc_sim_initialize": (None, [gp, spp, pp, ctypes.c_double]),
+            "orc_sim_step": (None, [gp, spp, pp, ctypes.c_double, _ip]),
+            "orc_time_steps": (ctypes.c_double, [gp, ctypes.c_int, ctypes.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def gsize(nc: int) -> int:
+    return 4 if nc == 2 else 18
+
+
+def asize(nc: int) -> int:
+    return 3 if nc == 2 else 9
+
+
+# ---- element-level helpers (operate on 1-D float64 arrays) ----
+
+def group_mult(nc, a, b):
+    o = np.zeros(gsize(nc))
+    lib().orc_group_mult(nc, _ptr(np.ascontiguousarray(a, np.float64)), _ptr(np.ascontiguousarray(b, np.float64)), _ptr(o))
+    return o
+
+
+def group_adj(nc, a):
+    o = np.zeros(gsize(nc))
+    lib().orc_group_adj(nc, _ptr(np.ascontiguousarray(a, np.float64)), _ptr(o))
+    return o
+
+
+def group_proj(nc, a):
+    o = np.zeros(asize(nc))
+    lib().orc_group_proj(nc, _ptr(np.ascontiguousarray(a, np.float64)), _ptr(o))
+    return o
+
+
+def alg_get_link(nc, v):
+    o = np.zeros(gsize(nc))
+    lib().orc_alg_get_link(nc, _ptr(np.ascontiguousarray(v, np.float64)), _ptr(o))
+    return o
+
+
+def alg_act(nc, v, g):
+    o = np.zeros(asize(nc))
+    lib().orc_alg_act(nc, _ptr(np.ascontiguousarray(v, np.float64)), _ptr(np.ascontiguousarray(g, np.float64)), _ptr(o))
+    return o
+
+
+def alg_square(nc, v):
+    return lib().orc_alg_square(nc, _ptr(np.ascontiguousarray(v, np.float64)))
+
+
+def alg_coeffs(nc, v):
+    """Generator coefficients of an algebra element in reference storage."""
+    v = np.ascontiguousarray(v, np.float64)
+    return np.array([lib().orc_alg_get(nc, _ptr(v), j) for j in range(nc * nc - 1)])
+
+
+def alg_from_coeffs(nc, c):
+    o = np.zeros(asize(nc))
+    lib().orc_alg_from_coeffs(nc, _ptr(np.ascontiguousarray(c, np.float64)), _ptr(o))
+    return o
+
+
+def links_from_coeffs(nc, coeffs):
+    """Group elements exp(i c_a t_a) for an array (..., nc*nc-1) of coefficients."""
+    c = np.ascontiguousarray(coeffs, np.float64)
+    shape = c.shape[:-1]
+    cnt = int(np.prod(shape)) if shape else 1
+    out = np.zeros(shape + (gsize(nc),))
+    lib().orc_links_from_coeffs(nc, cnt, _ptr(c.reshape(-1)), _ptr(out.reshape(-1)))
+    return out
+
+
+def algs_from_coeffs(nc, coeffs):
+    """Reference algebra storage for an array (..., nc*nc-1) of coefficients."""
+    c = np.ascontiguousarray(coeffs, np.float64)
+    shape = c.shape[:-1]
+    cnt = int(np.prod(shape)) if shape else 1
+    out = np.zeros(shape + (asize(nc),))
+    lib().orc_algs_from_coeffs(nc, cnt, _ptr(c.reshape(-1)), _ptr(out.reshape(-1)))
+    return out
+
+
+class OracleGrid:
+    """A reference-layout lattice (AoS per cell) advanced by the FP64 oracle."""
+
+    def __init__(self, nc, dims, spacing=(1.0, 1.0, 1.0), at=0.25, g=1.0, threads=1):
+        L = lib()
+        self.nc = nc
+        self.dims = tuple(int(d) for d in dims)
+        self.n = int(np.prod(self.dims))
+        d = (ctypes.c_int * 3)(*self.dims)
+        s = (ctypes.c_double * 3)(*[float(x) for x in spacing])
+        self._g = L.orc_grid_create(nc, d, s, float(at), float(g))
+        if not self._g:
+            raise ValueError("unsupported number of colours")
+        self.at = float(at)
+        self.spacing = tuple(float(x) for x in spacing)
+        self.coupling = float(g)
+        self.steps_done = 0
+        self._sp = None
+        self._sp_keep = None
+        self._p = None
+        self._p_keep = None
+        if threads > 1:
+            L.orc_grid_set_threads(self._g, int(threads))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_g", None):
+                lib().orc_grid_destroy(self._g)
+                self._g = None
+        except Exception:
+            pass
+
+    # field views (refresh after store_fields, which swaps pointers)
+    def _view(self, name, per):
+        ptr = getattr(self._g.contents, name)
+        return np.ctypeslib.as_array(ptr, shape=(self.n * per,))
+
+    @property
+    def U(self):
+        return self._view("U", 3 * gsize(self.nc)).reshape(self.n, 3, gsize(self.nc))
+
+    @property
+    def Unext(self):
+        return self._view("Unext", 3 * gsize(self.nc)).reshape(self.n, 3, gsize(self.nc))
+
+    @property
+    def E(self):
+        return self._view("E", 3 * asize(self.nc)).reshape(self.n, 3, asize(self.nc))
+
+    @property
+    def J(self):
+        return self._view("J", 3 * asize(self.nc)).reshape(self.n, 3, asize(self.nc))
+
+    @property
+    def rho(self):
+        return self._view("rho", asize(self.nc)).reshape(self.n, asize(self.nc))
+
+    def set_active_region(self, lo, hi):
+        lib().orc_set_active_region(self._g, (ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi))
+
+    def set_evaluation_region(self, lo, hi):
+        lib().orc_set_evaluation_region(self._g, (ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi))
+
+    def shift(self, index, d, o):
+        return lib().orc_shift(self._g, index, d, o)
+
+    def staple_sum(self, index, d):
+        o = np.zeros(gsize(self.nc))
+        lib().orc_staple_sum(self._g, index, d, _ptr(o))
+        return o
+
+    def step(self, at=None):
+        lib().orc_step(self._g, self.at if at is None else at)
+
+    def step_links(self, at=None):
+        lib().orc_step_links(self._g, self.at if at is None else at)
+
+    def store_fields(self):
+        lib().orc_store_fields(self._g)
+
+    def reset_current(self):
+        lib().orc_reset_current(self._g)
+
+    def reset_charge(self):
+        lib().orc_reset_charge(self._g)
+
+    def plaquette(self, index, d1, d2, o1, o2, t):
+        o = np.zeros(gsize(self.nc))
+        lib().orc_plaquette(self._g, index, d1, d2, o1, o2, t, _ptr(o))
+        return o
+
+    def get_B(self, index, d, t):
+        o = np.zeros(asize(self.nc))
+        lib().orc_get_B(self._g, index, d, t, _ptr(o))
+        return o
+
+    def gauss_constraint(self, index):
+        o = np.zeros(asize(self.nc))
+        lib().orc_gauss_constraint(self._g, index, _ptr(o))
+        return o
+
+    def gauss_constraint_squared(self, index):
+        return lib().orc_gauss_constraint_squared(self._g, index)
+
+    def measure(self):
+        """[E2_0,E2_1,E2_2,B2_0,B2_1,B2_2,gauss_mean_sq,total_charge_squared]."""
+        o = np.zeros(8)
+        lib().orc_measure(self._g, _ptr(o))
+        return o
+
+    def projected_energy_density(self, direction):
+        n = self.dims[direction]
+        arrs = [np.zeros(n) for _ in range(4)]
+        lib().orc_projected_energy_density(self._g, direction, *[_ptr(a) for a in arrs])
+        return arrs  # eT_el, eT_mag, eL_el, eL_mag
+
+    def gauge_transform(self, g_cells):
+        g_cells = np.ascontiguousarray(g_cells, np.float64).reshape(-1)
+        assert g_cells.size == self.n * gsize(self.nc)
+        lib().orc_gauge_transform(self._g, _ptr(g_cells))
+
+    # ---- super particles ----
+    def set_superparticles(self, orientation, num, index_offset, per_plane, shift, ppc, charges):
+        """charges: list of (num_p, A) float64 arrays in reference algebra storage."""
+        cnt = len(orientation)
+        arrs = {k: np.ascontiguousarray(v, np.int32) for k, v in dict(
+            orientation=orientation, num=num, index_offset=index_offset, per_plane=per_plane,
+            shift=shift, ppc=ppc).items()}
+        qoff = np.zeros(cnt, np.int64)
+        acc = 0
+        for i in range(cnt):
+            qoff[i] = acc
+            acc += int(num[i])
+        q = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float64).reshape(-1) for c in charges])
+                                 if cnt else np.zeros(0))
+        sp = _SP()
+        sp.count = cnt
+        for k in ("orientation", "num", "index_offset", "per_plane", "shift", "ppc"):
+            setattr(sp, k, arrs[k].ctypes.data_as(_ip))
+        sp.q_offset = qoff.ctypes.data_as(_i64p)
+        sp.q = q.ctypes.data_as(_dp)
+        sp.total_charges = acc
+        self._sp = sp
+        self._sp_keep = (arrs, qoff, q)
+
+    def superparticle_charges(self):
+        return self._sp_keep[2].reshape(-1, asize(self.nc))
+
+    def sp_interpolate_to_grid(self, t):
+        lib().orc_sp_interpolate_to_grid(self._g, ctypes.byref(self._sp), t)
+
+    def sp_interpolate_chargedensity(self, t):
+        lib().orc_sp_interpolate_chargedensity(self._g, ctypes.byref(self._sp), t)
+
+    # ---- per-particle NGP ----
+    def set_particles(self, direction, pos0, vel, Q0):
+        cnt = len(direction)
+        A, G = asize(self.nc), gsize(self.nc)
+        keep = dict(
+            direction=np.ascontiguousarray(direction, np.int32),
+            pos0=np.ascontiguousarray(pos0, np.float64).reshape(-1).copy(),
+            pos1=np.zeros(cnt * 3),
+            vel=np.ascontiguousarray(vel, np.float64).reshape(-1).copy(),
+            Q0=np.ascontiguousarray(Q0, np.float64).reshape(-1).copy(),
+            Q1=np.zeros(cnt * A),
+            Ulink=np.zeros(cnt * G),
+            update_charge=np.zeros(cnt, np.uint8),
+        )
+        p = _P()
+        p.count = cnt
+        p.direction = keep["direction"].ctypes.data_as(_ip)
+        for k in ("pos0", "pos1", "vel", "Q0", "Q1", "Ulink"):
+            setattr(p, k, keep[k].ctypes.data_as(_dp))
+        p.update_charge = keep["update_charge"].ctypes.data_as(_u8p)
+        self._p = p
+        self._p_keep = keep
+
+    def particle_state(self):
+        """Current (pos0, pos1, Q0, Q1) as stored (pointer swaps are value swaps here)."""
+        k = self._p_keep
+        A = asize(self.nc)
+        return (k["pos0"].reshape(-1, 3), k["pos1"].reshape(-1, 3), k["Q0"].reshape(-1, A), k["Q1"].reshape(-1, A))
+
+    # ---- driver ----
+    def initialize(self):
+        lib().orc_sim_initialize(self._g, ctypes.byref(self._sp) if self._sp else None,
+                                 ctypes.byref(self._p) if self._p else None, self.at)
+
+    def sim_step(self, n=1):
+        c = ctypes.c_int(self.steps_done)
+        for _ in range(n):
+            lib().orc_sim_step(self._g, ctypes.byref(self._sp) if self._sp else None,
+                               ctypes.byref(self._p) if self._p else None, self.at, ctypes.byref(c))
+        self.steps_done = c.value
+
+    def time_steps(self, steps, warmup=0):
+        return lib().orc_time_steps(self._g, steps, warmup)
+
+    def set_threads(self, n):
+        lib().orc_grid_set_threads(self._g, int(n))
+
+
+def sp_current_offset(o, off, pp, k, ppc, t):
+    return lib().orc_sp_current_offset(o, off, pp, k, ppc, t)
+
+
+def sp_ngp_offset(o, off, pp, k, ppc, t):
+    return lib().orc_sp_ngp_offset(o, off, pp, k, ppc, t)
+
+
+def sp_needs_update(o, k, ppc, t):
+    return bool(lib().orc_sp_needs_update(o, k, ppc, t))
+
+
+def nearest_grid_point(pos, a):
+    return lib().orc_nearest_grid_point(pos, a)

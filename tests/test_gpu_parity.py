@@ -1,0 +1,159 @@
+"""Parity of the gfx950 leapfrog (through the C ABI) against the FP64 oracle.
+
+FP64 device results must agree with the oracle within 1e-12 relative (max-norm
+over the lattice) after N steps; the FP32 form within the stated FP32 bounds.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from openpixi_amd import FieldMeasurements, FastTYMSolver
+
+from .helpers import device_grid, make_state, oracle_grid, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def run_both(nc, dims, steps, at=0.25, spacing=(1.0, 1.0, 1.0), sigma_a=0.1, sigma_e=0.05, seed=0, J=None,
+             precision="fp64", active=None):
+    U, E = make_state(nc, dims, sigma_a, sigma_e, seed)
+    og = oracle_grid(nc, dims, U, E, at, spacing, J=J)
+    dg = device_grid(nc, dims, U, E, at, spacing, precision=precision, J=J)
+    if active is not None:
+        og.set_active_region(*active)
+        dg.setActiveRegion(*active)
+    solver = FastTYMSolver()
+    for _ in range(steps):
+        og.store_fields()
+        og.step(at)
+        dg.storeFields()
+        solver.step(dg, at)
+    return og, dg
+
+
+@pytest.mark.parametrize("nc,dims,steps", [(2, (32, 32, 32), 100), (2, (5, 7, 6), 60), (3, (6, 6, 6), 40),
+                                           (3, (4, 5, 7), 30)])
+def test_leapfrog_fp64_parity(gpu, nc, dims, steps):
+    print(f"leapfrog parity nc={nc} dims={dims} steps={steps}", flush=True)
+    og, dg = run_both(nc, dims, steps)
+    assert rel_err(dg.getU(), og.U) < FP64_TOL
+    assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
+    assert rel_err(dg.getE(), og.E) < FP64_TOL
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_leapfrog_strong_fields_anisotropic(gpu, nc):
+    """Large amplitudes (SU(3) exp beyond the Taylor branch) and uneven spacings."""
+    print(f"strong fields nc={nc}", flush=True)
+    og, dg = run_both(nc, (4, 6, 5), 10, at=0.1, spacing=(1.0, 0.8, 1.3), sigma_a=1.0, sigma_e=3.0, seed=7)
+    assert rel_err(dg.getU(), og.U) < 1e-11
+    assert rel_err(dg.getE(), og.E) < 1e-11
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_current_and_active_region(gpu, nc):
+    dims = (6, 5, 4)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(3)
+    J = rng.normal(0, 0.1, size=(n, 3, O.asize(nc)))
+    if nc == 3:
+        from openpixi_amd.inputs import su3_coeffs_to_storage
+
+        J = su3_coeffs_to_storage(rng.normal(0, 0.1, size=(n, 3, 8)))
+    og, dg = run_both(nc, dims, 12, J=J, active=((1, 0, 1), (4, 4, 2)), seed=5)
+    assert rel_err(dg.getU(), og.U) < FP64_TOL
+    assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
+    assert rel_err(dg.getE(), og.E) < FP64_TOL
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_step_links_parity(gpu, nc):
+    dims = (4, 4, 6)
+    U, E = make_state(nc, dims, 0.3, 0.2, seed=11)
+    og = oracle_grid(nc, dims, U, E)
+    dg = device_grid(nc, dims, U, E)
+    og.step_links(0.25)
+    FastTYMSolver().stepLinks(dg, 0.25)
+    assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_measurements_parity(gpu, nc):
+    dims = (6, 5, 7)
+    og, dg = run_both(nc, dims, 5, sigma_a=0.3, sigma_e=0.2, seed=2)
+    m_o = og.measure()
+    m_d = FieldMeasurements().measure(dg)
+    for k in range(7):
+        assert abs(m_d[k] - m_o[k]) <= 1e-12 * max(abs(m_o[k]), 1e-30), (k, m_d[k], m_o[k])
+    # evaluation region
+    og.set_evaluation_region((1, 1, 1), (4, 3, 5))
+    dg.setEvaluationRegion((1, 1, 1), (4, 3, 5))
+    m_o = og.measure()
+    m_d = FieldMeasurements().measure(dg)
+    for k in range(7):
+        assert abs(m_d[k] - m_o[k]) <= 1e-12 * max(abs(m_o[k]), 1e-30), (k, m_d[k], m_o[k])
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_projected_energy_parity(gpu, nc):
+    dims = (6, 4, 5)
+    og, dg = run_both(nc, dims, 3, sigma_a=0.3, sigma_e=0.2, seed=4)
+    for d in range(3):
+        ref = og.projected_energy_density(d)
+        got = FieldMeasurements().projectedEnergyDensity(dg, d)
+        for a, b in zip(got, ref):
+            assert rel_err(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_fp32_form_tracks_fp64(gpu, nc):
+    """FP32 links/E with FP64 reunitarization every 10 steps stay within FP32
+    rounding of the FP64 oracle, and Gauss's law stays conserved."""
+    dims = (8, 8, 8)
+    steps = 50
+    U, E = make_state(nc, dims, 0.1, 0.05, seed=1)
+    og = oracle_grid(nc, dims, U, E)
+    dg = device_grid(nc, dims, U, E, precision="fp32")
+    g0 = og.measure()[6]
+    from openpixi_amd import _lib
+
+    L = _lib.load()
+    for s in range(steps):
+        og.store_fields()
+        og.step(0.25)
+    dg._check(L.opx_leapfrog(dg.handle, steps, 10))
+    assert rel_err(dg.getU(), og.U) < 2e-5
+    assert rel_err(dg.getE(), og.E) < 2e-4
+    m_d = FieldMeasurements().measure(dg)
+    m_o = og.measure()
+    assert abs(m_d[6] - m_o[6]) <= 1e-4 * abs(g0) + 1e-12
+    e_o = m_o[0:6].sum()
+    assert abs(m_d[0:6].sum() - e_o) <= 1e-5 * abs(e_o)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_reunitarize_and_health(gpu, nc):
+    dims = (4, 4, 4)
+    U, E = make_state(nc, dims, 0.5, 0.0, seed=9)
+    dg = device_grid(nc, dims, U, E)
+    nf, nu = dg.checkHealth(1e-12)
+    assert nf == 0 and nu == 0
+    # unitary input: reunitarization moves it by rounding only
+    dg.reunitarize()
+    assert rel_err(dg.getU(), U) < 1e-14
+    # a scaled (non-unitary) link is flagged and then repaired
+    bad = U.copy()
+    bad[5, 1] *= 1.01
+    dg.setU(bad)
+    nf, nu = dg.checkHealth(1e-6)
+    assert nf == 0 and nu >= 1
+    dg.reunitarize()
+    nf, nu = dg.checkHealth(1e-12)
+    assert nu == 0
+    # a NaN is flagged
+    bad[3, 0, 0] = np.nan
+    dg.setU(bad)
+    nf, nu = dg.checkHealth(1e-6)
+    assert nf >= 1
