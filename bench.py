@@ -45,7 +45,34 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--dims", type=int, nargs=3, default=list(DIMS))
     p.add_argument("--forms", default="fp32,fp64")
+    p.add_argument("--probe", action="store_true",
+                   help="counter-collection run: a few fused-kernel launches per form plus a 1 GiB device copy "
+                        "(the calibration kernel for FETCH_SIZE/WRITE_SIZE); prints no JSON line")
     return p.parse_args()
+
+
+def probe(dims, forms, reps=5):
+    """Launches for hardware-counter runs: `reps` fused updates per form, then a
+    1 GiB float32 device-to-device copy of known size."""
+    import torch
+    from openpixi_amd import _lib
+
+    L = _lib.load()
+    for f in forms:
+        g = make_grid(f, dims)
+        for _ in range(reps):
+            g._check(L.opx_store_fields(g.handle))
+            g._check(L.opx_step(g.handle, DT))
+        g.sync()
+        g.close()
+        print(f"# probe {f}: {reps} fused launches", file=sys.stderr, flush=True)
+    a = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"# probe copy: {reps} x 1 GiB read + 1 GiB write", file=sys.stderr, flush=True)
 
 
 def make_grid(precision, dims, seed=0):
@@ -156,8 +183,11 @@ def main():
     import torch
 
     dims = tuple(args.dims)
-    stream = torch.cuda.Stream()
     forms = args.forms.split(",")
+    if args.probe:
+        probe(dims, forms)
+        return
+    stream = torch.cuda.Stream()
     results = {}
     for f in forms:
         results[f] = time_form(f, dims, args.steps, args.warmup, args.kernel_reps, stream)
