@@ -45,27 +45,41 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--dims", type=int, nargs=3, default=list(DIMS))
     p.add_argument("--forms", default="fp32,fp64")
+    p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--probe-kernels", default="auto",
+                   help="kernel configurations for --probe: comma list of link | stream:TILE:CHUNKS | auto")
     p.add_argument("--probe", action="store_true",
                    help="counter-collection run: a few fused-kernel launches per form plus a 1 GiB device copy "
                         "(the calibration kernel for FETCH_SIZE/WRITE_SIZE); prints no JSON line")
     return p.parse_args()
 
 
-def probe(dims, forms, reps=5):
-    """Launches for hardware-counter runs: `reps` fused updates per form, then a
-    1 GiB float32 device-to-device copy of known size."""
+def parse_kernels(spec):
+    """"link,stream:1:16" -> [("link", None, None), ("stream", 1, 16)]; "auto" keeps the default."""
+    out = []
+    for item in spec.split(","):
+        parts = item.split(":")
+        out.append((parts[0], int(parts[1]) if len(parts) > 1 else None, int(parts[2]) if len(parts) > 2 else None))
+    return out
+
+
+def probe(dims, forms, kernels, reps=5):
+    """Launches for hardware-counter runs: `reps` fused updates per form and kernel
+    configuration, then a 1 GiB float32 device-to-device copy of known size."""
     import torch
     from openpixi_amd import _lib
 
     L = _lib.load()
     for f in forms:
         g = make_grid(f, dims)
-        for _ in range(reps):
-            g._check(L.opx_store_fields(g.handle))
-            g._check(L.opx_step(g.handle, DT))
-        g.sync()
+        for k in kernels:
+            g.setKernel(*k)
+            for _ in range(reps):
+                g._check(L.opx_store_fields(g.handle))
+                g._check(L.opx_step(g.handle, DT))
+            g.sync()
+            print(f"# probe {f} {g.kernelConfig()}: {reps} fused launches", file=sys.stderr, flush=True)
         g.close()
-        print(f"# probe {f}: {reps} fused launches", file=sys.stderr, flush=True)
     a = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").fill_(1.0)
     b = torch.empty_like(a)
     torch.cuda.synchronize()
@@ -89,13 +103,15 @@ def make_grid(precision, dims, seed=0):
     return g
 
 
-def time_form(precision, dims, steps, warmup, kernel_reps, stream):
+def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True):
     import torch
     from openpixi_amd import FieldMeasurements, _lib
 
     L = _lib.load()
     g = make_grid(precision, dims)
     g._check(L.opx_set_stream(g.handle, ctypes_void(stream.cuda_stream)))
+    tune_ms = g.autotune(reps=3) if autotune else None
+    kcfg = g.kernelConfig()
     reunit = REUNIT_EVERY if precision == "fp32" else 0
     # initial half step U(-dt/2) -> U(+dt/2), as Simulation.initialize
     g._check(L.opx_step_links(g.handle, DT))
@@ -138,6 +154,8 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream):
         "energy_start": energy0,
         "energy_end": energy1,
         "energy_rel_drift": abs(energy1 - energy0) / abs(energy0),
+        "kernel_config": kcfg,
+        "autotune_ms": tune_ms,
     }
     g.close()
     return res
@@ -185,12 +203,12 @@ def main():
     dims = tuple(args.dims)
     forms = args.forms.split(",")
     if args.probe:
-        probe(dims, forms)
+        probe(dims, forms, parse_kernels(args.probe_kernels))
         return
     stream = torch.cuda.Stream()
     results = {}
     for f in forms:
-        results[f] = time_form(f, dims, args.steps, args.warmup, args.kernel_reps, stream)
+        results[f] = time_form(f, dims, args.steps, args.warmup, args.kernel_reps, stream, not args.no_autotune)
         print(f"# {f}: {json.dumps(results[f])}", file=sys.stderr, flush=True)
     main_form = "fp32" if "fp32" in results else forms[0]
     r = results[main_form]
@@ -226,7 +244,7 @@ def main():
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": None,
-        "kernel": "k_combined_update<3,%s>" % ("float" if main_form == "fp32" else "double"),
+        "kernel": r["kernel_config"],
         "kernel_ms": r["kernel_ms"],
         "bytes_per_link": BYTES_PER_LINK[main_form],
     }

@@ -107,6 +107,23 @@ int opx_store_fields(opx_ctx *ctx);
 int opx_reset_current(opx_ctx *ctx);
 int opx_reset_charge(opx_ctx *ctx);
 
+/* Kernel form of the fused update (no counterpart in the reference):
+ *   OPX_KERNEL_LINK   one thread per link, neighbour links from global memory
+ *                     (any lattice, any active region);
+ *   OPX_KERNEL_STREAM tiles of (x1,x2) columns marching along x0, each plaquette
+ *                     computed once and exchanged through LDS;
+ *   OPX_KERNEL_AUTO   STREAM where supported, else LINK (default).
+ * OPX_OPT_TILE selects the STREAM tile shape (0: 16x16, 1: 8x32, 2: 16x32,
+ * 3: 8x64, 4: 8x16 threads over (x1,x2)); OPX_OPT_CHUNKS the number of x0
+ * chunks (0 = derived from the CU count). opx_autotune times every candidate
+ * on scratch copies of E and Unext (the lattice state is not changed), keeps
+ * the fastest and returns its time per launch in *best_ms. */
+enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2 };
+enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2 };
+int opx_set_option(opx_ctx *ctx, int option, int value);
+int opx_get_option(const opx_ctx *ctx, int option, int *value);
+int opx_autotune(opx_ctx *ctx, int reps, double *best_ms);
+
 /* Reprojects every link of U and Unext onto the group in double precision
  * (SU(2): quaternion normalisation; SU(3): Gram-Schmidt on rows 1-2, row 3 =
  * conj(row1 x row2)). No counterpart in the reference (which never

@@ -25,6 +25,9 @@ OPX_FP32 = 1
 
 FIELD_U, FIELD_UNEXT, FIELD_E, FIELD_J, FIELD_RHO = range(5)
 
+OPT_KERNEL, OPT_TILE, OPT_CHUNKS = range(3)
+KERNEL_AUTO, KERNEL_LINK, KERNEL_STREAM = range(3)
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -50,6 +53,9 @@ SIGNATURES = {
     "opx_store_fields": (ctypes.c_int, [_vp]),
     "opx_reset_current": (ctypes.c_int, [_vp]),
     "opx_reset_charge": (ctypes.c_int, [_vp]),
+    "opx_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "opx_get_option": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "opx_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "opx_reunitarize": (ctypes.c_int, [_vp]),
     "opx_leapfrog": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "opx_measure": (ctypes.c_int, [_vp, _dp]),

@@ -50,6 +50,11 @@ struct opx_ctx {
     bool rho_pending = false;       // charge density deposit deferred until rho is read
     int rho_pending_t = 0;
     int steps = 0;                  // Simulation.totalSimulationSteps
+    // fused-update kernel selection (opx_set_option / opx_autotune)
+    int kernel = OPX_KERNEL_AUTO;
+    int tile = 0;
+    int nchunks = 0;                // 0: derived from the CU count
+    int num_cus = 256;
     std::string err;
 };
 
@@ -182,6 +187,27 @@ int check_box(opx_ctx* ctx, const int* lo, const int* hi)
 
 int materialize_rho(opx_ctx* ctx);
 
+uint32_t auto_chunks(const opx_ctx* c, const opx::Lattice& L, int tile)
+{
+    const uint32_t bpc = opx::stream_blocks_per_chunk(L, tile);
+    uint32_t k = (uint32_t)((2 * c->num_cus + bpc / 2) / (bpc ? bpc : 1));
+    if (k < 1) k = 1;
+    if (k > L.d0) k = L.d0;
+    return k;
+}
+
+// The fused E-field/link update with the configured kernel form.
+hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E)
+{
+    if (c->kernel != OPX_KERNEL_LINK && opx::stream_supported(c->nc, c->prec, L, c->tile)) {
+        const uint32_t k = c->nchunks > 0 ? (uint32_t)c->nchunks : auto_chunks(c, L, c->tile);
+        return opx::launch_stream_update(c->nc, c->prec, L, c->active, c->tile, k, U, Unext, E, c->J, c->j_nonzero,
+                                         c->stream);
+    }
+    if (c->kernel == OPX_KERNEL_STREAM) return hipErrorInvalidConfiguration;
+    return opx::launch_combined_update(c->nc, c->prec, L, c->active, U, Unext, E, c->J, c->j_nonzero, c->stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -216,6 +242,12 @@ int opx_create(int num_colors, int num_dims, const int* num_cells, const double*
     ctx->AS = num_colors == 2 ? 3 : 9;
     ctx->tsz = precision == OPX_FP64 ? 8 : 4;
     int rc = hip_check(ctx, hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking), "stream");
+    if (rc == OPX_OK) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            ctx->num_cus = cus;
+    }
     if (rc == OPX_OK) {
         ctx->stream = ctx->own_stream;
         rc = hip_check(ctx, hipMalloc(&ctx->U, link_count(ctx) * ctx->tsz), "alloc U");
@@ -436,9 +468,7 @@ int opx_step(opx_ctx* ctx, double dt)
 {
     if (!ctx) return OPX_E_ARG;
     opx::Lattice L = lattice(ctx, dt);
-    OPX_TRY(opx::launch_combined_update(ctx->nc, ctx->prec, L, ctx->active, ctx->U, ctx->Unext, ctx->E, ctx->J,
-                                        ctx->j_nonzero, ctx->stream),
-            "step");
+    OPX_TRY(launch_update(ctx, L, ctx->U, ctx->Unext, ctx->E), "step");
     return OPX_OK;
 }
 
@@ -490,9 +520,7 @@ int opx_leapfrog(opx_ctx* ctx, int steps, int reunit_every)
     opx::Lattice L = lattice(ctx, ctx->at);
     for (int s = 0; s < steps; s++) {
         std::swap(ctx->U, ctx->Unext);
-        OPX_TRY(opx::launch_combined_update(ctx->nc, ctx->prec, L, ctx->active, ctx->U, ctx->Unext, ctx->E, ctx->J,
-                                            ctx->j_nonzero, ctx->stream),
-                "leapfrog step");
+        OPX_TRY(launch_update(ctx, L, ctx->U, ctx->Unext, ctx->E), "leapfrog step");
         ctx->steps++;
         if (reunit_every > 0 && ctx->steps % reunit_every == 0)
             OPX_TRY(opx::launch_reunitarize(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->Unext, ctx->stream),
@@ -741,6 +769,124 @@ int opx_check_health(opx_ctx* ctx, double tol, int64_t* non_finite, int64_t* non
     *non_finite = (int64_t)h[0];
     *non_unitary = (int64_t)h[1];
     return (h[0] || h[1]) ? fail(ctx, OPX_E_NUMERIC, "health monitor: non-finite or non-unitary links") : OPX_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int opx_set_option(opx_ctx* ctx, int option, int value)
+{
+    if (!ctx) return OPX_E_ARG;
+    switch (option) {
+    case OPX_OPT_KERNEL:
+        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_STREAM) return fail(ctx, OPX_E_ARG, "option: bad kernel");
+        ctx->kernel = value;
+        return OPX_OK;
+    case OPX_OPT_TILE:
+        if (value < 0 || value >= opx::kStreamTiles) return fail(ctx, OPX_E_ARG, "option: bad tile shape");
+        ctx->tile = value;
+        return OPX_OK;
+    case OPX_OPT_CHUNKS:
+        if (value < 0) return fail(ctx, OPX_E_ARG, "option: negative chunk count");
+        ctx->nchunks = value;
+        return OPX_OK;
+    default:
+        return fail(ctx, OPX_E_ARG, "option: unknown option");
+    }
+}
+
+int opx_get_option(const opx_ctx* ctx, int option, int* value)
+{
+    if (!ctx || !value) return OPX_E_ARG;
+    switch (option) {
+    case OPX_OPT_KERNEL: *value = ctx->kernel; return OPX_OK;
+    case OPX_OPT_TILE: *value = ctx->tile; return OPX_OK;
+    case OPX_OPT_CHUNKS: {
+        opx::Lattice L = lattice(ctx, ctx->at);
+        *value = ctx->nchunks > 0 ? ctx->nchunks : (int)auto_chunks(ctx, L, ctx->tile);
+        return OPX_OK;
+    }
+    default: return OPX_E_ARG;
+    }
+}
+
+int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (reps < 1) return fail(ctx, OPX_E_ARG, "autotune: reps must be >= 1");
+    opx::Lattice L = lattice(ctx, ctx->at);
+    // candidates run on copies of E and Unext so the lattice state is untouched
+    void* e2 = nullptr;
+    void* u2 = nullptr;
+    OPX_TRY(hipMalloc(&e2, alg_count(ctx) * ctx->tsz), "autotune alloc");
+    int rc = hip_check(ctx, hipMalloc(&u2, link_count(ctx) * ctx->tsz), "autotune alloc");
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (rc == OPX_OK) rc = hip_check(ctx, hipEventCreate(&ev0), "autotune event");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipEventCreate(&ev1), "autotune event");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemcpyAsync(e2, ctx->E, alg_count(ctx) * ctx->tsz, hipMemcpyDeviceToDevice, ctx->stream),
+                       "autotune copy");
+    const int saved_kernel = ctx->kernel, saved_tile = ctx->tile, saved_chunks = ctx->nchunks;
+    double best = 1e30;
+    int bk = OPX_KERNEL_LINK, bt = 0, bc = 0;
+    struct Cand {
+        int kernel, tile, chunks;
+    };
+    std::vector<Cand> cands;
+    cands.push_back({OPX_KERNEL_LINK, 0, 0});
+    for (int t = 0; t < opx::kStreamTiles; t++) {
+        if (!opx::stream_supported(ctx->nc, ctx->prec, L, t)) continue;
+        const uint32_t bpc = opx::stream_blocks_per_chunk(L, t);
+        std::vector<int> ks;
+        for (double f : {0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0}) {
+            int k = (int)(f * ctx->num_cus / (double)bpc + 0.5);
+            if (k < 1) k = 1;
+            if (k > (int)L.d0) k = (int)L.d0;
+            bool dup = false;
+            for (int q : ks) dup = dup || q == k;
+            if (!dup) ks.push_back(k);
+        }
+        for (int k : ks) cands.push_back({OPX_KERNEL_STREAM, t, k});
+    }
+    for (const Cand& cd : cands) {
+        if (rc != OPX_OK) break;
+        ctx->kernel = cd.kernel;
+        ctx->tile = cd.tile;
+        ctx->nchunks = cd.chunks;
+        hipError_t e = launch_update(ctx, L, ctx->U, u2, e2);  // warm-up
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        rc = hip_check(ctx, hipEventRecord(ev0, ctx->stream), "autotune record");
+        for (int r = 0; r < reps && rc == OPX_OK; r++) rc = hip_check(ctx, launch_update(ctx, L, ctx->U, u2, e2), "autotune launch");
+        if (rc == OPX_OK) rc = hip_check(ctx, hipEventRecord(ev1, ctx->stream), "autotune record");
+        if (rc == OPX_OK) rc = hip_check(ctx, hipEventSynchronize(ev1), "autotune sync");
+        float ms = 0.f;
+        if (rc == OPX_OK) rc = hip_check(ctx, hipEventElapsedTime(&ms, ev0, ev1), "autotune time");
+        if (rc == OPX_OK && ms / reps < best) {
+            best = ms / reps;
+            bk = cd.kernel;
+            bt = cd.tile;
+            bc = cd.chunks;
+        }
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (e2) (void)hipFree(e2);
+    if (u2) (void)hipFree(u2);
+    if (rc != OPX_OK) {
+        ctx->kernel = saved_kernel;
+        ctx->tile = saved_tile;
+        ctx->nchunks = saved_chunks;
+        return rc;
+    }
+    ctx->kernel = bk;
+    ctx->tile = bt;
+    ctx->nchunks = bc;
+    if (best_ms) *best_ms = best;
+    return OPX_OK;
 }
 
 }  // extern "C"

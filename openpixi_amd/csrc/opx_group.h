@@ -126,6 +126,10 @@ struct Alg3 {
     T c[8];
 };
 
+// Complex 3x3 products written as chains of single multiply-adds (one fused
+// multiply-add per real term), e.g. re = a*b; re -= c*d; re += e*f; ...
+
+// a * b
 template <typename T>
 OPX_HD Su3<T> mul(const Su3<T>& a, const Su3<T>& b)
 {
@@ -134,12 +138,16 @@ OPX_HD Su3<T> mul(const Su3<T>& a, const Su3<T>& b)
     for (int r = 0; r < 3; r++) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            T re = a.r[3 * r] * b.r[c] - a.i[3 * r] * b.i[c];
-            T im = a.r[3 * r] * b.i[c] + a.i[3 * r] * b.r[c];
+            T re = a.r[3 * r] * b.r[c];
+            re -= a.i[3 * r] * b.i[c];
+            T im = a.r[3 * r] * b.i[c];
+            im += a.i[3 * r] * b.r[c];
 #pragma unroll
             for (int k = 1; k < 3; k++) {
-                re += a.r[3 * r + k] * b.r[3 * k + c] - a.i[3 * r + k] * b.i[3 * k + c];
-                im += a.r[3 * r + k] * b.i[3 * k + c] + a.i[3 * r + k] * b.r[3 * k + c];
+                re += a.r[3 * r + k] * b.r[3 * k + c];
+                re -= a.i[3 * r + k] * b.i[3 * k + c];
+                im += a.r[3 * r + k] * b.i[3 * k + c];
+                im += a.i[3 * r + k] * b.r[3 * k + c];
             }
             o.r[3 * r + c] = re;
             o.i[3 * r + c] = im;
@@ -162,27 +170,32 @@ OPX_HD Su3<T> adj(const Su3<T>& a)
     return o;
 }
 
+// entry (r, c) of a * b^dagger: sum_k a[r][k] * conj(b[c][k])
+template <typename T>
+OPX_HD void mul_adj_entry(const Su3<T>& a, const Su3<T>& b, int r, int c, T& re, T& im)
+{
+    re = a.r[3 * r] * b.r[3 * c];
+    re += a.i[3 * r] * b.i[3 * c];
+    im = a.i[3 * r] * b.r[3 * c];
+    im -= a.r[3 * r] * b.i[3 * c];
+#pragma unroll
+    for (int k = 1; k < 3; k++) {
+        re += a.r[3 * r + k] * b.r[3 * c + k];
+        re += a.i[3 * r + k] * b.i[3 * c + k];
+        im += a.i[3 * r + k] * b.r[3 * c + k];
+        im -= a.r[3 * r + k] * b.i[3 * c + k];
+    }
+}
+
 // a * b^dagger
 template <typename T>
 OPX_HD Su3<T> mul_adj(const Su3<T>& a, const Su3<T>& b)
 {
     Su3<T> o;
 #pragma unroll
-    for (int r = 0; r < 3; r++) {
+    for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            // sum_k a[r][k] * conj(b[c][k])
-            T re = a.r[3 * r] * b.r[3 * c] + a.i[3 * r] * b.i[3 * c];
-            T im = a.i[3 * r] * b.r[3 * c] - a.r[3 * r] * b.i[3 * c];
-#pragma unroll
-            for (int k = 1; k < 3; k++) {
-                re += a.r[3 * r + k] * b.r[3 * c + k] + a.i[3 * r + k] * b.i[3 * c + k];
-                im += a.i[3 * r + k] * b.r[3 * c + k] - a.r[3 * r + k] * b.i[3 * c + k];
-            }
-            o.r[3 * r + c] = re;
-            o.i[3 * r + c] = im;
-        }
-    }
+        for (int c = 0; c < 3; c++) mul_adj_entry(a, b, r, c, o.r[3 * r + c], o.i[3 * r + c]);
     return o;
 }
 
@@ -196,12 +209,16 @@ OPX_HD Su3<T> adj_mul(const Su3<T>& a, const Su3<T>& b)
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             // sum_k conj(a[k][r]) * b[k][c]
-            T re = a.r[r] * b.r[c] + a.i[r] * b.i[c];
-            T im = a.r[r] * b.i[c] - a.i[r] * b.r[c];
+            T re = a.r[r] * b.r[c];
+            re += a.i[r] * b.i[c];
+            T im = a.r[r] * b.i[c];
+            im -= a.i[r] * b.r[c];
 #pragma unroll
             for (int k = 1; k < 3; k++) {
-                re += a.r[3 * k + r] * b.r[3 * k + c] + a.i[3 * k + r] * b.i[3 * k + c];
-                im += a.r[3 * k + r] * b.i[3 * k + c] - a.i[3 * k + r] * b.r[3 * k + c];
+                re += a.r[3 * k + r] * b.r[3 * k + c];
+                re += a.i[3 * k + r] * b.i[3 * k + c];
+                im += a.r[3 * k + r] * b.i[3 * k + c];
+                im -= a.i[3 * k + r] * b.r[3 * k + c];
             }
             o.r[3 * r + c] = re;
             o.i[3 * r + c] = im;
@@ -308,21 +325,60 @@ OPX_HD T square(const Alg3<T>& x)
 }
 
 template <typename T> struct ExpTerms;
-template <> struct ExpTerms<double> { static constexpr int n = 18; };
-template <> struct ExpTerms<float> { static constexpr int n = 10; };
+template <> struct ExpTerms<double> { static constexpr int n = 18; static constexpr double eps = 1e-18; };
+template <> struct ExpTerms<float> { static constexpr int n = 10; static constexpr double eps = 1e-9; };
+
+// Q^2 for Hermitian Q: real diagonal, upper triangle computed, lower = conj(upper)
+template <typename T>
+OPX_HD Su3<T> herm_square(const Su3<T>& q)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        T d = q.r[3 * r] * q.r[3 * r];
+        d += q.i[3 * r] * q.i[3 * r];
+        d += q.r[3 * r + 1] * q.r[3 * r + 1];
+        d += q.i[3 * r + 1] * q.i[3 * r + 1];
+        d += q.r[3 * r + 2] * q.r[3 * r + 2];
+        d += q.i[3 * r + 2] * q.i[3 * r + 2];
+        o.r[4 * r] = d;
+        o.i[4 * r] = 0;
+    }
+    const int rr[3] = {0, 0, 1}, cc[3] = {1, 2, 2};
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        const int r = rr[t], c = cc[t];
+        T re = q.r[3 * r] * q.r[c];
+        re -= q.i[3 * r] * q.i[c];
+        T im = q.r[3 * r] * q.i[c];
+        im += q.i[3 * r] * q.r[c];
+#pragma unroll
+        for (int k = 1; k < 3; k++) {
+            re += q.r[3 * r + k] * q.r[3 * k + c];
+            re -= q.i[3 * r + k] * q.i[3 * k + c];
+            im += q.r[3 * r + k] * q.i[3 * k + c];
+            im += q.i[3 * r + k] * q.r[3 * k + c];
+        }
+        o.r[3 * r + c] = re;
+        o.i[3 * r + c] = im;
+        o.r[3 * c + r] = re;
+        o.i[3 * c + r] = -im;
+    }
+    return o;
+}
 
 // exp(i Q) for Q = H(x) * f traceless Hermitian, via the Cayley-Hamilton reduced
 // power series: (iQ)^n / n! = a_n + b_n Q + c_n Q^2 with
 //   Q^3 = k1 Q + k0,  k1 = tr(Q^2)/2, k0 = det Q = tr(Q^3)/3,
 //   (a,b,c)_{n+1} = i (c_n k0, a_n + c_n k1, b_n) / (n+1).
-// When tr(Q^2) > 1 the argument is halved s times and the result squared s times,
-// so the truncated series always runs with spectral radius <= 1.
+// The series stops once a term is below the working precision (at most
+// ExpTerms<T>::n terms). When tr(Q^2) > 1 the argument is halved s times and the
+// result squared s times, so the series always runs with spectral radius <= 1.
 template <typename T>
 OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
 {
     Su3<T> q = herm(x, f);
-    // Q^2 (Hermitian)
-    Su3<T> q2 = mul(q, q);
+    Su3<T> q2 = herm_square(q);
     T trq2 = q2.r[0] + q2.r[4] + q2.r[8];
     int s = 0;
     T sc = T(1);
@@ -335,38 +391,62 @@ OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
         q2 = scaled(q2, sc * sc);
         trq2 = trq2 * sc * sc;
     }
-    T k1 = trq2 / 2;
-    // tr(Q^3) = sum_ij (Q^2)_ij Q_ji  (real)
-    T trq3 = 0;
+    const T k1 = trq2 / 2;
+    // tr(Q^3) = sum_ij (Q^2)_ij Q_ji, real for Hermitian Q
+    T trq3 = q2.r[0] * q.r[0];
+    trq3 += q2.r[4] * q.r[4];
+    trq3 += q2.r[8] * q.r[8];
+    {
+        const int rr[3] = {0, 0, 1}, cc[3] = {1, 2, 2};
 #pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            trq3 += q2.r[3 * r + c] * q.r[3 * c + r] - q2.i[3 * r + c] * q.i[3 * c + r];
-    T k0 = trq3 / 3;
-    // series
+        for (int t = 0; t < 3; t++) {
+            const int r = rr[t], c = cc[t];
+            // (Q^2)_rc Q_cr + (Q^2)_cr Q_rc = 2 Re((Q^2)_rc Q_cr)
+            T v = q2.r[3 * r + c] * q.r[3 * c + r];
+            v -= q2.i[3 * r + c] * q.i[3 * c + r];
+            trq3 += 2 * v;
+        }
+    }
+    const T k0 = trq3 / 3;
+    // bound on the size of the terms: |a| + |b| rho + |c| rho^2 with rho^2 <= tr Q^2
+    const T rho = fsqrt(trq2);
     T ar = 1, ai = 0, br = 0, bi = 0, cr = 0, ci = 0;
     T f0r = 1, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
-#pragma unroll
     for (int n = 0; n < ExpTerms<T>::n; n++) {
-        T inv = T(1) / T(n + 1);
+        const T inv = T(1) / T(n + 1);
         // new a = i c k0; new b = i (a + c k1); new c = i b
-        T nar = -ci * k0, nai = cr * k0;
-        T tbr = ar + cr * k1, tbi = ai + ci * k1;
-        T nbr = -tbi, nbi = tbr;
-        T ncr = -bi, nci = br;
-        ar = nar * inv; ai = nai * inv;
-        br = nbr * inv; bi = nbi * inv;
-        cr = ncr * inv; ci = nci * inv;
+        const T nar = -ci * k0, nai = cr * k0;
+        T tbr = ar;
+        tbr += cr * k1;
+        T tbi = ai;
+        tbi += ci * k1;
+        const T ncr = -bi, nci = br;
+        ar = nar * inv;
+        ai = nai * inv;
+        br = -tbi * inv;
+        bi = tbr * inv;
+        cr = ncr * inv;
+        ci = nci * inv;
         f0r += ar; f0i += ai;
         f1r += br; f1i += bi;
         f2r += cr; f2i += ci;
+        const T mag = (fabs(ar) + fabs(ai)) + (fabs(br) + fabs(bi)) * rho + (fabs(cr) + fabs(ci)) * trq2;
+        if (mag < T(ExpTerms<T>::eps)) break;
     }
+    // e = f0 + f1 Q + f2 Q^2 (Q and Q^2 Hermitian: real diagonals)
     Su3<T> e;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        e.r[k] = f1r * q.r[k] - f1i * q.i[k] + f2r * q2.r[k] - f2i * q2.i[k];
-        e.i[k] = f1r * q.i[k] + f1i * q.r[k] + f2r * q2.i[k] + f2i * q2.r[k];
+        T re = f1r * q.r[k];
+        re -= f1i * q.i[k];
+        re += f2r * q2.r[k];
+        re -= f2i * q2.i[k];
+        T im = f1r * q.i[k];
+        im += f1i * q.r[k];
+        im += f2r * q2.i[k];
+        im += f2i * q2.r[k];
+        e.r[k] = re;
+        e.i[k] = im;
     }
     e.r[0] += f0r; e.i[0] += f0i;
     e.r[4] += f0r; e.i[4] += f0i;

@@ -39,6 +39,15 @@ struct SuperParticles {
     int64_t total;     // charges in total
 };
 
+// Streaming (2.5D, LDS-exchanged plaquette) form of the fused update.
+// Tile shapes (TY x TX threads over (x1, x2)): 0: 16x16, 1: 8x32, 2: 16x32, 3: 8x64, 4: 8x16.
+constexpr int kStreamTiles = 5;
+bool stream_supported(int nc, int prec, const Lattice& L, int tile);
+void stream_tile_shape(int tile, int* ty, int* tx);
+uint32_t stream_blocks_per_chunk(const Lattice& L, int tile);
+hipError_t launch_stream_update(int nc, int prec, const Lattice& L, const Box& active, int tile, uint32_t nchunks,
+                                const void* U, void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
+
 // All launchers enqueue on `st` and return hipGetLastError().
 hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U,
                                   void* Unext, void* E, const void* J, bool has_j, hipStream_t st);

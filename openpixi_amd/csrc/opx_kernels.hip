@@ -25,6 +25,12 @@ struct Coord {
     uint32_t x0, x1, x2;
 };
 
+// block order of the per-link kernel (row tiles swept along x0)
+struct LinkOrder {
+    uint32_t rows;                   // x1 rows per tile
+    uint32_t blocks_per_tile_plane;  // blocks covering rows * d2 cells
+};
+
 __device__ __forceinline__ Coord coords(const Lattice& L, uint32_t site)
 {
     Coord c;
@@ -99,17 +105,41 @@ __device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t n, uint32_
 //   S   = sum_{j != dir} a_j^-2 [U_j(x+d) U_d(x+j)^+ U_j(x)^+ + (U_d(x-j) U_j(x+d-j))^+ U_j(x-j)]
 //   E  += at proj(U_d S) ;  E += uf_d J   (when a current is present)
 //   Unext = exp(-i at E) U_d
+// waves per SIMD requested from the register allocator (OPX_LINK_WAVES, default 4 for fp32)
+#ifndef OPX_LINK_WAVES
+#define OPX_LINK_WAVES 4
+#endif
+#ifndef OPX_LINK_WAVES_FP64
+#define OPX_LINK_WAVES_FP64 2
+#endif
 template <int NC, typename T, bool HAS_J>
-__global__ __launch_bounds__(kBlock) void k_combined_update(Lattice L, Box active, const T* __restrict__ U,
+__global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK_WAVES_FP64)) void k_combined_update(Lattice L, LinkOrder order, Box active, const T* __restrict__ U,
                                                             T* __restrict__ Unext, T* __restrict__ E,
                                                             const T* __restrict__ J)
 {
     using GR = Group<NC, T>;
     using M = typename GR::M;
     using A = typename GR::A;
-    const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
-    const int d = blockIdx.y;
-    if (site >= L.n) return;
+    // Block order: each x0 plane is cut into row tiles of `rows` rows (x1) by the
+    // full x2 extent; logical blocks run direction-fastest, then over the blocks of
+    // one tile-plane, then along x0, then over tiles. XCD k (= blockIdx % 8) takes
+    // a contiguous range of logical blocks, so it sweeps its tiles plane by plane
+    // and the x0 +- 1 neighbours of a block were (or are about to be) read by
+    // blocks a few dozen positions away on the same XCD, i.e. from its L2.
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
+    const uint32_t logical = xcd * q8 + (xcd < r8 ? xcd : r8) + (b >> 3);
+    const int d = (int)(logical % 3);
+    const uint32_t rest = logical / 3;
+    const uint32_t sb = rest % order.blocks_per_tile_plane;
+    const uint32_t rest2 = rest / order.blocks_per_tile_plane;
+    const uint32_t x0b = rest2 % L.d0;
+    const uint32_t tile = rest2 / L.d0;
+    const uint32_t row0 = tile * order.rows;
+    const uint32_t tile_rows = (L.d1 - row0 < order.rows) ? L.d1 - row0 : order.rows;
+    const uint32_t in_tile = sb * kBlock + threadIdx.x;
+    if (in_tile >= tile_rows * L.d2) return;
+    const uint32_t site = x0b * L.s0 + row0 * L.d2 + in_tile;
     const Coord c = coords(L, site);
     if (!in_box(active, c.x0, c.x1, c.x2)) return;
     const uint32_t n = L.n;
@@ -636,11 +666,16 @@ hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box&
                                   void* E, const void* J, bool has_j, hipStream_t st)
 {
     OPX_DISPATCH(nc, prec, {
+        LinkOrder order;
+        order.rows = L.d1 < 16 ? L.d1 : 16;
+        order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
+        const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
+        const dim3 g1(3 * order.blocks_per_tile_plane * L.d0 * tiles);
         if (has_j)
-            hipLaunchKernelGGL((k_combined_update<NC, T, true>), grid_links(L.n), dim3(kBlock), 0, st, L, active,
+            hipLaunchKernelGGL((k_combined_update<NC, T, true>), g1, dim3(kBlock), 0, st, L, order, active,
                                (const T*)U, (T*)Unext, (T*)E, (const T*)J);
         else
-            hipLaunchKernelGGL((k_combined_update<NC, T, false>), grid_links(L.n), dim3(kBlock), 0, st, L, active,
+            hipLaunchKernelGGL((k_combined_update<NC, T, false>), g1, dim3(kBlock), 0, st, L, order, active,
                                (const T*)U, (T*)Unext, (T*)E, (const T*)nullptr);
     });
     return hipGetLastError();

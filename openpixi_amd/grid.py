@@ -181,6 +181,32 @@ class Grid:
     def updateLinks(self, tstep, solver=None):
         (solver or FastTYMSolver()).stepLinks(self, tstep)
 
+    # ---- kernel form (no counterpart in the reference) ----
+    def setKernel(self, kernel="auto", tile=None, chunks=None):
+        """kernel: "auto", "link" or "stream"; tile: stream tile index; chunks: x0 chunk count (0 = auto)."""
+        L = _lib.load()
+        k = {"auto": _lib.KERNEL_AUTO, "link": _lib.KERNEL_LINK, "stream": _lib.KERNEL_STREAM}[kernel]
+        self._check(L.opx_set_option(self._h, _lib.OPT_KERNEL, k))
+        if tile is not None:
+            self._check(L.opx_set_option(self._h, _lib.OPT_TILE, int(tile)))
+        if chunks is not None:
+            self._check(L.opx_set_option(self._h, _lib.OPT_CHUNKS, int(chunks)))
+
+    def kernelConfig(self):
+        L = _lib.load()
+        out = []
+        for opt in (_lib.OPT_KERNEL, _lib.OPT_TILE, _lib.OPT_CHUNKS):
+            v = ctypes.c_int()
+            self._check(L.opx_get_option(self._h, opt, ctypes.byref(v)))
+            out.append(v.value)
+        return {"kernel": ["auto", "link", "stream"][out[0]], "tile": out[1], "chunks": out[2]}
+
+    def autotune(self, reps=5):
+        """Times every kernel form / tile / chunk candidate and keeps the fastest; returns ms per launch."""
+        ms = ctypes.c_double()
+        self._check(_lib.load().opx_autotune(self._h, int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def reunitarize(self):
         self._check(_lib.load().opx_reunitarize(self._h))
 

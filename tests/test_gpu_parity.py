@@ -16,15 +16,24 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-12
 
 
+# kernel forms of the fused update: (kernel, tile, chunks)
+KERNELS = [("link", None, None), ("stream", 0, None), ("stream", 1, 3), ("stream", 4, 1)]
+KERNEL_IDS = ["link", "stream16x16", "stream8x32c3", "stream8x16c1"]
+
+
 def run_both(nc, dims, steps, at=0.25, spacing=(1.0, 1.0, 1.0), sigma_a=0.1, sigma_e=0.05, seed=0, J=None,
-             precision="fp64", active=None):
+             precision="fp64", active=None, kernel=("auto", None, None)):
     U, E = make_state(nc, dims, sigma_a, sigma_e, seed)
     og = oracle_grid(nc, dims, U, E, at, spacing, J=J)
     dg = device_grid(nc, dims, U, E, at, spacing, precision=precision, J=J)
+    dg.setKernel(*kernel)
     if active is not None:
         og.set_active_region(*active)
         dg.setActiveRegion(*active)
     solver = FastTYMSolver()
+    # Simulation.initialize: U(-dt/2) -> Unext = U(+dt/2)
+    og.step_links(at)
+    solver.stepLinks(dg, at)
     for _ in range(steps):
         og.store_fields()
         og.step(at)
@@ -33,27 +42,31 @@ def run_both(nc, dims, steps, at=0.25, spacing=(1.0, 1.0, 1.0), sigma_a=0.1, sig
     return og, dg
 
 
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
 @pytest.mark.parametrize("nc,dims,steps", [(2, (32, 32, 32), 100), (2, (5, 7, 6), 60), (3, (6, 6, 6), 40),
-                                           (3, (4, 5, 7), 30)])
-def test_leapfrog_fp64_parity(gpu, nc, dims, steps):
-    print(f"leapfrog parity nc={nc} dims={dims} steps={steps}", flush=True)
-    og, dg = run_both(nc, dims, steps)
+                                           (3, (4, 5, 7), 30), (3, (12, 20, 17), 10)])
+def test_leapfrog_fp64_parity(gpu, nc, dims, steps, kernel):
+    print(f"leapfrog parity nc={nc} dims={dims} steps={steps} kernel={kernel}", flush=True)
+    og, dg = run_both(nc, dims, steps, kernel=kernel)
     assert rel_err(dg.getU(), og.U) < FP64_TOL
     assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
     assert rel_err(dg.getE(), og.E) < FP64_TOL
 
 
+@pytest.mark.parametrize("kernel", KERNELS[:2], ids=KERNEL_IDS[:2])
 @pytest.mark.parametrize("nc", [2, 3])
-def test_leapfrog_strong_fields_anisotropic(gpu, nc):
+def test_leapfrog_strong_fields_anisotropic(gpu, nc, kernel):
     """Large amplitudes (SU(3) exp beyond the Taylor branch) and uneven spacings."""
-    print(f"strong fields nc={nc}", flush=True)
-    og, dg = run_both(nc, (4, 6, 5), 10, at=0.1, spacing=(1.0, 0.8, 1.3), sigma_a=1.0, sigma_e=3.0, seed=7)
+    print(f"strong fields nc={nc} kernel={kernel}", flush=True)
+    og, dg = run_both(nc, (4, 6, 5), 10, at=0.1, spacing=(1.0, 0.8, 1.3), sigma_a=1.0, sigma_e=3.0, seed=7,
+                      kernel=kernel)
     assert rel_err(dg.getU(), og.U) < 1e-11
     assert rel_err(dg.getE(), og.E) < 1e-11
 
 
+@pytest.mark.parametrize("kernel", KERNELS[:2], ids=KERNEL_IDS[:2])
 @pytest.mark.parametrize("nc", [2, 3])
-def test_current_and_active_region(gpu, nc):
+def test_current_and_active_region(gpu, nc, kernel):
     dims = (6, 5, 4)
     n = int(np.prod(dims))
     rng = np.random.default_rng(3)
@@ -62,7 +75,7 @@ def test_current_and_active_region(gpu, nc):
         from openpixi_amd.inputs import su3_coeffs_to_storage
 
         J = su3_coeffs_to_storage(rng.normal(0, 0.1, size=(n, 3, 8)))
-    og, dg = run_both(nc, dims, 12, J=J, active=((1, 0, 1), (4, 4, 2)), seed=5)
+    og, dg = run_both(nc, dims, 12, J=J, active=((1, 0, 1), (4, 4, 2)), seed=5, kernel=kernel)
     assert rel_err(dg.getU(), og.U) < FP64_TOL
     assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
     assert rel_err(dg.getE(), og.E) < FP64_TOL
@@ -107,8 +120,9 @@ def test_projected_energy_parity(gpu, nc):
             assert rel_err(a, b) < 1e-12
 
 
+@pytest.mark.parametrize("kernel", KERNELS[:2] + [("stream", 2, None)], ids=KERNEL_IDS[:2] + ["stream16x32"])
 @pytest.mark.parametrize("nc", [2, 3])
-def test_fp32_form_tracks_fp64(gpu, nc):
+def test_fp32_form_tracks_fp64(gpu, nc, kernel):
     """FP32 links/E with FP64 reunitarization every 10 steps stay within FP32
     rounding of the FP64 oracle, and Gauss's law stays conserved."""
     dims = (8, 8, 8)
@@ -116,10 +130,13 @@ def test_fp32_form_tracks_fp64(gpu, nc):
     U, E = make_state(nc, dims, 0.1, 0.05, seed=1)
     og = oracle_grid(nc, dims, U, E)
     dg = device_grid(nc, dims, U, E, precision="fp32")
+    dg.setKernel(*kernel)
     g0 = og.measure()[6]
     from openpixi_amd import _lib
 
     L = _lib.load()
+    og.step_links(0.25)
+    FastTYMSolver().stepLinks(dg, 0.25)
     for s in range(steps):
         og.store_fields()
         og.step(0.25)
@@ -157,3 +174,28 @@ def test_reunitarize_and_health(gpu, nc):
     dg.setU(bad)
     nf, nu = dg.checkHealth(1e-6)
     assert nf >= 1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_autotune_keeps_state_and_parity(gpu, nc, precision):
+    """The autotuner runs on scratch buffers: the state is unchanged and the chosen kernel is exact."""
+    dims = (16, 12, 20)
+    print(f"autotune nc={nc} {precision}", flush=True)
+    U, E = make_state(nc, dims, 0.1, 0.05, seed=6)
+    og = oracle_grid(nc, dims, U, E)
+    dg = device_grid(nc, dims, U, E, precision=precision)
+    ms = dg.autotune(reps=2)
+    assert ms > 0
+    tol = FP64_TOL if precision == "fp64" else 2e-5
+    assert rel_err(dg.getU(), og.U) < tol
+    assert rel_err(dg.getE(), og.E) < (FP64_TOL if precision == "fp64" else 1e-6)
+    og.step_links(0.25)
+    FastTYMSolver().stepLinks(dg, 0.25)
+    for _ in range(5):
+        og.store_fields()
+        og.step(0.25)
+        dg.storeFields()
+        FastTYMSolver().step(dg, 0.25)
+    assert rel_err(dg.getU(), og.U) < tol
+    print("chosen", dg.kernelConfig(), flush=True)
