@@ -112,7 +112,7 @@ int opx_reset_charge(opx_ctx *ctx);
  *                     (any lattice, any active region);
  *   OPX_KERNEL_STREAM tiles of (x1,x2) columns marching along x0, each plaquette
  *                     computed once and exchanged through LDS;
- *   OPX_KERNEL_AUTO   STREAM where supported, else LINK (default).
+ *   OPX_KERNEL_AUTO   LINK until opx_autotune picks a form (default).
  * OPX_OPT_TILE selects the STREAM tile shape (0: 16x16, 1: 8x32, 2: 16x32,
  * 3: 8x64, 4: 8x16 threads over (x1,x2)); OPX_OPT_CHUNKS the number of x0
  * chunks (0 = derived from the CU count). opx_autotune times every candidate
@@ -129,6 +129,18 @@ int opx_autotune(opx_ctx *ctx, int reps, double *best_ms);
  * conj(row1 x row2)). No counterpart in the reference (which never
  * reunitarizes); used by the FP32 form every few steps. */
 int opx_reunitarize(opx_ctx *ctx);
+
+/* Projects E onto Gauss's law with respect to the links U (and rho):
+ * E' = E - D lambda with D^T D lambda = G, where G is the constraint of
+ * Grid.getGaussConstraint (physics/grid/Grid.java:944-968) and D its covariant
+ * difference; solved by conjugate gradients in double precision until
+ * |r| <= tol |r0| or max_iter iterations. Returns the iteration count and the
+ * mean |G|^2 before and after (of the stored field, in its precision). The
+ * reference's iterative GaussConstraintRestoration
+ * (diagnostics/methods/GaussConstraintRestoration.java:79-163) relaxes towards
+ * the same constraint surface; this solves for the nearest point on it. */
+int opx_gauss_project(opx_ctx *ctx, double tol, int max_iter, int *iterations, double *gauss_before,
+                      double *gauss_after);
 
 /* `steps` leapfrog steps of pure Yang-Mills: for each, storeFields then step
  * (physics/Simulation.java:389-395 with no particles), reunitarizing U and

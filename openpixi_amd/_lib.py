@@ -57,6 +57,7 @@ SIGNATURES = {
     "opx_get_option": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "opx_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "opx_reunitarize": (ctypes.c_int, [_vp]),
+    "opx_gauss_project": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp]),
     "opx_leapfrog": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "opx_measure": (ctypes.c_int, [_vp, _dp]),
     "opx_projected_energy_density": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),

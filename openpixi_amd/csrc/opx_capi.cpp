@@ -199,7 +199,7 @@ uint32_t auto_chunks(const opx_ctx* c, const opx::Lattice& L, int tile)
 // The fused E-field/link update with the configured kernel form.
 hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E)
 {
-    if (c->kernel != OPX_KERNEL_LINK && opx::stream_supported(c->nc, c->prec, L, c->tile)) {
+    if (c->kernel == OPX_KERNEL_STREAM && opx::stream_supported(c->nc, c->prec, L, c->tile)) {
         const uint32_t k = c->nchunks > 0 ? (uint32_t)c->nchunks : auto_chunks(c, L, c->tile);
         return opx::launch_stream_update(c->nc, c->prec, L, c->active, c->tile, k, U, Unext, E, c->J, c->j_nonzero,
                                          c->stream);
@@ -887,6 +887,79 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
     ctx->nchunks = bc;
     if (best_ms) *best_ms = best;
     return OPX_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int opx_gauss_project(opx_ctx* ctx, double tol, int max_iter, int* iterations, double* gauss_before,
+                      double* gauss_after)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!(tol > 0) || max_iter < 1) return fail(ctx, OPX_E_ARG, "gauss_project: tol must be > 0 and max_iter >= 1");
+    int rc = materialize_rho(ctx);
+    if (rc != OPX_OK) return rc;
+    opx::Lattice L = lattice(ctx, ctx->at);
+    const uint64_t ms = (uint64_t)ctx->n * ctx->AD;   // per-cell algebra values
+    const uint64_t ml = 3 * ms;                        // per-link algebra values
+    double *Ed = nullptr, *F = nullptr, *lam = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *scr = nullptr;
+    double* bufs_sz[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const size_t sizes[7] = {ml, ml, ms, ms, ms, ms, opx::gauss_dot_scratch()};
+    double** ptrs[7] = {&Ed, &F, &lam, &r, &p, &Ap, &scr};
+    for (int i = 0; i < 7 && rc == OPX_OK; i++) {
+        rc = hip_check(ctx, hipMalloc((void**)ptrs[i], sizes[i] * sizeof(double)), "gauss alloc");
+        bufs_sz[i] = *ptrs[i];
+    }
+    auto dot = [&](const double* a, const double* b, uint64_t m, double* out) -> int {
+        int r2 = hip_check(ctx, opx::gauss_dot(m, a, b, scr, scr + 2 * 1024 + 2, ctx->stream), "gauss dot");
+        if (r2 == OPX_OK)
+            r2 = hip_check(ctx, hipMemcpyAsync(out, scr + 2 * 1024 + 2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                           "gauss dot copy");
+        if (r2 == OPX_OK) r2 = hip_check(ctx, hipStreamSynchronize(ctx->stream), "gauss dot sync");
+        return r2;
+    };
+    double rr = 0, rr0 = 0;
+    int it = 0;
+    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_to_double(ctx->prec, ml, ctx->E, Ed, ctx->stream), "gauss E");
+    // r = G = D^T E - rho ; lambda = 0 ; p = r
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, opx::gauss_apply_Dt(ctx->nc, ctx->prec, L, ctx->U, Ed, ctx->rho, r, ctx->stream), "gauss G");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMemsetAsync(lam, 0, ms * sizeof(double), ctx->stream), "gauss zero");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemcpyAsync(p, r, ms * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "gauss copy");
+    if (rc == OPX_OK) rc = dot(r, r, ms, &rr);
+    rr0 = rr;
+    if (gauss_before) *gauss_before = rr0 / (double)ctx->n;
+    while (rc == OPX_OK && it < max_iter && rr > tol * tol * rr0 && rr > 0) {
+        double pAp = 0;
+        rc = hip_check(ctx, opx::gauss_apply_D(ctx->nc, ctx->prec, L, ctx->U, p, F, ctx->stream), "gauss D");
+        if (rc == OPX_OK)
+            rc = hip_check(ctx, opx::gauss_apply_Dt(ctx->nc, ctx->prec, L, ctx->U, F, nullptr, Ap, ctx->stream), "gauss DtD");
+        if (rc == OPX_OK) rc = dot(p, Ap, ms, &pAp);
+        if (rc != OPX_OK || !(pAp > 0)) break;
+        const double alpha = rr / pAp;
+        rc = hip_check(ctx, opx::gauss_axpy(ms, alpha, p, lam, ctx->stream), "gauss axpy");
+        if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_axpy(ms, -alpha, Ap, r, ctx->stream), "gauss axpy");
+        double rr_new = 0;
+        if (rc == OPX_OK) rc = dot(r, r, ms, &rr_new);
+        if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_xpby(ms, r, rr_new / rr, p, ctx->stream), "gauss xpby");
+        rr = rr_new;
+        it++;
+    }
+    // E' = E - D lambda, then the Gauss residual of the stored field
+    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_apply_D(ctx->nc, ctx->prec, L, ctx->U, lam, F, ctx->stream), "gauss D");
+    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_sub_into(ctx->prec, ml, Ed, F, ctx->E, ctx->stream), "gauss update");
+    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_to_double(ctx->prec, ml, ctx->E, Ed, ctx->stream), "gauss E");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, opx::gauss_apply_Dt(ctx->nc, ctx->prec, L, ctx->U, Ed, ctx->rho, r, ctx->stream), "gauss G");
+    double g2 = 0;
+    if (rc == OPX_OK) rc = dot(r, r, ms, &g2);
+    if (gauss_after) *gauss_after = g2 / (double)ctx->n;
+    if (iterations) *iterations = it;
+    for (int i = 0; i < 7; i++)
+        if (bufs_sz[i]) (void)hipFree(bufs_sz[i]);
+    return rc;
 }
 
 }  // extern "C"

@@ -48,6 +48,17 @@ uint32_t stream_blocks_per_chunk(const Lattice& L, int tile);
 hipError_t launch_stream_update(int nc, int prec, const Lattice& L, const Box& active, int tile, uint32_t nchunks,
                                 const void* U, void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
 
+// Gauss-law projection building blocks (double precision work arrays).
+hipError_t gauss_apply_D(int nc, int prec, const Lattice& L, const void* U, const double* phi, double* F, hipStream_t st);
+hipError_t gauss_apply_Dt(int nc, int prec, const Lattice& L, const void* U, const double* F, const void* rho,
+                          double* out, hipStream_t st);
+hipError_t gauss_to_double(int prec, uint64_t m, const void* a, double* b, hipStream_t st);
+hipError_t gauss_sub_into(int prec, uint64_t m, const double* Ed, const double* F, void* E, hipStream_t st);
+size_t gauss_dot_scratch();
+hipError_t gauss_dot(uint64_t m, const double* a, const double* b, double* scratch, double* out, hipStream_t st);
+hipError_t gauss_axpy(uint64_t m, double alpha, const double* x, double* y, hipStream_t st);
+hipError_t gauss_xpby(uint64_t m, const double* x, double beta, double* y, hipStream_t st);
+
 // All launchers enqueue on `st` and return hipGetLastError().
 hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U,
                                   void* Unext, void* E, const void* J, bool has_j, hipStream_t st);

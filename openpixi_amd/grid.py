@@ -207,6 +207,15 @@ class Grid:
         self._check(_lib.load().opx_autotune(self._h, int(reps), ctypes.byref(ms)))
         return ms.value
 
+    def projectGauss(self, tol=1e-10, max_iter=2000):
+        """Nearest E satisfying Gauss's law for the current U and rho (conjugate gradients, double
+        precision). Returns (iterations, mean |G|^2 before, mean |G|^2 after)."""
+        it = ctypes.c_int()
+        g0, g1 = ctypes.c_double(), ctypes.c_double()
+        self._check(_lib.load().opx_gauss_project(self._h, float(tol), int(max_iter), ctypes.byref(it),
+                                                  ctypes.byref(g0), ctypes.byref(g1)))
+        return it.value, g0.value, g1.value
+
     def reunitarize(self):
         self._check(_lib.load().opx_reunitarize(self._h))
 

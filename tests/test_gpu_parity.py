@@ -199,3 +199,32 @@ def test_autotune_keeps_state_and_parity(gpu, nc, precision):
         FastTYMSolver().step(dg, 0.25)
     assert rel_err(dg.getU(), og.U) < tol
     print("chosen", dg.kernelConfig(), flush=True)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_gauss_projection(gpu, nc, precision):
+    """E' = E - D lambda reaches Gauss's law (checked by the oracle's own Gauss
+    constraint), is the nearest such field (E - E' is orthogonal to E'), and the
+    leapfrog then keeps |G|^2 at rounding level."""
+    dims = (8, 6, 10)
+    print(f"gauss projection nc={nc} {precision}", flush=True)
+    U, E = make_state(nc, dims, 0.3, 0.2, seed=13)
+    dg = device_grid(nc, dims, U, E, precision=precision)
+    it, g0, g1 = dg.projectGauss(tol=1e-12, max_iter=3000)
+    assert g0 > 1e-3
+    assert g1 < (1e-22 if precision == "fp64" else 1e-12) * g0, (it, g0, g1)
+    Ep = dg.getE()
+    og = oracle_grid(nc, dims, dg.getU(), Ep)
+    gauss_oracle = og.measure()[6]
+    assert gauss_oracle < (1e-22 if precision == "fp64" else 1e-12) * g0
+    c = lambda X: np.array([[O.alg_coeffs(nc, X[i, d]) for d in range(3)] for i in range(X.shape[0])])  # noqa: E731
+    ce, cp = c(E), c(Ep)
+    assert abs(np.sum((ce - cp) * cp)) < (1e-10 if precision == "fp64" else 1e-4) * np.sum(cp * cp)
+    assert np.sum(cp * cp) < np.sum(ce * ce)
+    FastTYMSolver().stepLinks(dg, 0.25)
+    from openpixi_amd import _lib
+
+    dg._check(_lib.load().opx_leapfrog(dg.handle, 20, 10 if precision == "fp32" else 0))
+    g_end = FieldMeasurements().measure(dg)[6]
+    assert g_end < (1e-20 if precision == "fp64" else 1e-10) * g0
