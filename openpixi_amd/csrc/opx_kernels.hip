@@ -5,6 +5,9 @@
 // [dir][coefficient][cell], the charge density [coefficient][cell]. Consecutive
 // lanes of a wavefront touch consecutive cells, so every load and store is a
 // fully coalesced 256-byte (fp32) or 512-byte (fp64) wavefront access.
+#include <type_traits>
+
+#include "opx_buf.h"
 #include "opx_group.h"
 #include "opx_kernels.h"
 
@@ -99,6 +102,59 @@ __device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t n, uint32_
     for (int k = 0; k < GR::AD; k++) p[(size_t)k * n] = aref(a, k);
 }
 
+// Field accessors of the per-link kernel: component k of cell s of a field with
+// n cells. FieldB uses buffer instructions (component offset in an SGPR); FieldG
+// plain global pointers (fields of 4 GiB or more).
+template <typename T>
+struct FieldB {
+    Buf<T> b;
+    uint32_t n;
+    __device__ __forceinline__ FieldB(const T* p, uint32_t n_, uint32_t comps) : b(p, comps * n_ * (uint32_t)sizeof(T)), n(n_) {}
+    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return b.ld(s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T)); }
+    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { b.st(v, s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T)); }
+};
+template <typename T>
+struct FieldG {
+    T* p;
+    uint32_t n;
+    __device__ __forceinline__ FieldG(const T* p_, uint32_t n_, uint32_t) : p(const_cast<T*>(p_)), n(n_) {}
+    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return p[(size_t)k * n + s]; }
+    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { p[(size_t)k * n + s] = v; }
+};
+
+template <int NC, typename T, typename F>
+__device__ __forceinline__ typename Group<NC, T>::M fload_link(const F& f, uint32_t site, int dir)
+{
+    using GR = Group<NC, T>;
+    typename GR::M m;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) GR::gref(m, k) = f.ld((uint32_t)(dir * GR::G + k), site);
+    return m;
+}
+template <int NC, typename T, typename F>
+__device__ __forceinline__ void fstore_link(const F& f, uint32_t site, int dir, const typename Group<NC, T>::M& m)
+{
+    using GR = Group<NC, T>;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) f.st((uint32_t)(dir * GR::G + k), site, GR::gget(m, k));
+}
+template <int NC, typename T, typename F>
+__device__ __forceinline__ typename Group<NC, T>::A fload_alg(const F& f, uint32_t site, int dir)
+{
+    using GR = Group<NC, T>;
+    typename GR::A a;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = f.ld((uint32_t)(dir * GR::AD + k), site);
+    return a;
+}
+template <int NC, typename T, typename F>
+__device__ __forceinline__ void fstore_alg(const F& f, uint32_t site, int dir, typename Group<NC, T>::A a)
+{
+    using GR = Group<NC, T>;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) f.st((uint32_t)(dir * GR::AD + k), site, aref(a, k));
+}
+
 // ---------------------------------------------------------------- kernel 1
 // FastTYMSolver.CombinedUpdate.execute (physics/fields/FastTYMSolver.java:40-62)
 // for one link (cell, dir = blockIdx.y):
@@ -112,11 +168,14 @@ __device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t n, uint32_
 #ifndef OPX_LINK_WAVES_FP64
 #define OPX_LINK_WAVES_FP64 2
 #endif
-template <int NC, typename T, bool HAS_J>
-__global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK_WAVES_FP64)) void k_combined_update(Lattice L, LinkOrder order, Box active, const T* __restrict__ U,
-                                                            T* __restrict__ Unext, T* __restrict__ E,
-                                                            const T* __restrict__ J)
+template <int NC, typename T, bool HAS_J, bool BUF>
+__global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK_WAVES_FP64)) void k_combined_update(Lattice L, LinkOrder order, Box active, const T* __restrict__ Up,
+                                                            T* __restrict__ Unextp, T* __restrict__ Ep,
+                                                            const T* __restrict__ Jp)
 {
+    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+    const Fld U(Up, L.n, 3 * Group<NC, T>::G), Unext(Unextp, L.n, 3 * Group<NC, T>::G);
+    const Fld E(Ep, L.n, 3 * Group<NC, T>::AD), J(Jp, L.n, 3 * Group<NC, T>::AD);
     using GR = Group<NC, T>;
     using M = typename GR::M;
     using A = typename GR::A;
@@ -142,7 +201,6 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
     const uint32_t site = x0b * L.s0 + row0 * L.d2 + in_tile;
     const Coord c = coords(L, site);
     if (!in_box(active, c.x0, c.x1, c.x2)) return;
-    const uint32_t n = L.n;
     const uint32_t xpd = fwd(L, site, coord_of(c, d), d);
     M S = GR::zero();
 #pragma unroll
@@ -152,27 +210,27 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
         const uint32_t xpj = fwd(L, site, cj, j);
         const uint32_t xmj = bwd(L, site, cj, j);
         const uint32_t xpd_mj = bwd(L, xpd, cj, j);
-        M u1 = mul_adj(mul_adj(load_link<NC, T>(U, n, xpd, j), load_link<NC, T>(U, n, xpj, d)),
-                       load_link<NC, T>(U, n, site, j));
-        M u2 = adj_mul(mul(load_link<NC, T>(U, n, xmj, d), load_link<NC, T>(U, n, xpd_mj, j)),
-                       load_link<NC, T>(U, n, xmj, j));
+        M u1 = mul_adj(mul_adj(fload_link<NC, T>(U, xpd, j), fload_link<NC, T>(U, xpj, d)),
+                       fload_link<NC, T>(U, site, j));
+        M u2 = adj_mul(mul(fload_link<NC, T>(U, xmj, d), fload_link<NC, T>(U, xpd_mj, j)),
+                       fload_link<NC, T>(U, xmj, j));
         acc(u1, u2);
         acc(S, scaled(u1, (T)L.area[j]));
     }
-    const M Ud = load_link<NC, T>(U, n, site, d);
+    const M Ud = fload_link<NC, T>(U, site, d);
     const A pr = proj(mul(Ud, S));
-    A e = load_alg<NC, T>(E, n, site, d);
+    A e = fload_alg<NC, T>(E, site, d);
     const T at = (T)L.at;
 #pragma unroll
     for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * at;
     if (HAS_J) {
-        const A jj = load_alg<NC, T>(J, n, site, d);
+        const A jj = fload_alg<NC, T>(J, site, d);
         const T uf = (T)L.uf[d];
 #pragma unroll
         for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(jj, k) * uf;
     }
-    store_alg<NC, T>(E, n, site, d, e);
-    store_link<NC, T>(Unext, n, site, d, mul(exp_link(e, -at), Ud));
+    fstore_alg<NC, T>(E, site, d, e);
+    fstore_link<NC, T>(Unext, site, d, mul(exp_link(e, -at), Ud));
 }
 
 // ---------------------------------------------------------------- kernel 2
@@ -671,12 +729,19 @@ hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box&
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const dim3 g1(3 * order.blocks_per_tile_plane * L.d0 * tiles);
-        if (has_j)
-            hipLaunchKernelGGL((k_combined_update<NC, T, true>), g1, dim3(kBlock), 0, st, L, order, active,
+        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::G * sizeof(T) < (1ull << 32);
+        if (buf && has_j)
+            hipLaunchKernelGGL((k_combined_update<NC, T, true, true>), g1, dim3(kBlock), 0, st, L, order, active,
+                               (const T*)U, (T*)Unext, (T*)E, (const T*)J);
+        else if (buf)
+            hipLaunchKernelGGL((k_combined_update<NC, T, false, true>), g1, dim3(kBlock), 0, st, L, order, active,
+                               (const T*)U, (T*)Unext, (T*)E, (const T*)E);
+        else if (has_j)
+            hipLaunchKernelGGL((k_combined_update<NC, T, true, false>), g1, dim3(kBlock), 0, st, L, order, active,
                                (const T*)U, (T*)Unext, (T*)E, (const T*)J);
         else
-            hipLaunchKernelGGL((k_combined_update<NC, T, false>), g1, dim3(kBlock), 0, st, L, order, active,
-                               (const T*)U, (T*)Unext, (T*)E, (const T*)nullptr);
+            hipLaunchKernelGGL((k_combined_update<NC, T, false, false>), g1, dim3(kBlock), 0, st, L, order, active,
+                               (const T*)U, (T*)Unext, (T*)E, (const T*)E);
     });
     return hipGetLastError();
 }
