@@ -45,6 +45,7 @@ struct opx_ctx {
     int sp_max_num = 0;
     int64_t sp_total = 0;
     int* sp_desc = nullptr;         // 6 arrays of sp_count ints, back to back
+    std::vector<int> sp_host;       // host copy of the descriptors (same layout)
     int64_t* sp_qoff = nullptr;
     void* sp_q = nullptr;           // [coef][total]
     bool rho_pending = false;       // charge density deposit deferred until rho is read
@@ -631,6 +632,7 @@ int opx_superparticles_upload(opx_ctx* ctx, int count, const int* orientation, c
     ctx->sp_count = count;
     ctx->sp_total = total;
     ctx->sp_max_num = maxn;
+    ctx->sp_host = desc;
     if (total > 0) return to_device(ctx, ctx->sp_q, soa);
     return OPX_OK;
 }
@@ -679,12 +681,42 @@ opx::SuperParticles sp_view(const opx_ctx* c)
     return sp;
 }
 
+// CGCSuperParticle.getCurrentOffset / getCurrentNGPOffset on the host
+// (physics/particles/CGCSuperParticle.java:71-93), used to bound the cells a
+// charge-density gather has to visit.
+int host_floor_div(int a, int b)
+{
+    int q = a / b;
+    if ((a % b != 0) && ((a < 0) != (b < 0))) q--;
+    return q;
+}
+int host_ngp_offset(int o, int off, int pp, int k, int ppc, int t)
+{
+    const int cur = o > 0 ? off + ((t + k) / ppc) * pp : off + host_floor_div(-t + k, ppc) * pp;
+    if (o > 0) return cur + ((((t + k) % ppc) < ppc / 2) ? 0 : pp);
+    return cur + ((((t - k + ppc + ppc / 2 - 1) % ppc) < ppc / 2) ? 0 : pp);
+}
+
 int materialize_rho(opx_ctx* ctx)
 {
     if (!ctx->rho_pending) return OPX_OK;
     opx::Lattice L = lattice(ctx, ctx->at);
+    const int cnt = ctx->sp_count;
+    const int* h = ctx->sp_host.data();
+    int64_t lo = ctx->n, hi = 0;
+    for (int p = 0; p < cnt; p++) {
+        const int off = host_ngp_offset(h[p], h[2 * cnt + p], h[3 * cnt + p], h[4 * cnt + p], h[5 * cnt + p],
+                                        ctx->rho_pending_t);
+        int64_t a = off, b = (int64_t)off + h[cnt + p];
+        if (a < 0) a = 0;
+        if (b > ctx->n) b = ctx->n;
+        if (b > a) {
+            if (a < lo) lo = a;
+            if (b > hi) hi = b;
+        }
+    }
     OPX_TRY(opx::launch_sp_chargedensity(ctx->nc, ctx->prec, L, sp_view(ctx), ctx->rho_pending_t, ctx->sp_q, ctx->rho,
-                                         ctx->stream),
+                                         (uint32_t)lo, (uint32_t)(hi > lo ? hi : lo), ctx->stream),
             "charge density");
     ctx->rho_pending = false;
     return OPX_OK;

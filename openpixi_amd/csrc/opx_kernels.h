@@ -79,7 +79,8 @@ hipError_t launch_health_alg(int nc, int prec, uint32_t n_values, const void* X,
                              unsigned long long* counters, hipStream_t st);
 hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t, void* q,
                              const void* Unext, void* J, hipStream_t st);
+// gathers rho over cells [lo, hi) only (the cells the particles can occupy at t)
 hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
-                                   const void* q, void* rho, hipStream_t st);
+                                   const void* q, void* rho, uint32_t lo, uint32_t hi, hipStream_t st);
 
 }  // namespace opx

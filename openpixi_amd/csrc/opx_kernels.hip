@@ -683,11 +683,12 @@ __global__ __launch_bounds__(kBlock) void k_sp_to_grid(Lattice L, SuperParticles
 // the order of the sequential reference loop, with no atomics.
 template <int NC, typename T>
 __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, SuperParticles sp, int t,
-                                                             const T* __restrict__ q, T* __restrict__ rho)
+                                                             const T* __restrict__ q, T* __restrict__ rho,
+                                                             uint32_t lo, uint32_t hi)
 {
     using GR = Group<NC, T>;
-    const uint32_t cell = blockIdx.x * kBlock + threadIdx.x;
-    if (cell >= L.n) return;
+    const uint32_t cell = lo + blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= hi) return;
     typename GR::A r;
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) aref(r, m) = rho[(size_t)m * L.n + cell];
@@ -834,12 +835,13 @@ hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperPart
 }
 
 hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
-                                   const void* q, void* rho, hipStream_t st)
+                                   const void* q, void* rho, uint32_t lo, uint32_t hi, hipStream_t st)
 {
-    if (sp.count == 0) return hipSuccess;
-    dim3 g((L.n + kBlock - 1) / kBlock);
+    if (sp.count == 0 || hi <= lo) return hipSuccess;
+    dim3 g((hi - lo + kBlock - 1) / kBlock);
     OPX_DISPATCH(nc, prec, {
-        hipLaunchKernelGGL((k_sp_chargedensity<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (const T*)q, (T*)rho);
+        hipLaunchKernelGGL((k_sp_chargedensity<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (const T*)q, (T*)rho, lo,
+                           hi);
     });
     return hipGetLastError();
 }

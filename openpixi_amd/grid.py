@@ -230,6 +230,41 @@ class Grid:
         return nf.value, nu.value
 
 
+    # ---- colored particles: CGC super particles (physics/particles/CGCSuperParticle.java:16-46) ----
+    def setSuperParticles(self, orientation, number, index_offset, per_plane, sub_lattice_shift,
+                          particles_per_cell, charges):
+        """Upload a particle list; charges: (sum(number), A) reference algebra storage, particle after particle."""
+        L = _lib.load()
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (orientation, number, index_offset, per_plane,
+                                                             sub_lattice_shift, particles_per_cell)]
+        q = np.ascontiguousarray(charges, np.float64).reshape(-1)
+        self._check(L.opx_superparticles_upload(self._h, len(arrs[0]), *[_lib.iptr(a) for a in arrs],
+                                                 _lib.dptr(q), q.size))
+        self._sp_total = int(np.sum(arrs[1]))
+
+    def superParticleCharges(self):
+        A = algebra_size(self.num_colors)
+        out = np.zeros(self._sp_total * A)
+        self._check(_lib.load().opx_superparticles_download_charges(self._h, _lib.dptr(out), out.size))
+        return out.reshape(-1, A)
+
+    def interpolateToGrid(self, t):
+        """CGCSuperParticleInterpolationNGP.interpolateToGrid for every particle at integer clock t."""
+        self._check(_lib.load().opx_interpolate_to_grid(self._h, int(t)))
+
+    def interpolateChargedensity(self, t):
+        self._check(_lib.load().opx_interpolate_chargedensity(self._h, int(t)))
+
+    def simulationInitialize(self):
+        """Simulation.initialize (physics/Simulation.java:318-352) on the device."""
+        self._check(_lib.load().opx_simulation_initialize(self._h))
+
+    def simulationStep(self, n=1):
+        """n x Simulation.step (physics/Simulation.java:373-435) without diagnostics."""
+        self._check(_lib.load().opx_simulation_step(self._h, int(n)))
+        return _lib.load().opx_simulation_steps_done(self._h)
+
+
 class FieldSolver:
     """physics/fields/FieldSolver.java:114-156: step and stepLinks on a Grid."""
 
