@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--dims", type=int, nargs=3, default=list(DIMS))
     p.add_argument("--forms", default="fp32,fp64")
     p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--no-cpic", action="store_true", help="skip the SU(2) CPIC collision line")
+    p.add_argument("--cpic-steps", type=int, default=1000)
     p.add_argument("--probe-kernels", default="auto",
                    help="kernel configurations for --probe: comma list of link | stream:TILE:CHUNKS | auto")
     p.add_argument("--probe", action="store_true",
@@ -167,6 +169,66 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
     return res
 
 
+def time_cpic(steps, stream, autotune=True):
+    """SU(2) CPIC collision, 512 x 64 x 64, FP32 links/E/J/charges with FP64
+    reunitarization every 10 steps; two colour-neutral nuclei (Gaussian
+    longitudinal profile of width 4 cells, transverse rho_a ~ Gaussian(0, 1)),
+    64 particles per transverse cell each (16 cells x ppc = 4), moving at v = c
+    in +x0 and -x0; U = 1 and E projected onto Gauss's law for the initial
+    charges. A step is Simulation.step: storeFields, the fused update with the
+    current, the charge density, and the current deposit with parallel transport."""
+    import torch
+    from openpixi_amd import FieldMeasurements, Grid, _lib, workloads
+
+    L = _lib.load()
+    dims, ppc = (512, 64, 64), 4
+    g = Grid(2, dims, 1.0, 1.0 / ppc, 1.0, "fp32")
+    g._check(L.opx_set_stream(g.handle, ctypes_void(stream.cuda_stream)))
+    cols, q = workloads.cpic_collision(dims, ppc)
+    g.setSuperParticles(cols["orientation"], cols["number"], cols["index_offset"], cols["per_plane"],
+                        cols["sub_lattice_shift"], cols["particles_per_cell"], q)
+    g.resetCharge()
+    g.interpolateChargedensity(0)
+    cg_iters, gb, ga = g.projectGauss(tol=1e-8, max_iter=5000)
+    g.simulationInitialize()
+    tune_ms = g.autotune(reps=3) if autotune else None
+    m0 = FieldMeasurements().measure(g)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps // REUNIT_EVERY):
+        g.simulationStep(REUNIT_EVERY)
+        g._check(L.opx_reunitarize(g.handle))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    m1 = FieldMeasurements().measure(g)
+    non_finite, non_unitary = g.checkHealth(1e-4)
+    n = int(np.prod(dims))
+    done = (steps // REUNIT_EVERY) * REUNIT_EVERY
+    res = {
+        "workload": "SU(2) CPIC collision 512x64x64, FP32 + FP64 reunitarization every 10 steps, two nuclei of "
+                    "16 x 4 x 4096 NGP particles, v = c, dt/a = 0.25",
+        "steps": done,
+        "ms_per_step": ms / done,
+        "link_updates_per_s": 3 * n * done / (ms * 1e-3),
+        "particles": int(len(q)),
+        "gauss_projection": {"cg_iterations": cg_iters, "before": gb, "after": ga},
+        "gauss_residual_start": float(m0[6]),
+        "gauss_residual_end": float(m1[6]),
+        "energy_start": float(m0[:6].sum()),
+        "energy_end": float(m1[:6].sum()),
+        "charge_norm_start": float(m0[7]),
+        "charge_norm_end": float(m1[7]),
+        "health": {"non_finite": non_finite, "non_unitary_1e-4": non_unitary},
+        "kernel_config": g.kernelConfig(),
+        "autotune_ms": tune_ms,
+    }
+    g.close()
+    return res
+
+
 def ctypes_void(p):
     import ctypes
 
@@ -216,6 +278,10 @@ def main():
     for f in forms:
         results[f] = time_form(f, dims, args.steps, args.warmup, args.kernel_reps, stream, not args.no_autotune)
         print(f"# {f}: {json.dumps(results[f])}", file=sys.stderr, flush=True)
+    cpic = None
+    if not args.no_cpic:
+        cpic = time_cpic(args.cpic_steps, stream, not args.no_autotune)
+        print(f"# cpic: {json.dumps(cpic)}", file=sys.stderr, flush=True)
     main_form = "fp32" if "fp32" in results else forms[0]
     r = results[main_form]
     out = {
@@ -260,6 +326,8 @@ def main():
         "kernel_ms": r["kernel_ms"],
         "bytes_per_link": BYTES_PER_LINK[main_form],
     }
+    if cpic is not None:
+        out["cpic"] = cpic
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     out["host"] = platform.node()
