@@ -173,6 +173,39 @@ int opx_interpolate_to_grid(opx_ctx *ctx, int t);
 /* interpolateChargedensity at integer clock t: rho += Q at the NGP offset */
 int opx_interpolate_chargedensity(opx_ctx *ctx, int t);
 
+/* ---- batched events ------------------------------------------------------
+ * Many independent lattices (events) of uneven sizes advanced together: one
+ * persistent kernel per step runs the fused update of every event (a fixed
+ * grid striding over per-event work items), optionally replayed from a
+ * captured hipGraph. No counterpart in the reference (where ensembles are
+ * independent jobs); each event follows FastTYMSolver exactly as a single
+ * context does.
+ * opx_batch_event returns a context view of one event that borrows the batch
+ * memory: use it for opx_upload_field / opx_download_field /
+ * opx_set_links_from_coefficients / opx_measure / regions; stepping a view is
+ * rejected (OPX_E_STATE) and opx_destroy of a view is left to the batch.
+ * dims holds 3 cells per event; spacing, time step and coupling are shared. */
+typedef struct opx_batch opx_batch;
+enum { OPX_BATCH_OPT_GRID = 0 };   /* workgroups of the persistent kernel */
+int opx_batch_create(int num_colors, int count, const int *dims, const double *spacing, double time_step,
+                     double coupling, int precision, opx_batch **out);
+void opx_batch_destroy(opx_batch *batch);
+const char *opx_batch_last_error(const opx_batch *batch);
+opx_ctx *opx_batch_event(opx_batch *batch, int event);
+int opx_batch_set_stream(opx_batch *batch, void *hip_stream);
+int opx_batch_set_option(opx_batch *batch, int option, int value);
+int opx_batch_get_info(const opx_batch *batch, int *count, int64_t *total_cells, int *grid, int *steps_done);
+/* stepLinks on every event */
+int opx_batch_step_links(opx_batch *batch, double dt);
+/* `steps` x { storeFields, fused update } on every event, reunitarizing after
+ * every reunit_every-th step; use_graph != 0 replays captured periods of steps */
+int opx_batch_leapfrog(opx_batch *batch, int steps, int reunit_every, int use_graph);
+/* out[event * OPX_M_COUNT + k]: opx_measure of each event */
+int opx_batch_measure(opx_batch *batch, double *out);
+int opx_batch_sync(opx_batch *batch);
+/* times persistent-grid sizes (scratch copies of E and Unext) and keeps the fastest */
+int opx_batch_autotune(opx_batch *batch, int reps, double *best_ms);
+
 /* ---- Simulation driver (physics/Simulation.java:318-452) -----------------
  * opx_simulation_initialize: stepLinks, rho, J deposit at t = 0.
  * opx_simulation_step: n times { clock++, storeFields, step, resetCharge,

@@ -70,6 +70,19 @@ SIGNATURES = {
     "opx_simulation_step": (ctypes.c_int, [_vp, ctypes.c_int]),
     "opx_simulation_steps_done": (ctypes.c_int, [_vp]),
     "opx_check_health": (ctypes.c_int, [_vp, ctypes.c_double, _i64p, _i64p]),
+    "opx_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, ctypes.POINTER(_vp)]),
+    "opx_batch_destroy": (None, [_vp]),
+    "opx_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "opx_batch_event": (_vp, [_vp, ctypes.c_int]),
+    "opx_batch_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "opx_batch_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "opx_batch_get_info": (ctypes.c_int, [_vp, _ip, _i64p, _ip, _ip]),
+    "opx_batch_step_links": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "opx_batch_leapfrog": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "opx_batch_measure": (ctypes.c_int, [_vp, _dp]),
+    "opx_batch_sync": (ctypes.c_int, [_vp]),
+    "opx_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
 }
 
 

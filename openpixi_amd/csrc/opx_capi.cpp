@@ -56,6 +56,7 @@ struct opx_ctx {
     int tile = 0;
     int nchunks = 0;                // 0: derived from the CU count
     int num_cus = 256;
+    bool batch_view = false;        // event of an opx_batch: memory borrowed, stepping through the batch only
     std::string err;
 };
 
@@ -292,6 +293,14 @@ void opx_destroy(opx_ctx* ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->batch_view) {
+        // borrowed field memory belongs to the batch; only the view's own buffers are freed
+        void* own[] = {ctx->partials, ctx->dout, ctx->counters};
+        for (void* b : own)
+            if (b) (void)hipFree(b);
+        delete ctx;
+        return;
+    }
     void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->partials, ctx->dout, ctx->counters,
                     ctx->sp_desc, ctx->sp_qoff, ctx->sp_q};
     for (void* b : bufs)
@@ -468,6 +477,7 @@ int opx_set_evaluation_region(opx_ctx* ctx, const int* lo, const int* hi)
 int opx_step(opx_ctx* ctx, double dt)
 {
     if (!ctx) return OPX_E_ARG;
+    if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "step the batch, not one of its events");
     opx::Lattice L = lattice(ctx, dt);
     OPX_TRY(launch_update(ctx, L, ctx->U, ctx->Unext, ctx->E), "step");
     return OPX_OK;
@@ -476,6 +486,7 @@ int opx_step(opx_ctx* ctx, double dt)
 int opx_step_links(opx_ctx* ctx, double dt)
 {
     if (!ctx) return OPX_E_ARG;
+    if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "step the batch, not one of its events");
     opx::Lattice L = lattice(ctx, dt);
     OPX_TRY(opx::launch_step_links(ctx->nc, ctx->prec, L, ctx->active, ctx->U, ctx->Unext, ctx->E, ctx->stream),
             "step_links");
@@ -485,6 +496,7 @@ int opx_step_links(opx_ctx* ctx, double dt)
 int opx_store_fields(opx_ctx* ctx)
 {
     if (!ctx) return OPX_E_ARG;
+    if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "step the batch, not one of its events");
     std::swap(ctx->U, ctx->Unext);
     return OPX_OK;
 }
@@ -517,6 +529,7 @@ int opx_reunitarize(opx_ctx* ctx)
 int opx_leapfrog(opx_ctx* ctx, int steps, int reunit_every)
 {
     if (!ctx) return OPX_E_ARG;
+    if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "step the batch, not one of its events");
     if (steps < 0 || reunit_every < 0) return fail(ctx, OPX_E_ARG, "leapfrog: negative count");
     opx::Lattice L = lattice(ctx, ctx->at);
     for (int s = 0; s < steps; s++) {
@@ -992,6 +1005,373 @@ int opx_gauss_project(opx_ctx* ctx, double tol, int max_iter, int* iterations, d
     for (int i = 0; i < 7; i++)
         if (bufs_sz[i]) (void)hipFree(bufs_sz[i]);
     return rc;
+}
+
+}  // extern "C"
+
+// ============================================================== batched events
+struct opx_batch {
+    int nc = 2, prec = OPX_FP32, count = 0;
+    int G = 4, AD = 3;
+    size_t tsz = 4;
+    void* UA = nullptr;
+    void* UB = nullptr;
+    void* E = nullptr;
+    uint64_t link_total = 0, alg_total = 0;
+    std::vector<opx_ctx*> views;
+    std::vector<opx::BatchEvent> ev;
+    opx::BatchEvent* dev_ev = nullptr;
+    uint32_t total_items = 0;
+    uint32_t grid = 0;
+    int parity = 0;       // 0: U is UA; 1: U is UB
+    int steps = 0;
+    int num_cus = 256;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0, graph_reunit = 0, graph_parity = -1;
+    std::string err;
+};
+
+namespace {
+
+int bfail(opx_batch* b, int code, const std::string& m)
+{
+    if (b) b->err = m;
+    return code;
+}
+
+int bcheck(opx_batch* b, hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return OPX_OK;
+    return bfail(b, OPX_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define OPX_BTRY(expr, what)                          \
+    do {                                              \
+        int _rc = bcheck(b, (expr), (what));          \
+        if (_rc != OPX_OK) return _rc;                \
+    } while (0)
+
+void batch_point_views(opx_batch* b)
+{
+    for (int e = 0; e < b->count; e++) {
+        opx_ctx* v = b->views[e];
+        char* ua = (char*)b->UA + b->ev[e].link_offset * b->tsz;
+        char* ub = (char*)b->UB + b->ev[e].link_offset * b->tsz;
+        v->U = b->parity ? ub : ua;
+        v->Unext = b->parity ? ua : ub;
+        v->E = (char*)b->E + b->ev[e].alg_offset * b->tsz;
+        v->stream = b->stream;
+        v->steps = b->steps;
+    }
+}
+
+// one batch step: storeFields (parity flip) and the fused update of every event
+int batch_step(opx_batch* b, int reunit_every)
+{
+    b->parity ^= 1;
+    void* U = b->parity ? b->UB : b->UA;
+    void* Un = b->parity ? b->UA : b->UB;
+    OPX_BTRY(opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, U, Un, b->E,
+                                      b->stream),
+             "batch update");
+    b->steps++;
+    if (reunit_every > 0 && b->steps % reunit_every == 0)
+        OPX_BTRY(opx::launch_batch_reunitarize(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, Un,
+                                               b->stream),
+                 "batch reunitarize");
+    return OPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int opx_batch_create(int num_colors, int count, const int* dims, const double* spacing, double time_step,
+                     double coupling, int precision, opx_batch** out)
+{
+    if (!out || !dims || !spacing || count < 1) return OPX_E_ARG;
+    *out = nullptr;
+    if (num_colors != 2 && num_colors != 3) return OPX_E_UNSUPPORTED;
+    if (precision != OPX_FP64 && precision != OPX_FP32) return OPX_E_UNSUPPORTED;
+    opx_batch* b = new opx_batch();
+    b->nc = num_colors;
+    b->prec = precision;
+    b->count = count;
+    b->G = num_colors == 2 ? 4 : 18;
+    b->AD = num_colors == 2 ? 3 : 8;
+    b->tsz = precision == OPX_FP64 ? 8 : 4;
+    int rc = OPX_OK;
+    uint64_t lo = 0, ao = 0;
+    uint32_t items = 0;
+    for (int e = 0; e < count && rc == OPX_OK; e++) {
+        opx_ctx* v = new opx_ctx();
+        v->batch_view = true;
+        v->nc = num_colors;
+        v->prec = precision;
+        v->G = b->G;
+        v->AD = b->AD;
+        v->AS = num_colors == 2 ? 3 : 9;
+        v->tsz = b->tsz;
+        int64_t n = 1;
+        for (int i = 0; i < 3; i++) {
+            v->dims[i] = dims[3 * e + i];
+            v->as[i] = spacing[i];
+            if (v->dims[i] < 1) rc = bfail(b, OPX_E_ARG, "batch: event dimensions must be >= 1");
+            n *= v->dims[i];
+        }
+        v->n = n;
+        v->at = time_step;
+        v->g = coupling;
+        b->views.push_back(v);
+        opx::BatchEvent E;
+        E.L = lattice(v, time_step);
+        E.link_offset = lo;
+        E.alg_offset = ao;
+        E.item_start = items;
+        E.pad = 0;
+        b->ev.push_back(E);
+        lo += (uint64_t)n * 3 * b->G;
+        ao += (uint64_t)n * 3 * b->AD;
+        const uint64_t it = 3 * (((uint64_t)n + 255) / 256);
+        if ((uint64_t)items + it >= (1ull << 32)) rc = bfail(b, OPX_E_UNSUPPORTED, "batch: too many cells");
+        items += (uint32_t)it;
+    }
+    b->link_total = lo;
+    b->alg_total = ao;
+    b->total_items = items;
+    if (rc == OPX_OK) rc = bcheck(b, hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking), "batch stream");
+    b->stream = b->own_stream;
+    if (rc == OPX_OK) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            b->num_cus = cus;
+        b->grid = 4 * (uint32_t)b->num_cus;
+    }
+    if (rc == OPX_OK) rc = bcheck(b, hipMalloc(&b->UA, lo * b->tsz), "batch alloc U");
+    if (rc == OPX_OK) rc = bcheck(b, hipMalloc(&b->UB, lo * b->tsz), "batch alloc Unext");
+    if (rc == OPX_OK) rc = bcheck(b, hipMalloc(&b->E, ao * b->tsz), "batch alloc E");
+    if (rc == OPX_OK) rc = bcheck(b, hipMalloc(&b->dev_ev, b->ev.size() * sizeof(opx::BatchEvent)), "batch alloc events");
+    if (rc == OPX_OK)
+        rc = bcheck(b, hipMemcpy(b->dev_ev, b->ev.data(), b->ev.size() * sizeof(opx::BatchEvent), hipMemcpyHostToDevice),
+                    "batch upload events");
+    if (rc == OPX_OK) rc = bcheck(b, hipMemset(b->E, 0, ao * b->tsz), "batch zero E");
+    for (int e = 0; e < count && rc == OPX_OK; e++) {
+        opx_ctx* v = b->views[e];
+        v->num_cus = b->num_cus;
+        rc = bcheck(b, hipMalloc(&v->partials, opx::measure_partials_size((uint32_t)v->n) * sizeof(double)), "view alloc");
+        if (rc == OPX_OK) rc = bcheck(b, hipMalloc(&v->dout, 64 * sizeof(double)), "view alloc");
+        if (rc == OPX_OK) rc = bcheck(b, hipMalloc(&v->counters, 4 * sizeof(unsigned long long)), "view alloc");
+    }
+    if (rc == OPX_OK) {
+        batch_point_views(b);
+        // identity links in both buffers (physics/grid/Cell.java:63-68)
+        for (int e = 0; e < count && rc == OPX_OK; e++) {
+            opx_ctx* v = b->views[e];
+            std::vector<double> soa(link_count(v), 0.0);
+            for (int d = 0; d < 3; d++)
+                for (int k : (num_colors == 2 ? std::vector<int>{0} : std::vector<int>{0, 4, 8})) {
+                    double* p = soa.data() + ((size_t)d * v->G + k) * v->n;
+                    for (int64_t c = 0; c < v->n; c++) p[c] = 1.0;
+                }
+            rc = to_device(v, v->U, soa);
+            if (rc == OPX_OK) rc = to_device(v, v->Unext, soa);
+            if (rc != OPX_OK) b->err = v->err;
+        }
+    }
+    if (rc != OPX_OK) {
+        std::fprintf(stderr, "opx_batch_create: %s\n", b->err.c_str());
+        opx_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return OPX_OK;
+}
+
+void opx_batch_destroy(opx_batch* b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (opx_ctx* v : b->views) opx_destroy(v);
+    void* bufs[] = {b->UA, b->UB, b->E, b->dev_ev};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+    delete b;
+}
+
+const char* opx_batch_last_error(const opx_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+opx_ctx* opx_batch_event(opx_batch* b, int event)
+{
+    if (!b || event < 0 || event >= b->count) return nullptr;
+    return b->views[event];
+}
+
+int opx_batch_set_stream(opx_batch* b, void* hip_stream)
+{
+    if (!b) return OPX_E_ARG;
+    OPX_BTRY(hipStreamSynchronize(b->stream), "batch set_stream sync");
+    b->stream = hip_stream ? (hipStream_t)hip_stream : b->own_stream;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    batch_point_views(b);
+    return OPX_OK;
+}
+
+int opx_batch_step_links(opx_batch* b, double dt)
+{
+    if (!b) return OPX_E_ARG;
+    for (opx_ctx* v : b->views) {
+        opx::Lattice L = lattice(v, dt);
+        OPX_BTRY(opx::launch_step_links(v->nc, v->prec, L, v->active, v->U, v->Unext, v->E, b->stream),
+                 "batch step_links");
+    }
+    return OPX_OK;
+}
+
+int opx_batch_set_option(opx_batch* b, int option, int value)
+{
+    if (!b) return OPX_E_ARG;
+    if (option != OPX_BATCH_OPT_GRID) return bfail(b, OPX_E_ARG, "batch option: unknown option");
+    if (value < 1) return bfail(b, OPX_E_ARG, "batch option: grid must be >= 1");
+    b->grid = (uint32_t)value;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return OPX_OK;
+}
+
+int opx_batch_leapfrog(opx_batch* b, int steps, int reunit_every, int use_graph)
+{
+    if (!b) return OPX_E_ARG;
+    if (steps < 0 || reunit_every < 0) return bfail(b, OPX_E_ARG, "batch leapfrog: negative count");
+    int rc = OPX_OK;
+    // a captured period: reunit_every steps (rounded up to an even count so the
+    // U / Unext roles are the same at the start of every replay)
+    int period = reunit_every > 0 ? reunit_every : 2;
+    if (period % 2) period *= 2;
+    int done = 0;
+    // leading steps until the clock is aligned with the period
+    while (done < steps && (!use_graph || b->steps % period != 0) && rc == OPX_OK) {
+        rc = batch_step(b, reunit_every);
+        done++;
+    }
+    if (use_graph && rc == OPX_OK && steps - done >= period) {
+        if (!b->graph || b->graph_steps != period || b->graph_reunit != reunit_every || b->graph_parity != b->parity) {
+            if (b->graph) {
+                (void)hipGraphExecDestroy(b->graph);
+                b->graph = nullptr;
+            }
+            const int saved_parity = b->parity, saved_steps = b->steps;
+            hipGraph_t g = nullptr;
+            rc = bcheck(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch capture");
+            for (int s = 0; s < period && rc == OPX_OK; s++) rc = batch_step(b, reunit_every);
+            hipError_t ee = hipStreamEndCapture(b->stream, &g);
+            if (rc == OPX_OK) rc = bcheck(b, ee, "batch end capture");
+            if (rc == OPX_OK) rc = bcheck(b, hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0), "batch instantiate");
+            if (g) (void)hipGraphDestroy(g);
+            b->parity = saved_parity;
+            b->steps = saved_steps;
+            b->graph_steps = period;
+            b->graph_reunit = reunit_every;
+            b->graph_parity = saved_parity;
+        }
+        while (rc == OPX_OK && steps - done >= period) {
+            rc = bcheck(b, hipGraphLaunch(b->graph, b->stream), "batch graph launch");
+            b->steps += period;  // period is even: parity unchanged
+            done += period;
+        }
+    }
+    while (rc == OPX_OK && done < steps) {
+        rc = batch_step(b, reunit_every);
+        done++;
+    }
+    batch_point_views(b);
+    return rc;
+}
+
+int opx_batch_measure(opx_batch* b, double* out)
+{
+    if (!b || !out) return OPX_E_ARG;
+    for (int e = 0; e < b->count; e++) {
+        int rc = opx_measure(b->views[e], out + (size_t)OPX_M_COUNT * e);
+        if (rc != OPX_OK) return bfail(b, rc, b->views[e]->err);
+    }
+    return OPX_OK;
+}
+
+int opx_batch_sync(opx_batch* b)
+{
+    if (!b) return OPX_E_ARG;
+    OPX_BTRY(hipStreamSynchronize(b->stream), "batch sync");
+    return OPX_OK;
+}
+
+int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
+{
+    if (!b) return OPX_E_ARG;
+    if (reps < 1) return bfail(b, OPX_E_ARG, "batch autotune: reps must be >= 1");
+    void* e2 = nullptr;
+    void* u2 = nullptr;
+    OPX_BTRY(hipMalloc(&e2, b->alg_total * b->tsz), "batch autotune alloc");
+    int rc = bcheck(b, hipMalloc(&u2, b->link_total * b->tsz), "batch autotune alloc");
+    if (rc == OPX_OK) rc = bcheck(b, hipMemcpyAsync(e2, b->E, b->alg_total * b->tsz, hipMemcpyDeviceToDevice, b->stream), "batch autotune copy");
+    hipEvent_t a0 = nullptr, a1 = nullptr;
+    if (rc == OPX_OK) rc = bcheck(b, hipEventCreate(&a0), "batch autotune event");
+    if (rc == OPX_OK) rc = bcheck(b, hipEventCreate(&a1), "batch autotune event");
+    void* U = b->parity ? b->UB : b->UA;
+    double best = 1e30;
+    uint32_t bg = b->grid;
+    for (int m : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}) {
+        if (rc != OPX_OK) break;
+        const uint32_t grid = (uint32_t)(m * b->num_cus);
+        rc = bcheck(b, opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, grid, U, u2, e2, b->stream), "batch autotune");
+        if (rc == OPX_OK) rc = bcheck(b, hipEventRecord(a0, b->stream), "batch autotune record");
+        for (int r = 0; r < reps && rc == OPX_OK; r++)
+            rc = bcheck(b, opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, grid, U, u2, e2, b->stream), "batch autotune");
+        if (rc == OPX_OK) rc = bcheck(b, hipEventRecord(a1, b->stream), "batch autotune record");
+        if (rc == OPX_OK) rc = bcheck(b, hipEventSynchronize(a1), "batch autotune sync");
+        float ms = 0;
+        if (rc == OPX_OK) rc = bcheck(b, hipEventElapsedTime(&ms, a0, a1), "batch autotune time");
+        if (rc == OPX_OK && ms / reps < best) {
+            best = ms / reps;
+            bg = grid;
+        }
+    }
+    if (a0) (void)hipEventDestroy(a0);
+    if (a1) (void)hipEventDestroy(a1);
+    (void)hipFree(e2);
+    if (u2) (void)hipFree(u2);
+    if (rc != OPX_OK) return rc;
+    if (bg != b->grid) {
+        b->grid = bg;
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+    }
+    if (best_ms) *best_ms = best;
+    return OPX_OK;
+}
+
+int opx_batch_get_info(const opx_batch* b, int* count, int64_t* total_cells, int* grid, int* steps_done)
+{
+    if (!b) return OPX_E_ARG;
+    if (count) *count = b->count;
+    if (total_cells) {
+        int64_t t = 0;
+        for (const opx_ctx* v : b->views) t += v->n;
+        *total_cells = t;
+    }
+    if (grid) *grid = (int)b->grid;
+    if (steps_done) *steps_done = b->steps;
+    return OPX_OK;
 }
 
 }  // extern "C"

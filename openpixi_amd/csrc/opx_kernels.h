@@ -25,6 +25,17 @@ struct Lattice {
     double g;          // coupling
 };
 
+// One event of a batch: its lattice, the element offsets of its link block
+// (3*G*n values) and E block (3*AD*n values) in the batch buffers, and the first
+// of its 3*ceil(n/256) work items.
+struct BatchEvent {
+    Lattice L;
+    uint64_t link_offset;
+    uint64_t alg_offset;
+    uint32_t item_start;
+    uint32_t pad;
+};
+
 // Super-particle descriptors (device arrays of `count` entries).
 struct SuperParticles {
     int count;
@@ -58,6 +69,12 @@ size_t gauss_dot_scratch();
 hipError_t gauss_dot(uint64_t m, const double* a, const double* b, double* scratch, double* out, hipStream_t st);
 hipError_t gauss_axpy(uint64_t m, double alpha, const double* x, double* y, hipStream_t st);
 hipError_t gauss_xpby(uint64_t m, const double* x, double beta, double* y, hipStream_t st);
+
+// Batched events: persistent grid of `grid` workgroups over total_items work items.
+hipError_t launch_batch_update(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items, uint32_t grid,
+                               const void* U, void* Unext, void* E, hipStream_t st);
+hipError_t launch_batch_reunitarize(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items,
+                                    uint32_t grid, void* U, hipStream_t st);
 
 // All launchers enqueue on `st` and return hipGetLastError().
 hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U,

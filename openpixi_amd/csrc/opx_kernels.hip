@@ -168,39 +168,15 @@ __device__ __forceinline__ void fstore_alg(const F& f, uint32_t site, int dir, t
 #ifndef OPX_LINK_WAVES_FP64
 #define OPX_LINK_WAVES_FP64 2
 #endif
-template <int NC, typename T, bool HAS_J, bool BUF>
-__global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK_WAVES_FP64)) void k_combined_update(Lattice L, LinkOrder order, Box active, const T* __restrict__ Up,
-                                                            T* __restrict__ Unextp, T* __restrict__ Ep,
-                                                            const T* __restrict__ Jp)
+// The update of one link (cell `site`, direction d) on lattice L.
+template <int NC, typename T, bool HAS_J, typename Fld>
+__device__ __forceinline__ void link_update(const Lattice& L, const Fld& U, const Fld& Unext, const Fld& E,
+                                            const Fld& J, uint32_t site, int d)
 {
-    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
-    const Fld U(Up, L.n, 3 * Group<NC, T>::G), Unext(Unextp, L.n, 3 * Group<NC, T>::G);
-    const Fld E(Ep, L.n, 3 * Group<NC, T>::AD), J(Jp, L.n, 3 * Group<NC, T>::AD);
     using GR = Group<NC, T>;
     using M = typename GR::M;
     using A = typename GR::A;
-    // Block order: each x0 plane is cut into row tiles of `rows` rows (x1) by the
-    // full x2 extent; logical blocks run direction-fastest, then over the blocks of
-    // one tile-plane, then along x0, then over tiles. XCD k (= blockIdx % 8) takes
-    // a contiguous range of logical blocks, so it sweeps its tiles plane by plane
-    // and the x0 +- 1 neighbours of a block were (or are about to be) read by
-    // blocks a few dozen positions away on the same XCD, i.e. from its L2.
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
-    const uint32_t logical = xcd * q8 + (xcd < r8 ? xcd : r8) + (b >> 3);
-    const int d = (int)(logical % 3);
-    const uint32_t rest = logical / 3;
-    const uint32_t sb = rest % order.blocks_per_tile_plane;
-    const uint32_t rest2 = rest / order.blocks_per_tile_plane;
-    const uint32_t x0b = rest2 % L.d0;
-    const uint32_t tile = rest2 / L.d0;
-    const uint32_t row0 = tile * order.rows;
-    const uint32_t tile_rows = (L.d1 - row0 < order.rows) ? L.d1 - row0 : order.rows;
-    const uint32_t in_tile = sb * kBlock + threadIdx.x;
-    if (in_tile >= tile_rows * L.d2) return;
-    const uint32_t site = x0b * L.s0 + row0 * L.d2 + in_tile;
     const Coord c = coords(L, site);
-    if (!in_box(active, c.x0, c.x1, c.x2)) return;
     const uint32_t xpd = fwd(L, site, coord_of(c, d), d);
     M S = GR::zero();
 #pragma unroll
@@ -232,6 +208,77 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
     fstore_alg<NC, T>(E, site, d, e);
     fstore_link<NC, T>(Unext, site, d, mul(exp_link(e, -at), Ud));
 }
+
+template <int NC, typename T, bool HAS_J, bool BUF>
+__global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK_WAVES_FP64)) void k_combined_update(Lattice L, LinkOrder order, Box active, const T* __restrict__ Up,
+                                                            T* __restrict__ Unextp, T* __restrict__ Ep,
+                                                            const T* __restrict__ Jp)
+{
+    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+    const Fld U(Up, L.n, 3 * Group<NC, T>::G), Unext(Unextp, L.n, 3 * Group<NC, T>::G);
+    const Fld E(Ep, L.n, 3 * Group<NC, T>::AD), J(Jp, L.n, 3 * Group<NC, T>::AD);
+    // Block order: each x0 plane is cut into row tiles of `rows` rows (x1) by the
+    // full x2 extent; logical blocks run direction-fastest, then over the blocks of
+    // one tile-plane, then along x0, then over tiles. XCD k (= blockIdx % 8) takes
+    // a contiguous range of logical blocks, so it sweeps its tiles plane by plane
+    // and the x0 +- 1 neighbours of a block were (or are about to be) read by
+    // blocks a few dozen positions away on the same XCD, i.e. from its L2.
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
+    const uint32_t logical = xcd * q8 + (xcd < r8 ? xcd : r8) + (b >> 3);
+    const int d = (int)(logical % 3);
+    const uint32_t rest = logical / 3;
+    const uint32_t sb = rest % order.blocks_per_tile_plane;
+    const uint32_t rest2 = rest / order.blocks_per_tile_plane;
+    const uint32_t x0b = rest2 % L.d0;
+    const uint32_t tile = rest2 / L.d0;
+    const uint32_t row0 = tile * order.rows;
+    const uint32_t tile_rows = (L.d1 - row0 < order.rows) ? L.d1 - row0 : order.rows;
+    const uint32_t in_tile = sb * kBlock + threadIdx.x;
+    if (in_tile >= tile_rows * L.d2) return;
+    const uint32_t site = x0b * L.s0 + row0 * L.d2 + in_tile;
+    const Coord c = coords(L, site);
+    if (!in_box(active, c.x0, c.x1, c.x2)) return;
+    link_update<NC, T, HAS_J>(L, U, Unext, E, J, site, d);
+}
+
+// ---------------------------------------------------------------- batched events
+// Persistent form: a fixed grid loops over the work items of all events. A work
+// item is kBlock consecutive cells of one direction of one event; items are
+// numbered event by event (cells-block major, direction minor). XCD k
+// (= blockIdx % 8) owns the contiguous item range [k T/8, (k+1) T/8) and its
+// workgroups stride through it, so every workgroup exits after a bounded number
+// of items and neighbouring cells are handled by the same XCD.
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK_WAVES_FP64)) void k_batch_update(
+    const BatchEvent* __restrict__ ev, int nev, uint32_t total_items, const T* __restrict__ Ubase,
+    T* __restrict__ Unbase, T* __restrict__ Ebase)
+{
+    using GR = Group<NC, T>;
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
+    const uint32_t wg_on_xcd = q8 + (xcd < r8 ? 1u : 0u);
+    const uint32_t w = b >> 3;
+    const uint32_t i0 = (uint32_t)(((uint64_t)total_items * xcd) / 8), i1 = (uint32_t)(((uint64_t)total_items * (xcd + 1)) / 8);
+    for (uint32_t item = i0 + w; item < i1; item += wg_on_xcd) {
+        // event of this item: last e with ev[e].item_start <= item
+        int lo = 0, hi = nev - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ev[mid].item_start <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchEvent& E = ev[lo];
+        const uint32_t local = item - E.item_start;
+        const int d = (int)(local % 3);
+        const uint32_t site = (local / 3) * kBlock + threadIdx.x;
+        if (site >= E.L.n) continue;
+        const FieldG<T> U(Ubase + E.link_offset, E.L.n, 0), Un(Unbase + E.link_offset, E.L.n, 0);
+        const FieldG<T> Ef(Ebase + E.alg_offset, E.L.n, 0);
+        link_update<NC, T, false>(E.L, U, Un, Ef, Ef, site, d);
+    }
+}
+
 
 // ---------------------------------------------------------------- kernel 2
 // FastTYMSolver.UpdateLinks.execute (physics/fields/FastTYMSolver.java:65-84)
@@ -320,6 +367,42 @@ __global__ __launch_bounds__(kBlock) void k_reunitarize(uint32_t n, T* __restric
 #pragma unroll
     for (int k = 0; k < GR::G; k++) GR::gref(u, k) = (T)GD::gget(ud, k);
     store_link<NC, T>(U, n, site, d, u);
+}
+
+// reprojection of every link of a batch (same static schedule as k_batch_update)
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_batch_reunitarize(const BatchEvent* __restrict__ ev, int nev,
+                                                              uint32_t total_items, T* __restrict__ Ubase)
+{
+    using GR = Group<NC, T>;
+    using GD = Group<NC, double>;
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
+    const uint32_t wg_on_xcd = q8 + (xcd < r8 ? 1u : 0u);
+    const uint32_t w = b >> 3;
+    const uint32_t i0 = (uint32_t)(((uint64_t)total_items * xcd) / 8), i1 = (uint32_t)(((uint64_t)total_items * (xcd + 1)) / 8);
+    for (uint32_t item = i0 + w; item < i1; item += wg_on_xcd) {
+        int lo = 0, hi = nev - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ev[mid].item_start <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchEvent& E = ev[lo];
+        const uint32_t local = item - E.item_start;
+        const int d = (int)(local % 3);
+        const uint32_t site = (local / 3) * kBlock + threadIdx.x;
+        if (site >= E.L.n) continue;
+        T* U = Ubase + E.link_offset;
+        typename GR::M u = load_link<NC, T>(U, E.L.n, site, d);
+        typename GD::M ud;
+#pragma unroll
+        for (int k = 0; k < GR::G; k++) GD::gref(ud, k) = (double)GR::gget(u, k);
+        ud = reunit(ud);
+#pragma unroll
+        for (int k = 0; k < GR::G; k++) GR::gref(u, k) = (T)GD::gget(ud, k);
+        store_link<NC, T>(U, E.L.n, site, d, u);
+    }
 }
 
 // ---------------------------------------------------------------- init helper
@@ -842,6 +925,26 @@ hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const Sup
     OPX_DISPATCH(nc, prec, {
         hipLaunchKernelGGL((k_sp_chargedensity<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (const T*)q, (T*)rho, lo,
                            hi);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_update(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items, uint32_t grid,
+                               const void* U, void* Unext, void* E, hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_batch_update<NC, T>), dim3(grid), dim3(kBlock), 0, st, ev, nev, total_items, (const T*)U,
+                           (T*)Unext, (T*)E);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reunitarize(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items,
+                                    uint32_t grid, void* U, hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_batch_reunitarize<NC, T>), dim3(grid), dim3(kBlock), 0, st, ev, nev, total_items,
+                           (T*)U);
     });
     return hipGetLastError();
 }

@@ -1,0 +1,109 @@
+"""Batched events (one persistent kernel per step, optional captured hipGraph)
+against per-event oracle runs and against eager single-event stepping."""
+import numpy as np
+import pytest
+
+from openpixi_amd import FastTYMSolver
+from openpixi_amd.batch import Batch
+
+from .helpers import make_state, oracle_grid, rel_err
+
+pytestmark = pytest.mark.gpu
+
+EVENT_DIMS = [(6, 6, 6), (8, 5, 7), (4, 9, 6), (10, 4, 5), (5, 5, 11)]
+
+
+def filled_batch(nc, precision, dims_list=EVENT_DIMS, sigma_a=0.1, sigma_e=0.05):
+    b = Batch(nc, dims_list, 1.0, 0.25, 1.0, precision)
+    states = []
+    for e, dims in enumerate(dims_list):
+        U, E = make_state(nc, dims, sigma_a, sigma_e, seed=100 + e)
+        b.event(e).setU(U)
+        b.event(e).setE(E)
+        states.append((U, E))
+    return b, states
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_batch_matches_oracle_fp64(gpu, nc, graph):
+    print(f"batch fp64 nc={nc} graph={graph}", flush=True)
+    b, states = filled_batch(nc, "fp64")
+    b.step_links(0.25)
+    steps = 14
+    b.leapfrog(steps, 0, graph)
+    for e, dims in enumerate(EVENT_DIMS):
+        og = oracle_grid(nc, dims, *states[e])
+        og.step_links(0.25)
+        for _ in range(steps):
+            og.store_fields()
+            og.step(0.25)
+        assert rel_err(b.event(e).getU(), og.U) < 1e-12, e
+        assert rel_err(b.event(e).getE(), og.E) < 1e-12, e
+        m = b.measure()[e]
+        mo = og.measure()
+        assert np.allclose(m[:7], mo[:7], rtol=1e-11, atol=1e-300), e
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_graph_replays_match_eager(gpu, nc):
+    """Replays of the captured period (10 steps incl. reunitarization) interleaved with
+    eager work agree exactly with eager stepping after every replay."""
+    print(f"graph replays nc={nc}", flush=True)
+    bg, _ = filled_batch(nc, "fp32")
+    be, _ = filled_batch(nc, "fp32")
+    bg.step_links(0.25)
+    be.step_links(0.25)
+    for r in range(4):
+        bg.leapfrog(10, 10, graph=True)
+        be.leapfrog(10, 10, graph=False)
+        mg = bg.measure()   # eager work and synchronisation between replays
+        me = be.measure()
+        assert np.array_equal(mg, me), r
+        for e in range(len(EVENT_DIMS)):
+            assert np.array_equal(bg.event(e).getU(), be.event(e).getU()), (r, e)
+            assert np.array_equal(bg.event(e).getE(), be.event(e).getE()), (r, e)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_batch_matches_single_contexts(gpu, nc):
+    """An event of the batch evolves exactly as the same lattice in its own context."""
+    b, states = filled_batch(nc, "fp32")
+    b.step_links(0.25)
+    b.leapfrog(20, 10, graph=True)
+    from openpixi_amd import Grid, _lib
+
+    for e, dims in enumerate(EVENT_DIMS):
+        g = Grid(nc, dims, 1.0, 0.25, 1.0, "fp32")
+        g.setKernel("link")
+        g.setU(states[e][0])
+        g.setE(states[e][1])
+        FastTYMSolver().stepLinks(g, 0.25)
+        g._check(_lib.load().opx_leapfrog(g.handle, 20, 10))
+        assert rel_err(b.event(e).getU(), g.getU()) < 1e-6, e
+        assert rel_err(b.event(e).getE(), g.getE()) < 1e-5, e
+
+
+def test_batch_autotune_and_uneven_sizes(gpu):
+    dims_list = [(32, 32, 32), (48, 48, 48), (16, 24, 40)]
+    b, states = filled_batch(2, "fp32", dims_list)
+    ms = b.autotune(reps=2)
+    assert ms > 0
+    info = b.info()
+    assert info["count"] == 3 and info["total_cells"] == sum(int(np.prod(d)) for d in dims_list)
+    b.step_links(0.25)
+    b.leapfrog(6, 0, graph=True)
+    og = oracle_grid(2, dims_list[2], *states[2])
+    og.step_links(0.25)
+    for _ in range(6):
+        og.store_fields()
+        og.step(0.25)
+    assert rel_err(b.event(2).getU(), og.U) < 2e-6
+
+
+def test_stepping_an_event_view_is_rejected(gpu):
+    from openpixi_amd import OpxError
+
+    b, _ = filled_batch(2, "fp64", [(4, 4, 4), (5, 4, 3)])
+    with pytest.raises(OpxError):
+        FastTYMSolver().step(b.event(0), 0.25)
