@@ -52,6 +52,9 @@ def parse():
     p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--no-cpic", action="store_true", help="skip the SU(2) CPIC collision line")
     p.add_argument("--cpic-steps", type=int, default=1000)
+    p.add_argument("--no-batch", action="store_true", help="skip the batched-events line")
+    p.add_argument("--batch-events", type=int, default=64)
+    p.add_argument("--batch-steps", type=int, default=300)
     p.add_argument("--probe-kernels", default="auto",
                    help="kernel configurations for --probe: comma list of link | stream:TILE:CHUNKS | auto")
     p.add_argument("--probe", action="store_true",
@@ -229,6 +232,94 @@ def time_cpic(steps, stream, autotune=True):
     return res
 
 
+BATCH_SIDES = (32, 48, 64, 80, 96)
+
+
+def batch_event_sides(count, seed=0):
+    from openpixi_amd import inputs
+
+    return [int(s) for s in inputs.rng(seed, inputs.STREAM_SIZES).choice(BATCH_SIDES, size=count)]
+
+
+def make_batch(precision, sides, stream, autotune):
+    from openpixi_amd import _lib, inputs
+    from openpixi_amd.batch import Batch
+
+    b = Batch(2, [(s, s, s) for s in sides], 1.0, DT, 1.0, precision)
+    b.set_stream(stream.cuda_stream)
+    for e, s in enumerate(sides):
+        b.event(e).setLinksFromCoefficients(inputs.random_links_coefficients(s ** 3, 2, 0.1, seed=1000 + e))
+    tune_ms = b.autotune(reps=3) if autotune else None
+    # E = 0; the initial half step U(-dt/2) -> U(+dt/2) is the identity on the links then
+    b.step_links(DT)
+    return b, tune_ms
+
+
+def time_batch(count, steps, warmup, stream, autotune=True):
+    """Batched events: `count` independent SU(2) lattices with cubic sides drawn
+    uniformly (seeded Philox) from {32, 48, 64, 80, 96}; each starts as the SU(2)
+    32^3 parity case: links exp(i A_a T_a) with A_a ~ Gaussian(0, 0.1) per link,
+    E = 0, dt/a = 0.25. The FP32 batch reprojects its links onto SU(2) in double
+    precision every 10 steps; every step is one persistent kernel over all
+    events, replayed from a captured hipGraph. The FP64 batch runs the same steps
+    as the per-event reference the FP32 observables are compared with."""
+    import torch
+
+    sides = batch_event_sides(count)
+    cells = sum(s ** 3 for s in sides)
+    res = {
+        "workload": f"{count} SU(2) events, cubic sides from {{32,48,64,80,96}}, FP32 + FP64 reunitarization every "
+                    f"10 steps, {steps} steps, one persistent kernel per step in a captured hipGraph",
+        "events": count,
+        "sides": sides,
+        "cells": cells,
+        "steps": steps,
+    }
+    obs = {}
+    for precision in ("fp32", "fp64"):
+        b, tune_ms = make_batch(precision, sides, stream, autotune)
+        reunit = REUNIT_EVERY if precision == "fp32" else 0
+        m0 = b.measure()
+        b.leapfrog(warmup, reunit, graph=True)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        b.leapfrog(steps, reunit, graph=True)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        m1 = b.measure()
+        info = b.info()
+        obs[precision] = (m0, m1)
+        res[precision] = {
+            "ms_per_step": ms / steps,
+            "events_per_s": count / (ms * 1e-3),
+            "link_updates_per_s": 3 * cells * steps / (ms * 1e-3),
+            "grid_workgroups": info["grid"],
+            "autotune_ms": tune_ms,
+            "max_gauss_residual_end": float(m1[:, 6].max()),
+        }
+        b.close()
+        print(f"# batch {precision}: {json.dumps(res[precision])}", file=sys.stderr, flush=True)
+    (a0, a1), (d0, d1) = obs["fp32"], obs["fp64"]
+    ea, ed = a1[:, :6].sum(axis=1), d1[:, :6].sum(axis=1)
+    per_obs = np.abs(a1[:, :6] - d1[:, :6]) / np.abs(d1[:, :6])
+    res["value"] = res["fp32"]["events_per_s"]
+    res["unit"] = "events/s"
+    res["fp32_vs_fp64_per_event"] = {
+        "energy_rel_max": float((np.abs(ea - ed) / np.abs(ed)).max()),
+        "energy_rel_median": float(np.median(np.abs(ea - ed) / np.abs(ed))),
+        "observable_rel_max": float(per_obs.max()),
+        "gauss_abs_max": float(np.abs(a1[:, 6] - d1[:, 6]).max()),
+    }
+    # E^2 + B^2 read at staggered time levels (E at t, links at t + dt/2) moves by O(dt^2 omega^2)
+    # for these rough (UV-dominated) initial links; this is the FP64 run's own change, not FP32 error
+    res["fp64_energy_change_rel_max"] = float((np.abs(ed - d0[:, :6].sum(axis=1)) /
+                                               np.abs(d0[:, :6].sum(axis=1))).max())
+    return res
+
+
 def ctypes_void(p):
     import ctypes
 
@@ -282,6 +373,10 @@ def main():
     if not args.no_cpic:
         cpic = time_cpic(args.cpic_steps, stream, not args.no_autotune)
         print(f"# cpic: {json.dumps(cpic)}", file=sys.stderr, flush=True)
+    batch = None
+    if not args.no_batch:
+        batch = time_batch(args.batch_events, args.batch_steps, 20, stream, not args.no_autotune)
+        print(f"# batch: {json.dumps(batch)}", file=sys.stderr, flush=True)
     main_form = "fp32" if "fp32" in results else forms[0]
     r = results[main_form]
     out = {
@@ -328,6 +423,8 @@ def main():
     }
     if cpic is not None:
         out["cpic"] = cpic
+    if batch is not None:
+        out["batch"] = batch
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     out["host"] = platform.node()
