@@ -132,6 +132,11 @@ def lib():
             "orc_sim_initialize": (None, [gp, spp, pp, ctypes.c_double]),
             "orc_sim_step": (None, [gp, spp, pp, ctypes.c_double, _ip]),
             "orc_time_steps": (ctypes.c_double, [gp, ctypes.c_int, ctypes.c_int]),
+            "orc_adj_exp_vjp": (None, [_dp, _dp, _dp]),
+            "orc_adj_force_vjp": (None, [gp, _dp, _dp, ctypes.c_double, _dp]),
+            "orc_adj_bsq": (ctypes.c_double, [gp, _dp, ctypes.c_int, _dp]),
+            "orc_adj_loss": (ctypes.c_double, [gp, _dp, _dp, _dp, ctypes.c_int]),
+            "orc_adjoint_gradient": (ctypes.c_int, [gp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -422,6 +427,45 @@ class OracleGrid:
 
     def set_threads(self, n):
         lib().orc_grid_set_threads(self._g, int(n))
+
+    # ---- SU(2) adjoint (opx_oracle_adjoint.c) ----
+    def adj_force_vjp(self, U, Ebar, scale=1.0):
+        """(dF/dU)^T Ebar * scale for links U (n,3,4) and cotangent Ebar (n,3,3); F = proj(U S)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        Eb = np.ascontiguousarray(Ebar, dtype=np.float64)
+        out = np.zeros((self.n, 3, 4))
+        lib().orc_adj_force_vjp(self._g, _ptr(U), _ptr(Eb), float(scale), _ptr(out))
+        return out
+
+    def adj_bsq(self, U, direction, grad=True):
+        """Magnetic part of the loss for links U and (optionally) its gradient."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        out = np.zeros((self.n, 3, 4))
+        v = lib().orc_adj_bsq(self._g, _ptr(U), int(direction), _ptr(out) if grad else None)
+        return (v, out) if grad else v
+
+    def adj_loss(self, direction):
+        """Loss of the current state: E, U (= U^{N-1}) and Unext (= U^N)."""
+        U = np.ascontiguousarray(self.U)
+        Un = np.ascontiguousarray(self.Unext)
+        E = np.ascontiguousarray(self.E)
+        return lib().orc_adj_loss(self._g, _ptr(U), _ptr(Un), _ptr(E), int(direction))
+
+    def adjoint_gradient(self, steps, direction):
+        """(loss, dL/dE0) for the initial state in U (links) and E; leaves the final state."""
+        loss = ctypes.c_double()
+        grad = np.zeros((self.n, 3, 3))
+        rc = lib().orc_adjoint_gradient(self._g, int(steps), int(direction), ctypes.byref(loss), _ptr(grad))
+        if rc != 0:
+            raise ValueError(f"orc_adjoint_gradient failed ({rc})")
+        return loss.value, grad
+
+
+def adj_exp_vjp(w, Mbar):
+    o = np.zeros(3)
+    lib().orc_adj_exp_vjp(_ptr(np.ascontiguousarray(w, dtype=np.float64)),
+                          _ptr(np.ascontiguousarray(Mbar, dtype=np.float64)), _ptr(o))
+    return o
 
 
 def sp_current_offset(o, off, pp, k, ppc, t):

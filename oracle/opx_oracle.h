@@ -126,6 +126,13 @@ void orc_p_reassign(orc_particles *p);
 void orc_sim_initialize(orc_grid *gr, orc_superparticles *sp, orc_particles *p, double at);
 void orc_sim_step(orc_grid *gr, orc_superparticles *sp, orc_particles *p, double at, int *steps_done);
 
+/* ---- SU(2) adjoint of the leapfrog (opx_oracle_adjoint.c) ---- */
+void orc_adj_exp_vjp(const double *w, const double *Mbar, double *wbar);
+void orc_adj_force_vjp(const orc_grid *gr, const double *U, const double *Ebar, double scale, double *Ubar);
+double orc_adj_bsq(const orc_grid *gr, const double *U, int dir, double *Ubar);
+double orc_adj_loss(const orc_grid *gr, const double *Uprev, const double *Ulast, const double *E, int dir);
+int orc_adjoint_gradient(orc_grid *gr, int steps, int dir, double *loss, double *grad);
+
 /* ---- timing helper for the CPU baseline ---- */
 double orc_time_steps(orc_grid *gr, int steps, int warmup);   /* seconds for `steps` leapfrog steps */
 
