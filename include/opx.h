@@ -219,6 +219,25 @@ int opx_simulation_steps_done(const opx_ctx *ctx);
  * above tol (UnitarityTester, diagnostics/methods/UnitarityTester.java:31-46). */
 int opx_check_health(opx_ctx *ctx, double tol, int64_t *non_finite, int64_t *non_unitary);
 
+/* ---- SU(2) adjoint of the leapfrog (differentiation; no counterpart in the reference) ----
+ *
+ * Treats the context's current U as the initial links and E as the initial
+ * electric field E0, runs the forward map of opx_step_links followed by
+ * opx_leapfrog(ctx, steps, reunit_every) (FastTYMSolver.java:40-84), and
+ * evaluates the loss
+ *   L = lattice average of e_T - e_L
+ *     = (1/N_dir) sum_k [eT_el + eT_mag - eL_el - eL_mag](k)
+ * of opx_projected_energy_density(ctx, direction, ...) on the final state
+ * (ProjectedEnergyDensity.java:132-160). When grad is not NULL it then returns
+ * dL/dE0 (n_values = num_cells * 3 * 3, reference per-cell order) from a
+ * hand-written reverse sweep: (U, E) is checkpointed every checkpoint_every
+ * steps and each segment is recomputed before it is swept backwards. The
+ * context is left in the final forward state. SU(2), pure Yang-Mills, every
+ * cell active and evaluated; in FP32 contexts the sweep runs in float.
+ * With grad == NULL only the forward and the loss run (finite differences). */
+int opx_adjoint_gradient(opx_ctx *ctx, int steps, int reunit_every, int checkpoint_every, int direction,
+                         double *loss, double *grad, int64_t n_values);
+
 #ifdef __cplusplus
 }
 #endif

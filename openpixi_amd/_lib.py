@@ -83,6 +83,8 @@ SIGNATURES = {
     "opx_batch_measure": (ctypes.c_int, [_vp, _dp]),
     "opx_batch_sync": (ctypes.c_int, [_vp]),
     "opx_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "opx_adjoint_gradient": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                            ctypes.c_int64]),
 }
 
 

@@ -1,0 +1,76 @@
+// opx_ctx.h -- the context behind the opaque opx_ctx handle of include/opx.h
+// and the host helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "opx.h"
+#include "opx_kernels.h"
+
+namespace opx_host {
+struct AdjointWork;
+}
+
+struct opx_ctx {
+    int nc = 2;
+    int prec = OPX_FP64;
+    int dims[3] = {0, 0, 0};
+    double as[3] = {1, 1, 1};
+    double at = 0.0;
+    double g = 1.0;
+    int64_t n = 0;
+    int G = 4;     // reals per link
+    int AD = 3;    // algebra coefficients on the device
+    int AS = 3;    // algebra reals in reference storage
+    size_t tsz = 8;
+    void* U = nullptr;
+    void* Unext = nullptr;
+    void* E = nullptr;
+    void* J = nullptr;
+    void* rho = nullptr;
+    bool j_nonzero = false;
+    opx::Box active{{0, 0, 0}, {0, 0, 0}, 1};
+    opx::Box eval{{0, 0, 0}, {0, 0, 0}, 1};
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    double* partials = nullptr;
+    double* dout = nullptr;
+    unsigned long long* counters = nullptr;
+    // super particles
+    int sp_count = 0;
+    int sp_max_num = 0;
+    int64_t sp_total = 0;
+    int* sp_desc = nullptr;         // 6 arrays of sp_count ints, back to back
+    std::vector<int> sp_host;       // host copy of the descriptors (same layout)
+    int64_t* sp_qoff = nullptr;
+    void* sp_q = nullptr;           // [coef][total]
+    bool rho_pending = false;       // charge density deposit deferred until rho is read
+    int rho_pending_t = 0;
+    int steps = 0;                  // Simulation.totalSimulationSteps
+    // fused-update kernel selection (opx_set_option / opx_autotune)
+    int kernel = OPX_KERNEL_AUTO;
+    int tile = 0;
+    int nchunks = 0;                // 0: derived from the CU count
+    int num_cus = 256;
+    bool batch_view = false;        // event of an opx_batch: memory borrowed, stepping through the batch only
+    opx_host::AdjointWork* adj = nullptr;   // checkpoint and cotangent buffers of opx_adjoint_gradient
+    std::string err;
+};
+
+namespace opx_host {
+
+// record `msg` as the context's last error and return `code`
+int fail(opx_ctx* c, int code, const std::string& msg);
+// OPX_OK for hipSuccess, else OPX_E_DEVICE with the HIP error text
+int hip_check(opx_ctx* c, hipError_t e, const char* what);
+// geometry and step constants of the context's lattice for time step `at`
+opx::Lattice lattice(const opx_ctx* c, double at);
+// the fused E/link update with the context's configured kernel form
+hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E);
+// frees the adjoint workspace (opx_adjoint_capi.cpp)
+void adjoint_release(opx_ctx* c);
+
+}  // namespace opx_host

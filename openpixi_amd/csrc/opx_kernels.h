@@ -82,6 +82,15 @@ hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box&
 hipError_t launch_step_links(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* Unext,
                              const void* E, hipStream_t st);
 hipError_t launch_reunitarize(int nc, int prec, uint32_t n, void* U, hipStream_t st);
+
+// SU(2) adjoint (opx_adjoint.hip); prec 0 = fp64, 1 = fp32; Ubar 4 and Ebar 3 values per link
+hipError_t launch_adj_seed(int prec, uint32_t n, const void* E, void* Ebar, void* Ubar, const double coefE[3],
+                           hipStream_t st);
+hipError_t launch_adj_link_step(int prec, uint32_t n, const void* U, const void* E, void* Ubar, void* Ebar, double at,
+                                hipStream_t st);
+hipError_t launch_adj_force(int prec, const Lattice& L, const void* U, const void* Ebar, void* Ubar, double scale,
+                            hipStream_t st);
+hipError_t launch_adj_bsq(int prec, const Lattice& L, const void* U, void* Ubar, const double coef[3], hipStream_t st);
 hipError_t launch_links_from_coeffs(int nc, int prec, uint32_t n, const double* coeffs_soa, void* U,
                                     hipStream_t st);
 // partials: device buffer of at least measure_partials_size() doubles; out: 8 device doubles
