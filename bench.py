@@ -55,6 +55,9 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the batched-events line")
     p.add_argument("--batch-events", type=int, default=64)
     p.add_argument("--batch-steps", type=int, default=300)
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint line")
+    p.add_argument("--adjoint-steps", type=int, default=200)
+    p.add_argument("--fd-directions", type=int, default=256)
     p.add_argument("--probe-kernels", default="auto",
                    help="kernel configurations for --probe: comma list of link | stream:TILE:CHUNKS | auto")
     p.add_argument("--probe", action="store_true",
@@ -320,6 +323,82 @@ def time_batch(count, steps, warmup, stream, autotune=True):
     return res
 
 
+def time_adjoint(steps, directions, stream):
+    """Adjoint: SU(2) 64^3, links exp(i A_a T_a) with A_a ~ Gaussian(0, 0.1) and
+    E ~ Gaussian(0, 0.05) per algebra component (seeded Philox), dt/a = 0.25,
+    200 steps of the FP32 form with FP64 reunitarization every 10 steps. The loss
+    is the lattice average of e_T - e_L (longitudinal direction x0) of the final
+    state; its gradient with respect to the initial E comes from the hand-written
+    reverse sweep with checkpoints every 20 steps. Checked against central
+    finite differences of the FP64 forward map along `directions` random
+    directions, and against the FP64 adjoint."""
+    import torch
+    from openpixi_amd import Grid, _lib, inputs
+    from openpixi_amd.adjoint import loss_and_gradient
+
+    dims, K = (64, 64, 64), 20
+    n = int(np.prod(dims))
+    A = inputs.random_links_coefficients(n, 2, 0.1, seed=7)
+    Ec = inputs.random_efield_coefficients(n, 2, 0.05, seed=7)
+    g32 = Grid(2, dims, 1.0, DT, 1.0, "fp32")
+    g64 = Grid(2, dims, 1.0, DT, 1.0, "fp64")
+    for g in (g32, g64):
+        g._check(_lib.load().opx_set_stream(g.handle, ctypes_void(stream.cuda_stream)))
+    # reference-layout initial state from the coefficients (links through the library's getLink)
+    g64.setLinksFromCoefficients(A)
+    g64.setEFromCoefficients(Ec)
+    U0, E0 = g64.getU().copy(), g64.getE().copy()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    loss_and_gradient(g32, U0, E0, steps, REUNIT_EVERY, K, 0)   # warm-up (allocates the checkpoints)
+    (l32, gr32), t_grad = timed(lambda: loss_and_gradient(g32, U0, E0, steps, REUNIT_EVERY, K, 0))
+    (_, _), t_fwd = timed(lambda: loss_and_gradient(g32, U0, E0, steps, REUNIT_EVERY, K, 0, gradient=False))
+    (l64, gr64), t_grad64 = timed(lambda: loss_and_gradient(g64, U0, E0, steps, 0, K, 0))
+    rng = np.random.default_rng(2024)
+    h = 1e-5
+    fd = np.empty(directions)
+    dot32 = np.empty(directions)
+    dot64 = np.empty(directions)
+    t0 = time.perf_counter()
+    for k in range(directions):
+        v = rng.normal(size=E0.shape)
+        lp, _ = loss_and_gradient(g64, U0, E0 + h * v, steps, 0, K, 0, gradient=False)
+        lm, _ = loss_and_gradient(g64, U0, E0 - h * v, steps, 0, K, 0, gradient=False)
+        fd[k] = (lp - lm) / (2 * h)
+        dot32[k] = np.sum(gr32 * v)
+        dot64[k] = np.sum(gr64 * v)
+    t_fd = time.perf_counter() - t0
+    rms = float(np.sqrt(np.mean(fd ** 2)))
+    res = {
+        "workload": "SU(2) 64^3 adjoint, 200 steps FP32 + FP64 reunitarization every 10, checkpoints every 20, "
+                    "loss = lattice average of e_T - e_L (x0 longitudinal), gradient w.r.t. initial E",
+        "steps": steps,
+        "checkpoint_every": K,
+        "loss_fp32": l32,
+        "loss_fp64": l64,
+        "ms_forward": 1e3 * t_fwd,
+        "ms_gradient": 1e3 * t_grad,
+        "gradient_over_forward": t_grad / t_fwd,
+        "ms_gradient_fp64": 1e3 * t_grad64,
+        "fd_directions": directions,
+        "fd_step": h,
+        "fd_seconds": t_fd,
+        "fp32_vs_fd_rel_l2": float(np.linalg.norm(dot32 - fd) / np.linalg.norm(fd)),
+        "fp32_vs_fd_max_over_rms": float(np.max(np.abs(dot32 - fd)) / rms),
+        "fp64_vs_fd_rel_l2": float(np.linalg.norm(dot64 - fd) / np.linalg.norm(fd)),
+        "fp32_vs_fp64_adjoint_rel_l2": float(np.linalg.norm(gr32 - gr64) / np.linalg.norm(gr64)),
+    }
+    g32.close()
+    g64.close()
+    return res
+
+
 def ctypes_void(p):
     import ctypes
 
@@ -377,6 +456,10 @@ def main():
     if not args.no_batch:
         batch = time_batch(args.batch_events, args.batch_steps, 20, stream, not args.no_autotune)
         print(f"# batch: {json.dumps(batch)}", file=sys.stderr, flush=True)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = time_adjoint(args.adjoint_steps, args.fd_directions, stream)
+        print(f"# adjoint: {json.dumps(adjoint)}", file=sys.stderr, flush=True)
     main_form = "fp32" if "fp32" in results else forms[0]
     r = results[main_form]
     out = {
@@ -425,6 +508,8 @@ def main():
         out["cpic"] = cpic
     if batch is not None:
         out["batch"] = batch
+    if adjoint is not None:
+        out["adjoint"] = adjoint
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     out["host"] = platform.node()
