@@ -37,6 +37,12 @@ DT = 0.25
 REUNIT_EVERY = 10
 # algorithmic HBM bytes per link update of the fused kernel (read U, read E, write E, write Unext)
 BYTES_PER_LINK = {"fp32": 72 + 32 + 32 + 72, "fp64": 144 + 64 + 64 + 144}
+# HBM bytes per launch of the per-link kernel on the 128^3 lattice, counted with rocprofv3
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy; the profile
+# k_combined_update_su3_128cube.md holds the commands and raw values). Reported only when
+# the run uses that kernel on that lattice.
+MEASURED_TRAFFIC = {("fp32", "link", DIMS): (492762.0 * 2.0 + 638976.0) * 1024,
+                    ("fp64", "link", DIMS): (1006369.35 * 2.0 + 1277952.0) * 1024}
 
 
 def parse():
@@ -499,7 +505,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": MEASURED_TRAFFIC.get((main_form, r["kernel_config"]["kernel"], dims)),
         "kernel": r["kernel_config"],
         "kernel_ms": r["kernel_ms"],
         "bytes_per_link": BYTES_PER_LINK[main_form],
