@@ -187,8 +187,10 @@ def time_cpic(steps, stream, autotune=True):
     longitudinal profile of width 4 cells, transverse rho_a ~ Gaussian(0, 1)),
     64 particles per transverse cell each (16 cells x ppc = 4), moving at v = c
     in +x0 and -x0; U = 1 and E projected onto Gauss's law for the initial
-    charges. A step is Simulation.step: storeFields, the fused update with the
-    current, the charge density, and the current deposit with parallel transport."""
+    charges; active region x0 in [1, N0-2] and evaluation region x0 in [2, N0-3]
+    (input/CGC/MV model/MVModel_new_optimized.yaml). A step is Simulation.step:
+    storeFields, the fused update with the current, the charge density, and the
+    current deposit with parallel transport."""
     import torch
     from openpixi_amd import FieldMeasurements, Grid, _lib, workloads
 
@@ -199,6 +201,9 @@ def time_cpic(steps, stream, autotune=True):
     cols, q = workloads.cpic_collision(dims, ppc)
     g.setSuperParticles(cols["orientation"], cols["number"], cols["index_offset"], cols["per_plane"],
                         cols["sub_lattice_shift"], cols["particles_per_cell"], q)
+    # active x0 in [1, N0-2], evaluated x0 in [2, N0-3], as the reference's CGC run settings
+    g.setActiveRegion([1, 0, 0], [dims[0] - 2, dims[1] - 1, dims[2] - 1])
+    g.setEvaluationRegion([2, 0, 0], [dims[0] - 3, dims[1] - 1, dims[2] - 1])
     g.resetCharge()
     g.interpolateChargedensity(0)
     cg_iters, gb, ga = g.projectGauss(tol=1e-8, max_iter=5000)

@@ -54,6 +54,23 @@ def test_leapfrog_fp64_parity(gpu, nc, dims, steps, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS[:2], ids=KERNEL_IDS[:2])
+def test_su2_32cube_plain_method(gpu, kernel):
+    """SU(2) 32^3 FP64: links exp(i A_a sigma_a/2) with A_a ~ N(0, 0.1), E = 0, dt = 0.25,
+    100 steps; final links, E, total energy and Gauss residual against the oracle."""
+    print(f"su2 32^3 plain method kernel={kernel}", flush=True)
+    og, dg = run_both(2, (32, 32, 32), 100, sigma_a=0.1, sigma_e=0.0, seed=3, kernel=kernel)
+    assert rel_err(dg.getU(), og.U) < FP64_TOL
+    assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
+    assert rel_err(dg.getE(), og.E) < FP64_TOL
+    md = FieldMeasurements().measure(dg)
+    mo = og.measure()
+    energy_d, energy_o = md[:6].sum(), mo[:6].sum()
+    assert abs(energy_d - energy_o) < 1e-12 * abs(energy_o)
+    assert md[6] < 1e-26 and mo[6] < 1e-26   # E starts at 0: Gauss holds to rounding through 100 steps
+    assert abs(md[6] - mo[6]) < 1e-26
+
+
+@pytest.mark.parametrize("kernel", KERNELS[:2], ids=KERNEL_IDS[:2])
 @pytest.mark.parametrize("nc", [2, 3])
 def test_leapfrog_strong_fields_anisotropic(gpu, nc, kernel):
     """Large amplitudes (SU(3) exp beyond the Taylor branch) and uneven spacings."""
