@@ -173,6 +173,27 @@ int opx_interpolate_to_grid(opx_ctx *ctx, int t);
 /* interpolateChargedensity at integer clock t: rho += Q at the NGP offset */
 int opx_interpolate_chargedensity(opx_ctx *ctx, int t);
 
+/* ---- particle seam: per-particle CGC NGP particles ----------------------
+ * Replaces CGCParticleInterpolationNGP (physics/grid/CGCParticleInterpolationNGP.java:15-104)
+ * and CGCParticleSolver (physics/movement/solver/CGCParticleSolver.java:15-36) over a list of
+ * CGCParticle moving along grid axis `direction`. pos0 and vel hold 3 doubles per
+ * particle, charges one algebra element per particle in reference storage (Q0).
+ * Positions stay in double in every precision: the nearest grid point
+ * (int) rint(pos / a) (physics/util/GridFunctions.java:67-73) is bit-exact. */
+int opx_particles_upload(opx_ctx *ctx, int count, const int *direction, const double *pos0, const double *vel,
+                         const double *charges, int64_t n_values);
+/* pos0, pos1 (3 per particle) and Q0, Q1 (reference storage); any output may be NULL */
+int opx_particles_download(opx_ctx *ctx, double *pos0, double *pos1, double *Q0, double *Q1, int64_t count);
+/* interpolateChargedensity: rho(ngp(pos0)) += Q0 */
+int opx_particles_interpolate_chargedensity(opx_ctx *ctx);
+/* updatePosition (pos1 = pos0 + v dt), interpolateToParticle (the crossed link of Unext)
+ * and updateCharge (Q1 = U^dagger Q0 U on a crossing, else Q0 <-> Q1) */
+int opx_particles_push(opx_ctx *ctx, double dt);
+/* interpolateToGrid: on a crossing J_d += Q0 a_d/dt at the old cell (v > 0) or -Q1 a_d/dt at the new cell */
+int opx_particles_interpolate_to_grid(opx_ctx *ctx);
+/* YangMillsParticle.reassignValues (physics/particles/YangMillsParticle.java:117-125): pos0 <-> pos1, Q0 <-> Q1 */
+int opx_particles_reassign(opx_ctx *ctx);
+
 /* ---- batched events ------------------------------------------------------
  * Many independent lattices (events) of uneven sizes advanced together: one
  * persistent kernel per step runs the fused update of every event (a fixed

@@ -8,7 +8,7 @@ does, and fails loudly when it is missing.
 """
 from ._lib import LIB_PATH, OpxError, OpxLibraryError, load  # noqa: F401
 from .grid import FastTYMSolver, FieldMeasurements, FieldSolver, Grid  # noqa: F401
-from .simulation import Settings, Simulation, SuperParticleSet  # noqa: F401
+from .simulation import ParticleSet, Settings, Simulation, SuperParticleSet  # noqa: F401
 
 __all__ = [
     "Grid", "FieldSolver", "FastTYMSolver", "FieldMeasurements", "Settings", "Simulation", "SuperParticleSet",

@@ -83,6 +83,12 @@ SIGNATURES = {
     "opx_batch_measure": (ctypes.c_int, [_vp, _dp]),
     "opx_batch_sync": (ctypes.c_int, [_vp]),
     "opx_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "opx_particles_upload": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int64]),
+    "opx_particles_download": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, ctypes.c_int64]),
+    "opx_particles_interpolate_chargedensity": (ctypes.c_int, [_vp]),
+    "opx_particles_push": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "opx_particles_interpolate_to_grid": (ctypes.c_int, [_vp]),
+    "opx_particles_reassign": (ctypes.c_int, [_vp]),
     "opx_adjoint_gradient": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                             ctypes.c_int64]),
 }

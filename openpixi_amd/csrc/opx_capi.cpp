@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "opx_ctx.h"
@@ -146,6 +147,11 @@ int check_box(opx_ctx* ctx, const int* lo, const int* hi)
 
 int materialize_rho(opx_ctx* ctx);
 
+opx::ParticleArrays p_view(const opx_ctx* c)
+{
+    return opx::ParticleArrays{c->p_count, c->p_dir, c->p_pos0, c->p_pos1, c->p_vel, c->p_Q0, c->p_Q1};
+}
+
 uint32_t auto_chunks(const opx_ctx* c, const opx::Lattice& L, int tile)
 {
     const uint32_t bpc = opx::stream_blocks_per_chunk(L, tile);
@@ -270,7 +276,8 @@ void opx_destroy(opx_ctx* ctx)
     }
     opx_host::adjoint_release(ctx);
     void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->partials, ctx->dout, ctx->counters,
-                    ctx->sp_desc, ctx->sp_qoff, ctx->sp_q};
+                    ctx->sp_desc, ctx->sp_qoff, ctx->sp_q, ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel,
+                    ctx->p_Q0, ctx->p_Q1};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -640,6 +647,137 @@ int opx_superparticles_download_charges(opx_ctx* ctx, double* charges, int64_t n
     return OPX_OK;
 }
 
+int opx_particles_upload(opx_ctx* ctx, int count, const int* direction, const double* pos0, const double* vel,
+                         const double* charges, int64_t n_values)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "particles belong to a single context, not a batch event");
+    if (count < 0) return fail(ctx, OPX_E_ARG, "particles: negative count");
+    if (count > 0 && (!direction || !pos0 || !vel || !charges)) return fail(ctx, OPX_E_ARG, "particles: null array");
+    if (n_values != (int64_t)count * ctx->AS) return fail(ctx, OPX_E_ARG, "particles: charge buffer size mismatch");
+    for (int k = 0; k < count; k++)
+        if (direction[k] < 0 || direction[k] > 2) return fail(ctx, OPX_E_ARG, "particles: direction must be 0, 1 or 2");
+    void* old[] = {ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel, ctx->p_Q0, ctx->p_Q1};
+    for (void* b : old)
+        if (b) (void)hipFree(b);
+    ctx->p_dir = nullptr;
+    ctx->p_pos0 = ctx->p_pos1 = ctx->p_vel = nullptr;
+    ctx->p_Q0 = ctx->p_Q1 = nullptr;
+    ctx->p_count = 0;
+    if (count == 0) return OPX_OK;
+    const size_t c = (size_t)count;
+    OPX_TRY(hipMalloc(&ctx->p_dir, c * sizeof(int)), "alloc particles");
+    OPX_TRY(hipMalloc(&ctx->p_pos0, 3 * c * sizeof(double)), "alloc particles");
+    OPX_TRY(hipMalloc(&ctx->p_pos1, 3 * c * sizeof(double)), "alloc particles");
+    OPX_TRY(hipMalloc(&ctx->p_vel, 3 * c * sizeof(double)), "alloc particles");
+    OPX_TRY(hipMalloc(&ctx->p_Q0, c * ctx->AD * ctx->tsz), "alloc particles");
+    OPX_TRY(hipMalloc(&ctx->p_Q1, c * ctx->AD * ctx->tsz), "alloc particles");
+    std::vector<double> p0(3 * c), v(3 * c), q((size_t)ctx->AD * c);
+    for (size_t k = 0; k < c; k++)
+        for (int i = 0; i < 3; i++) {
+            p0[(size_t)i * c + k] = pos0[3 * k + i];
+            v[(size_t)i * c + k] = vel[3 * k + i];
+        }
+    for (size_t k = 0; k < c; k++) {
+        double cf[8];
+        const double* a = charges + k * ctx->AS;
+        if (ctx->nc == 2) {
+            cf[0] = a[0]; cf[1] = a[1]; cf[2] = a[2];
+        } else {
+            su3_storage_to_coeffs(a, cf);
+        }
+        for (int m = 0; m < ctx->AD; m++) q[(size_t)m * c + k] = cf[m];
+    }
+    OPX_TRY(hipMemcpyAsync(ctx->p_dir, direction, c * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "upload particles");
+    OPX_TRY(hipMemcpyAsync(ctx->p_pos0, p0.data(), 3 * c * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+            "upload particles");
+    OPX_TRY(hipMemcpyAsync(ctx->p_vel, v.data(), 3 * c * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+            "upload particles");
+    // pos1 and Q1 start at zero; the first push writes pos1 and Q1 before either is read
+    OPX_TRY(hipMemsetAsync(ctx->p_pos1, 0, 3 * c * sizeof(double), ctx->stream), "clear particles");
+    OPX_TRY(hipMemsetAsync(ctx->p_Q1, 0, c * ctx->AD * ctx->tsz, ctx->stream), "clear particles");
+    ctx->p_count = (uint32_t)count;
+    int rc = to_device(ctx, ctx->p_Q0, q);   // synchronizes the stream (host vectors stay alive until then)
+    if (rc != OPX_OK) ctx->p_count = 0;
+    return rc;
+}
+
+int opx_particles_download(opx_ctx* ctx, double* pos0, double* pos1, double* Q0, double* Q1, int64_t count)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (count != (int64_t)ctx->p_count) return fail(ctx, OPX_E_ARG, "particles: count mismatch");
+    if (count == 0) return OPX_OK;
+    const size_t c = (size_t)count;
+    std::vector<double> p0(3 * c), p1(3 * c), q0((size_t)ctx->AD * c), q1((size_t)ctx->AD * c);
+    OPX_TRY(hipMemcpyAsync(p0.data(), ctx->p_pos0, 3 * c * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+            "download particles");
+    OPX_TRY(hipMemcpyAsync(p1.data(), ctx->p_pos1, 3 * c * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+            "download particles");
+    int rc = from_device(ctx, ctx->p_Q0, q0);
+    if (rc == OPX_OK) rc = from_device(ctx, ctx->p_Q1, q1);
+    if (rc != OPX_OK) return rc;
+    for (size_t k = 0; k < c; k++) {
+        for (int i = 0; i < 3; i++) {
+            if (pos0) pos0[3 * k + i] = p0[(size_t)i * c + k];
+            if (pos1) pos1[3 * k + i] = p1[(size_t)i * c + k];
+        }
+        double a0[8], a1[8];
+        for (int m = 0; m < ctx->AD; m++) {
+            a0[m] = q0[(size_t)m * c + k];
+            a1[m] = q1[(size_t)m * c + k];
+        }
+        double* o0 = Q0 ? Q0 + k * ctx->AS : nullptr;
+        double* o1 = Q1 ? Q1 + k * ctx->AS : nullptr;
+        if (ctx->nc == 2) {
+            for (int m = 0; m < 3; m++) {
+                if (o0) o0[m] = a0[m];
+                if (o1) o1[m] = a1[m];
+            }
+        } else {
+            if (o0) su3_coeffs_to_storage(a0, o0);
+            if (o1) su3_coeffs_to_storage(a1, o1);
+        }
+    }
+    return OPX_OK;
+}
+
+int opx_particles_interpolate_chargedensity(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    int rc = materialize_rho(ctx);
+    if (rc != OPX_OK) return rc;
+    opx::Lattice L = lattice(ctx, ctx->at);
+    OPX_TRY(opx::launch_p_chargedensity(ctx->nc, ctx->prec, L, p_view(ctx), ctx->rho, ctx->stream),
+            "particle charge density");
+    return OPX_OK;
+}
+
+int opx_particles_push(opx_ctx* ctx, double dt)
+{
+    if (!ctx) return OPX_E_ARG;
+    opx::Lattice L = lattice(ctx, ctx->at);
+    OPX_TRY(opx::launch_p_push(ctx->nc, ctx->prec, L, p_view(ctx), dt, ctx->Unext, ctx->stream), "particle push");
+    return OPX_OK;
+}
+
+int opx_particles_interpolate_to_grid(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (ctx->p_count == 0) return OPX_OK;
+    opx::Lattice L = lattice(ctx, ctx->at);
+    OPX_TRY(opx::launch_p_to_grid(ctx->nc, ctx->prec, L, p_view(ctx), ctx->J, ctx->stream), "particle current");
+    ctx->j_nonzero = true;
+    return OPX_OK;
+}
+
+int opx_particles_reassign(opx_ctx* ctx)
+{
+    if (!ctx) return OPX_E_ARG;
+    std::swap(ctx->p_pos0, ctx->p_pos1);
+    std::swap(ctx->p_Q0, ctx->p_Q1);
+    return OPX_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -738,8 +876,13 @@ int opx_simulation_initialize(opx_ctx* ctx)
     if ((rc = opx_step_links(ctx, ctx->at)) != OPX_OK) return rc;
     if ((rc = opx_reset_charge(ctx)) != OPX_OK) return rc;
     if ((rc = opx_interpolate_chargedensity(ctx, 0)) != OPX_OK) return rc;
+    if (ctx->p_count > 0) {
+        if ((rc = opx_particles_interpolate_chargedensity(ctx)) != OPX_OK) return rc;
+        if ((rc = opx_particles_push(ctx, ctx->at)) != OPX_OK) return rc;
+    }
     if ((rc = opx_reset_current(ctx)) != OPX_OK) return rc;
     if ((rc = opx_interpolate_to_grid(ctx, 0)) != OPX_OK) return rc;
+    if ((rc = opx_particles_interpolate_to_grid(ctx)) != OPX_OK) return rc;
     return OPX_OK;
 }
 
@@ -751,13 +894,19 @@ int opx_simulation_step(opx_ctx* ctx, int n)
     for (int s = 0; s < n; s++) {
         ctx->steps++;
         const int t = ctx->steps;
+        if ((rc = opx_particles_reassign(ctx)) != OPX_OK) return rc;
         if ((rc = opx_store_fields(ctx)) != OPX_OK) return rc;
         if ((rc = opx_step(ctx, ctx->at)) != OPX_OK) return rc;
-        if (ctx->sp_count > 0) {
+        if (ctx->sp_count > 0 || ctx->p_count > 0) {
             if ((rc = opx_reset_charge(ctx)) != OPX_OK) return rc;
             if ((rc = opx_interpolate_chargedensity(ctx, t)) != OPX_OK) return rc;
+            if (ctx->p_count > 0) {
+                if ((rc = opx_particles_interpolate_chargedensity(ctx)) != OPX_OK) return rc;
+                if ((rc = opx_particles_push(ctx, ctx->at)) != OPX_OK) return rc;
+            }
             if ((rc = opx_reset_current(ctx)) != OPX_OK) return rc;
             if ((rc = opx_interpolate_to_grid(ctx, t)) != OPX_OK) return rc;
+            if ((rc = opx_particles_interpolate_to_grid(ctx)) != OPX_OK) return rc;
         }
     }
     return OPX_OK;

@@ -47,6 +47,14 @@ struct opx_ctx {
     std::vector<int> sp_host;       // host copy of the descriptors (same layout)
     int64_t* sp_qoff = nullptr;
     void* sp_q = nullptr;           // [coef][total]
+    // per-particle CGC NGP particles: [3][count] doubles, charges [coef][count]
+    uint32_t p_count = 0;
+    int* p_dir = nullptr;
+    double* p_pos0 = nullptr;
+    double* p_pos1 = nullptr;
+    double* p_vel = nullptr;
+    void* p_Q0 = nullptr;
+    void* p_Q1 = nullptr;
     bool rho_pending = false;       // charge density deposit deferred until rho is read
     int rho_pending_t = 0;
     int steps = 0;                  // Simulation.totalSimulationSteps

@@ -50,6 +50,18 @@ struct SuperParticles {
     int64_t total;     // charges in total
 };
 
+// Per-particle CGC NGP particles (opx_particles.hip): positions and velocities
+// [3][count] in double, charges [coefficient][count] in the context precision.
+struct ParticleArrays {
+    uint32_t count;
+    const int* dir;
+    const double* pos0;
+    double* pos1;
+    const double* vel;
+    void* Q0;
+    void* Q1;
+};
+
 // Streaming (2.5D, LDS-exchanged plaquette) form of the fused update.
 // Tile shapes (TY x TX threads over (x1, x2)): 0: 16x16, 1: 8x32, 2: 16x32, 3: 8x64, 4: 8x16.
 constexpr int kStreamTiles = 5;
@@ -71,6 +83,11 @@ hipError_t gauss_axpy(uint64_t m, double alpha, const double* x, double* y, hipS
 hipError_t gauss_xpby(uint64_t m, const double* x, double beta, double* y, hipStream_t st);
 
 // Batched events: persistent grid of `grid` workgroups over total_items work items.
+hipError_t launch_p_chargedensity(int nc, int prec, const Lattice& L, const ParticleArrays& p, void* rho,
+                                  hipStream_t st);
+hipError_t launch_p_push(int nc, int prec, const Lattice& L, const ParticleArrays& p, double dt, const void* Unext,
+                         hipStream_t st);
+hipError_t launch_p_to_grid(int nc, int prec, const Lattice& L, const ParticleArrays& p, void* J, hipStream_t st);
 hipError_t launch_batch_update(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items, uint32_t grid,
                                const void* U, void* Unext, void* E, hipStream_t st);
 hipError_t launch_batch_reunitarize(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items,

@@ -255,6 +255,26 @@ class Grid:
     def interpolateChargedensity(self, t):
         self._check(_lib.load().opx_interpolate_chargedensity(self._h, int(t)))
 
+    def setParticles(self, direction, pos0, vel, charges):
+        """Upload per-particle CGC NGP particles (CGCParticle): direction (count,), pos0 and
+        vel (count, 3), charges (count, A) reference algebra storage."""
+        d = np.ascontiguousarray(direction, np.int32)
+        p0 = np.ascontiguousarray(pos0, np.float64).reshape(-1)
+        v = np.ascontiguousarray(vel, np.float64).reshape(-1)
+        q = np.ascontiguousarray(charges, np.float64).reshape(-1)
+        self._check(_lib.load().opx_particles_upload(self._h, len(d), _lib.iptr(d), _lib.dptr(p0), _lib.dptr(v),
+                                                     _lib.dptr(q), q.size))
+        self._p_count = len(d)
+
+    def particleState(self):
+        """(pos0, pos1, Q0, Q1) of the per-particle list."""
+        c = getattr(self, "_p_count", 0)
+        A = algebra_size(self.num_colors)
+        p0, p1, q0, q1 = np.zeros(c * 3), np.zeros(c * 3), np.zeros(c * A), np.zeros(c * A)
+        self._check(_lib.load().opx_particles_download(self._h, _lib.dptr(p0), _lib.dptr(p1), _lib.dptr(q0),
+                                                       _lib.dptr(q1), c))
+        return p0.reshape(-1, 3), p1.reshape(-1, 3), q0.reshape(-1, A), q1.reshape(-1, A)
+
     def simulationInitialize(self):
         """Simulation.initialize (physics/Simulation.java:318-352) on the device."""
         self._check(_lib.load().opx_simulation_initialize(self._h))

@@ -113,6 +113,15 @@ class Settings:
 
 
 @dataclass
+class ParticleSet:
+    """A list of CGCParticle (physics/particles/CGCParticle.java) for CGCParticleInterpolationNGP."""
+    direction: np.ndarray        # (count,) beam axis of each particle
+    pos0: np.ndarray             # (count, 3) position at t = 0
+    vel: np.ndarray              # (count, 3)
+    charges: np.ndarray          # (count, A) reference algebra storage (Q0)
+
+
+@dataclass
 class SuperParticleSet:
     """A list of CGCSuperParticle (physics/particles/CGCSuperParticle.java:16-46)."""
     orientation: np.ndarray
@@ -127,7 +136,8 @@ class SuperParticleSet:
 class Simulation:
     """physics/Simulation.java driver over the device grid (FastTYMSolver types only)."""
 
-    def __init__(self, settings: Settings, superparticles: Optional[SuperParticleSet] = None):
+    def __init__(self, settings: Settings, superparticles: Optional[SuperParticleSet] = None,
+                 particles: Optional[ParticleSet] = None):
         if settings.simulation_type not in FAST_TYM_TYPES:
             raise NotImplementedError(f"{settings.simulation_type} does not use FastTYMSolver")
         if settings.number_of_dimensions != 3:
@@ -146,6 +156,13 @@ class Simulation:
         self.superparticles = None
         if superparticles is not None:
             self.set_superparticles(superparticles)
+        self.particles = None
+        if particles is not None:
+            self.set_particles(particles)
+
+    def set_particles(self, p: ParticleSet):
+        self.grid.setParticles(p.direction, p.pos0, p.vel, p.charges)
+        self.particles = p
 
     def set_superparticles(self, sp: SuperParticleSet):
         L = _lib.load()

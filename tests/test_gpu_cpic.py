@@ -68,3 +68,70 @@ def test_cpic_fp32_conserves_gauss(gpu, nc):
     m1 = FieldMeasurements().measure(dg)
     assert m0[6] > 1e-3
     assert abs(m1[6] - m0[6]) < 1e-5 * m0[6]
+
+
+def setup_particle_pair(nc, dims, ppc, seed, precision="fp64"):
+    from .test_oracle_cpic import particle_nuclei
+
+    U, E = make_state(nc, dims, 0.3, 0.0, seed=seed)
+    at = 1.0 / ppc
+    og = O.OracleGrid(nc, dims, (1.0, 1.0, 1.0), at, 1.0)
+    og.U[:] = U
+    og.E[:] = E
+    parts = particle_nuclei(nc, dims, ppc, seed)
+    og.set_particles(*parts)
+    dg = device_grid(nc, dims, U, E, at, precision=precision)
+    dg.setParticles(*parts)
+    return og, dg
+
+
+@pytest.mark.parametrize("kernel", ["link", "stream"])
+@pytest.mark.parametrize("nc,ppc", [(2, 4), (3, 4), (2, 1), (3, 2)])
+def test_per_particle_simulation_parity(gpu, nc, ppc, kernel):
+    """Per-particle CGC NGP (CGCParticleInterpolationNGP + CGCParticleSolver) through
+    Simulation steps: positions bit-exact, fields and charges within 1e-12."""
+    dims = (12, 5, 4)
+    print(f"per-particle parity nc={nc} ppc={ppc} kernel={kernel}", flush=True)
+    og, dg = setup_particle_pair(nc, dims, ppc, seed=5)
+    dg.setKernel(kernel)
+    og.initialize()
+    dg.simulationInitialize()
+    steps = 3 * ppc + 2
+    og.sim_step(steps)
+    dg.simulationStep(steps)
+    po0, po1, qo0, qo1 = og.particle_state()
+    pd0, pd1, qd0, qd1 = dg.particleState()
+    assert np.array_equal(pd0, po0) and np.array_equal(pd1, po1)
+    assert np.array_equal(np.rint(pd1), np.rint(po1))
+    tol = 1e-12
+    assert rel_err(qd0, qo0) < tol and rel_err(qd1, qo1) < tol
+    Jo, Jd = og.J, dg.getJ()
+    ro, rd = og.rho, dg.getRho()
+    assert np.array_equal(np.abs(Jo).sum(axis=2) > 0, np.abs(Jd).sum(axis=2) > 0)
+    assert np.array_equal(np.abs(ro).sum(axis=1) > 0, np.abs(rd).sum(axis=1) > 0)
+    assert np.max(np.abs(Jo)) > 0 and np.max(np.abs(ro)) > 0
+    assert rel_err(Jd, Jo) < tol and rel_err(rd, ro) < tol
+    assert rel_err(dg.getE(), og.E) < tol
+    assert rel_err(dg.getU(), og.U) < tol
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_per_particle_fp32(gpu, nc):
+    """FP32 fields and charges, double positions: positions stay bit-exact with the FP64
+    oracle and G(x) keeps its initial value to FP32 rounding."""
+    dims, ppc = (16, 6, 5), 4
+    og, dg = setup_particle_pair(nc, dims, ppc, seed=6, precision="fp32")
+    og.initialize()
+    dg.simulationInitialize()
+    m0 = FieldMeasurements().measure(dg)
+    for _ in range(5):
+        og.sim_step(4)
+        dg.simulationStep(4)
+        dg.reunitarize()
+    m1 = FieldMeasurements().measure(dg)
+    po0, po1, _, qo1 = og.particle_state()
+    pd0, pd1, _, qd1 = dg.particleState()
+    assert np.array_equal(pd0, po0) and np.array_equal(pd1, po1)
+    assert rel_err(qd1, qo1) < 1e-5
+    assert m0[6] > 1e-3
+    assert abs(m1[6] - m0[6]) < 1e-5 * m0[6]

@@ -104,3 +104,77 @@ def test_nearest_grid_point_rounds_half_to_even():
     assert O.nearest_grid_point(-0.5, 1.0) == 0
     assert O.nearest_grid_point(1.25, 0.5) == 2
     assert O.nearest_grid_point(0.7499999, 0.5) == 1
+
+
+def particle_nuclei(nc, dims, ppc, seed, block=2):
+    """Per-particle CGC nuclei laid out as LightConeNGPParticleCreator does
+    (physics/initial/CGC/LightConeNGPParticleCreator.java:155-200): ppc particles per cell
+    at x0 = z + (j - ppc/2)/ppc, every coordinate shifted by 1e-11 a, v = +-1 along x0;
+    a +x nucleus near x0 = 2 and a -x nucleus near x0 = N0 - 4, `block` cells long."""
+    rng = np.random.default_rng(seed)
+    direction, pos, vel, q = [], [], [], []
+    for o, z0 in ((1, 2), (-1, dims[0] - 2 - block)):
+        for z in range(z0, z0 + block):
+            for x1 in range(dims[1]):
+                for x2 in range(dims[2]):
+                    for j in range(ppc):
+                        x = (1.0 * j - ppc // 2) / ppc
+                        direction.append(0)
+                        pos.append([z + 10e-12 + x, x1 + 10e-12, x2 + 10e-12])
+                        vel.append([float(o), 0.0, 0.0])
+                        q.append(O.alg_from_coeffs(nc, rng.normal(0, 1, nc * nc - 1) / ppc))
+    return np.array(direction, np.int32), np.array(pos), np.array(vel), np.array(q)
+
+
+def oracle_with_per_particle(nc, dims, ppc, seed=0, sigma_a=0.3):
+    U, E = make_state(nc, dims, sigma_a, 0.0, seed=seed)
+    og = O.OracleGrid(nc, dims, (1.0, 1.0, 1.0), 1.0 / ppc, 1.0)
+    og.U[:] = U
+    og.E[:] = E
+    og.set_particles(*particle_nuclei(nc, dims, ppc, seed))
+    return og
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+@pytest.mark.parametrize("ppc", [1, 2, 4])
+def test_per_particle_gauss_law_conserved(nc, ppc):
+    """CGCParticleInterpolationNGP + CGCParticleSolver keep G(x) (with rho) constant."""
+    dims = (12, 3, 2)
+    og = oracle_with_per_particle(nc, dims, ppc, seed=3)
+    og.initialize()
+    G0 = np.array([O.alg_coeffs(nc, og.gauss_constraint(i)) for i in range(og.n)])
+    assert np.max(np.abs(G0)) > 0.1
+    assert np.max(np.abs(og.J)) > 0.1
+    for _ in range(3 * ppc):
+        og.sim_step(1)
+        G = np.array([O.alg_coeffs(nc, og.gauss_constraint(i)) for i in range(og.n)])
+        assert np.max(np.abs(G - G0)) < 1e-13
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_per_particle_positions_and_charges(nc):
+    """Each particle advances by v dt per step (pos0 after n steps is the n-fold sum),
+    |Q|^2 is conserved, and the charge is rotated only when its NGP changes. At the
+    end of a step the current charge is Q1 (reassignValues swaps it into Q0 next step)."""
+    dims, ppc = (10, 2, 2), 4
+    d, p0, v, q = particle_nuclei(nc, dims, ppc, seed=4)
+    og = O.OracleGrid(nc, dims, (1.0, 1.0, 1.0), 1.0 / ppc, 1.0)
+    U, _ = make_state(nc, dims, 0.3, 0.0, seed=4)
+    og.U[:] = U
+    og.set_particles(d, p0, v, q)
+    og.initialize()
+    steps = 2   # the charges are current after steps + 1 pushes (initialize pushes once)
+    og.sim_step(steps)
+    pos0, pos1, _, Q1 = og.particle_state()
+    expect = p0.copy()
+    for _ in range(steps):
+        expect = expect + v * (1.0 / ppc)
+    assert np.array_equal(pos0, expect)
+    assert np.array_equal(pos1, expect + v * (1.0 / ppc))
+    n0 = np.array([O.alg_square(nc, x) for x in q])
+    n1 = np.array([O.alg_square(nc, x) for x in Q1])
+    assert np.allclose(n0, n1, rtol=1e-13, atol=1e-15)
+    crossed = np.rint(pos1[:, 0]) != np.rint(p0[:, 0])
+    assert crossed.any() and (~crossed).any()
+    assert np.array_equal(Q1[~crossed], q[~crossed])
+    assert np.min(np.max(np.abs(Q1[crossed] - q[crossed]), axis=1)) > 1e-6
