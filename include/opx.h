@@ -112,6 +112,9 @@ int opx_reset_charge(opx_ctx *ctx);
  *                     (any lattice, any active region);
  *   OPX_KERNEL_STREAM tiles of (x1,x2) columns marching along x0, each plaquette
  *                     computed once and exchanged through LDS;
+ *   OPX_KERNEL_PLAQ   two kernels: proj of the three plaquettes of every cell into a
+ *                     scratch buffer (3 algebra elements per cell), then one thread per
+ *                     link builds the force from four of them and two adjoint actions;
  *   OPX_KERNEL_AUTO   LINK until opx_autotune picks a form (default).
  * OPX_OPT_TILE selects the STREAM tile shape (0: 16x16, 1: 8x32, 2: 16x32,
  * 3: 8x64, 4: 8x16 threads over (x1,x2)); OPX_OPT_CHUNKS the number of x0
@@ -119,7 +122,7 @@ int opx_reset_charge(opx_ctx *ctx);
  * on scratch copies of E and Unext (the lattice state is not changed), keeps
  * the fastest and returns its time per launch in *best_ms. */
 enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2 };
-enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2 };
+enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2, OPX_KERNEL_PLAQ = 3 };
 int opx_set_option(opx_ctx *ctx, int option, int value);
 int opx_get_option(const opx_ctx *ctx, int option, int *value);
 int opx_autotune(opx_ctx *ctx, int reps, double *best_ms);

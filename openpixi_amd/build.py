@@ -5,6 +5,7 @@ declared in ``include/opx.h`` over hand-written HIP kernels.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -46,14 +47,22 @@ def build(verbose: bool = False) -> str:
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
     build_dir = os.path.join(PKG, "_build")
     os.makedirs(build_dir, exist_ok=True)
+    cmds = []
     for src in SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"]
-        cmd = [cc, *HIP_FLAGS, *inc, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmds.append([cc, *HIP_FLAGS, *inc, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    # translation units compile independently: up to 8 at a time
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(cmds))) as pool:
+        for f in [pool.submit(run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

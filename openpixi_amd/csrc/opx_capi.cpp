@@ -170,6 +170,13 @@ hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void*
                                          c->stream);
     }
     if (c->kernel == OPX_KERNEL_STREAM) return hipErrorInvalidConfiguration;
+    if (c->kernel == OPX_KERNEL_PLAQ) {
+        if (!c->P) {
+            hipError_t e = hipMalloc(&c->P, (size_t)c->n * 3 * c->AD * c->tsz);
+            if (e != hipSuccess) return e;
+        }
+        return opx::launch_plaq_update(c->nc, c->prec, L, c->active, U, c->P, Unext, E, c->J, c->j_nonzero, c->stream);
+    }
     return opx::launch_combined_update(c->nc, c->prec, L, c->active, U, Unext, E, c->J, c->j_nonzero, c->stream);
 }
 
@@ -275,7 +282,7 @@ void opx_destroy(opx_ctx* ctx)
         return;
     }
     opx_host::adjoint_release(ctx);
-    void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->partials, ctx->dout, ctx->counters,
+    void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->P, ctx->partials, ctx->dout, ctx->counters,
                     ctx->sp_desc, ctx->sp_qoff, ctx->sp_q, ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel,
                     ctx->p_Q0, ctx->p_Q1};
     for (void* b : bufs)
@@ -942,7 +949,7 @@ int opx_set_option(opx_ctx* ctx, int option, int value)
     if (!ctx) return OPX_E_ARG;
     switch (option) {
     case OPX_OPT_KERNEL:
-        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_STREAM) return fail(ctx, OPX_E_ARG, "option: bad kernel");
+        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_PLAQ) return fail(ctx, OPX_E_ARG, "option: bad kernel");
         ctx->kernel = value;
         return OPX_OK;
     case OPX_OPT_TILE:
@@ -997,6 +1004,7 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
     };
     std::vector<Cand> cands;
     cands.push_back({OPX_KERNEL_LINK, 0, 0});
+    cands.push_back({OPX_KERNEL_PLAQ, 0, 0});
     for (int t = 0; t < opx::kStreamTiles; t++) {
         if (!opx::stream_supported(ctx->nc, ctx->prec, L, t)) continue;
         const uint32_t bpc = opx::stream_blocks_per_chunk(L, t);

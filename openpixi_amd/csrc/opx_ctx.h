@@ -31,6 +31,7 @@ struct opx_ctx {
     void* E = nullptr;
     void* J = nullptr;
     void* rho = nullptr;
+    void* P = nullptr;              // proj of the three plaquettes per cell (plaquette form scratch)
     bool j_nonzero = false;
     opx::Box active{{0, 0, 0}, {0, 0, 0}, 1};
     opx::Box eval{{0, 0, 0}, {0, 0, 0}, 1};

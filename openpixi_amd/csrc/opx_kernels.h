@@ -96,6 +96,10 @@ hipError_t launch_batch_reunitarize(int nc, int prec, const BatchEvent* ev, int 
 // All launchers enqueue on `st` and return hipGetLastError().
 hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U,
                                   void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
+// Plaquette form of the fused update: proj P of the three planes of every cell into P
+// (3 * AD * n values), then the per-link update from them.
+hipError_t launch_plaq_update(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* P,
+                              void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
 hipError_t launch_step_links(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* Unext,
                              const void* E, hipStream_t st);
 hipError_t launch_reunitarize(int nc, int prec, uint32_t n, void* U, hipStream_t st);

@@ -242,6 +242,113 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
     link_update<NC, T, HAS_J>(L, U, Unext, E, J, site, d);
 }
 
+// ---------------------------------------------------------------- plaquette form
+// The force of the fused update from plaquettes computed once per cell. With
+// P_dj(x) = U_d(x) U_j(x+d) U_d(x+j)^+ U_j(x)^+ the staple term is
+//   proj(U_d S_d)(x) = sum_{j != d} a_j^-2 [proj P_dj(x) - act(proj P_dj(x-j), U_j(x-j)^+)],
+// the second term being the backward staple U_d(x) U_j(x+d-j)^+ U_d(x-j)^+ U_j(x-j)
+// = U_j(x-j)^+ P_dj(x-j)^+ U_j(x-j). Kernel A stores proj P for the three planes
+// (0,1), (0,2), (1,2) of every cell; P_jd = P_dj^+ flips the sign. Kernel B does one
+// link: the force from four stored plaquettes and two adjoint actions, then the
+// E and link updates of FastTYMSolver.CombinedUpdate.
+
+__device__ __forceinline__ int plane_of(int a, int b) { return a + b - 1; }   // (0,1)->0, (0,2)->1, (1,2)->2
+
+// Block order of kernel A: as the per-link kernel's, without the direction.
+__device__ __forceinline__ bool tile_site(const Lattice& L, const LinkOrder& order, uint32_t logical, uint32_t& site)
+{
+    const uint32_t sb = logical % order.blocks_per_tile_plane;
+    const uint32_t rest2 = logical / order.blocks_per_tile_plane;
+    const uint32_t x0b = rest2 % L.d0;
+    const uint32_t tile = rest2 / L.d0;
+    const uint32_t row0 = tile * order.rows;
+    const uint32_t tile_rows = (L.d1 - row0 < order.rows) ? L.d1 - row0 : order.rows;
+    const uint32_t in_tile = sb * kBlock + threadIdx.x;
+    if (in_tile >= tile_rows * L.d2) return false;
+    site = x0b * L.s0 + row0 * L.d2 + in_tile;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nb)
+{
+    const uint32_t q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
+    return xcd * q8 + (xcd < r8 ? xcd : r8) + (b >> 3);
+}
+
+#ifndef OPX_PLAQ_WAVES
+#define OPX_PLAQ_WAVES 4
+#endif
+#ifndef OPX_PLAQ_UPDATE_WAVES
+#define OPX_PLAQ_UPDATE_WAVES 6
+#endif
+template <int NC, typename T, bool BUF>
+__global__ __launch_bounds__(kBlock, OPX_PLAQ_WAVES) void k_plaquettes(Lattice L, LinkOrder order, const T* __restrict__ Up,
+                                                       T* __restrict__ Pp)
+{
+    using GR = Group<NC, T>;
+    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+    const Fld U(Up, L.n, 3 * GR::G), P(Pp, L.n, 3 * GR::AD);
+    uint32_t site;
+    if (!tile_site(L, order, xcd_logical(blockIdx.x, gridDim.x), site)) return;
+    const Coord c = coords(L, site);
+    const uint32_t x0p = fwd(L, site, c.x0, 0), x1p = fwd(L, site, c.x1, 1), x2p = fwd(L, site, c.x2, 2);
+    const typename GR::M u0 = fload_link<NC, T>(U, site, 0);
+    const typename GR::M u1 = fload_link<NC, T>(U, site, 1);
+    // P_01 = U_0(x) U_1(x+0) [U_1(x) U_0(x+1)]^+
+    fstore_alg<NC, T>(P, site, 0,
+                      proj_mul_adj(mul(u0, fload_link<NC, T>(U, x0p, 1)), mul(u1, fload_link<NC, T>(U, x1p, 0))));
+    const typename GR::M u2 = fload_link<NC, T>(U, site, 2);
+    // P_02 = U_0(x) U_2(x+0) [U_2(x) U_0(x+2)]^+
+    fstore_alg<NC, T>(P, site, 1,
+                      proj_mul_adj(mul(u0, fload_link<NC, T>(U, x0p, 2)), mul(u2, fload_link<NC, T>(U, x2p, 0))));
+    // P_12 = U_1(x) U_2(x+1) [U_2(x) U_1(x+2)]^+
+    fstore_alg<NC, T>(P, site, 2,
+                      proj_mul_adj(mul(u1, fload_link<NC, T>(U, x1p, 2)), mul(u2, fload_link<NC, T>(U, x2p, 1))));
+}
+
+template <int NC, typename T, bool HAS_J, bool BUF>
+__global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_PLAQ_UPDATE_WAVES : OPX_LINK_WAVES_FP64)) void k_update_plaq(
+    Lattice L, LinkOrder order, Box active, const T* __restrict__ Up, const T* __restrict__ Pp,
+    T* __restrict__ Unextp, T* __restrict__ Ep, const T* __restrict__ Jp)
+{
+    using GR = Group<NC, T>;
+    using A = typename GR::A;
+    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+    const Fld U(Up, L.n, 3 * GR::G), Unext(Unextp, L.n, 3 * GR::G);
+    const Fld P(Pp, L.n, 3 * GR::AD), E(Ep, L.n, 3 * GR::AD), J(Jp, L.n, 3 * GR::AD);
+    const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
+    const int d = (int)(logical % 3);
+    uint32_t site;
+    if (!tile_site(L, order, logical / 3, site)) return;
+    const Coord c = coords(L, site);
+    if (!in_box(active, c.x0, c.x1, c.x2)) return;
+    A f = GR::azero();
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (j == d) continue;
+        const int pl = d < j ? plane_of(d, j) : plane_of(j, d);
+        const uint32_t xmj = bwd(L, site, coord_of(c, j), j);
+        const A back = act_adj(fload_alg<NC, T>(P, xmj, pl), fload_link<NC, T>(U, xmj, j));
+        const A here = fload_alg<NC, T>(P, site, pl);
+        const T w = d < j ? (T)L.area[j] : -(T)L.area[j];
+#pragma unroll
+        for (int k = 0; k < GR::AD; k++) aref(f, k) += w * (aget(here, k) - aget(back, k));
+    }
+    A e = fload_alg<NC, T>(E, site, d);
+    const T at = (T)L.at;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(f, k) * at;
+    if (HAS_J) {
+        const A jj = fload_alg<NC, T>(J, site, d);
+        const T uf = (T)L.uf[d];
+#pragma unroll
+        for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(jj, k) * uf;
+    }
+    fstore_alg<NC, T>(E, site, d, e);
+    const typename GR::M Ud = fload_link<NC, T>(U, site, d);
+    fstore_link<NC, T>(Unext, site, d, mul(exp_link(e, -at), Ud));
+}
+
 // ---------------------------------------------------------------- batched events
 // Persistent form: a fixed grid loops over the work items of all events. A work
 // item is kBlock consecutive cells of one direction of one event; items are
@@ -826,6 +933,40 @@ hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box&
         else
             hipLaunchKernelGGL((k_combined_update<NC, T, false, false>), g1, dim3(kBlock), 0, st, L, order, active,
                                (const T*)U, (T*)Unext, (T*)E, (const T*)E);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_plaq_update(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* P,
+                              void* Unext, void* E, const void* J, bool has_j, hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        LinkOrder order;
+        order.rows = L.d1 < 16 ? L.d1 : 16;
+        order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
+        const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
+        const uint32_t cell_blocks = order.blocks_per_tile_plane * L.d0 * tiles;
+        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::G * sizeof(T) < (1ull << 32);
+        const T* Jt = has_j ? (const T*)J : (const T*)E;
+        if (buf) {
+            hipLaunchKernelGGL((k_plaquettes<NC, T, true>), dim3(cell_blocks), dim3(kBlock), 0, st, L, order,
+                               (const T*)U, (T*)P);
+            if (has_j)
+                hipLaunchKernelGGL((k_update_plaq<NC, T, true, true>), dim3(3 * cell_blocks), dim3(kBlock), 0, st, L,
+                                   order, active, (const T*)U, (const T*)P, (T*)Unext, (T*)E, Jt);
+            else
+                hipLaunchKernelGGL((k_update_plaq<NC, T, false, true>), dim3(3 * cell_blocks), dim3(kBlock), 0, st, L,
+                                   order, active, (const T*)U, (const T*)P, (T*)Unext, (T*)E, Jt);
+        } else {
+            hipLaunchKernelGGL((k_plaquettes<NC, T, false>), dim3(cell_blocks), dim3(kBlock), 0, st, L, order,
+                               (const T*)U, (T*)P);
+            if (has_j)
+                hipLaunchKernelGGL((k_update_plaq<NC, T, true, false>), dim3(3 * cell_blocks), dim3(kBlock), 0, st, L,
+                                   order, active, (const T*)U, (const T*)P, (T*)Unext, (T*)E, Jt);
+            else
+                hipLaunchKernelGGL((k_update_plaq<NC, T, false, false>), dim3(3 * cell_blocks), dim3(kBlock), 0, st, L,
+                                   order, active, (const T*)U, (const T*)P, (T*)Unext, (T*)E, Jt);
+        }
     });
     return hipGetLastError();
 }

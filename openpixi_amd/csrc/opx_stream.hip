@@ -28,123 +28,6 @@ namespace opx {
 
 namespace {
 
-// ---------------------------------------------------------------- reduced products
-// proj(a b^dagger): only the entries the projection reads.
-template <typename T>
-__device__ __forceinline__ Alg2<T> proj_mul_adj(const Su2<T>& a, const Su2<T>& b)
-{
-    return proj(mul(a, adj(b)));
-}
-
-template <typename T>
-__device__ __forceinline__ Alg3<T> proj_mul_adj(const Su3<T>& a, const Su3<T>& b)
-{
-    // projection of m = a b^dagger: imaginary diagonal and the off-diagonal pairs
-    T d[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        T im = a.i[3 * r] * b.r[3 * r];
-        im -= a.r[3 * r] * b.i[3 * r];
-#pragma unroll
-        for (int k = 1; k < 3; k++) {
-            im += a.i[3 * r + k] * b.r[3 * r + k];
-            im -= a.r[3 * r + k] * b.i[3 * r + k];
-        }
-        d[r] = im;
-    }
-    T m01r, m01i, m10r, m10i, m02r, m02i, m20r, m20i, m12r, m12i, m21r, m21i;
-    mul_adj_entry(a, b, 0, 1, m01r, m01i);
-    mul_adj_entry(a, b, 1, 0, m10r, m10i);
-    mul_adj_entry(a, b, 0, 2, m02r, m02i);
-    mul_adj_entry(a, b, 2, 0, m20r, m20i);
-    mul_adj_entry(a, b, 1, 2, m12r, m12i);
-    mul_adj_entry(a, b, 2, 1, m21r, m21i);
-    Alg3<T> o;
-    o.c[0] = m01i + m10i;
-    o.c[1] = m01r - m10r;
-    o.c[2] = d[0] - d[1];
-    o.c[3] = m02i + m20i;
-    o.c[4] = m02r - m20r;
-    o.c[5] = m12i + m21i;
-    o.c[6] = m12r - m21r;
-    o.c[7] = (d[0] + d[1] - 2 * d[2]) * inv_sqrt3<T>();
-    return o;
-}
-
-// act(x, v^dagger) = v^dagger X v, X = sum_a x_a t_a
-template <typename T>
-__device__ __forceinline__ Alg2<T> act_adj(const Alg2<T>& x, const Su2<T>& v)
-{
-    return act(x, adj(v));
-}
-
-template <typename T>
-__device__ __forceinline__ Alg3<T> act_adj(const Alg3<T>& x, const Su3<T>& v)
-{
-    // W = H v, then R = v^dagger W (Hermitian: diagonal real parts and upper triangle)
-    const Su3<T> h = herm(x, T(1));
-    T wr[9], wi[9];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            // h_rr real
-            T re = h.r[4 * r] * v.r[3 * r + c];
-            T im = h.r[4 * r] * v.i[3 * r + c];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                if (k == r) continue;
-                re += h.r[3 * r + k] * v.r[3 * k + c];
-                re -= h.i[3 * r + k] * v.i[3 * k + c];
-                im += h.r[3 * r + k] * v.i[3 * k + c];
-                im += h.i[3 * r + k] * v.r[3 * k + c];
-            }
-            wr[3 * r + c] = re;
-            wi[3 * r + c] = im;
-        }
-    // R_rc = sum_k conj(v_kr) w_kc
-    T d[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        T s = v.r[r] * wr[r];
-        s += v.i[r] * wi[r];
-        s += v.r[3 + r] * wr[3 + r];
-        s += v.i[3 + r] * wi[3 + r];
-        s += v.r[6 + r] * wr[6 + r];
-        s += v.i[6 + r] * wi[6 + r];
-        d[r] = s;
-    }
-    T orr[3], oii[3];  // R_01, R_02, R_12
-    const int rr[3] = {0, 0, 1}, cc[3] = {1, 2, 2};
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const int r = rr[q], c = cc[q];
-        T re = v.r[r] * wr[c];
-        re += v.i[r] * wi[c];
-        T im = v.r[r] * wi[c];
-        im -= v.i[r] * wr[c];
-#pragma unroll
-        for (int k = 1; k < 3; k++) {
-            re += v.r[3 * k + r] * wr[3 * k + c];
-            re += v.i[3 * k + r] * wi[3 * k + c];
-            im += v.r[3 * k + r] * wi[3 * k + c];
-            im -= v.i[3 * k + r] * wr[3 * k + c];
-        }
-        orr[q] = re;
-        oii[q] = im;
-    }
-    // coefficients of the Hermitian R (R_10 = conj R_01, ...)
-    Alg3<T> o;
-    o.c[0] = 2 * orr[0];
-    o.c[1] = -2 * oii[0];
-    o.c[2] = d[0] - d[1];
-    o.c[3] = 2 * orr[1];
-    o.c[4] = -2 * oii[1];
-    o.c[5] = 2 * orr[2];
-    o.c[6] = -2 * oii[2];
-    o.c[7] = (d[0] + d[1] - 2 * d[2]) * inv_sqrt3<T>();
-    return o;
-}
 
 __device__ __forceinline__ bool in_box_dev(const Box& b, uint32_t x0, uint32_t x1, uint32_t x2)
 {

@@ -183,9 +183,10 @@ class Grid:
 
     # ---- kernel form (no counterpart in the reference) ----
     def setKernel(self, kernel="auto", tile=None, chunks=None):
-        """kernel: "auto", "link" or "stream"; tile: stream tile index; chunks: x0 chunk count (0 = auto)."""
+        """kernel: "auto", "link", "stream" or "plaq"; tile: stream tile index; chunks: x0 chunk count (0 = auto)."""
         L = _lib.load()
-        k = {"auto": _lib.KERNEL_AUTO, "link": _lib.KERNEL_LINK, "stream": _lib.KERNEL_STREAM}[kernel]
+        k = {"auto": _lib.KERNEL_AUTO, "link": _lib.KERNEL_LINK, "stream": _lib.KERNEL_STREAM,
+             "plaq": _lib.KERNEL_PLAQ}[kernel]
         self._check(L.opx_set_option(self._h, _lib.OPT_KERNEL, k))
         if tile is not None:
             self._check(L.opx_set_option(self._h, _lib.OPT_TILE, int(tile)))
@@ -199,7 +200,7 @@ class Grid:
             v = ctypes.c_int()
             self._check(L.opx_get_option(self._h, opt, ctypes.byref(v)))
             out.append(v.value)
-        return {"kernel": ["auto", "link", "stream"][out[0]], "tile": out[1], "chunks": out[2]}
+        return {"kernel": ["auto", "link", "stream", "plaq"][out[0]], "tile": out[1], "chunks": out[2]}
 
     def autotune(self, reps=5):
         """Times every kernel form / tile / chunk candidate and keeps the fastest; returns ms per launch."""

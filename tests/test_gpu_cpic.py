@@ -26,7 +26,7 @@ def setup_pair(nc, dims, ppc, seed, precision="fp64", sigma_a=0.3, subdivisions=
     return og, dg
 
 
-@pytest.mark.parametrize("kernel", ["link", "stream"])
+@pytest.mark.parametrize("kernel", ["link", "plaq", "stream"])
 @pytest.mark.parametrize("nc,ppc", [(2, 4), (3, 4), (2, 8), (2, 1), (3, 2)])
 def test_cpic_simulation_parity(gpu, nc, ppc, kernel):
     dims = (14, 4, 3)
@@ -85,7 +85,7 @@ def setup_particle_pair(nc, dims, ppc, seed, precision="fp64"):
     return og, dg
 
 
-@pytest.mark.parametrize("kernel", ["link", "stream"])
+@pytest.mark.parametrize("kernel", ["link", "plaq", "stream"])
 @pytest.mark.parametrize("nc,ppc", [(2, 4), (3, 4), (2, 1), (3, 2)])
 def test_per_particle_simulation_parity(gpu, nc, ppc, kernel):
     """Per-particle CGC NGP (CGCParticleInterpolationNGP + CGCParticleSolver) through

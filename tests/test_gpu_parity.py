@@ -17,8 +17,8 @@ FP64_TOL = 1e-12
 
 
 # kernel forms of the fused update: (kernel, tile, chunks)
-KERNELS = [("link", None, None), ("stream", 0, None), ("stream", 1, 3), ("stream", 4, 1)]
-KERNEL_IDS = ["link", "stream16x16", "stream8x32c3", "stream8x16c1"]
+KERNELS = [("link", None, None), ("plaq", None, None), ("stream", 0, None), ("stream", 1, 3), ("stream", 4, 1)]
+KERNEL_IDS = ["link", "plaq", "stream16x16", "stream8x32c3", "stream8x16c1"]
 
 
 def run_both(nc, dims, steps, at=0.25, spacing=(1.0, 1.0, 1.0), sigma_a=0.1, sigma_e=0.05, seed=0, J=None,
@@ -53,7 +53,7 @@ def test_leapfrog_fp64_parity(gpu, nc, dims, steps, kernel):
     assert rel_err(dg.getE(), og.E) < FP64_TOL
 
 
-@pytest.mark.parametrize("kernel", KERNELS[:2], ids=KERNEL_IDS[:2])
+@pytest.mark.parametrize("kernel", KERNELS[:3], ids=KERNEL_IDS[:3])
 def test_su2_32cube_plain_method(gpu, kernel):
     """SU(2) 32^3 FP64: links exp(i A_a sigma_a/2) with A_a ~ N(0, 0.1), E = 0, dt = 0.25,
     100 steps; final links, E, total energy and Gauss residual against the oracle."""
@@ -70,7 +70,7 @@ def test_su2_32cube_plain_method(gpu, kernel):
     assert abs(md[6] - mo[6]) < 1e-26
 
 
-@pytest.mark.parametrize("kernel", KERNELS[:2], ids=KERNEL_IDS[:2])
+@pytest.mark.parametrize("kernel", KERNELS[:3], ids=KERNEL_IDS[:3])
 @pytest.mark.parametrize("nc", [2, 3])
 def test_leapfrog_strong_fields_anisotropic(gpu, nc, kernel):
     """Large amplitudes (SU(3) exp beyond the Taylor branch) and uneven spacings."""
@@ -81,7 +81,7 @@ def test_leapfrog_strong_fields_anisotropic(gpu, nc, kernel):
     assert rel_err(dg.getE(), og.E) < 1e-11
 
 
-@pytest.mark.parametrize("kernel", KERNELS[:2], ids=KERNEL_IDS[:2])
+@pytest.mark.parametrize("kernel", KERNELS[:3], ids=KERNEL_IDS[:3])
 @pytest.mark.parametrize("nc", [2, 3])
 def test_current_and_active_region(gpu, nc, kernel):
     dims = (6, 5, 4)
