@@ -102,6 +102,12 @@ __device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t n, uint32_
     for (int k = 0; k < GR::AD; k++) p[(size_t)k * n] = aref(a, k);
 }
 
+// Non-temporal cache policy on the per-link kernel's once-touched data (E read and
+// written, Unext written): 1 to enable.
+#ifndef OPX_STREAMING
+#define OPX_STREAMING 1
+#endif
+
 // Field accessors of the per-link kernel: component k of cell s of a field with
 // n cells. FieldB uses buffer instructions (component offset in an SGPR); FieldG
 // plain global pointers (fields of 4 GiB or more).
@@ -112,6 +118,8 @@ struct FieldB {
     __device__ __forceinline__ FieldB(const T* p, uint32_t n_, uint32_t comps) : b(p, comps * n_ * (uint32_t)sizeof(T)), n(n_) {}
     __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return b.ld(s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T)); }
     __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { b.st(v, s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T)); }
+    __device__ __forceinline__ T ld_once(uint32_t k, uint32_t s) const { return b.ld(s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T), OPX_STREAMING); }
+    __device__ __forceinline__ void st_once(uint32_t k, uint32_t s, T v) const { b.st(v, s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T), OPX_STREAMING); }
 };
 template <typename T>
 struct FieldG {
@@ -120,6 +128,8 @@ struct FieldG {
     __device__ __forceinline__ FieldG(const T* p_, uint32_t n_, uint32_t) : p(const_cast<T*>(p_)), n(n_) {}
     __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return p[(size_t)k * n + s]; }
     __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { p[(size_t)k * n + s] = v; }
+    __device__ __forceinline__ T ld_once(uint32_t k, uint32_t s) const { return ld(k, s); }
+    __device__ __forceinline__ void st_once(uint32_t k, uint32_t s, T v) const { st(k, s, v); }
 };
 
 template <int NC, typename T, typename F>
@@ -146,6 +156,29 @@ __device__ __forceinline__ typename Group<NC, T>::A fload_alg(const F& f, uint32
 #pragma unroll
     for (int k = 0; k < GR::AD; k++) aref(a, k) = f.ld((uint32_t)(dir * GR::AD + k), site);
     return a;
+}
+template <int NC, typename T, typename F>
+__device__ __forceinline__ typename Group<NC, T>::A fload_alg_once(const F& f, uint32_t site, int dir)
+{
+    using GR = Group<NC, T>;
+    typename GR::A a;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = f.ld_once((uint32_t)(dir * GR::AD + k), site);
+    return a;
+}
+template <int NC, typename T, typename F>
+__device__ __forceinline__ void fstore_alg_once(const F& f, uint32_t site, int dir, typename Group<NC, T>::A a)
+{
+    using GR = Group<NC, T>;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) f.st_once((uint32_t)(dir * GR::AD + k), site, aref(a, k));
+}
+template <int NC, typename T, typename F>
+__device__ __forceinline__ void fstore_link_once(const F& f, uint32_t site, int dir, const typename Group<NC, T>::M& m)
+{
+    using GR = Group<NC, T>;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) f.st_once((uint32_t)(dir * GR::G + k), site, GR::gget(m, k));
 }
 template <int NC, typename T, typename F>
 __device__ __forceinline__ void fstore_alg(const F& f, uint32_t site, int dir, typename Group<NC, T>::A a)
@@ -195,7 +228,7 @@ __device__ __forceinline__ void link_update(const Lattice& L, const Fld& U, cons
     }
     const M Ud = fload_link<NC, T>(U, site, d);
     const A pr = proj(mul(Ud, S));
-    A e = fload_alg<NC, T>(E, site, d);
+    A e = fload_alg_once<NC, T>(E, site, d);
     const T at = (T)L.at;
 #pragma unroll
     for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * at;
@@ -205,8 +238,8 @@ __device__ __forceinline__ void link_update(const Lattice& L, const Fld& U, cons
 #pragma unroll
         for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(jj, k) * uf;
     }
-    fstore_alg<NC, T>(E, site, d, e);
-    fstore_link<NC, T>(Unext, site, d, mul(exp_link(e, -at), Ud));
+    fstore_alg_once<NC, T>(E, site, d, e);
+    fstore_link_once<NC, T>(Unext, site, d, mul(exp_link(e, -at), Ud));
 }
 
 template <int NC, typename T, bool HAS_J, bool BUF>
