@@ -115,6 +115,9 @@ int opx_reset_charge(opx_ctx *ctx);
  *   OPX_KERNEL_PLAQ   two kernels: proj of the three plaquettes of every cell into a
  *                     scratch buffer (3 algebra elements per cell), then one thread per
  *                     link builds the force from four of them and two adjoint actions;
+ *   OPX_KERNEL_TILE   SU(2) only: tiles of (x1,x2) cells marching along x0 with the
+ *                     links of three planes staged in LDS; OPX_OPT_CHUNKS sets the
+ *                     number of x0 chunks (0 = derived from the CU count);
  *   OPX_KERNEL_AUTO   LINK until opx_autotune picks a form (default).
  * OPX_OPT_TILE selects the STREAM tile shape (0: 16x16, 1: 8x32, 2: 16x32,
  * 3: 8x64, 4: 8x16 threads over (x1,x2)); OPX_OPT_CHUNKS the number of x0
@@ -122,7 +125,7 @@ int opx_reset_charge(opx_ctx *ctx);
  * on scratch copies of E and Unext (the lattice state is not changed), keeps
  * the fastest and returns its time per launch in *best_ms. */
 enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2 };
-enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2, OPX_KERNEL_PLAQ = 3 };
+enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2, OPX_KERNEL_PLAQ = 3, OPX_KERNEL_TILE = 4 };
 int opx_set_option(opx_ctx *ctx, int option, int value);
 int opx_get_option(const opx_ctx *ctx, int option, int *value);
 int opx_autotune(opx_ctx *ctx, int reps, double *best_ms);
@@ -210,7 +213,10 @@ int opx_particles_reassign(opx_ctx *ctx);
  * rejected (OPX_E_STATE) and opx_destroy of a view is left to the batch.
  * dims holds 3 cells per event; spacing, time step and coupling are shared. */
 typedef struct opx_batch opx_batch;
-enum { OPX_BATCH_OPT_GRID = 0 };   /* workgroups of the persistent kernel */
+enum {
+    OPX_BATCH_OPT_GRID = 0,   /* workgroups of the persistent kernel */
+    OPX_BATCH_OPT_FORM = 1    /* 0: per-link kernel; 1: LDS-tile kernel (SU(2) only) */
+};
 int opx_batch_create(int num_colors, int count, const int *dims, const double *spacing, double time_step,
                      double coupling, int precision, opx_batch **out);
 void opx_batch_destroy(opx_batch *batch);
@@ -218,6 +224,7 @@ const char *opx_batch_last_error(const opx_batch *batch);
 opx_ctx *opx_batch_event(opx_batch *batch, int event);
 int opx_batch_set_stream(opx_batch *batch, void *hip_stream);
 int opx_batch_set_option(opx_batch *batch, int option, int value);
+int opx_batch_get_option(const opx_batch *batch, int option, int *value);
 int opx_batch_get_info(const opx_batch *batch, int *count, int64_t *total_cells, int *grid, int *steps_done);
 /* stepLinks on every event */
 int opx_batch_step_links(opx_batch *batch, double dt);

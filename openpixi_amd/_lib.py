@@ -26,7 +26,8 @@ OPX_FP32 = 1
 FIELD_U, FIELD_UNEXT, FIELD_E, FIELD_J, FIELD_RHO = range(5)
 
 OPT_KERNEL, OPT_TILE, OPT_CHUNKS = range(3)
-KERNEL_AUTO, KERNEL_LINK, KERNEL_STREAM, KERNEL_PLAQ = range(4)
+KERNEL_AUTO, KERNEL_LINK, KERNEL_STREAM, KERNEL_PLAQ, KERNEL_TILE = range(5)
+BATCH_OPT_GRID, BATCH_OPT_FORM = range(2)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -77,6 +78,7 @@ SIGNATURES = {
     "opx_batch_event": (_vp, [_vp, ctypes.c_int]),
     "opx_batch_set_stream": (ctypes.c_int, [_vp, _vp]),
     "opx_batch_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "opx_batch_get_option": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "opx_batch_get_info": (ctypes.c_int, [_vp, _ip, _i64p, _ip, _ip]),
     "opx_batch_step_links": (ctypes.c_int, [_vp, ctypes.c_double]),
     "opx_batch_leapfrog": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),

@@ -99,14 +99,23 @@ class Batch:
         return ms.value
 
     def set_grid(self, workgroups):
-        self._check(_lib.load().opx_batch_set_option(self._h, 0, int(workgroups)))
+        self._check(_lib.load().opx_batch_set_option(self._h, _lib.BATCH_OPT_GRID, int(workgroups)))
+
+    def set_form(self, form):
+        """"link" (per-link persistent kernel) or "tile" (LDS-tile kernel, SU(2) only)."""
+        self._check(_lib.load().opx_batch_set_option(self._h, _lib.BATCH_OPT_FORM, {"link": 0, "tile": 1}[form]))
+
+    def form(self):
+        v = ctypes.c_int()
+        self._check(_lib.load().opx_batch_get_option(self._h, _lib.BATCH_OPT_FORM, ctypes.byref(v)))
+        return ["link", "tile"][v.value]
 
     def info(self):
         c, g, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         t = ctypes.c_int64()
         self._check(_lib.load().opx_batch_get_info(self._h, ctypes.byref(c), ctypes.byref(t), ctypes.byref(g),
                                                    ctypes.byref(s)))
-        return {"count": c.value, "total_cells": t.value, "grid": g.value, "steps": s.value}
+        return {"count": c.value, "total_cells": t.value, "grid": g.value, "steps": s.value, "form": self.form()}
 
     def set_stream(self, stream_handle):
         self._check(_lib.load().opx_batch_set_stream(self._h, ctypes.c_void_p(int(stream_handle))))

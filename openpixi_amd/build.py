@@ -16,7 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libopx.so")
 
-SOURCES = ["opx_kernels.hip", "opx_stream.hip", "opx_gauss.hip", "opx_adjoint.hip", "opx_particles.hip",
+SOURCES = ["opx_kernels.hip", "opx_stream.hip", "opx_gauss.hip", "opx_adjoint.hip", "opx_particles.hip", "opx_tile.hip",
            "opx_capi.cpp",
            "opx_adjoint_capi.cpp"]
 

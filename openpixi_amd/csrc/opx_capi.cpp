@@ -161,6 +161,17 @@ uint32_t auto_chunks(const opx_ctx* c, const opx::Lattice& L, int tile)
     return k;
 }
 
+// x0 chunks of the tile form: about three blocks per CU
+uint32_t auto_tile_chunks(const opx_ctx* c, const opx::Lattice& L)
+{
+    const opx::TileGeom g = opx::tile_geom(c->prec, L, 1);
+    const uint32_t tiles = g.tiles1 * g.tiles2;
+    uint32_t k = (uint32_t)((3 * c->num_cus + tiles - 1) / tiles);
+    if (k < 1) k = 1;
+    if (k > L.d0) k = L.d0;
+    return k;
+}
+
 // The fused E-field/link update with the configured kernel form.
 hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E)
 {
@@ -170,6 +181,12 @@ hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void*
                                          c->stream);
     }
     if (c->kernel == OPX_KERNEL_STREAM) return hipErrorInvalidConfiguration;
+    if (c->kernel == OPX_KERNEL_TILE) {
+        if (!opx::tile_supported(c->nc)) return hipErrorInvalidConfiguration;
+        return opx::launch_tile_update(c->nc, c->prec, L, c->active,
+                                       c->nchunks > 0 ? (uint32_t)c->nchunks : auto_tile_chunks(c, L), U, Unext, E,
+                                       c->J, c->j_nonzero, c->stream);
+    }
     if (c->kernel == OPX_KERNEL_PLAQ) {
         if (!c->P) {
             hipError_t e = hipMalloc(&c->P, (size_t)c->n * 3 * c->AD * c->tsz);
@@ -949,7 +966,9 @@ int opx_set_option(opx_ctx* ctx, int option, int value)
     if (!ctx) return OPX_E_ARG;
     switch (option) {
     case OPX_OPT_KERNEL:
-        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_PLAQ) return fail(ctx, OPX_E_ARG, "option: bad kernel");
+        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_TILE) return fail(ctx, OPX_E_ARG, "option: bad kernel");
+        if (value == OPX_KERNEL_TILE && !opx::tile_supported(ctx->nc))
+            return fail(ctx, OPX_E_UNSUPPORTED, "option: the tile form is SU(2) only");
         ctx->kernel = value;
         return OPX_OK;
     case OPX_OPT_TILE:
@@ -1005,6 +1024,20 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
     std::vector<Cand> cands;
     cands.push_back({OPX_KERNEL_LINK, 0, 0});
     cands.push_back({OPX_KERNEL_PLAQ, 0, 0});
+    if (opx::tile_supported(ctx->nc)) {
+        const opx::TileGeom g = opx::tile_geom(ctx->prec, L, 1);
+        const uint32_t tiles = g.tiles1 * g.tiles2;
+        std::vector<int> ks;
+        for (double f : {1.0, 2.0, 3.0, 4.0, 6.0, 8.0}) {
+            int k = (int)((f * ctx->num_cus + tiles - 1) / tiles);
+            if (k < 1) k = 1;
+            if (k > (int)L.d0) k = (int)L.d0;
+            bool dup = false;
+            for (int q : ks) dup = dup || q == k;
+            if (!dup) ks.push_back(k);
+        }
+        for (int k : ks) cands.push_back({OPX_KERNEL_TILE, 0, k});
+    }
     for (int t = 0; t < opx::kStreamTiles; t++) {
         if (!opx::stream_supported(ctx->nc, ctx->prec, L, t)) continue;
         const uint32_t bpc = opx::stream_blocks_per_chunk(L, t);
@@ -1154,6 +1187,9 @@ struct opx_batch {
     hipStream_t own_stream = nullptr, stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0, graph_reunit = 0, graph_parity = -1;
+    int form = 0;                      // 0: per-link kernel, 1: LDS-tile kernel
+    opx::TileItem* dev_items = nullptr;
+    uint32_t nitems = 0;
     std::string err;
 };
 
@@ -1197,9 +1233,13 @@ int batch_step(opx_batch* b, int reunit_every)
     b->parity ^= 1;
     void* U = b->parity ? b->UB : b->UA;
     void* Un = b->parity ? b->UA : b->UB;
-    OPX_BTRY(opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, U, Un, b->E,
-                                      b->stream),
-             "batch update");
+    if (b->form == 1)
+        OPX_BTRY(opx::launch_batch_tile(b->prec, b->dev_ev, b->dev_items, b->nitems, b->grid, U, Un, b->E, b->stream),
+                 "batch tile update");
+    else
+        OPX_BTRY(opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, U, Un, b->E,
+                                          b->stream),
+                 "batch update");
     b->steps++;
     if (reunit_every > 0 && b->steps % reunit_every == 0)
         OPX_BTRY(opx::launch_batch_reunitarize(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, Un,
@@ -1281,6 +1321,28 @@ int opx_batch_create(int num_colors, int count, const int* dims, const double* s
     if (rc == OPX_OK)
         rc = bcheck(b, hipMemcpy(b->dev_ev, b->ev.data(), b->ev.size() * sizeof(opx::BatchEvent), hipMemcpyHostToDevice),
                     "batch upload events");
+    if (rc == OPX_OK && opx::tile_supported(num_colors)) {
+        // tile-form work items: per event, chunks of up to 8 planes, tiles fastest
+        std::vector<opx::TileItem> items_h;
+        const int ty = precision == OPX_FP64 ? opx::kTileTY64 : opx::kTileTY32, tx = opx::kTileTX;
+        for (int e = 0; e < count; e++) {
+            const opx::Lattice& L = b->ev[e].L;
+            const opx::TileGeom g = opx::tile_geom(precision, L, 1);
+            const uint32_t nch = (L.d0 + 7) / 8;
+            for (uint32_t ch = 0; ch < nch; ch++)
+                for (uint32_t t1 = 0; t1 < g.tiles1; t1++)
+                    for (uint32_t t2 = 0; t2 < g.tiles2; t2++)
+                        items_h.push_back(opx::TileItem{(uint32_t)e, t1 * (uint32_t)ty, t2 * (uint32_t)tx,
+                                                        (uint32_t)(((uint64_t)ch * L.d0) / nch),
+                                                        (uint32_t)(((uint64_t)(ch + 1) * L.d0) / nch)});
+        }
+        b->nitems = (uint32_t)items_h.size();
+        rc = bcheck(b, hipMalloc(&b->dev_items, items_h.size() * sizeof(opx::TileItem)), "batch alloc items");
+        if (rc == OPX_OK)
+            rc = bcheck(b, hipMemcpy(b->dev_items, items_h.data(), items_h.size() * sizeof(opx::TileItem),
+                                     hipMemcpyHostToDevice),
+                        "batch upload items");
+    }
     if (rc == OPX_OK) rc = bcheck(b, hipMemset(b->E, 0, ao * b->tsz), "batch zero E");
     for (int e = 0; e < count && rc == OPX_OK; e++) {
         opx_ctx* v = b->views[e];
@@ -1320,7 +1382,7 @@ void opx_batch_destroy(opx_batch* b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (opx_ctx* v : b->views) opx_destroy(v);
-    void* bufs[] = {b->UA, b->UB, b->E, b->dev_ev};
+    void* bufs[] = {b->UA, b->UB, b->E, b->dev_ev, b->dev_items};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
@@ -1362,13 +1424,29 @@ int opx_batch_step_links(opx_batch* b, double dt)
 int opx_batch_set_option(opx_batch* b, int option, int value)
 {
     if (!b) return OPX_E_ARG;
-    if (option != OPX_BATCH_OPT_GRID) return bfail(b, OPX_E_ARG, "batch option: unknown option");
-    if (value < 1) return bfail(b, OPX_E_ARG, "batch option: grid must be >= 1");
-    b->grid = (uint32_t)value;
+    if (option == OPX_BATCH_OPT_GRID) {
+        if (value < 1) return bfail(b, OPX_E_ARG, "batch option: grid must be >= 1");
+        b->grid = (uint32_t)value;
+    } else if (option == OPX_BATCH_OPT_FORM) {
+        if (value != 0 && value != 1) return bfail(b, OPX_E_ARG, "batch option: form must be 0 or 1");
+        if (value == 1 && !b->dev_items) return bfail(b, OPX_E_UNSUPPORTED, "batch option: the tile form is SU(2) only");
+        b->form = value;
+    } else {
+        return bfail(b, OPX_E_ARG, "batch option: unknown option");
+    }
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
+    return OPX_OK;
+}
+
+int opx_batch_get_option(const opx_batch* b, int option, int* value)
+{
+    if (!b || !value) return OPX_E_ARG;
+    if (option == OPX_BATCH_OPT_GRID) *value = (int)b->grid;
+    else if (option == OPX_BATCH_OPT_FORM) *value = b->form;
+    else return OPX_E_ARG;
     return OPX_OK;
 }
 
@@ -1453,13 +1531,27 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
     void* U = b->parity ? b->UB : b->UA;
     double best = 1e30;
     uint32_t bg = b->grid;
-    for (int m : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}) {
+    int bf = b->form;
+    struct Cand {
+        int form;
+        uint32_t grid;
+    };
+    std::vector<Cand> cands;
+    for (int m : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}) cands.push_back({0, (uint32_t)(m * b->num_cus)});
+    if (b->dev_items)
+        for (int m : {1, 2, 3, 4, 6, 8}) cands.push_back({1, (uint32_t)(m * b->num_cus)});
+    auto run = [&](const Cand& c) {
+        return c.form == 1 ? opx::launch_batch_tile(b->prec, b->dev_ev, b->dev_items, b->nitems, c.grid, U, u2, e2,
+                                                    b->stream)
+                           : opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, U,
+                                                      u2, e2, b->stream);
+    };
+    for (const Cand& c : cands) {
         if (rc != OPX_OK) break;
-        const uint32_t grid = (uint32_t)(m * b->num_cus);
-        rc = bcheck(b, opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, grid, U, u2, e2, b->stream), "batch autotune");
+        const uint32_t grid = c.grid;
+        rc = bcheck(b, run(c), "batch autotune");
         if (rc == OPX_OK) rc = bcheck(b, hipEventRecord(a0, b->stream), "batch autotune record");
-        for (int r = 0; r < reps && rc == OPX_OK; r++)
-            rc = bcheck(b, opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, grid, U, u2, e2, b->stream), "batch autotune");
+        for (int r = 0; r < reps && rc == OPX_OK; r++) rc = bcheck(b, run(c), "batch autotune");
         if (rc == OPX_OK) rc = bcheck(b, hipEventRecord(a1, b->stream), "batch autotune record");
         if (rc == OPX_OK) rc = bcheck(b, hipEventSynchronize(a1), "batch autotune sync");
         float ms = 0;
@@ -1467,6 +1559,7 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
         if (rc == OPX_OK && ms / reps < best) {
             best = ms / reps;
             bg = grid;
+            bf = c.form;
         }
     }
     if (a0) (void)hipEventDestroy(a0);
@@ -1474,8 +1567,9 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
     (void)hipFree(e2);
     if (u2) (void)hipFree(u2);
     if (rc != OPX_OK) return rc;
-    if (bg != b->grid) {
+    if (bg != b->grid || bf != b->form) {
         b->grid = bg;
+        b->form = bf;
         if (b->graph) {
             (void)hipGraphExecDestroy(b->graph);
             b->graph = nullptr;

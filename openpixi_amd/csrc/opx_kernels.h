@@ -96,6 +96,23 @@ hipError_t launch_batch_reunitarize(int nc, int prec, const BatchEvent* ev, int 
 // All launchers enqueue on `st` and return hipGetLastError().
 hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U,
                                   void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
+// SU(2) LDS-tile form of the fused update (opx_tile.hip): blocks of TY x TX cells
+// march along x0 over nchunks chunks of planes.
+constexpr int kTileTY32 = 16, kTileTY64 = 8, kTileTX = 16;
+struct TileGeom {
+    uint32_t tiles1, tiles2, nchunks;
+};
+// One work item of the batched tile form: planes [pb, pe) of the tile at (row0, col0) of an event.
+struct TileItem {
+    uint32_t event, row0, col0, pb, pe;
+};
+bool tile_supported(int nc);
+TileGeom tile_geom(int prec, const Lattice& L, uint32_t nchunks);
+hipError_t launch_tile_update(int nc, int prec, const Lattice& L, const Box& active, uint32_t nchunks, const void* U,
+                              void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
+hipError_t launch_batch_tile(int prec, const BatchEvent* ev, const TileItem* items, uint32_t nitems, uint32_t grid,
+                             const void* U, void* Unext, void* E, hipStream_t st);
+
 // Plaquette form of the fused update: proj P of the three planes of every cell into P
 // (3 * AD * n values), then the per-link update from them.
 hipError_t launch_plaq_update(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* P,

@@ -107,3 +107,36 @@ def test_stepping_an_event_view_is_rejected(gpu):
     b, _ = filled_batch(2, "fp64", [(4, 4, 4), (5, 4, 3)])
     with pytest.raises(OpxError):
         FastTYMSolver().step(b.event(0), 0.25)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_tile_form_matches_oracle(gpu, graph):
+    """The LDS-tile batch form (SU(2)) against per-event oracle runs."""
+    dims_list = [(6, 6, 6), (8, 5, 7), (20, 17, 33), (1, 9, 6), (10, 18, 5)]
+    b, states = filled_batch(2, "fp64", dims_list)
+    b.set_form("tile")
+    assert b.form() == "tile"
+    b.step_links(0.25)
+    b.leapfrog(12, 0, graph)
+    for e, dims in enumerate(dims_list):
+        og = oracle_grid(2, dims, *states[e])
+        og.step_links(0.25)
+        for _ in range(12):
+            og.store_fields()
+            og.step(0.25)
+        assert rel_err(b.event(e).getU(), og.U) < 1e-12, e
+        assert rel_err(b.event(e).getE(), og.E) < 1e-12, e
+
+
+def test_batch_tile_graph_replays_match_eager(gpu):
+    bg, _ = filled_batch(2, "fp32")
+    be, _ = filled_batch(2, "fp32")
+    for b in (bg, be):
+        b.set_form("tile")
+        b.step_links(0.25)
+    for r in range(3):
+        bg.leapfrog(10, 10, graph=True)
+        be.leapfrog(10, 10, graph=False)
+        assert np.array_equal(bg.measure(), be.measure()), r
+        for e in range(len(EVENT_DIMS)):
+            assert np.array_equal(bg.event(e).getU(), be.event(e).getU()), (r, e)

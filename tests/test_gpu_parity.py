@@ -245,3 +245,48 @@ def test_gauss_projection(gpu, nc, precision):
     dg._check(_lib.load().opx_leapfrog(dg.handle, 20, 10 if precision == "fp32" else 0))
     g_end = FieldMeasurements().measure(dg)[6]
     assert g_end < (1e-20 if precision == "fp64" else 1e-10) * g0
+
+
+@pytest.mark.parametrize("chunks", [None, 1, 3])
+@pytest.mark.parametrize("dims,steps", [((6, 5, 7), 20), ((20, 33, 17), 8), ((3, 40, 70), 6), ((1, 18, 20), 5),
+                                        ((32, 32, 32), 10)])
+def test_tile_form_su2_parity(gpu, dims, steps, chunks):
+    """SU(2) LDS-tile form: lattices smaller than a tile, not multiples of it, a single
+    x0 plane, and several x0 chunkings, against the oracle (FP64, 1e-12)."""
+    print(f"tile parity dims={dims} steps={steps} chunks={chunks}", flush=True)
+    og, dg = run_both(2, dims, steps, sigma_a=0.3, sigma_e=0.2, seed=12, kernel=("tile", None, chunks))
+    assert rel_err(dg.getU(), og.U) < FP64_TOL
+    assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
+    assert rel_err(dg.getE(), og.E) < FP64_TOL
+
+
+def test_tile_form_current_active_region_fp32(gpu):
+    dims = (9, 18, 21)
+    n = int(np.prod(dims))
+    J = np.random.default_rng(4).normal(0, 0.1, size=(n, 3, 3))
+    og, dg = run_both(2, dims, 12, J=J, active=((1, 2, 0), (7, 15, 19)), seed=6, kernel=("tile", None, 2))
+    assert rel_err(dg.getU(), og.U) < FP64_TOL
+    assert rel_err(dg.getE(), og.E) < FP64_TOL
+    U, E = make_state(2, (16, 16, 16), 0.1, 0.05, seed=2)
+    og = oracle_grid(2, (16, 16, 16), U, E)
+    dg = device_grid(2, (16, 16, 16), U, E, precision="fp32")
+    dg.setKernel("tile")
+    from openpixi_amd import _lib
+
+    og.step_links(0.25)
+    FastTYMSolver().stepLinks(dg, 0.25)
+    for _ in range(40):
+        og.store_fields()
+        og.step(0.25)
+    dg._check(_lib.load().opx_leapfrog(dg.handle, 40, 10))
+    assert rel_err(dg.getU(), og.U) < 2e-5
+    assert rel_err(dg.getE(), og.E) < 2e-4
+
+
+def test_tile_form_rejected_for_su3(gpu):
+    from openpixi_amd import OpxError
+
+    U, E = make_state(3, (4, 4, 4), 0.1, 0.0, seed=1)
+    dg = device_grid(3, (4, 4, 4), U, E)
+    with pytest.raises(OpxError):
+        dg.setKernel("tile")
