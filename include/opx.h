@@ -117,7 +117,9 @@ int opx_reset_charge(opx_ctx *ctx);
  *                     link builds the force from four of them and two adjoint actions;
  *   OPX_KERNEL_TILE   SU(2) only: tiles of (x1,x2) cells marching along x0 with the
  *                     links of three planes staged in LDS; OPX_OPT_CHUNKS sets the
- *                     number of x0 chunks (0 = derived from the CU count);
+ *                     number of x0 chunks (0 = derived from the CU count) and
+ *                     OPX_OPT_TILE the store policy of E and Unext (0 non-temporal,
+ *                     1 cached);
  *   OPX_KERNEL_AUTO   LINK until opx_autotune picks a form (default).
  * OPX_OPT_TILE selects the STREAM tile shape (0: 16x16, 1: 8x32, 2: 16x32,
  * 3: 8x64, 4: 8x16 threads over (x1,x2)); OPX_OPT_CHUNKS the number of x0
@@ -215,7 +217,8 @@ int opx_particles_reassign(opx_ctx *ctx);
 typedef struct opx_batch opx_batch;
 enum {
     OPX_BATCH_OPT_GRID = 0,   /* workgroups of the persistent kernel */
-    OPX_BATCH_OPT_FORM = 1    /* 0: per-link kernel; 1: LDS-tile kernel (SU(2) only) */
+    OPX_BATCH_OPT_FORM = 1    /* 0: per-link kernel; 1, 2: LDS-tile kernel with non-temporal / cached
+                                 stores of E and Unext (SU(2) only) */
 };
 int opx_batch_create(int num_colors, int count, const int *dims, const double *spacing, double time_step,
                      double coupling, int precision, opx_batch **out);

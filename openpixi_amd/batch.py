@@ -101,14 +101,17 @@ class Batch:
     def set_grid(self, workgroups):
         self._check(_lib.load().opx_batch_set_option(self._h, _lib.BATCH_OPT_GRID, int(workgroups)))
 
+    FORMS = ["link", "tile", "tile-cached"]
+
     def set_form(self, form):
-        """"link" (per-link persistent kernel) or "tile" (LDS-tile kernel, SU(2) only)."""
-        self._check(_lib.load().opx_batch_set_option(self._h, _lib.BATCH_OPT_FORM, {"link": 0, "tile": 1}[form]))
+        """"link" (per-link persistent kernel), "tile" or "tile-cached" (LDS-tile kernel with
+        non-temporal or cached stores of E and Unext; SU(2) only)."""
+        self._check(_lib.load().opx_batch_set_option(self._h, _lib.BATCH_OPT_FORM, self.FORMS.index(form)))
 
     def form(self):
         v = ctypes.c_int()
         self._check(_lib.load().opx_batch_get_option(self._h, _lib.BATCH_OPT_FORM, ctypes.byref(v)))
-        return ["link", "tile"][v.value]
+        return self.FORMS[v.value]
 
     def info(self):
         c, g, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

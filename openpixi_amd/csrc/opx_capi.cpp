@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -183,9 +184,10 @@ hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void*
     if (c->kernel == OPX_KERNEL_STREAM) return hipErrorInvalidConfiguration;
     if (c->kernel == OPX_KERNEL_TILE) {
         if (!opx::tile_supported(c->nc)) return hipErrorInvalidConfiguration;
+        // OPX_OPT_TILE selects the store policy of this form: 0 non-temporal, 1 cached
         return opx::launch_tile_update(c->nc, c->prec, L, c->active,
-                                       c->nchunks > 0 ? (uint32_t)c->nchunks : auto_tile_chunks(c, L), U, Unext, E,
-                                       c->J, c->j_nonzero, c->stream);
+                                       c->nchunks > 0 ? (uint32_t)c->nchunks : auto_tile_chunks(c, L), c->tile == 0,
+                                       U, Unext, E, c->J, c->j_nonzero, c->stream);
     }
     if (c->kernel == OPX_KERNEL_PLAQ) {
         if (!c->P) {
@@ -846,20 +848,35 @@ int materialize_rho(opx_ctx* ctx)
     opx::Lattice L = lattice(ctx, ctx->at);
     const int cnt = ctx->sp_count;
     const int* h = ctx->sp_host.data();
-    int64_t lo = ctx->n, hi = 0;
+    // the cells the particles sit on now: union of their ranges, merged
+    std::vector<std::pair<int64_t, int64_t>> iv;
     for (int p = 0; p < cnt; p++) {
         const int off = host_ngp_offset(h[p], h[2 * cnt + p], h[3 * cnt + p], h[4 * cnt + p], h[5 * cnt + p],
                                         ctx->rho_pending_t);
         int64_t a = off, b = (int64_t)off + h[cnt + p];
         if (a < 0) a = 0;
         if (b > ctx->n) b = ctx->n;
-        if (b > a) {
-            if (a < lo) lo = a;
-            if (b > hi) hi = b;
-        }
+        if (b > a) iv.push_back({a, b});
+    }
+    std::sort(iv.begin(), iv.end());
+    std::vector<std::pair<int64_t, int64_t>> merged;
+    for (const auto& r : iv) {
+        if (!merged.empty() && r.first <= merged.back().second) merged.back().second = std::max(merged.back().second, r.second);
+        else merged.push_back(r);
+    }
+    if ((int)merged.size() > opx::kMaxCellRanges) {   // one covering range instead
+        const int64_t lo = merged.front().first, hi = merged.back().second;
+        merged.assign(1, {lo, hi});
+    }
+    opx::CellRanges cr{};
+    cr.count = (uint32_t)merged.size();
+    cr.prefix[0] = 0;
+    for (size_t r = 0; r < merged.size(); r++) {
+        cr.start[r] = (uint32_t)merged[r].first;
+        cr.prefix[r + 1] = cr.prefix[r] + (uint32_t)(merged[r].second - merged[r].first);
     }
     OPX_TRY(opx::launch_sp_chargedensity(ctx->nc, ctx->prec, L, sp_view(ctx), ctx->rho_pending_t, ctx->sp_q, ctx->rho,
-                                         (uint32_t)lo, (uint32_t)(hi > lo ? hi : lo), ctx->stream),
+                                         cr, ctx->stream),
             "charge density");
     ctx->rho_pending = false;
     return OPX_OK;
@@ -1004,11 +1021,15 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
     if (!ctx) return OPX_E_ARG;
     if (reps < 1) return fail(ctx, OPX_E_ARG, "autotune: reps must be >= 1");
     opx::Lattice L = lattice(ctx, ctx->at);
-    // candidates run on copies of E and Unext so the lattice state is untouched
+    // candidates run on copies of E and two scratch link fields so the lattice state
+    // is untouched; timed launches alternate between the scratch fields, so each
+    // reads the links the previous one wrote, as consecutive leapfrog steps do
     void* e2 = nullptr;
     void* u2 = nullptr;
+    void* u3 = nullptr;
     OPX_TRY(hipMalloc(&e2, alg_count(ctx) * ctx->tsz), "autotune alloc");
     int rc = hip_check(ctx, hipMalloc(&u2, link_count(ctx) * ctx->tsz), "autotune alloc");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&u3, link_count(ctx) * ctx->tsz), "autotune alloc");
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (rc == OPX_OK) rc = hip_check(ctx, hipEventCreate(&ev0), "autotune event");
     if (rc == OPX_OK) rc = hip_check(ctx, hipEventCreate(&ev1), "autotune event");
@@ -1036,7 +1057,10 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
             for (int q : ks) dup = dup || q == k;
             if (!dup) ks.push_back(k);
         }
-        for (int k : ks) cands.push_back({OPX_KERNEL_TILE, 0, k});
+        for (int k : ks) {
+            cands.push_back({OPX_KERNEL_TILE, 0, k});
+            cands.push_back({OPX_KERNEL_TILE, 1, k});
+        }
     }
     for (int t = 0; t < opx::kStreamTiles; t++) {
         if (!opx::stream_supported(ctx->nc, ctx->prec, L, t)) continue;
@@ -1063,7 +1087,8 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
             continue;
         }
         rc = hip_check(ctx, hipEventRecord(ev0, ctx->stream), "autotune record");
-        for (int r = 0; r < reps && rc == OPX_OK; r++) rc = hip_check(ctx, launch_update(ctx, L, ctx->U, u2, e2), "autotune launch");
+        for (int r = 0; r < reps && rc == OPX_OK; r++)
+            rc = hip_check(ctx, launch_update(ctx, L, r % 2 ? u3 : u2, r % 2 ? u2 : u3, e2), "autotune launch");
         if (rc == OPX_OK) rc = hip_check(ctx, hipEventRecord(ev1, ctx->stream), "autotune record");
         if (rc == OPX_OK) rc = hip_check(ctx, hipEventSynchronize(ev1), "autotune sync");
         float ms = 0.f;
@@ -1079,6 +1104,7 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
     if (ev1) (void)hipEventDestroy(ev1);
     if (e2) (void)hipFree(e2);
     if (u2) (void)hipFree(u2);
+    if (u3) (void)hipFree(u3);
     if (rc != OPX_OK) {
         ctx->kernel = saved_kernel;
         ctx->tile = saved_tile;
@@ -1233,8 +1259,9 @@ int batch_step(opx_batch* b, int reunit_every)
     b->parity ^= 1;
     void* U = b->parity ? b->UB : b->UA;
     void* Un = b->parity ? b->UA : b->UB;
-    if (b->form == 1)
-        OPX_BTRY(opx::launch_batch_tile(b->prec, b->dev_ev, b->dev_items, b->nitems, b->grid, U, Un, b->E, b->stream),
+    if (b->form >= 1)
+        OPX_BTRY(opx::launch_batch_tile(b->prec, b->form == 1, b->dev_ev, b->dev_items, b->nitems, b->grid, U, Un,
+                                        b->E, b->stream),
                  "batch tile update");
     else
         OPX_BTRY(opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, U, Un, b->E,
@@ -1428,8 +1455,8 @@ int opx_batch_set_option(opx_batch* b, int option, int value)
         if (value < 1) return bfail(b, OPX_E_ARG, "batch option: grid must be >= 1");
         b->grid = (uint32_t)value;
     } else if (option == OPX_BATCH_OPT_FORM) {
-        if (value != 0 && value != 1) return bfail(b, OPX_E_ARG, "batch option: form must be 0 or 1");
-        if (value == 1 && !b->dev_items) return bfail(b, OPX_E_UNSUPPORTED, "batch option: the tile form is SU(2) only");
+        if (value < 0 || value > 2) return bfail(b, OPX_E_ARG, "batch option: form must be 0, 1 or 2");
+        if (value >= 1 && !b->dev_items) return bfail(b, OPX_E_UNSUPPORTED, "batch option: the tile form is SU(2) only");
         b->form = value;
     } else {
         return bfail(b, OPX_E_ARG, "batch option: unknown option");
@@ -1522,8 +1549,10 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
     if (reps < 1) return bfail(b, OPX_E_ARG, "batch autotune: reps must be >= 1");
     void* e2 = nullptr;
     void* u2 = nullptr;
+    void* u3 = nullptr;
     OPX_BTRY(hipMalloc(&e2, b->alg_total * b->tsz), "batch autotune alloc");
     int rc = bcheck(b, hipMalloc(&u2, b->link_total * b->tsz), "batch autotune alloc");
+    if (rc == OPX_OK) rc = bcheck(b, hipMalloc(&u3, b->link_total * b->tsz), "batch autotune alloc");
     if (rc == OPX_OK) rc = bcheck(b, hipMemcpyAsync(e2, b->E, b->alg_total * b->tsz, hipMemcpyDeviceToDevice, b->stream), "batch autotune copy");
     hipEvent_t a0 = nullptr, a1 = nullptr;
     if (rc == OPX_OK) rc = bcheck(b, hipEventCreate(&a0), "batch autotune event");
@@ -1539,19 +1568,22 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
     std::vector<Cand> cands;
     for (int m : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}) cands.push_back({0, (uint32_t)(m * b->num_cus)});
     if (b->dev_items)
-        for (int m : {1, 2, 3, 4, 6, 8}) cands.push_back({1, (uint32_t)(m * b->num_cus)});
-    auto run = [&](const Cand& c) {
-        return c.form == 1 ? opx::launch_batch_tile(b->prec, b->dev_ev, b->dev_items, b->nitems, c.grid, U, u2, e2,
-                                                    b->stream)
-                           : opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, U,
-                                                      u2, e2, b->stream);
+        for (int m : {1, 2, 3, 4, 6, 8})
+            for (int f : {1, 2}) cands.push_back({f, (uint32_t)(m * b->num_cus)});
+    // first launch from the batch's links, timed launches alternating between two scratch fields
+    auto run = [&](const Cand& c, const void* src, void* dst) {
+        return c.form >= 1 ? opx::launch_batch_tile(b->prec, c.form == 1, b->dev_ev, b->dev_items, b->nitems, c.grid,
+                                                    src, dst, e2, b->stream)
+                           : opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, src,
+                                                      dst, e2, b->stream);
     };
     for (const Cand& c : cands) {
         if (rc != OPX_OK) break;
         const uint32_t grid = c.grid;
-        rc = bcheck(b, run(c), "batch autotune");
+        rc = bcheck(b, run(c, U, u2), "batch autotune");
         if (rc == OPX_OK) rc = bcheck(b, hipEventRecord(a0, b->stream), "batch autotune record");
-        for (int r = 0; r < reps && rc == OPX_OK; r++) rc = bcheck(b, run(c), "batch autotune");
+        for (int r = 0; r < reps && rc == OPX_OK; r++)
+            rc = bcheck(b, run(c, r % 2 ? u3 : u2, r % 2 ? u2 : u3), "batch autotune");
         if (rc == OPX_OK) rc = bcheck(b, hipEventRecord(a1, b->stream), "batch autotune record");
         if (rc == OPX_OK) rc = bcheck(b, hipEventSynchronize(a1), "batch autotune sync");
         float ms = 0;
@@ -1566,6 +1598,7 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
     if (a1) (void)hipEventDestroy(a1);
     (void)hipFree(e2);
     if (u2) (void)hipFree(u2);
+    if (u3) (void)hipFree(u3);
     if (rc != OPX_OK) return rc;
     if (bg != b->grid || bf != b->form) {
         b->grid = bg;

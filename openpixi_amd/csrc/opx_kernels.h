@@ -36,6 +36,15 @@ struct BatchEvent {
     uint32_t pad;
 };
 
+// Disjoint cell ranges [start[r], start[r] + prefix[r+1] - prefix[r]) covered by a
+// kernel, passed by value; prefix[count] cells in total.
+constexpr int kMaxCellRanges = 16;
+struct CellRanges {
+    uint32_t count;
+    uint32_t start[kMaxCellRanges];
+    uint32_t prefix[kMaxCellRanges + 1];
+};
+
 // Super-particle descriptors (device arrays of `count` entries).
 struct SuperParticles {
     int count;
@@ -108,10 +117,11 @@ struct TileItem {
 };
 bool tile_supported(int nc);
 TileGeom tile_geom(int prec, const Lattice& L, uint32_t nchunks);
-hipError_t launch_tile_update(int nc, int prec, const Lattice& L, const Box& active, uint32_t nchunks, const void* U,
-                              void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
-hipError_t launch_batch_tile(int prec, const BatchEvent* ev, const TileItem* items, uint32_t nitems, uint32_t grid,
-                             const void* U, void* Unext, void* E, hipStream_t st);
+// nt: non-temporal stores of E and Unext
+hipError_t launch_tile_update(int nc, int prec, const Lattice& L, const Box& active, uint32_t nchunks, bool nt,
+                              const void* U, void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
+hipError_t launch_batch_tile(int prec, bool nt, const BatchEvent* ev, const TileItem* items, uint32_t nitems,
+                             uint32_t grid, const void* U, void* Unext, void* E, hipStream_t st);
 
 // Plaquette form of the fused update: proj P of the three planes of every cell into P
 // (3 * AD * n values), then the per-link update from them.
@@ -145,6 +155,6 @@ hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperPart
                              const void* Unext, void* J, hipStream_t st);
 // gathers rho over cells [lo, hi) only (the cells the particles can occupy at t)
 hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
-                                   const void* q, void* rho, uint32_t lo, uint32_t hi, hipStream_t st);
+                                   const void* q, void* rho, const CellRanges& cr, hipStream_t st);
 
 }  // namespace opx

@@ -903,18 +903,22 @@ __global__ __launch_bounds__(kBlock) void k_sp_to_grid(Lattice L, SuperParticles
 
 // interpolateChargedensity (physics/grid/CGCSuperParticleInterpolationNGP.java:75-93)
 // as a gather: each cell sums the charges that sit on it in particle-list order,
-// the order of the sequential reference loop, with no atomics.
+// the order of the sequential reference loop, with no atomics. Threads cover the
+// cell ranges in `cr` (the union of the particles' current cells) back to back.
 template <int NC, typename T>
 __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, SuperParticles sp, int t,
                                                              const T* __restrict__ q, T* __restrict__ rho,
-                                                             uint32_t lo, uint32_t hi)
+                                                             CellRanges cr)
 {
     using GR = Group<NC, T>;
-    const uint32_t cell = lo + blockIdx.x * kBlock + threadIdx.x;
-    if (cell >= hi) return;
-    typename GR::A r;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= cr.prefix[cr.count]) return;
+    int r = 0;
+    while (r + 1 < (int)cr.count && i >= cr.prefix[r + 1]) r++;
+    const uint32_t cell = cr.start[r] + (i - cr.prefix[r]);
+    typename GR::A a;
 #pragma unroll
-    for (int m = 0; m < GR::AD; m++) aref(r, m) = rho[(size_t)m * L.n + cell];
+    for (int m = 0; m < GR::AD; m++) aref(a, m) = rho[(size_t)m * L.n + cell];
     bool touched = false;
     for (int p = 0; p < sp.count; p++) {
         const int off = sp_ngp_offset(sp.orientation[p], sp.index_offset[p], sp.per_plane[p], sp.shift[p], sp.ppc[p], t);
@@ -922,12 +926,12 @@ __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, SuperPar
         if (j < 0 || j >= sp.num[p]) continue;
         const size_t qi = (size_t)sp.q_offset[p] + j;
 #pragma unroll
-        for (int m = 0; m < GR::AD; m++) aref(r, m) += q[(size_t)m * sp.total + qi];
+        for (int m = 0; m < GR::AD; m++) aref(a, m) += q[(size_t)m * sp.total + qi];
         touched = true;
     }
     if (touched) {
 #pragma unroll
-        for (int m = 0; m < GR::AD; m++) rho[(size_t)m * L.n + cell] = aref(r, m);
+        for (int m = 0; m < GR::AD; m++) rho[(size_t)m * L.n + cell] = aref(a, m);
     }
 }
 
@@ -1092,13 +1096,13 @@ hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperPart
 }
 
 hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
-                                   const void* q, void* rho, uint32_t lo, uint32_t hi, hipStream_t st)
+                                   const void* q, void* rho, const CellRanges& cr, hipStream_t st)
 {
-    if (sp.count == 0 || hi <= lo) return hipSuccess;
-    dim3 g((hi - lo + kBlock - 1) / kBlock);
+    const uint32_t total = cr.prefix[cr.count];
+    if (sp.count == 0 || total == 0) return hipSuccess;
+    dim3 g((total + kBlock - 1) / kBlock);
     OPX_DISPATCH(nc, prec, {
-        hipLaunchKernelGGL((k_sp_chargedensity<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (const T*)q, (T*)rho, lo,
-                           hi);
+        hipLaunchKernelGGL((k_sp_chargedensity<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (const T*)q, (T*)rho, cr);
     });
     return hipGetLastError();
 }

@@ -19,6 +19,16 @@ namespace {
 template <typename T>
 using Q2 = Su2<T>;
 
+// E and Unext are written once per step: non-temporal stores (NT) keep them out of
+// the caches, ordinary ones let a lattice that fits the last-level cache be re-read
+// from it by the next step; the autotuner picks per lattice.
+template <bool NT, typename T>
+__device__ __forceinline__ void st_once(T v, T* p)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 // v mod n for v >= -1 (tiles may be wider than the lattice, so v can exceed n several times)
 __device__ __forceinline__ uint32_t wrap(int v, uint32_t n)
 {
@@ -40,19 +50,19 @@ __device__ __forceinline__ Q2<T> ld_link2(const T* __restrict__ U, uint32_t n, u
 }
 
 // One block's march over planes [pb, pe) of the tile at (row0, col0) of lattice L.
-template <typename T, int TY, int TX, bool HAS_J>
+template <typename T, int TY, int TX, bool HAS_J, bool NT>
 __device__ __forceinline__ void tile_march(const Lattice& L, const Box& active, const T* __restrict__ U,
                                            T* __restrict__ Un, T* __restrict__ E, const T* __restrict__ J,
                                            uint32_t row0, uint32_t col0, uint32_t pb, uint32_t pe,
                                            Q2<T> (*sl)[3][(TY + 2) * (TX + 2)])
 {
     using A = Alg2<T>;
-    constexpr int PW = TX + 2, NS = (TY + 2) * (TX + 2), NT = TY * TX;
+    constexpr int PW = TX + 2, NS = (TY + 2) * (TX + 2), NTHREADS = TY * TX;
     const int tid = threadIdx.x;
     const int ty = tid / TX, tx = tid - ty * TX;
     const uint32_t n = L.n;
     auto load_plane = [&](uint32_t plane, int slot) {
-        for (int i = tid; i < NS; i += NT) {
+        for (int i = tid; i < NS; i += NTHREADS) {
             const int hy = i / PW, hx = i - hy * PW;
             const uint32_t x1 = wrap((int)row0 - 1 + hy, L.d1), x2 = wrap((int)col0 - 1 + hx, L.d2);
             const uint32_t site = plane * L.s0 + x1 * L.s1 + x2;
@@ -106,13 +116,13 @@ __device__ __forceinline__ void tile_march(const Lattice& L, const Box& active, 
                     for (int k = 0; k < 3; k++) e.v[k] += pj[(size_t)k * n] * uf;
                 }
 #pragma unroll
-                for (int k = 0; k < 3; k++) __builtin_nontemporal_store(e.v[k], pe_ + (size_t)k * n);
+                for (int k = 0; k < 3; k++) st_once<NT>(e.v[k], pe_ + (size_t)k * n);
                 const Q2<T> un = mul(exp_link(e, -at), Ud);
                 T* pu = Un + (size_t)d * 4 * n + site;
-                __builtin_nontemporal_store(un.e0, pu);
-                __builtin_nontemporal_store(un.e1, pu + (size_t)n);
-                __builtin_nontemporal_store(un.e2, pu + 2 * (size_t)n);
-                __builtin_nontemporal_store(un.e3, pu + 3 * (size_t)n);
+                st_once<NT>(un.e0, pu);
+                st_once<NT>(un.e1, pu + (size_t)n);
+                st_once<NT>(un.e2, pu + 2 * (size_t)n);
+                st_once<NT>(un.e3, pu + 3 * (size_t)n);
             }
         }
         __syncthreads();   // plane p-1 is read before it is overwritten
@@ -123,7 +133,7 @@ __device__ __forceinline__ void tile_march(const Lattice& L, const Box& active, 
     }
 }
 
-template <typename T, int TY, int TX, bool HAS_J>
+template <typename T, int TY, int TX, bool HAS_J, bool NT>
 __global__ __launch_bounds__(TY* TX) void k_tile_update(Lattice L, Box active, TileGeom tg, const T* __restrict__ U,
                                                         T* __restrict__ Un, T* __restrict__ E,
                                                         const T* __restrict__ J)
@@ -139,12 +149,12 @@ __global__ __launch_bounds__(TY* TX) void k_tile_update(Lattice L, Box active, T
     const uint32_t chunk = rest / tg.tiles1;
     const uint32_t pb = (uint32_t)(((uint64_t)chunk * L.d0) / tg.nchunks);
     const uint32_t pe = (uint32_t)(((uint64_t)(chunk + 1) * L.d0) / tg.nchunks);
-    tile_march<T, TY, TX, HAS_J>(L, active, U, Un, E, J, t1 * TY, t2 * TX, pb, pe, sl);
+    tile_march<T, TY, TX, HAS_J, NT>(L, active, U, Un, E, J, t1 * TY, t2 * TX, pb, pe, sl);
 }
 
 // Persistent form over the (event, tile, chunk) items of a batch: a fixed grid
 // strides through the item table, XCD k owning a contiguous eighth of it.
-template <typename T, int TY, int TX>
+template <typename T, int TY, int TX, bool NT>
 __global__ __launch_bounds__(TY* TX) void k_batch_tile(const BatchEvent* __restrict__ ev, const TileItem* __restrict__ items,
                                                        uint32_t nitems, const T* __restrict__ Ubase,
                                                        T* __restrict__ Unbase, T* __restrict__ Ebase)
@@ -158,7 +168,7 @@ __global__ __launch_bounds__(TY* TX) void k_batch_tile(const BatchEvent* __restr
     for (uint32_t it = i0 + (b >> 3); it < i1; it += wg_on_xcd) {
         const TileItem w = items[it];
         const BatchEvent& e = ev[w.event];
-        tile_march<T, TY, TX, false>(e.L, all, Ubase + e.link_offset, Unbase + e.link_offset, Ebase + e.alg_offset,
+        tile_march<T, TY, TX, false, NT>(e.L, all, Ubase + e.link_offset, Unbase + e.link_offset, Ebase + e.alg_offset,
                                      nullptr, w.row0, w.col0, w.pb, w.pe, sl);
         __syncthreads();
     }
@@ -178,42 +188,57 @@ TileGeom tile_geom(int prec, const Lattice& L, uint32_t nchunks)
     return g;
 }
 
-hipError_t launch_tile_update(int nc, int prec, const Lattice& L, const Box& active, uint32_t nchunks, const void* U,
-                              void* Unext, void* E, const void* J, bool has_j, hipStream_t st)
+namespace {
+
+template <typename T, int TY, bool NT>
+void tile_launch(const Lattice& L, const Box& active, const TileGeom& tg, const void* U, void* Unext, void* E,
+                 const void* J, bool has_j, hipStream_t st)
+{
+    const dim3 grid(tg.tiles1 * tg.tiles2 * tg.nchunks), block(TY * kTileTX);
+    if (has_j)
+        hipLaunchKernelGGL((k_tile_update<T, TY, kTileTX, true, NT>), grid, block, 0, st, L, active, tg, (const T*)U,
+                           (T*)Unext, (T*)E, (const T*)J);
+    else
+        hipLaunchKernelGGL((k_tile_update<T, TY, kTileTX, false, NT>), grid, block, 0, st, L, active, tg, (const T*)U,
+                           (T*)Unext, (T*)E, (const T*)E);
+}
+
+template <typename T, int TY, bool NT>
+void batch_tile_launch(const BatchEvent* ev, const TileItem* items, uint32_t nitems, uint32_t grid, const void* U,
+                       void* Unext, void* E, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_batch_tile<T, TY, kTileTX, NT>), dim3(grid), dim3(TY * kTileTX), 0, st, ev, items, nitems,
+                       (const T*)U, (T*)Unext, (T*)E);
+}
+
+}  // namespace
+
+hipError_t launch_tile_update(int nc, int prec, const Lattice& L, const Box& active, uint32_t nchunks, bool nt,
+                              const void* U, void* Unext, void* E, const void* J, bool has_j, hipStream_t st)
 {
     if (nc != 2) return hipErrorInvalidConfiguration;
     const TileGeom tg = tile_geom(prec, L, nchunks);
-    const dim3 grid(tg.tiles1 * tg.tiles2 * tg.nchunks);
     if (prec == 0) {
-        const dim3 block(kTileTY64 * kTileTX);
-        if (has_j)
-            hipLaunchKernelGGL((k_tile_update<double, kTileTY64, kTileTX, true>), grid, block, 0, st, L, active, tg,
-                               (const double*)U, (double*)Unext, (double*)E, (const double*)J);
-        else
-            hipLaunchKernelGGL((k_tile_update<double, kTileTY64, kTileTX, false>), grid, block, 0, st, L, active, tg,
-                               (const double*)U, (double*)Unext, (double*)E, (const double*)E);
+        if (nt) tile_launch<double, kTileTY64, true>(L, active, tg, U, Unext, E, J, has_j, st);
+        else tile_launch<double, kTileTY64, false>(L, active, tg, U, Unext, E, J, has_j, st);
     } else {
-        const dim3 block(kTileTY32 * kTileTX);
-        if (has_j)
-            hipLaunchKernelGGL((k_tile_update<float, kTileTY32, kTileTX, true>), grid, block, 0, st, L, active, tg,
-                               (const float*)U, (float*)Unext, (float*)E, (const float*)J);
-        else
-            hipLaunchKernelGGL((k_tile_update<float, kTileTY32, kTileTX, false>), grid, block, 0, st, L, active, tg,
-                               (const float*)U, (float*)Unext, (float*)E, (const float*)E);
+        if (nt) tile_launch<float, kTileTY32, true>(L, active, tg, U, Unext, E, J, has_j, st);
+        else tile_launch<float, kTileTY32, false>(L, active, tg, U, Unext, E, J, has_j, st);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_batch_tile(int prec, const BatchEvent* ev, const TileItem* items, uint32_t nitems, uint32_t grid,
-                             const void* U, void* Unext, void* E, hipStream_t st)
+hipError_t launch_batch_tile(int prec, bool nt, const BatchEvent* ev, const TileItem* items, uint32_t nitems,
+                             uint32_t grid, const void* U, void* Unext, void* E, hipStream_t st)
 {
     if (nitems == 0) return hipSuccess;
-    if (prec == 0)
-        hipLaunchKernelGGL((k_batch_tile<double, kTileTY64, kTileTX>), dim3(grid), dim3(kTileTY64 * kTileTX), 0, st, ev,
-                           items, nitems, (const double*)U, (double*)Unext, (double*)E);
-    else
-        hipLaunchKernelGGL((k_batch_tile<float, kTileTY32, kTileTX>), dim3(grid), dim3(kTileTY32 * kTileTX), 0, st, ev,
-                           items, nitems, (const float*)U, (float*)Unext, (float*)E);
+    if (prec == 0) {
+        if (nt) batch_tile_launch<double, kTileTY64, true>(ev, items, nitems, grid, U, Unext, E, st);
+        else batch_tile_launch<double, kTileTY64, false>(ev, items, nitems, grid, U, Unext, E, st);
+    } else {
+        if (nt) batch_tile_launch<float, kTileTY32, true>(ev, items, nitems, grid, U, Unext, E, st);
+        else batch_tile_launch<float, kTileTY32, false>(ev, items, nitems, grid, U, Unext, E, st);
+    }
     return hipGetLastError();
 }
 

@@ -109,13 +109,14 @@ def test_stepping_an_event_view_is_rejected(gpu):
         FastTYMSolver().step(b.event(0), 0.25)
 
 
+@pytest.mark.parametrize("form", ["tile", "tile-cached"])
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_tile_form_matches_oracle(gpu, graph):
+def test_batch_tile_form_matches_oracle(gpu, graph, form):
     """The LDS-tile batch form (SU(2)) against per-event oracle runs."""
     dims_list = [(6, 6, 6), (8, 5, 7), (20, 17, 33), (1, 9, 6), (10, 18, 5)]
     b, states = filled_batch(2, "fp64", dims_list)
-    b.set_form("tile")
-    assert b.form() == "tile"
+    b.set_form(form)
+    assert b.form() == form
     b.step_links(0.25)
     b.leapfrog(12, 0, graph)
     for e, dims in enumerate(dims_list):

@@ -247,14 +247,15 @@ def test_gauss_projection(gpu, nc, precision):
     assert g_end < (1e-20 if precision == "fp64" else 1e-10) * g0
 
 
+@pytest.mark.parametrize("policy", [0, 1], ids=["nontemporal", "cached"])
 @pytest.mark.parametrize("chunks", [None, 1, 3])
 @pytest.mark.parametrize("dims,steps", [((6, 5, 7), 20), ((20, 33, 17), 8), ((3, 40, 70), 6), ((1, 18, 20), 5),
                                         ((32, 32, 32), 10)])
-def test_tile_form_su2_parity(gpu, dims, steps, chunks):
+def test_tile_form_su2_parity(gpu, dims, steps, chunks, policy):
     """SU(2) LDS-tile form: lattices smaller than a tile, not multiples of it, a single
-    x0 plane, and several x0 chunkings, against the oracle (FP64, 1e-12)."""
-    print(f"tile parity dims={dims} steps={steps} chunks={chunks}", flush=True)
-    og, dg = run_both(2, dims, steps, sigma_a=0.3, sigma_e=0.2, seed=12, kernel=("tile", None, chunks))
+    x0 plane, and several x0 chunkings, both store policies, against the oracle (FP64, 1e-12)."""
+    print(f"tile parity dims={dims} steps={steps} chunks={chunks} policy={policy}", flush=True)
+    og, dg = run_both(2, dims, steps, sigma_a=0.3, sigma_e=0.2, seed=12, kernel=("tile", policy, chunks))
     assert rel_err(dg.getU(), og.U) < FP64_TOL
     assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
     assert rel_err(dg.getE(), og.E) < FP64_TOL
