@@ -36,13 +36,13 @@ NC = 3
 DT = 0.25
 REUNIT_EVERY = 10
 # algorithmic HBM bytes per link update of the fused kernel (read U, read E, write E, write Unext)
-BYTES_PER_LINK = {"fp32": 72 + 32 + 32 + 72, "fp64": 144 + 64 + 64 + 144}
+# in the stored formats: FP32 SU(3) links hold their first two rows (12 reals), FP64 links all 18
+BYTES_PER_LINK = {"fp32": 48 + 32 + 32 + 48, "fp64": 144 + 64 + 64 + 144}
 # HBM bytes per launch of the per-link kernel on the 128^3 lattice, counted with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over `bench.py --probe` (KiB per dispatch,
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy). Reported only
 # when the run uses that kernel on that lattice.
-MEASURED_TRAFFIC = {("fp32", "link", DIMS): (356947.2 * 2.0 + 638976.0) * 1024,
-                    ("fp64", "link", DIMS): (794607.9 * 2.0 + 1277952.0) * 1024}
+MEASURED_TRAFFIC = {("fp64", "link", DIMS): (794607.9 * 2.0 + 1277952.0) * 1024}
 
 
 def parse():

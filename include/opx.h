@@ -120,6 +120,11 @@ int opx_reset_charge(opx_ctx *ctx);
  *                     number of x0 chunks (0 = derived from the CU count) and
  *                     OPX_OPT_TILE the store policy of E and Unext (0 non-temporal,
  *                     1 cached);
+ *   OPX_KERNEL_PLANE  a block updates the three links of every cell of a tile of one
+ *                     x0 plane, the plane's links (tile plus a one-cell halo) staged in
+ *                     LDS and the x0 +- 1 neighbours read from global memory;
+ *                     OPX_OPT_TILE the tile shape (0: 8x16, 1: 8x8, 2: 4x16 cells
+ *                     over (x1,x2)), one thread per link;
  *   OPX_KERNEL_AUTO   LINK until opx_autotune picks a form (default).
  * OPX_OPT_TILE selects the STREAM tile shape (0: 16x16, 1: 8x32, 2: 16x32,
  * 3: 8x64, 4: 8x16 threads over (x1,x2)); OPX_OPT_CHUNKS the number of x0
@@ -127,7 +132,8 @@ int opx_reset_charge(opx_ctx *ctx);
  * on scratch copies of E and Unext (the lattice state is not changed), keeps
  * the fastest and returns its time per launch in *best_ms. */
 enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2 };
-enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2, OPX_KERNEL_PLAQ = 3, OPX_KERNEL_TILE = 4 };
+enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2, OPX_KERNEL_PLAQ = 3, OPX_KERNEL_TILE = 4,
+       OPX_KERNEL_PLANE = 5 };
 int opx_set_option(opx_ctx *ctx, int option, int value);
 int opx_get_option(const opx_ctx *ctx, int option, int *value);
 int opx_autotune(opx_ctx *ctx, int reps, double *best_ms);

@@ -58,7 +58,42 @@ opx::Lattice lattice(const opx_ctx* c, double at)
     return L;
 }
 
-size_t link_count(const opx_ctx* c) { return (size_t)c->n * 3 * c->G; }
+size_t link_count(const opx_ctx* c) { return (size_t)c->n * 3 * c->S; }
+
+// Reals stored per link on the device: FP32 SU(3) links keep their first two rows
+// (real parts then imaginary parts of u00..u12); the third row is rebuilt on load.
+int stored_reals(int nc, int prec) { return nc == 2 ? 4 : (prec == OPX_FP64 ? 18 : 12); }
+// element index (reference storage) of stored value k
+int stored_slot(int S, int k) { return S == 12 ? (k < 6 ? k : k + 3) : k; }
+
+// third row of an SU(3) link in reference storage from its first two rows
+void su3_rebuild_third_row(double* u)
+{
+    const double* r = u;
+    const double* im = u + 9;
+    for (int j = 0; j < 3; j++) {
+        const int a = (j + 1) % 3, b = (j + 2) % 3;
+        const double cr = (r[a] * r[3 + b] - im[a] * im[3 + b]) - (r[b] * r[3 + a] - im[b] * im[3 + a]);
+        const double ci = (r[a] * im[3 + b] + im[a] * r[3 + b]) - (r[b] * im[3 + a] + im[b] * r[3 + a]);
+        u[6 + j] = cr;
+        u[9 + 6 + j] = -ci;
+    }
+}
+
+// identity links (physics/grid/Cell.java:63-68) in device SoA layout
+std::vector<double> identity_links(const opx_ctx* c)
+{
+    std::vector<double> soa(link_count(c), 0.0);
+    for (int d = 0; d < 3; d++)
+        for (int k = 0; k < c->S; k++) {
+            const int e = stored_slot(c->S, k);
+            const bool diag = c->nc == 2 ? e == 0 : (e == 0 || e == 4 || e == 8);
+            if (!diag) continue;
+            double* p = soa.data() + ((size_t)d * c->S + k) * c->n;
+            for (int64_t s = 0; s < c->n; s++) p[s] = 1.0;
+        }
+    return soa;
+}
 size_t alg_count(const opx_ctx* c) { return (size_t)c->n * 3 * c->AD; }
 size_t rho_count(const opx_ctx* c) { return (size_t)c->n * c->AD; }
 
@@ -189,6 +224,9 @@ hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void*
                                        c->nchunks > 0 ? (uint32_t)c->nchunks : auto_tile_chunks(c, L), c->tile == 0,
                                        U, Unext, E, c->J, c->j_nonzero, c->stream);
     }
+    if (c->kernel == OPX_KERNEL_PLANE)
+        return opx::launch_plane_update(c->nc, c->prec, L, c->active, c->tile, U, Unext, E, c->J, c->j_nonzero,
+                                        c->stream);
     if (c->kernel == OPX_KERNEL_PLAQ) {
         if (!c->P) {
             hipError_t e = hipMalloc(&c->P, (size_t)c->n * 3 * c->AD * c->tsz);
@@ -239,6 +277,7 @@ int opx_create(int num_colors, int num_dims, const int* num_cells, const double*
     ctx->g = coupling;
     ctx->n = n;
     ctx->G = num_colors == 2 ? 4 : 18;
+    ctx->S = stored_reals(num_colors, precision);
     ctx->AD = num_colors == 2 ? 3 : 8;
     ctx->AS = num_colors == 2 ? 3 : 9;
     ctx->tsz = precision == OPX_FP64 ? 8 : 4;
@@ -264,13 +303,7 @@ int opx_create(int num_colors, int num_dims, const int* num_cells, const double*
     if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&ctx->counters, 4 * sizeof(unsigned long long)), "alloc counters");
     if (rc == OPX_OK) {
         // identities (physics/grid/Cell.java:63-68) and zero fields
-        std::vector<double> soa(link_count(ctx), 0.0);
-        for (int d = 0; d < 3; d++) {
-            for (int k : (num_colors == 2 ? std::vector<int>{0} : std::vector<int>{0, 4, 8})) {
-                double* p = soa.data() + ((size_t)d * ctx->G + k) * n;
-                for (int64_t s = 0; s < n; s++) p[s] = 1.0;
-            }
-        }
+        const std::vector<double> soa = identity_links(ctx);
         rc = to_device(ctx, ctx->U, soa);
         if (rc == OPX_OK) rc = to_device(ctx, ctx->Unext, soa);
     }
@@ -349,8 +382,8 @@ int opx_upload_field(opx_ctx* ctx, int field, const double* host, int64_t n_valu
         soa.resize(link_count(ctx));
         for (int64_t s = 0; s < n; s++)
             for (int d = 0; d < 3; d++)
-                for (int k = 0; k < ctx->G; k++)
-                    soa[((size_t)d * ctx->G + k) * n + s] = host[(s * 3 + d) * ctx->G + k];
+                for (int k = 0; k < ctx->S; k++)
+                    soa[((size_t)d * ctx->S + k) * n + s] = host[(s * 3 + d) * ctx->G + stored_slot(ctx->S, k)];
     } else {
         const int ndir = field == OPX_FIELD_RHO ? 1 : 3;
         if (n_values != n * ndir * ctx->AS) return fail(ctx, OPX_E_ARG, "upload: algebra buffer size mismatch");
@@ -389,8 +422,11 @@ int opx_download_field(opx_ctx* ctx, int field, double* host, int64_t n_values)
         int rc = from_device(ctx, src, soa);
         if (rc != OPX_OK) return rc;
         for (int64_t s = 0; s < n; s++)
-            for (int d = 0; d < 3; d++)
-                for (int k = 0; k < ctx->G; k++) host[(s * 3 + d) * ctx->G + k] = soa[((size_t)d * ctx->G + k) * n + s];
+            for (int d = 0; d < 3; d++) {
+                double* u = host + (s * 3 + d) * ctx->G;
+                for (int k = 0; k < ctx->S; k++) u[stored_slot(ctx->S, k)] = soa[((size_t)d * ctx->S + k) * n + s];
+                if (ctx->S != ctx->G) su3_rebuild_third_row(u);
+            }
         return OPX_OK;
     }
     const int ndir = field == OPX_FIELD_RHO ? 1 : 3;
@@ -983,7 +1019,7 @@ int opx_set_option(opx_ctx* ctx, int option, int value)
     if (!ctx) return OPX_E_ARG;
     switch (option) {
     case OPX_OPT_KERNEL:
-        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_TILE) return fail(ctx, OPX_E_ARG, "option: bad kernel");
+        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_PLANE) return fail(ctx, OPX_E_ARG, "option: bad kernel");
         if (value == OPX_KERNEL_TILE && !opx::tile_supported(ctx->nc))
             return fail(ctx, OPX_E_UNSUPPORTED, "option: the tile form is SU(2) only");
         ctx->kernel = value;
@@ -1062,6 +1098,8 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
             cands.push_back({OPX_KERNEL_TILE, 1, k});
         }
     }
+    for (int t = 0; t < opx::kPlaneShapes; t++)
+        if (opx::plane_supported(ctx->nc, ctx->prec, L, t)) cands.push_back({OPX_KERNEL_PLANE, t, 0});
     for (int t = 0; t < opx::kStreamTiles; t++) {
         if (!opx::stream_supported(ctx->nc, ctx->prec, L, t)) continue;
         const uint32_t bpc = opx::stream_blocks_per_chunk(L, t);
@@ -1196,7 +1234,7 @@ int opx_gauss_project(opx_ctx* ctx, double tol, int max_iter, int* iterations, d
 // ============================================================== batched events
 struct opx_batch {
     int nc = 2, prec = OPX_FP32, count = 0;
-    int G = 4, AD = 3;
+    int G = 4, S = 4, AD = 3;
     size_t tsz = 4;
     void* UA = nullptr;
     void* UB = nullptr;
@@ -1291,6 +1329,7 @@ int opx_batch_create(int num_colors, int count, const int* dims, const double* s
     b->prec = precision;
     b->count = count;
     b->G = num_colors == 2 ? 4 : 18;
+    b->S = stored_reals(num_colors, precision);
     b->AD = num_colors == 2 ? 3 : 8;
     b->tsz = precision == OPX_FP64 ? 8 : 4;
     int rc = OPX_OK;
@@ -1302,6 +1341,7 @@ int opx_batch_create(int num_colors, int count, const int* dims, const double* s
         v->nc = num_colors;
         v->prec = precision;
         v->G = b->G;
+        v->S = b->S;
         v->AD = b->AD;
         v->AS = num_colors == 2 ? 3 : 9;
         v->tsz = b->tsz;
@@ -1323,7 +1363,7 @@ int opx_batch_create(int num_colors, int count, const int* dims, const double* s
         E.item_start = items;
         E.pad = 0;
         b->ev.push_back(E);
-        lo += (uint64_t)n * 3 * b->G;
+        lo += (uint64_t)n * 3 * b->S;
         ao += (uint64_t)n * 3 * b->AD;
         const uint64_t it = 3 * (((uint64_t)n + 255) / 256);
         if ((uint64_t)items + it >= (1ull << 32)) rc = bfail(b, OPX_E_UNSUPPORTED, "batch: too many cells");
@@ -1383,12 +1423,7 @@ int opx_batch_create(int num_colors, int count, const int* dims, const double* s
         // identity links in both buffers (physics/grid/Cell.java:63-68)
         for (int e = 0; e < count && rc == OPX_OK; e++) {
             opx_ctx* v = b->views[e];
-            std::vector<double> soa(link_count(v), 0.0);
-            for (int d = 0; d < 3; d++)
-                for (int k : (num_colors == 2 ? std::vector<int>{0} : std::vector<int>{0, 4, 8})) {
-                    double* p = soa.data() + ((size_t)d * v->G + k) * v->n;
-                    for (int64_t c = 0; c < v->n; c++) p[c] = 1.0;
-                }
+            const std::vector<double> soa = identity_links(v);
             rc = to_device(v, v->U, soa);
             if (rc == OPX_OK) rc = to_device(v, v->Unext, soa);
             if (rc != OPX_OK) b->err = v->err;

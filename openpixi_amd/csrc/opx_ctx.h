@@ -22,7 +22,8 @@ struct opx_ctx {
     double at = 0.0;
     double g = 1.0;
     int64_t n = 0;
-    int G = 4;     // reals per link
+    int G = 4;     // reals per link in reference storage
+    int S = 4;     // reals stored per link on the device (FP32 SU(3): the first two rows, 12)
     int AD = 3;    // algebra coefficients on the device
     int AS = 3;    // algebra reals in reference storage
     size_t tsz = 8;

@@ -43,9 +43,11 @@ template <int NC, typename T>
 __device__ __forceinline__ typename Group<NC, double>::M link_d(const T* __restrict__ U, uint32_t n, uint32_t site, int d)
 {
     using GD = Group<NC, double>;
+    using GR = Group<NC, T>;
     typename GD::M m;
 #pragma unroll
-    for (int k = 0; k < GD::G; k++) GD::gref(m, k) = (double)U[((size_t)d * GD::G + k) * n + site];
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)U[((size_t)d * GR::S + k) * n + site];
+    if constexpr (GR::S != GR::G) su3_third_row(m);   // rebuilt in double
     return m;
 }
 

@@ -455,7 +455,26 @@ OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
     return e;
 }
 
+// Third row of an SU(3) matrix from its first two: u_2j = conj(u_0a u_1b - u_0b u_1a),
+// (j, a, b) cyclic. Exact for det = 1 unitary matrices.
+template <typename T>
+OPX_HD void su3_third_row(Su3<T>& m)
+{
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int a = (j + 1) % 3, b = (j + 2) % 3;
+        const T cr = (m.r[a] * m.r[3 + b] - m.i[a] * m.i[3 + b]) - (m.r[b] * m.r[3 + a] - m.i[b] * m.i[3 + a]);
+        const T ci = (m.r[a] * m.i[3 + b] + m.i[a] * m.r[3 + b]) - (m.r[b] * m.i[3 + a] + m.i[b] * m.r[3 + a]);
+        m.r[6 + j] = cr;
+        m.i[6 + j] = -ci;
+    }
+}
+
 // --------------------------------------------------- group-generic traits
+// G: reals of a group element; S: reals stored per link in device fields, slot(k) the
+// element index of stored value k and complete() what rebuilds the rest after a load.
+// FP32 SU(3) links store their first two rows (12 reals, S = 12); every other form
+// stores the whole element.
 template <int NC, typename T> struct Group;
 
 template <typename T>
@@ -463,7 +482,10 @@ struct Group<2, T> {
     using M = Su2<T>;
     using A = Alg2<T>;
     static constexpr int G = 4;   // reals per group element
+    static constexpr int S = 4;   // reals stored per link
     static constexpr int AD = 3;  // algebra coefficients
+    OPX_HD static constexpr int slot(int k) { return k; }
+    OPX_HD static void complete(M&) {}
     OPX_HD static M zero() { return M{0, 0, 0, 0}; }
     OPX_HD static M identity() { return M{1, 0, 0, 0}; }
     OPX_HD static A azero() { return A{{0, 0, 0}}; }
@@ -476,7 +498,13 @@ struct Group<3, T> {
     using M = Su3<T>;
     using A = Alg3<T>;
     static constexpr int G = 18;
+    static constexpr int S = sizeof(T) == 4 ? 12 : 18;
     static constexpr int AD = 8;
+    OPX_HD static constexpr int slot(int k) { return S == G ? k : (k < 6 ? k : k + 3); }
+    OPX_HD static void complete(M& m)
+    {
+        if constexpr (S != G) su3_third_row(m);
+    }
     OPX_HD static M zero()
     {
         M m;

@@ -123,6 +123,16 @@ hipError_t launch_tile_update(int nc, int prec, const Lattice& L, const Box& act
 hipError_t launch_batch_tile(int prec, bool nt, const BatchEvent* ev, const TileItem* items, uint32_t nitems,
                              uint32_t grid, const void* U, void* Unext, void* E, hipStream_t st);
 
+// Plane-tile form of the fused update (opx_plane.hip): a block updates the three
+// links of a TY x TX tile of one x0 plane, with the plane's links (tile plus halo) in LDS.
+constexpr int kPlaneShapes = 3;
+struct PlaneGeom {
+    uint32_t tiles1, tiles2;
+};
+bool plane_supported(int nc, int prec, const Lattice& L, int shape);
+hipError_t launch_plane_update(int nc, int prec, const Lattice& L, const Box& active, int shape, const void* U,
+                               void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
+
 // Plaquette form of the fused update: proj P of the three planes of every cell into P
 // (3 * AD * n values), then the per-link update from them.
 hipError_t launch_plaq_update(int nc, int prec, const Lattice& L, const Box& active, const void* U, void* P,

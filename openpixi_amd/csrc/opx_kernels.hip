@@ -65,9 +65,10 @@ __device__ __forceinline__ typename Group<NC, T>::M load_link(const T* __restric
 {
     using GR = Group<NC, T>;
     typename GR::M m;
-    const T* p = U + (size_t)dir * GR::G * n + site;
+    const T* p = U + (size_t)dir * GR::S * n + site;
 #pragma unroll
-    for (int k = 0; k < GR::G; k++) GR::gref(m, k) = p[(size_t)k * n];
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[(size_t)k * n];
+    GR::complete(m);
     return m;
 }
 
@@ -76,9 +77,9 @@ __device__ __forceinline__ void store_link(T* __restrict__ U, uint32_t n, uint32
                                            const typename Group<NC, T>::M& m)
 {
     using GR = Group<NC, T>;
-    T* p = U + (size_t)dir * GR::G * n + site;
+    T* p = U + (size_t)dir * GR::S * n + site;
 #pragma unroll
-    for (int k = 0; k < GR::G; k++) p[(size_t)k * n] = GR::gget(m, k);
+    for (int k = 0; k < GR::S; k++) p[(size_t)k * n] = GR::gget(m, GR::slot(k));
 }
 
 template <int NC, typename T>
@@ -138,7 +139,8 @@ __device__ __forceinline__ typename Group<NC, T>::M fload_link(const F& f, uint3
     using GR = Group<NC, T>;
     typename GR::M m;
 #pragma unroll
-    for (int k = 0; k < GR::G; k++) GR::gref(m, k) = f.ld((uint32_t)(dir * GR::G + k), site);
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = f.ld((uint32_t)(dir * GR::S + k), site);
+    GR::complete(m);
     return m;
 }
 template <int NC, typename T, typename F>
@@ -146,7 +148,7 @@ __device__ __forceinline__ void fstore_link(const F& f, uint32_t site, int dir, 
 {
     using GR = Group<NC, T>;
 #pragma unroll
-    for (int k = 0; k < GR::G; k++) f.st((uint32_t)(dir * GR::G + k), site, GR::gget(m, k));
+    for (int k = 0; k < GR::S; k++) f.st((uint32_t)(dir * GR::S + k), site, GR::gget(m, GR::slot(k)));
 }
 template <int NC, typename T, typename F>
 __device__ __forceinline__ typename Group<NC, T>::A fload_alg(const F& f, uint32_t site, int dir)
@@ -178,7 +180,7 @@ __device__ __forceinline__ void fstore_link_once(const F& f, uint32_t site, int 
 {
     using GR = Group<NC, T>;
 #pragma unroll
-    for (int k = 0; k < GR::G; k++) f.st_once((uint32_t)(dir * GR::G + k), site, GR::gget(m, k));
+    for (int k = 0; k < GR::S; k++) f.st_once((uint32_t)(dir * GR::S + k), site, GR::gget(m, GR::slot(k)));
 }
 template <int NC, typename T, typename F>
 __device__ __forceinline__ void fstore_alg(const F& f, uint32_t site, int dir, typename Group<NC, T>::A a)
@@ -248,7 +250,7 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
                                                             const T* __restrict__ Jp)
 {
     using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
-    const Fld U(Up, L.n, 3 * Group<NC, T>::G), Unext(Unextp, L.n, 3 * Group<NC, T>::G);
+    const Fld U(Up, L.n, 3 * Group<NC, T>::S), Unext(Unextp, L.n, 3 * Group<NC, T>::S);
     const Fld E(Ep, L.n, 3 * Group<NC, T>::AD), J(Jp, L.n, 3 * Group<NC, T>::AD);
     // Block order: each x0 plane is cut into row tiles of `rows` rows (x1) by the
     // full x2 extent; logical blocks run direction-fastest, then over the blocks of
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(kBlock, OPX_PLAQ_WAVES) void k_plaquettes(Lattice L
 {
     using GR = Group<NC, T>;
     using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
-    const Fld U(Up, L.n, 3 * GR::G), P(Pp, L.n, 3 * GR::AD);
+    const Fld U(Up, L.n, 3 * GR::S), P(Pp, L.n, 3 * GR::AD);
     uint32_t site;
     if (!tile_site(L, order, xcd_logical(blockIdx.x, gridDim.x), site)) return;
     const Coord c = coords(L, site);
@@ -347,7 +349,7 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_PLAQ_UPDATE_WAVES : O
     using GR = Group<NC, T>;
     using A = typename GR::A;
     using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
-    const Fld U(Up, L.n, 3 * GR::G), Unext(Unextp, L.n, 3 * GR::G);
+    const Fld U(Up, L.n, 3 * GR::S), Unext(Unextp, L.n, 3 * GR::S);
     const Fld P(Pp, L.n, 3 * GR::AD), E(Ep, L.n, 3 * GR::AD), J(Jp, L.n, 3 * GR::AD);
     const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
     const int d = (int)(logical % 3);
@@ -592,24 +594,31 @@ __device__ __forceinline__ void merge(double& s1, double& c1, double s2, double 
 constexpr int kMeasureBlocks = 1024;
 constexpr int kNQ = 8;
 
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, double>::M load_link_d(const T* __restrict__ U, uint32_t n, uint32_t site,
+                                                                     int dir)
+{
+    using GD = Group<NC, double>;
+    using GR = Group<NC, T>;
+    typename GD::M m;
+    const T* p = U + (size_t)dir * GR::S * n + site;
+#pragma unroll
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[(size_t)k * n];
+    if constexpr (GR::S != GR::G) su3_third_row(m);   // rebuilt in double
+    return m;
+}
+
 // plaquette P_{jk}(x) = U_j(x) U_k(x+j) U_j(x+k)^+ U_k(x)^+ (Grid.getPlaquette with o1 = o2 = +1)
 template <int NC, typename T>
 __device__ __forceinline__ typename Group<NC, double>::M plaquette_d(const Lattice& L, const T* __restrict__ U,
                                                                      uint32_t site, const Coord& c, int j, int k)
 {
-    using GD = Group<NC, double>;
-    using GR = Group<NC, T>;
     const uint32_t n = L.n;
     uint32_t xpj = fwd(L, site, coord_of(c, j), j);
     uint32_t xpk = fwd(L, site, coord_of(c, k), k);
-    typename GR::M l[4] = {load_link<NC, T>(U, n, site, j), load_link<NC, T>(U, n, xpj, k),
-                           load_link<NC, T>(U, n, xpk, j), load_link<NC, T>(U, n, site, k)};
-    typename GD::M ld[4];
-#pragma unroll
-    for (int m = 0; m < 4; m++)
-#pragma unroll
-        for (int q = 0; q < GR::G; q++) GD::gref(ld[m], q) = (double)GR::gget(l[m], q);
-    return mul_adj(mul_adj(mul(ld[0], ld[1]), ld[2]), ld[3]);
+    return mul_adj(mul_adj(mul(load_link_d<NC, T>(U, n, site, j), load_link_d<NC, T>(U, n, xpj, k)),
+                           load_link_d<NC, T>(U, n, xpk, j)),
+                   load_link_d<NC, T>(U, n, site, k));
 }
 
 template <int NC, typename T>
@@ -624,17 +633,6 @@ __device__ __forceinline__ typename Group<NC, double>::A load_alg_d(const T* __r
     return a;
 }
 
-template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, double>::M load_link_d(const T* __restrict__ U, uint32_t n, uint32_t site,
-                                                                     int dir)
-{
-    using GD = Group<NC, double>;
-    typename GD::M m;
-    const T* p = U + (size_t)dir * GD::G * n + site;
-#pragma unroll
-    for (int k = 0; k < GD::G; k++) GD::gref(m, k) = (double)p[(size_t)k * n];
-    return m;
-}
 
 // B_i^2 at one time level: proj(P_jk) a_i/(a_j a_k), (j,k) cyclic (Grid.getB, Grid.java:877-900)
 template <int NC, typename T>
@@ -957,7 +955,7 @@ hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box&
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const dim3 g1(3 * order.blocks_per_tile_plane * L.d0 * tiles);
-        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::G * sizeof(T) < (1ull << 32);
+        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
         if (buf && has_j)
             hipLaunchKernelGGL((k_combined_update<NC, T, true, true>), g1, dim3(kBlock), 0, st, L, order, active,
                                (const T*)U, (T*)Unext, (T*)E, (const T*)J);
@@ -983,7 +981,7 @@ hipError_t launch_plaq_update(int nc, int prec, const Lattice& L, const Box& act
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const uint32_t cell_blocks = order.blocks_per_tile_plane * L.d0 * tiles;
-        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::G * sizeof(T) < (1ull << 32);
+        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
         const T* Jt = has_j ? (const T*)J : (const T*)E;
         if (buf) {
             hipLaunchKernelGGL((k_plaquettes<NC, T, true>), dim3(cell_blocks), dim3(kBlock), 0, st, L, order,

@@ -104,9 +104,10 @@ __global__ __launch_bounds__(kPBlock) void k_p_push(Lattice L, uint32_t count, c
         const bool forward = vel[(size_t)d * count + k] > 0;
         const uint32_t cell = cell_of(L, forward ? no : nn);
         typename GR::M U;
-        const T* p = Unext + (size_t)d * GR::G * L.n + cell;
+        const T* p = Unext + (size_t)d * GR::S * L.n + cell;
 #pragma unroll
-        for (int c = 0; c < GR::G; c++) GR::gref(U, c) = p[(size_t)c * L.n];
+        for (int c = 0; c < GR::S; c++) GR::gref(U, GR::slot(c)) = p[(size_t)c * L.n];
+        GR::complete(U);
         if (!forward) U = adj(U);
         store_charge<NC, T>(Q1, count, k, act(q0, adj(U)));
     } else {

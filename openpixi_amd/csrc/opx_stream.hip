@@ -71,8 +71,8 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
 
     const uint32_t n = L.n;
     const uint32_t sz = sizeof(T);
-    const Buf<T> bu(Up, 3u * G * n * sz);
-    const Buf<T> bun(Unp, 3u * G * n * sz);
+    const Buf<T> bu(Up, 3u * GR::S * n * sz);
+    const Buf<T> bun(Unp, 3u * GR::S * n * sz);
     const Buf<T> be(Ep, 3u * AD * n * sz);
 
     // XCD-aware block order: XCD k (= blockIdx % 8) gets a contiguous range of tiles
@@ -115,9 +115,11 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
     auto load_links = [&](uint32_t site, M* u) {
         const uint32_t vo = site * sz;
 #pragma unroll
-        for (int l = 0; l < 3; l++)
+        for (int l = 0; l < 3; l++) {
 #pragma unroll
-            for (int k = 0; k < G; k++) GR::gref(u[l], k) = bu.ld(vo, (uint32_t)(l * G + k) * n * sz);
+            for (int k = 0; k < GR::S; k++) GR::gref(u[l], GR::slot(k)) = bu.ld(vo, (uint32_t)(l * GR::S + k) * n * sz);
+            GR::complete(u[l]);
+        }
     };
     auto lds_link = [&](int s, int l) {
         M m;
@@ -217,7 +219,8 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
                 for (int k = 0; k < AD; k++) be.st(aget(e[d], k), site * sz, (uint32_t)(d * AD + k) * n * sz);
                 const M un = mul(exp_link(e[d], -at), ucur[d]);
 #pragma unroll
-                for (int k = 0; k < G; k++) bun.st(GR::gget(un, k), site * sz, (uint32_t)(d * G + k) * n * sz);
+                for (int k = 0; k < GR::S; k++)
+                    bun.st(GR::gget(un, GR::slot(k)), site * sz, (uint32_t)(d * GR::S + k) * n * sz);
             }
         }
         carry1 = act_adj(a01, ucur[0]);
@@ -274,7 +277,7 @@ hipError_t launch_stream_tile(int nc, int prec, const Lattice& L, const Box& act
 
 bool stream_supported(int nc, int prec, const Lattice& L, int tile)
 {
-    const uint64_t G = nc == 2 ? 4 : 18;
+    const uint64_t G = nc == 2 ? 4 : (prec == 0 ? 18 : 12);   // reals stored per link
     const uint64_t sz = prec == 0 ? 8 : 4;
     if ((uint64_t)L.n * 3 * G * sz >= (1ull << 32)) return false;  // one buffer descriptor per field
     if (tile < 0 || tile >= kStreamTiles) return false;

@@ -291,3 +291,53 @@ def test_tile_form_rejected_for_su3(gpu):
     dg = device_grid(3, (4, 4, 4), U, E)
     with pytest.raises(OpxError):
         dg.setKernel("tile")
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("nc,dims,steps", [(2, (6, 5, 7), 20), (2, (3, 40, 70), 6), (2, (1, 18, 20), 5),
+                                           (3, (4, 5, 7), 12), (3, (12, 20, 17), 6), (3, (2, 9, 35), 4)])
+def test_plane_form_parity(gpu, nc, dims, steps, shape):
+    """Plane-tile form (LDS tile of one x0 plane with halo): lattices smaller than a tile,
+    not multiples of it, one or two x0 planes, every tile shape, against the oracle (FP64)."""
+    print(f"plane parity nc={nc} dims={dims} steps={steps} shape={shape}", flush=True)
+    og, dg = run_both(nc, dims, steps, sigma_a=0.3, sigma_e=0.2, seed=13, kernel=("plane", shape, None))
+    assert rel_err(dg.getU(), og.U) < FP64_TOL
+    assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
+    assert rel_err(dg.getE(), og.E) < FP64_TOL
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_plane_form_current_active_region(gpu, nc):
+    dims = (7, 18, 37)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(8)
+    J = rng.normal(0, 0.1, size=(n, 3, O.asize(nc)))
+    if nc == 3:
+        from openpixi_amd.inputs import su3_coeffs_to_storage
+
+        J = su3_coeffs_to_storage(rng.normal(0, 0.1, size=(n, 3, 8)))
+    og, dg = run_both(nc, dims, 6, J=J, active=((1, 2, 0), (5, 15, 33)), seed=9, kernel=("plane", 0, None))
+    assert rel_err(dg.getU(), og.U) < FP64_TOL
+    assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
+    assert rel_err(dg.getE(), og.E) < FP64_TOL
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_plane_form_fp32_matches_link_form(gpu, nc, shape):
+    """FP32: the plane form does the per-link kernel's arithmetic in the same order, so
+    after 30 steps (reunitarization every 10) it agrees with it to FP32 rounding."""
+    dims = (9, 20, 40)
+    U, E = make_state(nc, dims, 0.1, 0.05, seed=2)
+    from openpixi_amd import _lib
+
+    L = _lib.load()
+    out = []
+    for kernel in (("link", None, None), ("plane", shape, None)):
+        dg = device_grid(nc, dims, U, E, precision="fp32")
+        dg.setKernel(*kernel)
+        FastTYMSolver().stepLinks(dg, 0.25)
+        dg._check(L.opx_leapfrog(dg.handle, 30, 10))
+        out.append((dg.getU(), dg.getE()))
+    assert rel_err(out[1][0], out[0][0]) < 1e-5
+    assert rel_err(out[1][1], out[0][1]) < 1e-4
