@@ -168,12 +168,18 @@ __global__ __launch_bounds__(kB) void k_dot_partial(uint64_t m, const double* __
     }
 }
 
-__global__ void k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out)
+// one wave: lane l merges partials l, l + 64, ... in order, then a fixed shuffle tree
+__global__ __launch_bounds__(64) void k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out)
 {
-    if (threadIdx.x != 0) return;
+    const int lane = threadIdx.x;
     double s = 0.0, c = 0.0;
-    for (int b = 0; b < nb; b++) two_sum_merge(s, c, partial[2 * b], partial[2 * b + 1]);
-    *out = s + c;
+    for (int b = lane; b < nb; b += 64) two_sum_merge(s, c, partial[2 * b], partial[2 * b + 1]);
+    for (int off = 32; off > 0; off >>= 1) {
+        const double s2 = __shfl_down(s, off, 64);
+        const double c2 = __shfl_down(c, off, 64);
+        two_sum_merge(s, c, s2, c2);
+    }
+    if (lane == 0) *out = s + c;
 }
 
 // y += alpha x ; or p = r + beta p

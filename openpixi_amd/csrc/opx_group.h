@@ -77,6 +77,13 @@ OPX_HD void acc(Su2<T>& s, const Su2<T>& a)
     s.e0 += a.e0; s.e1 += a.e1; s.e2 += a.e2; s.e3 += a.e3;
 }
 
+// s += a f
+template <typename T>
+OPX_HD void acc_scaled(Su2<T>& s, const Su2<T>& a, T f)
+{
+    s.e0 += a.e0 * f; s.e1 += a.e1 * f; s.e2 += a.e2 * f; s.e3 += a.e3 * f;
+}
+
 template <typename T>
 OPX_HD Su2<T> scaled(const Su2<T>& a, T f)
 {
@@ -234,6 +241,17 @@ OPX_HD void acc(Su3<T>& s, const Su3<T>& a)
     for (int k = 0; k < 9; k++) {
         s.r[k] += a.r[k];
         s.i[k] += a.i[k];
+    }
+}
+
+// s += a f
+template <typename T>
+OPX_HD void acc_scaled(Su3<T>& s, const Su3<T>& a, T f)
+{
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        s.r[k] += a.r[k] * f;
+        s.i[k] += a.i[k] * f;
     }
 }
 
@@ -412,6 +430,7 @@ OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
     const T rho = fsqrt(trq2);
     T ar = 1, ai = 0, br = 0, bi = 0, cr = 0, ci = 0;
     T f0r = 1, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
+#pragma unroll
     for (int n = 0; n < ExpTerms<T>::n; n++) {
         const T inv = T(1) / T(n + 1);
         // new a = i c k0; new b = i (a + c k1); new c = i b

@@ -226,7 +226,7 @@ __device__ __forceinline__ void link_update(const Lattice& L, const Fld& U, cons
         M u2 = adj_mul(mul(fload_link<NC, T>(U, xmj, d), fload_link<NC, T>(U, xpd_mj, j)),
                        fload_link<NC, T>(U, xmj, j));
         acc(u1, u2);
-        acc(S, scaled(u1, (T)L.area[j]));
+        acc_scaled(S, u1, (T)L.area[j]);
     }
     const M Ud = fload_link<NC, T>(U, site, d);
     const A pr = proj(mul(Ud, S));
@@ -591,7 +591,7 @@ __device__ __forceinline__ void merge(double& s1, double& c1, double s2, double 
     c1 = c1 + c2 - err;
 }
 
-constexpr int kMeasureBlocks = 1024;
+constexpr int kMeasureBlocks = 4096;
 constexpr int kNQ = 8;
 
 template <int NC, typename T>
@@ -651,6 +651,7 @@ __device__ __forceinline__ double gauss2(const Lattice& L, const T* __restrict__
 {
     using GD = Group<NC, double>;
     typename GD::A g = GD::azero();
+#pragma unroll 1
     for (int i = 0; i < 3; i++) {
         uint32_t xm = bwd(L, site, coord_of(c, i), i);
         auto em = load_alg_d<NC, T>(E, L.n, xm, i);
@@ -684,9 +685,17 @@ __global__ __launch_bounds__(kBlock) void k_measure(Lattice L, Box eval, const T
         if (!in_box(eval, c.x0, c.x1, c.x2)) continue;
 #pragma unroll
         for (int i = 0; i < 3; i++) acc[i].add(square(load_alg_d<NC, T>(E, L.n, site, i)) * uf[i]);
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-            acc[3 + i].add(0.5 * (b2<NC, T>(L, U, site, c, i) + b2<NC, T>(L, Unext, site, c, i)) * uf[i]);
+        // one direction at a time: the plaquettes of both time levels are the largest live state
+#pragma unroll 1
+        for (int i = 0; i < 3; i++) {
+            const double v = 0.5 * (b2<NC, T>(L, U, site, c, i) + b2<NC, T>(L, Unext, site, c, i)) * uf[i];
+            if (i == 0)
+                acc[3].add(v);
+            else if (i == 1)
+                acc[4].add(v);
+            else
+                acc[5].add(v);
+        }
         acc[6].add(gauss2<NC, T>(L, U, E, rho, site, c));
         if (rho) acc[7].add(square(load_alg_d<NC, T>(rho, L.n, site, 0)));
     }
@@ -716,14 +725,23 @@ __global__ __launch_bounds__(kBlock) void k_measure(Lattice L, Box eval, const T
     }
 }
 
-// Fixed-order reduction of the per-block partials; writes normalised results.
-__global__ __launch_bounds__(64) void k_measure_final(int nblocks, double inv_n, const double* __restrict__ partials,
-                                                      double* __restrict__ out)
+// Fixed-order reduction of the per-block partials; writes normalised results. Wave q
+// reduces quantity q: lane l merges blocks l, l + 64, ... in order, then a fixed
+// shuffle tree.
+__global__ __launch_bounds__(64 * kNQ) void k_measure_final(int nblocks, double inv_n,
+                                                            const double* __restrict__ partials,
+                                                            double* __restrict__ out)
 {
-    const int q = threadIdx.x;
-    if (q >= kNQ) return;
+    const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double s = 0.0, cc = 0.0;
-    for (int b = 0; b < nblocks; b++) merge(s, cc, partials[((size_t)b * kNQ + q) * 2], partials[((size_t)b * kNQ + q) * 2 + 1]);
+    for (int b = lane; b < nblocks; b += 64)
+        merge(s, cc, partials[((size_t)b * kNQ + q) * 2], partials[((size_t)b * kNQ + q) * 2 + 1]);
+    for (int off = 32; off > 0; off >>= 1) {
+        const double s2 = __shfl_down(s, off, 64);
+        const double c2 = __shfl_down(cc, off, 64);
+        merge(s, cc, s2, c2);
+    }
+    if (lane != 0) return;
     double tot = s + cc;
     if (q == 7)
         out[q] = sqrt(tot) * inv_n;
@@ -1045,7 +1063,7 @@ hipError_t launch_measure(int nc, int prec, const Lattice& L, const Box& eval, c
     });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_measure_final, dim3(1), dim3(64), 0, st, nb, 1.0 / (double)L.n, partials, out);
+    hipLaunchKernelGGL(k_measure_final, dim3(1), dim3(64 * kNQ), 0, st, nb, 1.0 / (double)L.n, partials, out);
     return hipGetLastError();
 }
 

@@ -87,7 +87,7 @@ __device__ __forceinline__ void plane_link(const Lattice& L, const T (*sl)[NS], 
                        link_at(std::integral_constant<int, -EJ0>(), std::integral_constant<int, -EJ1>(),
                                std::integral_constant<int, -EJ2>(), jj));
         acc(u1, u2);
-        acc(St, scaled(u1, (T)L.area[J]));
+        acc_scaled(St, u1, (T)L.area[J]);
     };
     if constexpr (D != 0) staple(std::integral_constant<int, 0>());
     if constexpr (D != 1) staple(std::integral_constant<int, 1>());
