@@ -19,6 +19,8 @@
  *     cell = ((x0*N1)+x1)*N2+x2 (physics/grid/Grid.java:757-770).
  *     The device keeps structure-of-arrays [dir][component][cell] in the
  *     context's precision; SU(3) algebra is kept as 8 Gell-Mann coefficients.
+ *     OPX_FP32 SU(3) contexts store the first two rows of each link (12 reals)
+ *     and rebuild the third as conj(row0 x row1), on the device and on download.
  *   - One host thread per context; calls are ordered on the context's HIP
  *     stream. Entry points that return host data synchronize that stream.
  */
@@ -172,6 +174,12 @@ int opx_measure(opx_ctx *ctx, double *out);
  * longitudinal electric/magnetic energy per slice. */
 int opx_projected_energy_density(opx_ctx *ctx, int direction, double *eT_el, double *eT_mag,
                                  double *eL_el, double *eL_mag);
+/* Longitudinal Poynting vector along `direction` (PoyntingComputation,
+ * diagnostics/methods/ProjectedEnergyDensity.java:178-262): two arrays of
+ * num_cells[direction] doubles, spatially and time averaged and only time averaged,
+ * summed over every cell of each slice. SU(2) only: returns OPX_E_UNSUPPORTED for
+ * SU(3), where the reference's SU3AlgebraElement.mult(AlgebraElement) throws. */
+int opx_projected_poynting(opx_ctx *ctx, int direction, double *averaged, double *time_averaged);
 
 /* ---- particle seam: CGC super particles (light-cone NGP) ----------------
  * Replaces CGCSuperParticleInterpolationNGP (physics/grid/CGCSuperParticleInterpolationNGP.java:19-93)

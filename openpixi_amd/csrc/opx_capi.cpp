@@ -623,6 +623,32 @@ int opx_projected_energy_density(opx_ctx* ctx, int direction, double* eT_el, dou
     return OPX_OK;
 }
 
+int opx_projected_poynting(opx_ctx* ctx, int direction, double* averaged, double* time_averaged)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (direction < 0 || direction > 2) return fail(ctx, OPX_E_ARG, "projected Poynting vector: bad direction");
+    if (!averaged || !time_averaged) return fail(ctx, OPX_E_ARG, "projected Poynting vector: null output");
+    if (ctx->nc != 2)
+        return fail(ctx, OPX_E_UNSUPPORTED, "projected Poynting vector: SU(3) code has not been tested yet.");
+    const int nd = ctx->dims[direction];
+    double* dbuf = nullptr;
+    OPX_TRY(hipMalloc(&dbuf, (size_t)2 * nd * sizeof(double)), "alloc projected Poynting vector");
+    opx::Lattice L = lattice(ctx, ctx->at);
+    int rc = hip_check(ctx, opx::launch_projected_poynting(ctx->nc, ctx->prec, L, direction, ctx->U, ctx->Unext,
+                                                           ctx->E, dbuf, ctx->stream),
+                       "projected Poynting vector");
+    std::vector<double> h((size_t)2 * nd);
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemcpyAsync(h.data(), dbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                       "projected Poynting vector copy");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "projected Poynting vector sync");
+    (void)hipFree(dbuf);
+    if (rc != OPX_OK) return rc;
+    std::memcpy(averaged, h.data(), nd * sizeof(double));
+    std::memcpy(time_averaged, h.data() + nd, nd * sizeof(double));
+    return OPX_OK;
+}
+
 int opx_superparticles_upload(opx_ctx* ctx, int count, const int* orientation, const int* num,
                               const int* index_offset, const int* per_plane, const int* sub_lattice_shift,
                               const int* particles_per_cell, const double* charges, int64_t n_values)

@@ -155,6 +155,9 @@ hipError_t launch_links_from_coeffs(int nc, int prec, uint32_t n, const double* 
 size_t measure_partials_size(uint32_t n);
 hipError_t launch_measure(int nc, int prec, const Lattice& L, const Box& eval, const void* U, const void* Unext,
                           const void* E, const void* rho, double* partials, double* out, hipStream_t st);
+// SU(2) longitudinal Poynting vector per slice: out2[0, nd) averaged, out2[nd, 2 nd) time averaged
+hipError_t launch_projected_poynting(int nc, int prec, const Lattice& L, int dir, const void* U, const void* Unext,
+                                     const void* E, double* out2, hipStream_t st);
 hipError_t launch_projected_energy(int nc, int prec, const Lattice& L, const Box& eval, int dir, const void* U,
                                    const void* Unext, const void* E, double* out4, hipStream_t st);
 hipError_t launch_health(int nc, int prec, uint32_t n, const void* U, double tol,

@@ -809,6 +809,137 @@ __global__ __launch_bounds__(kBlock) void k_projected_energy(Lattice L, Box eval
     }
 }
 
+// ---------------------------------------------------------------- Poynting vector
+// ProjectedEnergyDensity.PoyntingComputation (diagnostics/methods/ProjectedEnergyDensity.java:178-262):
+// the longitudinal Poynting vector per slice along `dir`, spatially and time averaged
+// (out[0, nd)) and only time averaged (out[nd, 2 nd)). Every cell counts: the reference's
+// computation has no evaluation-region test. SU(2) only, as in the reference, whose SU(3)
+// algebra dot product throws.
+__device__ __forceinline__ Coord shifted(const Lattice& L, Coord c, int d, int o)
+{
+    const uint32_t n = dim_of(L, d);
+    uint32_t& v = d == 0 ? c.x0 : (d == 1 ? c.x1 : c.x2);
+    v = o > 0 ? (v + 1 == n ? 0 : v + 1) : (v == 0 ? n - 1 : v - 1);
+    return c;
+}
+__device__ __forceinline__ uint32_t site_of(const Lattice& L, const Coord& c) { return c.x0 * L.s0 + c.x1 * L.s1 + c.x2; }
+
+// Grid.getLink(index, d, o, t) (Grid.java:712-727): U_d(x), or U_d(x - d)^+ for o < 0
+template <typename T>
+__device__ __forceinline__ Su2<double> link_or(const Lattice& L, const T* __restrict__ U, const Coord& c, int d, int o)
+{
+    if (o > 0) return load_link_d<2, T>(U, L.n, site_of(L, c), d);
+    return adj(load_link_d<2, T>(U, L.n, site_of(L, shifted(L, c, d, -1)), d));
+}
+
+// Grid.getPlaquette(index, d1, d2, o1, o2, t) (Grid.java:609-642)
+template <typename T>
+__device__ __forceinline__ Su2<double> plaq_or(const Lattice& L, const T* __restrict__ U, const Coord& c1, int d1,
+                                               int d2, int o1, int o2)
+{
+    const Coord c2 = shifted(L, c1, d1, o1), c3 = shifted(L, c2, d2, o2), c4 = shifted(L, c3, d1, -o1);
+    return mul(mul(mul(link_or<T>(L, U, c1, d1, o1), link_or<T>(L, U, c2, d2, o2)), link_or<T>(L, U, c3, d1, -o1)),
+               link_or<T>(L, U, c4, d2, -o2));
+}
+
+__device__ __forceinline__ double dot(const Alg2<double>& a, const Alg2<double>& b)
+{
+    return a.v[0] * b.v[0] + a.v[1] * b.v[1] + a.v[2] * b.v[2];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_projected_poynting(Lattice L, int dir, const T* __restrict__ U,
+                                                               const T* __restrict__ Unext, const T* __restrict__ E,
+                                                               double* __restrict__ out)
+{
+    const uint32_t slice = blockIdx.x;
+    const uint32_t nd = dim_of(L, dir);
+    const uint32_t per = L.n / nd;
+    const double root_area = sqrt((double)L.n / (double)nd);
+    double ufF[3], ufEB[3];
+    for (int i = 0; i < 3; i++) {
+        ufF[i] = 1.0 / ((L.as[i] * L.as[dir]) * L.g * root_area);
+        ufEB[i] = 1.0 / ((L.g * L.as[i]) * root_area);
+    }
+    const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+    Kahan a[2] = {{0, 0}, {0, 0}};
+    const int da = dir == 0 ? 1 : 0, db = dir == 2 ? 1 : 2;
+    const uint32_t nb = dim_of(L, db);
+    for (uint32_t m = threadIdx.x; m < per; m += kBlock) {
+        uint32_t x[3];
+        x[dir] = slice;
+        x[da] = m / nb;
+        x[db] = m - (m / nb) * nb;
+        const Coord c{x[0], x[1], x[2]};
+        const uint32_t site = site_of(L, c);
+        double s_avg = 0.0;
+#pragma unroll 1
+        for (int i = 0; i < 3; i++) {
+            if (i == dir) continue;
+            Su2<double> fg{0, 0, 0, 0};
+            for (int t = 0; t < 2; t++) {
+                const T* Ut = t == 0 ? U : Unext;
+                acc(fg, plaq_or<T>(L, Ut, c, dir, i, 1, 1));
+                acc(fg, plaq_or<T>(L, Ut, c, i, dir, -1, 1));
+                acc(fg, plaq_or<T>(L, Ut, c, i, dir, 1, -1));
+                acc(fg, plaq_or<T>(L, Ut, c, dir, i, -1, -1));
+            }
+            Alg2<double> F = proj(fg);
+            const double ff = ufF[i] / 8.0;
+            for (int k = 0; k < 3; k++) F.v[k] *= ff;
+            const Coord cm = shifted(L, c, i, -1);
+            const Alg2<double> e2 = act(load_alg_d<2, T>(E, L.n, site_of(L, cm), i), link_or<T>(L, U, c, i, -1));
+            const Alg2<double> e1 = load_alg_d<2, T>(E, L.n, site, i);
+            Alg2<double> e;
+            for (int k = 0; k < 3; k++) e.v[k] = (e1.v[k] + e2.v[k]) * (0.5 * ufEB[i]);
+            s_avg += dot(e, F);
+        }
+        Alg2<double> E1 = load_alg_d<2, T>(E, L.n, site, d1), E2 = load_alg_d<2, T>(E, L.n, site, d2);
+        for (int k = 0; k < 3; k++) {
+            E1.v[k] *= ufEB[d1];
+            E2.v[k] *= ufEB[d2];
+        }
+        // Grid.getB(index, k, t) (Grid.java:877-900)
+        auto bfield = [&](const T* Ut, int k) {
+            const int j = (k + 1) % 3, l = (k + 2) % 3;
+            Alg2<double> b = proj(plaq_or<T>(L, Ut, c, j, l, 1, 1));
+            const double f = L.as[k] / (L.as[j] * L.as[l]);
+            for (int q = 0; q < 3; q++) b.v[q] *= f;
+            return b;
+        };
+        const Alg2<double> b10 = bfield(U, d1), b11 = bfield(Unext, d1);
+        const Alg2<double> b20 = bfield(U, d2), b21 = bfield(Unext, d2);
+        Alg2<double> B1, B2;
+        for (int k = 0; k < 3; k++) {
+            B1.v[k] = (b10.v[k] + b11.v[k]) * (ufEB[d1] * 0.5);
+            B2.v[k] = (b20.v[k] + b21.v[k]) * (ufEB[d2] * 0.5);
+        }
+        a[0].add(s_avg);
+        a[1].add(dot(E1, B2) - dot(E2, B1));
+    }
+    __shared__ double sh[kBlock / 64][2][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q < 2; q++) {
+        double s = a[q].s, cc = -a[q].c;
+        for (int off = 32; off > 0; off >>= 1) {
+            double s2 = __shfl_down(s, off, 64);
+            double c2 = __shfl_down(cc, off, 64);
+            merge(s, cc, s2, c2);
+        }
+        if (lane == 0) {
+            sh[wave][q][0] = s;
+            sh[wave][q][1] = cc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int q = threadIdx.x;
+        double s = sh[0][q][0], cc = sh[0][q][1];
+        for (int w = 1; w < kBlock / 64; w++) merge(s, cc, sh[w][q][0], sh[w][q][1]);
+        out[(size_t)q * nd + slice] = s + cc;
+    }
+}
+
 // ---------------------------------------------------------------- health monitor
 template <int NC, typename T>
 __global__ __launch_bounds__(kBlock) void k_health(uint32_t n, const T* __restrict__ U, double tol,
@@ -1064,6 +1195,20 @@ hipError_t launch_measure(int nc, int prec, const Lattice& L, const Box& eval, c
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_measure_final, dim3(1), dim3(64 * kNQ), 0, st, nb, 1.0 / (double)L.n, partials, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_projected_poynting(int nc, int prec, const Lattice& L, int dir, const void* U, const void* Unext,
+                                     const void* E, double* out2, hipStream_t st)
+{
+    if (nc != 2) return hipErrorInvalidValue;
+    const uint32_t nd = dir == 0 ? L.d0 : (dir == 1 ? L.d1 : L.d2);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_projected_poynting<double>, dim3(nd), dim3(kBlock), 0, st, L, dir, (const double*)U,
+                           (const double*)Unext, (const double*)E, out2);
+    else
+        hipLaunchKernelGGL(k_projected_poynting<float>, dim3(nd), dim3(kBlock), 0, st, L, dir, (const float*)U,
+                           (const float*)Unext, (const float*)E, out2);
     return hipGetLastError();
 }
 

@@ -338,6 +338,14 @@ class FieldMeasurements:
     def calculateTotalChargeSquared(self, grid: Grid):
         return float(self._all(grid)[7])
 
+    def projectedPoynting(self, grid: Grid, direction: int):
+        """Longitudinal Poynting vector per slice, (averaged, time averaged)
+        (diagnostics/methods/ProjectedEnergyDensity.java:178-262). SU(2) only."""
+        n = grid.num_cells[direction]
+        avg, tavg = np.zeros(n), np.zeros(n)
+        grid._check(_lib.load().opx_projected_poynting(grid.handle, direction, _lib.dptr(avg), _lib.dptr(tavg)))
+        return avg, tavg
+
     def projectedEnergyDensity(self, grid: Grid, direction: int):
         n = grid.num_cells[direction]
         arrs = [np.zeros(n) for _ in range(4)]

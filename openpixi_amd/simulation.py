@@ -2,10 +2,12 @@
 FastTYMSolver path.
 
 ``Settings.from_yaml`` accepts the reference's YAML keys
-(ui/util/yaml/YamlSettings.java:14-191) that concern this path and ignores
-output/panel sections. ``Simulation`` keeps the reference's step ordering
+(ui/util/yaml/YamlSettings.java:14-191) that concern this path, including the
+``output`` writers of this path (bulkQuantitiesInTime, projectedEnergyDensity), and
+ignores panel sections. ``Simulation`` keeps the reference's step ordering
 (physics/Simulation.java:318-452): ``initialize()`` once, then ``step()`` while
-``continues()``, which runs ceil(duration/timeStep) + 1 steps like the reference.
+``continues()``, which runs ceil(duration/timeStep) + 1 steps like the reference;
+diagnostics run before the first step and after every step.
 """
 from __future__ import annotations
 
@@ -61,6 +63,7 @@ class Settings:
     active_region: Region = field(default_factory=Region)
     precision: str = "fp64"          # device precision (not a reference key)
     reunitarize_every: int = 0       # FP64 reprojection period (not a reference key)
+    diagnostics: list = field(default_factory=list)   # Settings.addDiagnostics (YamlOutput.applyTo)
 
     @property
     def iterations(self) -> int:
@@ -104,6 +107,10 @@ class Settings:
             if r and r.get("point1") is not None and r.get("point2") is not None and \
                     len(r["point1"]) == s.number_of_dimensions and len(r["point2"]) == s.number_of_dimensions:
                 setattr(s, attr, Region(bool(r.get("enabled", False)), list(r["point1"]), list(r["point2"])))
+        if y.get("output"):
+            from .diagnostics import from_yaml_output
+
+            s.diagnostics = from_yaml_output(y["output"])
         dev = y.get("device") or {}
         if "precision" in dev:
             s.precision = str(dev["precision"])
@@ -146,6 +153,7 @@ class Simulation:
         self.tstep = settings.time_step
         self.iterations = settings.iterations
         self.totalSimulationSteps = 0
+        self._diagnostics_started = False
         self.grid = Grid(settings.number_of_colors, settings.grid_cells, settings.grid_steps, settings.time_step,
                          settings.coupling_constant, settings.precision)
         self.fsolver = FastTYMSolver()
@@ -185,11 +193,31 @@ class Simulation:
         self.grid._check(_lib.load().opx_simulation_initialize(self.grid.handle))
 
     def step(self, n: int = 1):
-        """physics/Simulation.java:373-435 (diagnostics and external currents excluded)."""
-        self.grid._check(_lib.load().opx_simulation_step(self.grid.handle, int(n)))
+        """physics/Simulation.java:373-435 (external currents excluded). Diagnostics
+        are initialized and run before the first step and run after every step."""
+        diags = self.settings.diagnostics
+        if diags and self.totalSimulationSteps == 0 and not self._diagnostics_started:
+            for d in diags:
+                d.initialize(self)
+            self.runDiagnostics()
+            self._diagnostics_started = True
+        if diags or self.settings.reunitarize_every:
+            for _ in range(int(n)):
+                self._one_step(1)
+                self.runDiagnostics()
+        else:
+            self._one_step(int(n))
+
+    def _one_step(self, n: int):
+        self.grid._check(_lib.load().opx_simulation_step(self.grid.handle, n))
         self.totalSimulationSteps = _lib.load().opx_simulation_steps_done(self.grid.handle)
         if self.settings.reunitarize_every and self.totalSimulationSteps % self.settings.reunitarize_every == 0:
             self.grid._check(_lib.load().opx_reunitarize(self.grid.handle))
+
+    def runDiagnostics(self):
+        """physics/Simulation.java:454-460."""
+        for d in self.settings.diagnostics:
+            d.calculate(self.grid, self.totalSimulationSteps)
 
     def continues(self) -> bool:
         """physics/Simulation.java:441-443: steps <= iterations (iterations + 1 steps in total)."""

@@ -122,6 +122,7 @@ def lib():
             "orc_gauss_constraint_squared": (ctypes.c_double, [gp, ctypes.c_int64]),
             "orc_measure": (None, [gp, _dp]),
             "orc_projected_energy_density": (None, [gp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+            "orc_projected_poynting": (ctypes.c_int, [gp, ctypes.c_int, _dp, _dp]),
             "orc_gauge_transform": (None, [gp, _dp]),
             "orc_sp_current_offset": (ctypes.c_int, [ctypes.c_int] * 6),
             "orc_sp_ngp_offset": (ctypes.c_int, [ctypes.c_int] * 6),
@@ -342,6 +343,14 @@ class OracleGrid:
         arrs = [np.zeros(n) for _ in range(4)]
         lib().orc_projected_energy_density(self._g, direction, *[_ptr(a) for a in arrs])
         return arrs  # eT_el, eT_mag, eL_el, eL_mag
+
+    def projected_poynting(self, direction):
+        """Longitudinal Poynting vector per slice (averaged, time averaged); SU(2) only."""
+        n = self.dims[direction]
+        avg, tavg = np.zeros(n), np.zeros(n)
+        if lib().orc_projected_poynting(self._g, direction, _ptr(avg), _ptr(tavg)) != 0:
+            raise RuntimeError("SU(3) code has not been tested yet.")
+        return avg, tavg
 
     def gauge_transform(self, g_cells):
         g_cells = np.ascontiguousarray(g_cells, np.float64).reshape(-1)

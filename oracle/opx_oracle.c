@@ -726,6 +726,77 @@ void orc_projected_energy_density(const orc_grid *gr, int dir, double *eT_el, do
     }
 }
 
+/* diagnostics/methods/ProjectedEnergyDensity.java:178-262 (PoyntingComputation), summed over
+ * every cell in cell order. The longitudinal Poynting vector, spatially and time averaged
+ * (avg) and only time averaged (tavg). SU(2) only: SU3AlgebraElement.mult(AlgebraElement)
+ * throws in the reference (math/SU3AlgebraElement.java:196-201); returns -1 for SU(3). */
+int orc_projected_poynting(const orc_grid *gr, int dir, double *avg, double *tavg)
+{
+    if (gr->nc != 2) return -1;
+    const int A = gr->asz, G = gr->gsz;
+    const int ncell = gr->dims[dir];
+    const double area = (double)gr->n / (double)gr->dims[dir];
+    double ufF[3], ufEB[3];
+    for (int i = 0; i < 3; i++) {
+        ufF[i] = 1.0 / ((gr->as[i] * gr->as[dir]) * gr->g * sqrt(area));
+        ufEB[i] = 1.0 / ((gr->g * gr->as[i]) * sqrt(area));
+    }
+    for (int k = 0; k < ncell; k++) avg[k] = tavg[k] = 0.0;
+    for (int64_t c = 0; c < gr->n; c++) {
+        double local_avg = 0.0;
+        for (int i = 0; i < 3; i++) {
+            if (i == dir) continue;
+            /* plaquettes around the (dir, i) face, both time levels */
+            double FG[4] = {0, 0, 0, 0}, P[4];
+            for (int t = 0; t < 2; t++) {
+                orc_plaquette(gr, c, dir, i, 1, 1, t, P);
+                for (int m = 0; m < G; m++) FG[m] += P[m];
+                orc_plaquette(gr, c, i, dir, -1, 1, t, P);
+                for (int m = 0; m < G; m++) FG[m] += P[m];
+                orc_plaquette(gr, c, i, dir, 1, -1, t, P);
+                for (int m = 0; m < G; m++) FG[m] += P[m];
+                orc_plaquette(gr, c, dir, i, -1, -1, t, P);
+                for (int m = 0; m < G; m++) FG[m] += P[m];
+            }
+            double F[3];
+            orc_group_proj(2, FG, F);
+            for (int m = 0; m < A; m++) F[m] *= ufF[i] / 8.0;
+            /* E_i averaged over x and x - i (transported to x) */
+            int64_t sh = orc_shift(gr, c, i, -1);
+            double L[4], E2[3], E[3];
+            get_link(gr, c, i, -1, 0, L);
+            orc_alg_act(2, GA(gr, gr->E, sh, i), L, E2);
+            const double *E1 = GA(gr, gr->E, c, i);
+            for (int m = 0; m < A; m++) E[m] = (E1[m] + E2[m]) * (0.5 * ufEB[i]);
+            double dot = 0.0;
+            for (int m = 0; m < A; m++) dot += E[m] * F[m];
+            local_avg += dot;
+        }
+        const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+        double E1[3], E2[3], B1[3], B2[3], b0[3], b1[3];
+        for (int m = 0; m < A; m++) {
+            E1[m] = GA(gr, gr->E, c, d1)[m] * ufEB[d1];
+            E2[m] = GA(gr, gr->E, c, d2)[m] * ufEB[d2];
+        }
+        orc_get_B(gr, c, d1, 0, b0);
+        orc_get_B(gr, c, d1, 1, b1);
+        for (int m = 0; m < A; m++) B1[m] = (b0[m] + b1[m]) * (ufEB[d1] * 0.5);
+        orc_get_B(gr, c, d2, 0, b0);
+        orc_get_B(gr, c, d2, 1, b1);
+        for (int m = 0; m < A; m++) B2[m] = (b0[m] + b1[m]) * (ufEB[d2] * 0.5);
+        double e1b2 = 0.0, e2b1 = 0.0;
+        for (int m = 0; m < A; m++) {
+            e1b2 += E1[m] * B2[m];
+            e2b1 += E2[m] * B1[m];
+        }
+        int p[3];
+        cell_pos(gr, c, p);
+        avg[p[dir]] += local_avg;
+        tavg[p[dir]] += e1b2 - e2b1;
+    }
+    return 0;
+}
+
 /* physics/gauge/GaugeTransformation.java:56-86: U -> g(x) U g(x+i)^dagger on U and Unext,
  * E -> proj(g(x) linearized(E) g(x)^dagger). g_per_cell holds one group element per cell. */
 void orc_gauge_transform(orc_grid *gr, const double *gpc)

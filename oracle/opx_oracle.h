@@ -87,6 +87,7 @@ void orc_measure(const orc_grid *gr, double *out);
 /* ProjectedEnergyDensity per longitudinal slice: out arrays of length dims[dir] */
 void orc_projected_energy_density(const orc_grid *gr, int dir, double *eT_el, double *eT_mag,
                                   double *eL_el, double *eL_mag);
+int orc_projected_poynting(const orc_grid *gr, int dir, double *avg, double *tavg);
 void orc_gauge_transform(orc_grid *gr, const double *g_per_cell);
 
 /* ---- colored particles (CGC super particles, light-cone NGP) ---- */
