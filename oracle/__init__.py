@@ -123,6 +123,9 @@ def lib():
             "orc_measure": (None, [gp, _dp]),
             "orc_projected_energy_density": (None, [gp, ctypes.c_int, _dp, _dp, _dp, _dp]),
             "orc_projected_poynting": (ctypes.c_int, [gp, ctypes.c_int, _dp, _dp]),
+            "orc_implicit_step": (None, [gp, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
+            "orc_group_log": (None, [ctypes.c_int, _dp, _dp]),
+            "orc_group_pow": (None, [ctypes.c_int, _dp, ctypes.c_double, _dp]),
             "orc_gauge_transform": (None, [gp, _dp]),
             "orc_sp_current_offset": (ctypes.c_int, [ctypes.c_int] * 6),
             "orc_sp_ngp_offset": (ctypes.c_int, [ctypes.c_int] * 6),
@@ -189,6 +192,20 @@ def alg_get_link(nc, v):
 def alg_act(nc, v, g):
     o = np.zeros(asize(nc))
     lib().orc_alg_act(nc, _ptr(np.ascontiguousarray(v, np.float64)), _ptr(np.ascontiguousarray(g, np.float64)), _ptr(o))
+    return o
+
+
+def group_log(nc, u):
+    """GroupElement.getAlgebraElement: H (reference algebra storage) with U = getLink(H)."""
+    o = np.zeros(asize(nc))
+    lib().orc_group_log(nc, _ptr(np.ascontiguousarray(u, np.float64)), _ptr(o))
+    return o
+
+
+def group_pow(nc, u, x):
+    """GroupElement.pow(x) = getLink(x getAlgebraElement())."""
+    o = np.zeros(gsize(nc))
+    lib().orc_group_pow(nc, _ptr(np.ascontiguousarray(u, np.float64)), float(x), _ptr(o))
     return o
 
 
@@ -343,6 +360,10 @@ class OracleGrid:
         arrs = [np.zeros(n) for _ in range(4)]
         lib().orc_projected_energy_density(self._g, direction, *[_ptr(a) for a in arrs])
         return arrs  # eT_el, eT_mag, eL_el, eL_mag
+
+    def implicit_step(self, at=None, iterations=0, damping=0.0):
+        """ImplicitTYMSolver.step (physics/fields/ImplicitTYMSolver.java:35-58)."""
+        lib().orc_implicit_step(self._g, self.at if at is None else at, int(iterations), float(damping))
 
     def projected_poynting(self, direction):
         """Longitudinal Poynting vector per slice (averaged, time averaged); SU(2) only."""

@@ -73,6 +73,10 @@ void orc_set_evaluation_region(orc_grid *gr, const int lo[3], const int hi[3]);
 void orc_staple_sum(const orc_grid *gr, int64_t index, int d, double *S_out);
 void orc_step(orc_grid *gr, double at);         /* CombinedUpdate over all cells */
 void orc_step_links(orc_grid *gr, double at);   /* UpdateLinks over all cells */
+/* ImplicitTYMSolver.step (opx_oracle_implicit.c) and the group log / power it uses */
+void orc_implicit_step(orc_grid *gr, double at, int iterations, double damping);
+void orc_group_log(int nc, const double *u, double *v);
+void orc_group_pow(int nc, const double *u, double x, double *o);
 void orc_store_fields(orc_grid *gr);            /* U <-> Unext */
 void orc_reset_current(orc_grid *gr);
 void orc_reset_charge(orc_grid *gr);
