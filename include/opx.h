@@ -105,6 +105,23 @@ int opx_set_evaluation_region(opx_ctx *ctx, const int *lo, const int *hi);
  * opx_reset_current / opx_reset_charge: Grid.resetCurrent / resetCharge (:570-586) */
 int opx_step(opx_ctx *ctx, double dt);
 int opx_step_links(opx_ctx *ctx, double dt);
+/* Field solver used by opx_step, opx_leapfrog and opx_simulation_step:
+ *   OPX_SOLVER_FAST      FastTYMSolver (default);
+ *   OPX_SOLVER_IMPLICIT  ImplicitTYMSolver(iterations, damping)
+ *                        (physics/fields/ImplicitTYMSolver.java:18-58), the solver of the
+ *                        "temporal implicit cgc ngp" and "temporal optimized implicit cgc ngp"
+ *                        simulation types (physics/Settings.java:560-577): the explicit update,
+ *                        then `iterations` passes that rebuild E from plaquettes averaged over
+ *                        the past and future links (implicit in the transverse plane and along
+ *                        the beam direction x0), each followed by the damping
+ *                        Unext = U^damping Unext^(1 - damping) (GroupElement.pow). */
+enum { OPX_SOLVER_FAST = 0, OPX_SOLVER_IMPLICIT = 1 };
+int opx_set_field_solver(opx_ctx *ctx, int solver, int iterations, double damping);
+/* GroupElement.pow(x) = getLink(x getAlgebraElement()) (math/SU2GroupElement.java:234-271,
+ * math/SU3GroupElement.java:239-520) on the device for `count` elements in reference storage
+ * (4 or 18 reals each), rounded to the given precision and raised in double precision, as the
+ * damping of the implicit solver evaluates it in every precision. */
+int opx_group_pow(int num_colors, int precision, int count, const double *elements, double x, double *out);
 int opx_store_fields(opx_ctx *ctx);
 int opx_reset_current(opx_ctx *ctx);
 int opx_reset_charge(opx_ctx *ctx);

@@ -2,15 +2,15 @@
 
 The product is the shared library ``libopx.so`` (C ABI in ``include/opx.h``)
 built from ``csrc/`` by ``python -m openpixi_amd.build``. This package mirrors
-OpenPixi's Grid / FastTYMSolver / FieldMeasurements / Simulation seams over
+OpenPixi's Grid / FastTYMSolver / ImplicitTYMSolver / FieldMeasurements / Simulation seams over
 that ABI. Importing the package does not require the library; creating a Grid
 does, and fails loudly when it is missing.
 """
 from ._lib import LIB_PATH, OpxError, OpxLibraryError, load  # noqa: F401
-from .grid import FastTYMSolver, FieldMeasurements, FieldSolver, Grid  # noqa: F401
+from .grid import FastTYMSolver, FieldMeasurements, FieldSolver, Grid, ImplicitTYMSolver  # noqa: F401
 from .simulation import ParticleSet, Settings, Simulation, SuperParticleSet  # noqa: F401
 
 __all__ = [
-    "Grid", "FieldSolver", "FastTYMSolver", "FieldMeasurements", "Settings", "Simulation", "SuperParticleSet",
+    "Grid", "FieldSolver", "FastTYMSolver", "ImplicitTYMSolver", "FieldMeasurements", "Settings", "Simulation", "SuperParticleSet",
     "OpxError", "OpxLibraryError", "load", "LIB_PATH",
 ]

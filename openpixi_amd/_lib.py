@@ -27,6 +27,7 @@ FIELD_U, FIELD_UNEXT, FIELD_E, FIELD_J, FIELD_RHO = range(5)
 
 OPT_KERNEL, OPT_TILE, OPT_CHUNKS = range(3)
 KERNEL_AUTO, KERNEL_LINK, KERNEL_STREAM, KERNEL_PLAQ, KERNEL_TILE, KERNEL_PLANE = range(6)
+SOLVER_FAST, SOLVER_IMPLICIT = 0, 1
 BATCH_OPT_GRID, BATCH_OPT_FORM = range(2)
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -51,6 +52,8 @@ SIGNATURES = {
     "opx_set_evaluation_region": (ctypes.c_int, [_vp, _ip, _ip]),
     "opx_step": (ctypes.c_int, [_vp, ctypes.c_double]),
     "opx_step_links": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "opx_set_field_solver": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    "opx_group_pow": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, _dp]),
     "opx_store_fields": (ctypes.c_int, [_vp]),
     "opx_reset_current": (ctypes.c_int, [_vp]),
     "opx_reset_charge": (ctypes.c_int, [_vp]),

@@ -237,6 +237,40 @@ hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void*
     return opx::launch_combined_update(c->nc, c->prec, L, c->active, U, Unext, E, c->J, c->j_nonzero, c->stream);
 }
 
+// ImplicitTYMSolver.step (physics/fields/ImplicitTYMSolver.java:35-58): the implicit grid is
+// a copy of E (and of U and Unext where an active region leaves cells untouched); ImplicitBegin
+// is the fused explicit update into the copies; each iteration swaps the implicit links and
+// recomputes the active ones (ImplicitStep + ImplicitDamping); ImplicitEnd copies E and Unext
+// of the active links back.
+hipError_t implicit_step(opx_ctx* c, const opx::Lattice& L)
+{
+    const size_t lb = link_count(c) * c->tsz, ab = alg_count(c) * c->tsz;
+    hipError_t e = hipSuccess;
+    if (!c->iU && (e = hipMalloc(&c->iU, lb)) != hipSuccess) return e;
+    if (!c->iUn && (e = hipMalloc(&c->iUn, lb)) != hipSuccess) return e;
+    if (!c->iE && (e = hipMalloc(&c->iE, ab)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(c->iE, c->E, ab, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    if (!c->active.all) {
+        if ((e = hipMemcpyAsync(c->iU, c->U, lb, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(c->iUn, c->Unext, lb, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    }
+    if ((e = launch_update(c, L, c->U, c->iUn, c->iE)) != hipSuccess) return e;
+    for (int it = 0; it < c->implicit_iterations; it++) {
+        std::swap(c->iU, c->iUn);
+        e = opx::launch_implicit_iteration(c->nc, c->prec, L, c->active, c->U, c->Unext, c->iU, c->E, c->J,
+                                           c->j_nonzero, c->iE, c->iUn, c->implicit_damping, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    return opx::launch_implicit_end(c->nc, c->prec, L, c->active, c->iE, c->iUn, c->E, c->Unext, c->stream);
+}
+
+// FieldSolver.step with the context's field solver
+hipError_t field_step(opx_ctx* c, const opx::Lattice& L)
+{
+    if (c->solver == OPX_SOLVER_IMPLICIT) return implicit_step(c, L);
+    return launch_update(c, L, c->U, c->Unext, c->E);
+}
+
 }  // namespace
 
 namespace opx_host {
@@ -334,7 +368,8 @@ void opx_destroy(opx_ctx* ctx)
         return;
     }
     opx_host::adjoint_release(ctx);
-    void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->P, ctx->partials, ctx->dout, ctx->counters,
+    void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->P, ctx->iU, ctx->iUn, ctx->iE,
+                    ctx->partials, ctx->dout, ctx->counters,
                     ctx->sp_desc, ctx->sp_qoff, ctx->sp_q, ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel,
                     ctx->p_Q0, ctx->p_Q1};
     for (void* b : bufs)
@@ -516,7 +551,7 @@ int opx_step(opx_ctx* ctx, double dt)
     if (!ctx) return OPX_E_ARG;
     if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "step the batch, not one of its events");
     opx::Lattice L = lattice(ctx, dt);
-    OPX_TRY(launch_update(ctx, L, ctx->U, ctx->Unext, ctx->E), "step");
+    OPX_TRY(field_step(ctx, L), "step");
     return OPX_OK;
 }
 
@@ -527,6 +562,49 @@ int opx_step_links(opx_ctx* ctx, double dt)
     opx::Lattice L = lattice(ctx, dt);
     OPX_TRY(opx::launch_step_links(ctx->nc, ctx->prec, L, ctx->active, ctx->U, ctx->Unext, ctx->E, ctx->stream),
             "step_links");
+    return OPX_OK;
+}
+
+int opx_set_field_solver(opx_ctx* ctx, int solver, int iterations, double damping)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "the events of a batch use the explicit solver");
+    if (solver != OPX_SOLVER_FAST && solver != OPX_SOLVER_IMPLICIT) return fail(ctx, OPX_E_ARG, "field solver: unknown");
+    if (iterations < 0) return fail(ctx, OPX_E_ARG, "field solver: negative iteration count");
+    if (!std::isfinite(damping)) return fail(ctx, OPX_E_ARG, "field solver: damping must be finite");
+    ctx->solver = solver;
+    ctx->implicit_iterations = iterations;
+    ctx->implicit_damping = damping;
+    return OPX_OK;
+}
+
+int opx_group_pow(int num_colors, int precision, int count, const double* elements, double x, double* out)
+{
+    if ((num_colors != 2 && num_colors != 3) || (precision != OPX_FP64 && precision != OPX_FP32))
+        return OPX_E_UNSUPPORTED;
+    if (count < 0 || (count > 0 && (!elements || !out))) return OPX_E_ARG;
+    if (count == 0) return OPX_OK;
+    const size_t m = (size_t)count * (num_colors == 2 ? 4 : 18);
+    const bool f64 = precision == OPX_FP64;
+    const size_t tsz = f64 ? 8 : 4;
+    std::vector<double> hd(f64 ? m : 0);
+    std::vector<float> hf(f64 ? 0 : m);
+    for (size_t k = 0; k < m; k++) {
+        if (f64) hd[k] = elements[k];
+        else hf[k] = (float)elements[k];
+    }
+    void* host = f64 ? (void*)hd.data() : (void*)hf.data();
+    void *din = nullptr, *dout = nullptr;
+    int rc = OPX_OK;
+    if (hipMalloc(&din, m * tsz) != hipSuccess || hipMalloc(&dout, m * tsz) != hipSuccess) rc = OPX_E_DEVICE;
+    if (rc == OPX_OK && hipMemcpy(din, host, m * tsz, hipMemcpyHostToDevice) != hipSuccess) rc = OPX_E_DEVICE;
+    if (rc == OPX_OK && opx::launch_group_pow(num_colors, f64 ? 0 : 1, count, din, x, dout, nullptr) != hipSuccess)
+        rc = OPX_E_DEVICE;
+    if (rc == OPX_OK && hipMemcpy(host, dout, m * tsz, hipMemcpyDeviceToHost) != hipSuccess) rc = OPX_E_DEVICE;
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (rc != OPX_OK) return rc;
+    for (size_t k = 0; k < m; k++) out[k] = f64 ? hd[k] : (double)hf[k];
     return OPX_OK;
 }
 
@@ -571,7 +649,7 @@ int opx_leapfrog(opx_ctx* ctx, int steps, int reunit_every)
     opx::Lattice L = lattice(ctx, ctx->at);
     for (int s = 0; s < steps; s++) {
         std::swap(ctx->U, ctx->Unext);
-        OPX_TRY(launch_update(ctx, L, ctx->U, ctx->Unext, ctx->E), "leapfrog step");
+        OPX_TRY(field_step(ctx, L), "leapfrog step");
         ctx->steps++;
         if (reunit_every > 0 && ctx->steps % reunit_every == 0)
             OPX_TRY(opx::launch_reunitarize(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->Unext, ctx->stream),

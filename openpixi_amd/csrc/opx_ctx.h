@@ -33,6 +33,14 @@ struct opx_ctx {
     void* J = nullptr;
     void* rho = nullptr;
     void* P = nullptr;              // proj of the three plaquettes per cell (plaquette form scratch)
+    // field solver: OPX_SOLVER_FAST or OPX_SOLVER_IMPLICIT with its iterations and damping;
+    // the implicit solver's copies of U, Unext and E (allocated on first use)
+    int solver = 0;
+    int implicit_iterations = 0;
+    double implicit_damping = 0.0;
+    void* iU = nullptr;
+    void* iUn = nullptr;
+    void* iE = nullptr;
     bool j_nonzero = false;
     opx::Box active{{0, 0, 0}, {0, 0, 0}, 1};
     opx::Box eval{{0, 0, 0}, {0, 0, 0}, 1};

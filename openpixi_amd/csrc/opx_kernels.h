@@ -155,6 +155,17 @@ hipError_t launch_links_from_coeffs(int nc, int prec, uint32_t n, const double* 
 size_t measure_partials_size(uint32_t n);
 hipError_t launch_measure(int nc, int prec, const Lattice& L, const Box& eval, const void* U, const void* Unext,
                           const void* E, const void* rho, double* partials, double* out, hipStream_t st);
+// ImplicitTYMSolver (opx_implicit.hip): one iteration (ImplicitStep + ImplicitDamping) from the
+// links U (time 0), Uold (time -1) and the implicit estimate iU (time +1) into iE, iUn; and
+// ImplicitEnd (E, Unext of the active links from iE, iUn).
+hipError_t launch_implicit_iteration(int nc, int prec, const Lattice& L, const Box& active, const void* U,
+                                     const void* Uold, const void* iU, const void* E, const void* J, bool has_j,
+                                     void* iE, void* iUn, double damping, hipStream_t st);
+hipError_t launch_implicit_end(int nc, int prec, const Lattice& L, const Box& active, const void* iE, const void* iUn,
+                               void* E, void* Unext, hipStream_t st);
+// GroupElement.pow(x) of `count` elements (reference storage, G reals each, precision prec)
+hipError_t launch_group_pow(int nc, int prec, int count, const void* U, double x, void* out, hipStream_t st);
+
 // SU(2) longitudinal Poynting vector per slice: out2[0, nd) averaged, out2[nd, 2 nd) time averaged
 hipError_t launch_projected_poynting(int nc, int prec, const Lattice& L, int dir, const void* U, const void* Unext,
                                      const void* E, double* out2, hipStream_t st);

@@ -439,59 +439,6 @@ __global__ __launch_bounds__(kBlock) void k_step_links(Lattice L, Box active, co
 }
 
 // ---------------------------------------------------------------- kernel 3
-// Reprojection onto the group in double precision.
-__device__ __forceinline__ Su2<double> reunit(Su2<double> u)
-{
-    double nrm = sqrt(u.e0 * u.e0 + u.e1 * u.e1 + u.e2 * u.e2 + u.e3 * u.e3);
-    double f = 1.0 / nrm;
-    return Su2<double>{u.e0 * f, u.e1 * f, u.e2 * f, u.e3 * f};
-}
-
-__device__ __forceinline__ Su3<double> reunit(Su3<double> u)
-{
-    // row 0
-    double n0 = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) n0 += u.r[k] * u.r[k] + u.i[k] * u.i[k];
-    double f0 = 1.0 / sqrt(n0);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        u.r[k] *= f0;
-        u.i[k] *= f0;
-    }
-    // row 1 -= <row0, row1> row0
-    double pr = 0, pi = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        // conj(r0_k) * r1_k
-        pr += u.r[k] * u.r[3 + k] + u.i[k] * u.i[3 + k];
-        pi += u.r[k] * u.i[3 + k] - u.i[k] * u.r[3 + k];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        u.r[3 + k] -= pr * u.r[k] - pi * u.i[k];
-        u.i[3 + k] -= pr * u.i[k] + pi * u.r[k];
-    }
-    double n1 = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) n1 += u.r[3 + k] * u.r[3 + k] + u.i[3 + k] * u.i[3 + k];
-    double f1 = 1.0 / sqrt(n1);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        u.r[3 + k] *= f1;
-        u.i[3 + k] *= f1;
-    }
-    // row 2 = conj(row0 x row1)
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        int a = (k + 1) % 3, b = (k + 2) % 3;
-        double cr = (u.r[a] * u.r[3 + b] - u.i[a] * u.i[3 + b]) - (u.r[b] * u.r[3 + a] - u.i[b] * u.i[3 + a]);
-        double ci = (u.r[a] * u.i[3 + b] + u.i[a] * u.r[3 + b]) - (u.r[b] * u.i[3 + a] + u.i[b] * u.r[3 + a]);
-        u.r[6 + k] = cr;
-        u.i[6 + k] = -ci;
-    }
-    return u;
-}
 
 template <int NC, typename T>
 __global__ __launch_bounds__(kBlock) void k_reunitarize(uint32_t n, T* __restrict__ U)

@@ -306,7 +306,31 @@ class FastTYMSolver(FieldSolver):
     """physics/fields/FastTYMSolver.java:7-85 on the gfx950 kernels."""
 
     def step(self, grid: Grid, timeStep: float):
-        grid._check(_lib.load().opx_step(grid.handle, float(timeStep)))
+        L = _lib.load()
+        grid._check(L.opx_set_field_solver(grid.handle, _lib.SOLVER_FAST, 0, 0.0))
+        grid._check(L.opx_step(grid.handle, float(timeStep)))
+
+    def stepLinks(self, grid: Grid, timeStep: float):
+        grid._check(_lib.load().opx_step_links(grid.handle, float(timeStep)))
+
+
+class ImplicitTYMSolver(FieldSolver):
+    """physics/fields/ImplicitTYMSolver.java:7-263 on the gfx950 kernels: the explicit update,
+    then `implicitIterations` passes with plaquettes averaged over the past and future links
+    (beam direction x0), each damped by Unext = U^damping Unext^(1 - damping)."""
+
+    def __init__(self, implicitIterations=0, implicitDampingFactor=0.0):
+        self.implicitIterations = int(implicitIterations)
+        self.implicitDampingFactor = float(implicitDampingFactor)
+
+    def clone(self):
+        return ImplicitTYMSolver(self.implicitIterations, self.implicitDampingFactor)
+
+    def step(self, grid: Grid, timeStep: float):
+        L = _lib.load()
+        grid._check(L.opx_set_field_solver(grid.handle, _lib.SOLVER_IMPLICIT, self.implicitIterations,
+                                           self.implicitDampingFactor))
+        grid._check(L.opx_step(grid.handle, float(timeStep)))
 
     def stepLinks(self, grid: Grid, timeStep: float):
         grid._check(_lib.load().opx_step_links(grid.handle, float(timeStep)))

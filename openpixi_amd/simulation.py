@@ -18,7 +18,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _lib
-from .grid import FieldMeasurements, FastTYMSolver, Grid
+from .grid import FieldMeasurements, FastTYMSolver, Grid, ImplicitTYMSolver
 
 SIMULATION_TYPES = {
     "temporal yang-mills": "TemporalYangMills",
@@ -30,8 +30,9 @@ SIMULATION_TYPES = {
     "boost-invariant cgc": "BoostInvariantCGC",
 }
 
-# simulation types whose field solver is FastTYMSolver (physics/Settings.java:540-581)
+# simulation types whose field solver is FastTYMSolver or ImplicitTYMSolver (physics/Settings.java:540-581)
 FAST_TYM_TYPES = {"TemporalYangMills", "TemporalCGC", "TemporalCGCNGP", "TemporalOptimizedCGCNGP"}
+IMPLICIT_TYM_TYPES = {"TemporalImplicitCGCNGP", "TemporalOptimizedImplicitCGCNGP"}
 
 
 @dataclass
@@ -64,6 +65,8 @@ class Settings:
     precision: str = "fp64"          # device precision (not a reference key)
     reunitarize_every: int = 0       # FP64 reprojection period (not a reference key)
     diagnostics: list = field(default_factory=list)   # Settings.addDiagnostics (YamlOutput.applyTo)
+    implicit_solver_iterations: int = 0               # physics/Settings.java:79
+    implicit_solver_damping: float = 0.0              # physics/Settings.java:80
 
     @property
     def iterations(self) -> int:
@@ -107,6 +110,10 @@ class Settings:
             if r and r.get("point1") is not None and r.get("point2") is not None and \
                     len(r["point1"]) == s.number_of_dimensions and len(r["point2"]) == s.number_of_dimensions:
                 setattr(s, attr, Region(bool(r.get("enabled", False)), list(r["point1"]), list(r["point2"])))
+        if y.get("implicitSolverIterations") is not None:
+            s.implicit_solver_iterations = int(y["implicitSolverIterations"])
+        if y.get("implicitSolverDamping") is not None:
+            s.implicit_solver_damping = float(y["implicitSolverDamping"])
         if y.get("output"):
             from .diagnostics import from_yaml_output
 
@@ -145,8 +152,8 @@ class Simulation:
 
     def __init__(self, settings: Settings, superparticles: Optional[SuperParticleSet] = None,
                  particles: Optional[ParticleSet] = None):
-        if settings.simulation_type not in FAST_TYM_TYPES:
-            raise NotImplementedError(f"{settings.simulation_type} does not use FastTYMSolver")
+        if settings.simulation_type not in FAST_TYM_TYPES | IMPLICIT_TYM_TYPES:
+            raise NotImplementedError(f"{settings.simulation_type} uses neither FastTYMSolver nor ImplicitTYMSolver")
         if settings.number_of_dimensions != 3:
             raise NotImplementedError("only 3 spatial dimensions are supported")
         self.settings = settings
@@ -156,7 +163,13 @@ class Simulation:
         self._diagnostics_started = False
         self.grid = Grid(settings.number_of_colors, settings.grid_cells, settings.grid_steps, settings.time_step,
                          settings.coupling_constant, settings.precision)
-        self.fsolver = FastTYMSolver()
+        if settings.simulation_type in IMPLICIT_TYM_TYPES:
+            self.fsolver = ImplicitTYMSolver(settings.implicit_solver_iterations, settings.implicit_solver_damping)
+            self.grid._check(_lib.load().opx_set_field_solver(self.grid.handle, _lib.SOLVER_IMPLICIT,
+                                                              settings.implicit_solver_iterations,
+                                                              settings.implicit_solver_damping))
+        else:
+            self.fsolver = FastTYMSolver()
         if settings.evaluation_region.enabled:
             self.grid.setEvaluationRegion(*settings.evaluation_region.resolved(settings.grid_cells))
         if settings.active_region.enabled:
