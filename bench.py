@@ -63,6 +63,8 @@ def parse():
     p.add_argument("--batch-events", type=int, default=64)
     p.add_argument("--batch-steps", type=int, default=300)
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint line")
+    p.add_argument("--no-implicit", action="store_true", help="skip the implicit-solver line")
+    p.add_argument("--implicit-steps", type=int, default=20)
     p.add_argument("--adjoint-steps", type=int, default=200)
     p.add_argument("--fd-directions", type=int, default=256)
     p.add_argument("--probe-kernels", default="auto",
@@ -411,6 +413,44 @@ def time_adjoint(steps, directions, stream):
     return res
 
 
+def time_implicit(steps, iterations, damping, stream):
+    """ImplicitTYMSolver on the SU(3) 128^3 lattice (the solver of the implicit CGC simulation
+    types): FP32 + FP64 reunitarization every 10 steps, and FP64; link-updates/s of the
+    implicit step and the energy change of FP32 against FP64."""
+    import torch
+    from openpixi_amd import FieldMeasurements, _lib
+
+    L = _lib.load()
+    out = {"workload": f"SU(3) {DIMS[0]}x{DIMS[1]}x{DIMS[2]} ImplicitTYMSolver, {iterations} iterations, damping "
+                       f"{damping}, beam direction x0, dt/a = {DT}", "steps": steps, "iterations": iterations,
+           "damping": damping}
+    n = int(np.prod(DIMS))
+    energies = {}
+    for precision in ("fp32", "fp64"):
+        g = make_grid(precision, DIMS)
+        g._check(L.opx_set_stream(g.handle, ctypes_void(stream.cuda_stream)))
+        g.projectGauss(tol=1e-10, max_iter=4000)
+        reunit = REUNIT_EVERY if precision == "fp32" else 0
+        g._check(L.opx_step_links(g.handle, DT))
+        g._check(L.opx_set_field_solver(g.handle, _lib.SOLVER_IMPLICIT, iterations, damping))
+        g._check(L.opx_leapfrog(g.handle, 2, reunit))
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        g._check(L.opx_leapfrog(g.handle, steps, reunit))
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / steps
+        m = FieldMeasurements().measure(g)
+        energies[precision] = float(m[0:6].sum())
+        out[precision] = {"ms_per_step": ms, "link_updates_per_s": 3 * n / (ms * 1e-3),
+                          "gauss_residual_end": float(m[6])}
+        g.close()
+    out["fp32_energy_vs_fp64"] = abs(energies["fp32"] - energies["fp64"]) / abs(energies["fp64"])
+    return out
+
+
 def ctypes_void(p):
     import ctypes
 
@@ -468,6 +508,10 @@ def main():
     if not args.no_batch:
         batch = time_batch(args.batch_events, args.batch_steps, 20, stream, not args.no_autotune)
         print(f"# batch: {json.dumps(batch)}", file=sys.stderr, flush=True)
+    implicit = None
+    if not args.no_implicit:
+        implicit = time_implicit(args.implicit_steps, 2, 0.25, stream)
+        print(f"# implicit: {json.dumps(implicit)}", file=sys.stderr, flush=True)
     adjoint = None
     if not args.no_adjoint:
         adjoint = time_adjoint(args.adjoint_steps, args.fd_directions, stream)
@@ -522,6 +566,8 @@ def main():
         out["batch"] = batch
     if adjoint is not None:
         out["adjoint"] = adjoint
+    if implicit is not None:
+        out["implicit"] = implicit
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     out["host"] = platform.node()
