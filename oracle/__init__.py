@@ -124,6 +124,14 @@ def lib():
             "orc_projected_energy_density": (None, [gp, ctypes.c_int, _dp, _dp, _dp, _dp]),
             "orc_projected_poynting": (ctypes.c_int, [gp, ctypes.c_int, _dp, _dp]),
             "orc_implicit_step": (None, [gp, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
+            "orc_regulate_charge_gaussian": (None, [_ip, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_double, _dp]),
+            "orc_poisson_2d": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]),
+            "orc_mv_charge": (None, [gp] + [ctypes.c_double] * 6 + [_dp, _dp]),
+            "orc_wilson_segment": (None, [gp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]),
+            "orc_lightcone_solve": (None, [gp, _dp, ctypes.c_int, _dp]),
+            "orc_lightcone_block": (None, [gp, _dp, _ip, _ip]),
+            "orc_lightcone_superparticles": (None, [gp, _dp] + [ctypes.c_int] * 5 + [_ip] * 6 + [_dp]),
             "orc_group_log": (None, [ctypes.c_int, _dp, _dp]),
             "orc_group_pow": (None, [ctypes.c_int, _dp, ctypes.c_double, _dp]),
             "orc_gauge_transform": (None, [gp, _dp]),
@@ -207,6 +215,21 @@ def group_pow(nc, u, x):
     o = np.zeros(gsize(nc))
     lib().orc_group_pow(nc, _ptr(np.ascontiguousarray(u, np.float64)), float(x), _ptr(o))
     return o
+
+
+def regulate_charge_gaussian(dims, uvt, long_width, ir, aT, aL, rho):
+    """FourierFunctions.regulateChargeDensityGaussian, beam along x0, on a (cells,) array."""
+    r = np.ascontiguousarray(rho, np.float64).copy()
+    lib().orc_regulate_charge_gaussian((ctypes.c_int * 3)(*dims), uvt, long_width, ir, aT, aL, _ptr(r))
+    return r
+
+
+def poisson_2d(n1, n2, aT, rho):
+    """FourierFunctions.solvePoisson2D on one (n1 * n2,) sheet."""
+    r = np.ascontiguousarray(rho, np.float64)
+    phi = np.zeros(n1 * n2)
+    lib().orc_poisson_2d(n1, n2, aT, _ptr(r), _ptr(phi))
+    return phi
 
 
 def alg_square(nc, v):
@@ -364,6 +387,43 @@ class OracleGrid:
     def implicit_step(self, at=None, iterations=0, damping=0.0):
         """ImplicitTYMSolver.step (physics/fields/ImplicitTYMSolver.java:35-58)."""
         lib().orc_implicit_step(self._g, self.at if at is None else at, int(iterations), float(damping))
+
+    # ---- CGC initial condition, beam along x0 (opx_oracle_cgc.c) ----
+    def mv_charge(self, location, width, mu, uvt, long_width, ir, noise):
+        """MVModel.initialize with unit Gaussian noise[(components, cells)]; returns rho (cells, A)."""
+        noise = np.ascontiguousarray(noise, np.float64).reshape(-1)
+        assert noise.size == (self.nc * self.nc - 1) * self.n
+        rho = np.zeros((self.n, asize(self.nc)))
+        lib().orc_mv_charge(self._g, location, width, mu, uvt, long_width, ir, _ptr(noise), _ptr(rho))
+        return rho
+
+    def lightcone_solve(self, rho, orientation):
+        """LightConePoissonSolverImprovedFull.solve: sets U, Unext, E; returns the Gauss violation (cells, A)."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        gauss = np.zeros((self.n, asize(self.nc)))
+        lib().orc_lightcone_solve(self._g, _ptr(rho), int(orientation), _ptr(gauss))
+        return gauss
+
+    def lightcone_superparticles(self, gauss, orientation, subdivisions):
+        """LightConeNGPSuperParticleCreator.initializeParticles: (descriptor dict, charges list per super particle)."""
+        gauss = np.ascontiguousarray(gauss, np.float64)
+        zs, bw = ctypes.c_int(), ctypes.c_int()
+        lib().orc_lightcone_block(self._g, _ptr(gauss), ctypes.byref(zs), ctypes.byref(bw))
+        ppc = int(round(self.spacing[0] / self.at))
+        sub = min(int(subdivisions), bw.value)
+        cnt = sub * ppc
+        arrs = {k: np.zeros(cnt, np.int32) for k in ("orientation", "num", "index_offset", "per_plane", "shift", "ppc")}
+        T = self.dims[1] * self.dims[2]
+        q = np.zeros((bw.value * T * ppc, asize(self.nc)))
+        lib().orc_lightcone_superparticles(self._g, _ptr(gauss), int(orientation), zs.value, bw.value, sub, ppc,
+                                           *[arrs[k].ctypes.data_as(_ip) for k in ("orientation", "num", "index_offset",
+                                                                                    "per_plane", "shift", "ppc")],
+                                           _ptr(q))
+        charges, pos = [], 0
+        for s_ in range(cnt):
+            charges.append(q[pos:pos + arrs["num"][s_]])
+            pos += arrs["num"][s_]
+        return arrs, charges, (zs.value, bw.value)
 
     def projected_poynting(self, direction):
         """Longitudinal Poynting vector per slice (averaged, time averaged); SU(2) only."""

@@ -77,6 +77,18 @@ void orc_step_links(orc_grid *gr, double at);   /* UpdateLinks over all cells */
 void orc_implicit_step(orc_grid *gr, double at, int iterations, double damping);
 void orc_group_log(int nc, const double *u, double *v);
 void orc_group_pow(int nc, const double *u, double x, double *o);
+/* CGC initial condition, beam along x0 (opx_oracle_cgc.c) */
+void orc_regulate_charge_gaussian(const int dims[3], double uvt, double long_width, double ir, double aT,
+                                  double aL, double *rho);
+void orc_poisson_2d(int n1, int n2, double aT, const double *rho, double *phi);
+void orc_mv_charge(const orc_grid *gr, double location, double width, double mu, double uvt, double long_width,
+                   double ir, const double *noise, double *rho);
+void orc_wilson_segment(const orc_grid *gr, double z1, double z2, int M, const double *P1, const double *P2, double *W);
+void orc_lightcone_solve(orc_grid *gr, const double *rho, int orientation, double *gauss);
+void orc_lightcone_block(const orc_grid *gr, const double *gauss, int *z_start, int *block_width);
+void orc_lightcone_superparticles(const orc_grid *gr, const double *gauss, int orientation, int z_start,
+                                  int block_width, int subdivisions, int ppc, int *sp_orientation, int *sp_num,
+                                  int *sp_index_offset, int *sp_per_plane, int *sp_shift, int *sp_ppc, double *charges);
 void orc_store_fields(orc_grid *gr);            /* U <-> Unext */
 void orc_reset_current(orc_grid *gr);
 void orc_reset_charge(orc_grid *gr);
