@@ -211,6 +211,54 @@ int opx_superparticles_download_charges(opx_ctx *ctx, double *charges, int64_t n
 int opx_interpolate_to_grid(opx_ctx *ctx, int t);
 /* interpolateChargedensity at integer clock t: rho += Q at the NGP offset */
 int opx_interpolate_chargedensity(opx_ctx *ctx, int t);
+/* the list's size: super particles and charges in total (CGC super particles of the
+ * simulation's particle list, Simulation.particles) */
+int opx_superparticles_count(const opx_ctx *ctx, int *count, int64_t *total_charges);
+/* the list's descriptors, `count` entries each (the constructor arguments of CGCSuperParticle,
+ * physics/particles/CGCSuperParticle.java:33-46) */
+int opx_superparticles_download_descriptors(opx_ctx *ctx, int count, int *orientation, int *num,
+                                            int *index_offset, int *per_plane, int *sub_lattice_shift,
+                                            int *particles_per_cell);
+
+/* ---- CGC initial condition (beam along x0) ------------------------------
+ * Replaces CGCInitialCondition.applyInitialCondition (physics/initial/CGC/CGCInitialCondition.java:55-93)
+ * for direction 0 with an MVModel charge density, LightConePoissonSolverImprovedFull and
+ * LightConeNGPSuperParticleCreator. One call of each, in this order, per nucleus. The charge
+ * density and the Gauss violation of the last solve stay on the device between the calls
+ * (double precision); every axis may hold at most 4096 cells. Entry points that take host
+ * buffers synchronize the context's stream. */
+/* MVModel.initialize (physics/initial/CGC/MVModel.java:101-157): noise holds the unit Gaussian
+ * draws, (colours^2 - 1) * cells values, component after component and cells in index order
+ * within a component (the order the reference takes them from java.util.Random.nextGaussian).
+ * uv_cutoff = ultravioletCutoffTransverse, coherence_length = longitudinalCoherenceLength,
+ * ir_coefficient = infraredCoefficient. rho_out (cells * A reals, reference storage) may be NULL. */
+int opx_cgc_mv_charge(opx_ctx *ctx, double location, double width, double mu, double uv_cutoff,
+                      double coherence_length, double ir_coefficient, const double *noise, int64_t n_noise,
+                      double *rho_out, int64_t n_rho);
+/* an arbitrary charge density for the next solve (IInitialChargeDensity.getChargeDensity),
+ * cells * A reals in reference storage */
+int opx_cgc_set_charge(opx_ctx *ctx, const double *rho, int64_t n_values);
+/* LightConePoissonSolverImprovedFull.initialize + solve (physics/initial/CGC/
+ * LightConePoissonSolverImprovedFull.java:34-263): transverse Poisson equation per x0 sheet,
+ * path-ordered Wilson lines V at t = -at/2 and +at/2, transverse links U and Unext rotated by
+ * them, E = -proj(Unext U^+)/at in every direction, and the Gauss violation of the fields the
+ * Wilson lines alone produce (getGaussViolation). orientation is +1 or -1; gauss_out (cells * A
+ * reals, may be NULL) receives that violation. */
+int opx_cgc_lightcone_solve(opx_ctx *ctx, int orientation, double *gauss_out, int64_t n_values);
+/* LightConeNGPSuperParticleCreator.initialize (physics/initial/CGC/
+ * LightConeNGPSuperParticleCreator.java:76-245) from the last solve's Gauss violation: the block
+ * of planes whose largest |G| exceeds 1e-9 of the largest, min(num_threads, block) subdivisions
+ * of round(a_0/at) super particles each, their charges refined to 2nd then 4th order along the
+ * beam (100 sweeps each), APPENDED to the context's super-particle list (Simulation.particles.add).
+ * added, z_start and block_width (each may be NULL) report the particles added and the block. */
+int opx_cgc_create_superparticles(opx_ctx *ctx, int orientation, int num_threads, int *added, int *z_start,
+                                  int *block_width);
+/* the charge density of the next solve (MVModel.getChargeDensity) or the Gauss violation of the
+ * last one (LightConePoissonSolverImprovedFull.getGaussViolation), cells * A reals */
+enum { OPX_CGC_CHARGE = 0, OPX_CGC_GAUSS = 1 };
+int opx_cgc_download(opx_ctx *ctx, int which, double *out, int64_t n_values);
+/* frees the charge density and Gauss violation (IInitialChargeDensity.clear) */
+int opx_cgc_clear(opx_ctx *ctx);
 
 /* ---- particle seam: per-particle CGC NGP particles ----------------------
  * Replaces CGCParticleInterpolationNGP (physics/grid/CGCParticleInterpolationNGP.java:15-104)

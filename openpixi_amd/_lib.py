@@ -28,6 +28,7 @@ FIELD_U, FIELD_UNEXT, FIELD_E, FIELD_J, FIELD_RHO = range(5)
 OPT_KERNEL, OPT_TILE, OPT_CHUNKS = range(3)
 KERNEL_AUTO, KERNEL_LINK, KERNEL_STREAM, KERNEL_PLAQ, KERNEL_TILE, KERNEL_PLANE = range(6)
 SOLVER_FAST, SOLVER_IMPLICIT = 0, 1
+CGC_CHARGE, CGC_GAUSS = 0, 1
 BATCH_OPT_GRID, BATCH_OPT_FORM = range(2)
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -71,6 +72,15 @@ SIGNATURES = {
     "opx_superparticles_download_charges": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     "opx_interpolate_to_grid": (ctypes.c_int, [_vp, ctypes.c_int]),
     "opx_interpolate_chargedensity": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "opx_superparticles_count": (ctypes.c_int, [_vp, _ip, _i64p]),
+    "opx_superparticles_download_descriptors": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip, _ip]),
+    "opx_cgc_mv_charge": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int64, _dp, ctypes.c_int64]),
+    "opx_cgc_set_charge": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
+    "opx_cgc_lightcone_solve": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64]),
+    "opx_cgc_create_superparticles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip]),
+    "opx_cgc_download": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64]),
+    "opx_cgc_clear": (ctypes.c_int, [_vp]),
     "opx_simulation_initialize": (ctypes.c_int, [_vp]),
     "opx_simulation_step": (ctypes.c_int, [_vp, ctypes.c_int]),
     "opx_simulation_steps_done": (ctypes.c_int, [_vp]),

@@ -17,8 +17,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libopx.so")
 
 SOURCES = ["opx_kernels.hip", "opx_stream.hip", "opx_gauss.hip", "opx_adjoint.hip", "opx_particles.hip", "opx_tile.hip",
-           "opx_plane.hip", "opx_implicit.hip", "opx_capi.cpp",
-           "opx_adjoint_capi.cpp"]
+           "opx_plane.hip", "opx_implicit.hip", "opx_cgc.hip", "opx_capi.cpp",
+           "opx_adjoint_capi.cpp", "opx_cgc_capi.cpp"]
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 are generated.
 HIP_FLAGS = [

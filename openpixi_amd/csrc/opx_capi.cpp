@@ -277,6 +277,22 @@ namespace opx_host {
 int fail(opx_ctx* c, int code, const std::string& msg) { return ::fail(c, code, msg); }
 int hip_check(opx_ctx* c, hipError_t e, const char* what) { return ::hip_check(c, e, what); }
 opx::Lattice lattice(const opx_ctx* c, double at) { return ::lattice(c, at); }
+void storage_to_coeffs(int nc, const double* v, double* c)
+{
+    if (nc == 2) {
+        c[0] = v[0]; c[1] = v[1]; c[2] = v[2];
+    } else {
+        su3_storage_to_coeffs(v, c);
+    }
+}
+void coeffs_to_storage(int nc, const double* c, double* v)
+{
+    if (nc == 2) {
+        v[0] = c[0]; v[1] = c[1]; v[2] = c[2];
+    } else {
+        su3_coeffs_to_storage(c, v);
+    }
+}
 hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E)
 {
     return ::launch_update(c, L, U, Unext, E);
@@ -368,6 +384,7 @@ void opx_destroy(opx_ctx* ctx)
         return;
     }
     opx_host::adjoint_release(ctx);
+    opx_host::cgc_release(ctx);
     void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->P, ctx->iU, ctx->iUn, ctx->iE,
                     ctx->partials, ctx->dout, ctx->counters,
                     ctx->sp_desc, ctx->sp_qoff, ctx->sp_q, ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel,
@@ -788,6 +805,27 @@ int opx_superparticles_upload(opx_ctx* ctx, int count, const int* orientation, c
     ctx->sp_max_num = maxn;
     ctx->sp_host = desc;
     if (total > 0) return to_device(ctx, ctx->sp_q, soa);
+    return OPX_OK;
+}
+
+int opx_superparticles_count(const opx_ctx* ctx, int* count, int64_t* total_charges)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (count) *count = ctx->sp_count;
+    if (total_charges) *total_charges = ctx->sp_total;
+    return OPX_OK;
+}
+
+int opx_superparticles_download_descriptors(opx_ctx* ctx, int count, int* orientation, int* num, int* index_offset,
+                                            int* per_plane, int* sub_lattice_shift, int* particles_per_cell)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (count != ctx->sp_count) return fail(ctx, OPX_E_ARG, "superparticles: count mismatch");
+    int* out[6] = {orientation, num, index_offset, per_plane, sub_lattice_shift, particles_per_cell};
+    for (int f = 0; f < 6; f++) {
+        if (count > 0 && !out[f]) return fail(ctx, OPX_E_ARG, "superparticles: null array");
+        for (int p = 0; p < count; p++) out[f][p] = ctx->sp_host[(size_t)f * count + p];
+    }
     return OPX_OK;
 }
 

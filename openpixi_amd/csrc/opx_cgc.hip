@@ -1,0 +1,735 @@
+// opx_cgc.hip -- gfx950 kernels of the CGC initial condition with the beam along x0:
+//   - MV charges regulated in Fourier space (physics/initial/CGC/MVModel.java:101-157,
+//     FourierFunctions.java:86-128),
+//   - the light-cone Poisson solve with path-ordered Wilson lines
+//     (LightConePoissonSolverImprovedFull.java:40-291),
+//   - the super-particle charges with their 2nd/4th-order refinement
+//     (LightConeNGPSuperParticleCreator.java:104-307).
+//
+// Arithmetic is double throughout; the context's links, E and rho are read and written in its
+// own precision. Work arrays are component-major [component][cell] (complex ones as double2),
+// group-element arrays [real][cell]. The discrete Fourier transforms run one x-axis at a time:
+// a workgroup stages a few lines in LDS and runs radix-2 butterflies for power-of-two lengths,
+// or the direct sum against a twiddle table for other lengths.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "opx_group.h"
+#include "opx_kernels.h"
+
+namespace opx {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kFftThreads = 256;
+constexpr int kWilsonM = 32;   // midpoints of one lattice spacing (LightConePoissonSolverImprovedFull.java:72)
+
+// ------------------------------------------------------------------ lattice helpers
+__device__ __forceinline__ void site_xyz(const Lattice& L, uint32_t site, uint32_t x[3])
+{
+    x[0] = site / L.s0;
+    const uint32_t r = site - x[0] * L.s0;
+    x[1] = r / L.s1;
+    x[2] = r - x[1] * L.s1;
+}
+
+// periodic neighbour of `site` along d (Grid.shift, physics/grid/Grid.java:794-823)
+__device__ __forceinline__ uint32_t nb(const Lattice& L, uint32_t site, int d, int o)
+{
+    uint32_t x[3];
+    site_xyz(L, site, x);
+    const uint32_t len = d == 0 ? L.d0 : (d == 1 ? L.d1 : L.d2);
+    const uint32_t st = d == 0 ? L.s0 : (d == 1 ? L.s1 : 1u);
+    const uint32_t c = x[d];
+    if (o > 0) return c + 1 == len ? site - c * st : site + st;
+    return c == 0 ? site + (len - 1) * st : site - st;
+}
+
+__device__ __forceinline__ bool in_active(const Box& b, const uint32_t x[3])
+{
+    if (b.all) return true;
+    for (int d = 0; d < 3; d++)
+        if ((int)x[d] < b.lo[d] || (int)x[d] > b.hi[d]) return false;
+    return true;
+}
+
+// a context link (precision T, S stored reals) in double
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, double>::M ld_link_d(const T* __restrict__ U, uint32_t n, uint32_t site,
+                                                                  int dir)
+{
+    using GR = Group<NC, T>;
+    using GD = Group<NC, double>;
+    typename GD::M m = GD::zero();
+    const T* p = U + (size_t)dir * GR::S * n + site;
+#pragma unroll
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[(size_t)k * n];
+    if constexpr (GR::S != GR::G) su3_third_row(m);
+    return m;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void st_link_d(T* __restrict__ U, uint32_t n, uint32_t site, int dir,
+                                          const typename Group<NC, double>::M& m)
+{
+    using GR = Group<NC, T>;
+    using GD = Group<NC, double>;
+    T* p = U + (size_t)dir * GR::S * n + site;
+#pragma unroll
+    for (int k = 0; k < GR::S; k++) p[(size_t)k * n] = (T)GD::gget(m, GR::slot(k));
+}
+
+// group element of a double [real][cell] array
+template <int NC>
+__device__ __forceinline__ typename Group<NC, double>::M ld_g(const double* __restrict__ V, uint32_t n, uint32_t site)
+{
+    using GD = Group<NC, double>;
+    typename GD::M m;
+#pragma unroll
+    for (int k = 0; k < GD::G; k++) GD::gref(m, k) = V[(size_t)k * n + site];
+    return m;
+}
+
+template <int NC>
+__device__ __forceinline__ void st_g(double* __restrict__ V, uint32_t n, uint32_t site,
+                                     const typename Group<NC, double>::M& m)
+{
+    using GD = Group<NC, double>;
+#pragma unroll
+    for (int k = 0; k < GD::G; k++) V[(size_t)k * n + site] = GD::gget(m, k);
+}
+
+// algebra coefficients of a [component][cell] array (precision T), optionally offset by dir blocks
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, double>::A ld_coef(const T* __restrict__ X, uint32_t n, uint32_t site,
+                                                                int dir = 0)
+{
+    using GD = Group<NC, double>;
+    typename GD::A a;
+    const T* p = X + (size_t)dir * GD::AD * n + site;
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[(size_t)k * n];
+    return a;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void st_coef(T* __restrict__ X, uint32_t n, uint32_t site, int dir,
+                                        const typename Group<NC, double>::A& a)
+{
+    using GD = Group<NC, double>;
+    T* p = X + (size_t)dir * GD::AD * n + site;
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) p[(size_t)k * n] = (T)aget(a, k);
+}
+
+// ------------------------------------------------------------------ discrete Fourier transforms
+// tw[t] = (cos 2 pi t / n, sin 2 pi t / n), t < n
+__global__ __launch_bounds__(kB) void k_twiddles(uint32_t n, double2* __restrict__ tw)
+{
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= n) return;
+    double s, c;
+    sincospi(2.0 * (double)t / (double)n, &s, &c);
+    tw[t] = make_double2(c, s);
+}
+
+// X_k = sum_m x_m exp(sign 2 pi i k m / n) along every line of a strided complex array: line l starts
+// at (l / stride) * n * stride + l % stride, its elements are `stride` apart. A workgroup handles B
+// consecutive lines, staged in LDS with row pitch `pitch`.
+template <bool POW2>
+__global__ __launch_bounds__(kFftThreads) void k_fft(double2* __restrict__ x, uint64_t lines, uint32_t n,
+                                                     uint32_t stride, uint32_t B, uint32_t pitch, uint32_t logn,
+                                                     const double2* __restrict__ tw, double sign)
+{
+    extern __shared__ double2 sm[];
+    const uint64_t l0 = (uint64_t)blockIdx.x * B;
+    const uint32_t tot = B * n;
+    auto addr = [&](uint32_t b, uint32_t k) -> uint64_t {
+        const uint64_t l = l0 + b;
+        return (l / stride) * (uint64_t)n * stride + (l % stride) + (uint64_t)k * stride;
+    };
+    // element f of the workgroup: consecutive threads take consecutive addresses
+    auto split = [&](uint32_t f, uint32_t& b, uint32_t& k) {
+        if (stride == 1) {
+            b = f / n;
+            k = f - b * n;
+        } else {
+            k = f / B;
+            b = f - k * B;
+        }
+    };
+    for (uint32_t f = threadIdx.x; f < tot; f += kFftThreads) {
+        uint32_t b, k;
+        split(f, b, k);
+        double2 v = make_double2(0.0, 0.0);
+        if (l0 + b < lines) v = x[addr(b, k)];
+        const uint32_t pos = POW2 ? (logn == 0 ? 0u : (__brev(k) >> (32 - logn))) : k;
+        sm[b * pitch + pos] = v;
+    }
+    __syncthreads();
+    if constexpr (POW2) {
+        const uint32_t hn = n / 2;
+        for (uint32_t half = 1; half < n; half <<= 1) {
+            const uint32_t step = n / (2 * half);
+            for (uint32_t u = threadIdx.x; u < B * hn; u += kFftThreads) {
+                const uint32_t b = u / hn, j = u - b * hn;
+                const uint32_t grp = j / half, pos = j - grp * half;
+                const uint32_t i0 = b * pitch + grp * 2 * half + pos, i1 = i0 + half;
+                const double2 w = tw[pos * step];
+                const double wr = w.x, wi = sign * w.y;
+                const double2 a = sm[i0], c = sm[i1];
+                const double vr = c.x * wr - c.y * wi, vi = c.x * wi + c.y * wr;
+                sm[i0] = make_double2(a.x + vr, a.y + vi);
+                sm[i1] = make_double2(a.x - vr, a.y - vi);
+            }
+            __syncthreads();
+        }
+        for (uint32_t f = threadIdx.x; f < tot; f += kFftThreads) {
+            uint32_t b, k;
+            split(f, b, k);
+            if (l0 + b < lines) x[addr(b, k)] = sm[b * pitch + k];
+        }
+    } else {
+        for (uint32_t f = threadIdx.x; f < tot; f += kFftThreads) {
+            uint32_t b, k;
+            split(f, b, k);
+            if (l0 + b >= lines) continue;
+            const double2* line = sm + b * pitch;
+            double sr = 0.0, si = 0.0;
+            uint32_t t = 0;
+            for (uint32_t m = 0; m < n; m++) {
+                const double2 w = tw[t];
+                const double wr = w.x, wi = sign * w.y;
+                const double2 v = line[m];
+                sr += v.x * wr - v.y * wi;
+                si += v.x * wi + v.y * wr;
+                t += k;
+                if (t >= n) t -= n;
+            }
+            x[addr(b, k)] = make_double2(sr, si);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ momentum-space factors
+// FourierFunctions.computeEffectiveTransverseMomentumSquared / computeTransverseMomentumSquared /
+// computeLatticeMomentum1D (FourierFunctions.java:182-235) on the transverse plane (x1, x2)
+__device__ __forceinline__ double kt_eff2(uint32_t p1, uint32_t p2, uint32_t n1, uint32_t n2, double aT)
+{
+    double m = 4.0;
+    m -= 2.0 * cos((2.0 * M_PI * p1) / n1);
+    m -= 2.0 * cos((2.0 * M_PI * p2) / n2);
+    return m / (aT * aT);
+}
+
+__device__ __forceinline__ double kt2(uint32_t p1, uint32_t p2, uint32_t n1, uint32_t n2, double aT)
+{
+    const double twopi = 2.0 * M_PI;
+    const double c1 = p1 < n1 / 2 ? twopi * p1 / (aT * n1) : twopi * (double)(n1 - p1) / (aT * n1);
+    const double c2 = p2 < n2 / 2 ? twopi * p2 / (aT * n2) : twopi * (double)(n2 - p2) / (aT * n2);
+    return c1 * c1 + c2 * c2;
+}
+
+__device__ __forceinline__ double k_long(uint32_t pos, uint32_t n, double aL)
+{
+    const double delta = pos / (double)n;
+    if (delta < 0.5) return 2.0 * delta * M_PI / aL;
+    return 2.0 * (delta - 1.0) * M_PI / aL;
+}
+
+// MVModel.initialize (MVModel.java:119-128): noise * mu g / sqrt(aL aT^2) * sqrt(Gaussian(location, width)(x0 aL))
+__global__ __launch_bounds__(kB) void k_mv_fill(Lattice L, uint64_t m, const double* __restrict__ noise, double rcw,
+                                                double location, double norm, double i2s2, double2* __restrict__ x)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t c = (uint32_t)(i % L.n);
+    const double xm = (double)(c / L.s0) * L.as[0] - location;
+    const double profile = sqrt(norm * exp(-xm * xm * i2s2));
+    x[i] = make_double2(noise[i] * rcw * profile, 0.0);
+}
+
+// FourierFunctions.regulateChargeDensityGaussian (FourierFunctions.java:104-123); inv_n folds in the
+// normalization of the inverse transform
+__global__ __launch_bounds__(kB) void k_mv_regulate(Lattice L, uint64_t m, double uvt2, double long_width, double ir2,
+                                                    double inv_n, double2* __restrict__ x)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    uint32_t p[3];
+    site_xyz(L, (uint32_t)(i % L.n), p);
+    const double aL = L.as[0], aT = L.as[1];
+    const double kte = kt_eff2(p[1], p[2], L.d1, L.d2, aT), kt = kt2(p[1], p[2], L.d1, L.d2, aT);
+    const double kl = fabs(k_long(p[0], L.d0, aL));
+    double f = 0.0;
+    if (kt <= uvt2 && kt > 0) f = kte / (kte + ir2) * exp(-0.25 * kl * kl * long_width * long_width) * inv_n;
+    const double2 v = x[i];
+    x[i] = make_double2(v.x * f, v.y * f);
+}
+
+// the box exclusion of MVModel.initialize (MVModel.java:138-149)
+__global__ __launch_bounds__(kB) void k_mv_store(Lattice L, Box active, uint64_t m, int lmin, int lmax,
+                                                 const double2* __restrict__ x, double* __restrict__ rho)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    uint32_t p[3];
+    site_xyz(L, (uint32_t)(i % L.n), p);
+    const int z = (int)p[0];
+    rho[i] = (lmin < z && z < lmax && in_active(active, p)) ? x[i].x : 0.0;
+}
+
+__global__ __launch_bounds__(kB) void k_to_complex(uint64_t m, const double* __restrict__ a, double2* __restrict__ x)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < m) x[i] = make_double2(a[i], 0.0);
+}
+
+__global__ __launch_bounds__(kB) void k_real_part(uint64_t m, const double2* __restrict__ x, double* __restrict__ a)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < m) a[i] = x[i].x;
+}
+
+// FourierFunctions.solvePoisson2D (FourierFunctions.java:146-154) on every x0 sheet: 1 / kT_eff^2,
+// the zero mode removed; inv_n folds in the normalization of the inverse transform
+__global__ __launch_bounds__(kB) void k_poisson_scale(Lattice L, uint64_t m, double inv_n, double2* __restrict__ x)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    uint32_t p[3];
+    site_xyz(L, (uint32_t)(i % L.n), p);
+    double f = 0.0;
+    if (p[1] != 0 || p[2] != 0) f = 1.0 / kt_eff2(p[1], p[2], L.d1, L.d2, L.as[1]) * inv_n;
+    const double2 v = x[i];
+    x[i] = make_double2(v.x * f, v.y * f);
+}
+
+// ------------------------------------------------------------------ Wilson lines
+// W(z1, z2, M, P1, P2) (LightConePoissonSolverImprovedFull.java:271-291): the ordered product of the
+// links of the linear interpolation between P1 and P2 at M midpoints, centred on round((z1 + z2) / 2)
+template <int NC>
+__device__ typename Group<NC, double>::M wilson_segment(double z1, double z2, int M,
+                                                        const typename Group<NC, double>::A& P1,
+                                                        const typename Group<NC, double>::A& P2, double g, double aL)
+{
+    using GD = Group<NC, double>;
+    typename GD::M W = GD::identity();
+    if (M <= 0) return W;
+    const double dz = fabs(z2 - z1) / (double)M * aL;
+    const double gf = -g * dz;
+    const double ui = floor((z1 + z2) * 0.5 + 0.5);
+    const double u1 = z1 - ui, u2 = z2 - ui;
+    typename GD::A PM, PD, P;
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) {
+        aref(PM, k) = (aget(P1, k) + aget(P2, k)) * 0.5;
+        aref(PD, k) = aget(P2, k) - aget(P1, k);
+    }
+#pragma unroll 1
+    for (int m = 0; m < M; m++) {
+        const double u = u1 + (u2 - u1) * (m + 0.5) / (double)M;
+#pragma unroll
+        for (int k = 0; k < GD::AD; k++) aref(P, k) = (aget(PM, k) + aget(PD, k) * u) * gf;
+        W = mul(W, exp_link(P, 1.0));
+    }
+    return W;
+}
+
+// the step of the Wilson line into plane x0 from its neighbour against the orientation
+// (LightConePoissonSolverImprovedFull.java:88-118); identity on the plane the line starts from
+template <int NC>
+__global__ __launch_bounds__(kB) void k_wilson_steps(Lattice L, int orientation, const double* __restrict__ phi,
+                                                     double* __restrict__ Wz)
+{
+    using GD = Group<NC, double>;
+    const uint32_t c = blockIdx.x * kB + threadIdx.x;
+    if (c >= L.n) return;
+    const uint32_t z = c / L.s0, i = c - z * L.s0;
+    const int Lz = (int)L.d0;
+    auto plane = [&](int zz) -> uint32_t { return (uint32_t)(((zz % Lz) + Lz) % Lz) * L.s0 + i; };
+    const int h = kWilsonM / 2;
+    const double aL = L.as[0];
+    typename GD::M W = GD::identity();
+    const double zd = (double)z;
+    if (orientation < 0) {
+        if (z != 0) {
+            const uint32_t i3 = c, i2 = plane((int)z - 1), i1 = plane((int)z - 2);
+            const auto p1 = ld_coef<NC, double>(phi, L.n, i1), p2 = ld_coef<NC, double>(phi, L.n, i2),
+                       p3 = ld_coef<NC, double>(phi, L.n, i3);
+            W = mul(wilson_segment<NC>(zd - 1, zd - 0.5, h, p1, p2, L.g, aL),
+                    wilson_segment<NC>(zd - 0.5, zd, h, p2, p3, L.g, aL));
+        }
+    } else {
+        if ((int)z != Lz - 1) {
+            const uint32_t i2 = c, i1 = plane((int)z - 1), i3 = plane((int)z + 1);
+            const auto p1 = ld_coef<NC, double>(phi, L.n, i1), p2 = ld_coef<NC, double>(phi, L.n, i2),
+                       p3 = ld_coef<NC, double>(phi, L.n, i3);
+            W = mul(wilson_segment<NC>(zd + 1, zd + 0.5, h, p2, p3, L.g, aL),
+                    wilson_segment<NC>(zd + 0.5, zd, h, p1, p2, L.g, aL));
+        }
+    }
+    st_g<NC>(Wz, L.n, c, W);
+}
+
+// V(x0) = V(x0 - orientation) W(x0), from the identity on the starting plane: one thread per transverse cell
+template <int NC>
+__global__ __launch_bounds__(kB) void k_wilson_scan(Lattice L, int orientation, const double* __restrict__ Wz,
+                                                    double* __restrict__ V)
+{
+    using GD = Group<NC, double>;
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= L.s0) return;
+    const uint32_t Lz = L.d0;
+    typename GD::M v = GD::identity();
+    st_g<NC>(V, L.n, (orientation < 0 ? 0u : Lz - 1) * L.s0 + i, v);
+#pragma unroll 1
+    for (uint32_t k = 1; k < Lz; k++) {
+        const uint32_t z = orientation < 0 ? k : Lz - k - 1;
+        const uint32_t c = z * L.s0 + i;
+        v = mul(v, ld_g<NC>(Wz, L.n, c));
+        st_g<NC>(V, L.n, c, v);
+    }
+}
+
+// V at t = +at/2: the extra path of length at along the beam (LightConePoissonSolverImprovedFull.java:167-190)
+template <int NC>
+__global__ __launch_bounds__(kB) void k_wilson_extra(Lattice L, int orientation, int mfrac, const double* __restrict__ phi,
+                                                     const double* __restrict__ Vm, double* __restrict__ Vp)
+{
+    const uint32_t c = blockIdx.x * kB + threadIdx.x;
+    if (c >= L.n) return;
+    const uint32_t i1 = nb(L, c, 0, -1);
+    const double z1 = (double)(c / L.s0);
+    const double z2 = orientation == -1 ? z1 + L.at / L.as[0] : z1 - L.at / L.as[0];
+    const auto W = wilson_segment<NC>(z1, z2, mfrac, ld_coef<NC, double>(phi, L.n, i1),
+                                      ld_coef<NC, double>(phi, L.n, c), L.g, L.as[0]);
+    st_g<NC>(Vp, L.n, c, mul(ld_g<NC>(Vm, L.n, c), W));
+}
+
+// transverse links U -> V_x U V_{x+i}^+ at both time levels and E = -proj(Unext U^+) / at in every
+// direction (LightConePoissonSolverImprovedFull.java:142-165, 192-230)
+template <int NC, typename T>
+__global__ __launch_bounds__(kB) void k_cgc_links(Lattice L, const double* __restrict__ Vm, const double* __restrict__ Vp,
+                                                  T* __restrict__ U, T* __restrict__ Unext, T* __restrict__ E)
+{
+    using GD = Group<NC, double>;
+    const uint32_t c = blockIdx.x * kB + threadIdx.x;
+    if (c >= L.n) return;
+    const double f = -1.0 / L.at;
+#pragma unroll 1
+    for (int d = 0; d < 3; d++) {
+        typename GD::M u = ld_link_d<NC, T>(U, L.n, c, d), un = ld_link_d<NC, T>(Unext, L.n, c, d);
+        if (d > 0) {
+            const uint32_t xd = nb(L, c, d, 1);
+            u = mul_adj(mul(ld_g<NC>(Vm, L.n, c), u), ld_g<NC>(Vm, L.n, xd));
+            un = mul_adj(mul(ld_g<NC>(Vp, L.n, c), un), ld_g<NC>(Vp, L.n, xd));
+            st_link_d<NC, T>(U, L.n, c, d, u);
+            st_link_d<NC, T>(Unext, L.n, c, d, un);
+        }
+        typename GD::A e = proj(mul_adj(un, u));
+#pragma unroll
+        for (int k = 0; k < GD::AD; k++) aref(e, k) *= f;
+        st_coef<NC, T>(E, L.n, c, d, e);
+    }
+}
+
+// Gauss constraint of the Wilson-line copy of the fields (transverse links V_x V_{x+i}^+, the grid's
+// own longitudinal links, E from its temporal plaquettes) minus the grid's rho
+// (LightConePoissonSolverImprovedFull.java:232-262, Grid.getGaussConstraint)
+template <int NC, typename T>
+__global__ __launch_bounds__(kB) void k_cgc_gauss(Lattice L, Box active, const double* __restrict__ Vm,
+                                                  const double* __restrict__ Vp, const T* __restrict__ U,
+                                                  const T* __restrict__ Unext, const T* __restrict__ rho,
+                                                  double* __restrict__ gauss)
+{
+    using GD = Group<NC, double>;
+    using M = typename GD::M;
+    using A = typename GD::A;
+    const uint32_t c = blockIdx.x * kB + threadIdx.x;
+    if (c >= L.n) return;
+    uint32_t x[3];
+    site_xyz(L, c, x);
+    A g = GD::azero();
+    if (in_active(active, x)) {
+        auto copy_u = [&](const double* V, const T* grid, uint32_t s, int d) -> M {
+            if (d == 0) return ld_link_d<NC, T>(grid, L.n, s, 0);
+            return mul_adj(ld_g<NC>(V, L.n, s), ld_g<NC>(V, L.n, nb(L, s, d, 1)));
+        };
+        auto copy_e = [&](uint32_t s, int d, const M& u) -> A {
+            A e = proj(mul_adj(copy_u(Vp, Unext, s, d), u));
+#pragma unroll
+            for (int k = 0; k < GD::AD; k++) aref(e, k) *= -1.0 / L.at;
+            return e;
+        };
+#pragma unroll 1
+        for (int d = 0; d < 3; d++) {
+            const uint32_t xm = nb(L, c, d, -1);
+            const M um = copy_u(Vm, U, xm, d);
+            const A t = act(copy_e(xm, d, um), adj(um));
+            const A ex = copy_e(c, d, copy_u(Vm, U, c, d));
+            const double f = 1.0 / (L.as[d] * (L.g * L.as[d]));
+#pragma unroll
+            for (int k = 0; k < GD::AD; k++) aref(g, k) += (aget(ex, k) - aget(t, k)) * f;
+        }
+        if (rho) {
+            const A r = ld_coef<NC, T>(rho, L.n, c);
+#pragma unroll
+            for (int k = 0; k < GD::AD; k++) aref(g, k) -= aget(r, k);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) gauss[(size_t)k * L.n + c] = aget(g, k);
+}
+
+// ------------------------------------------------------------------ super particles
+// largest |G| of each x0 plane (LightConeNGPSuperParticleCreator.java:110-127)
+__global__ __launch_bounds__(kB) void k_plane_max(uint32_t nT, uint32_t n, int ad, const double* __restrict__ gauss,
+                                                  double* __restrict__ mx)
+{
+    __shared__ double red[kB];
+    const uint32_t z = blockIdx.x;
+    double m = 0.0;
+    for (uint32_t j = threadIdx.x; j < nT; j += kB) {
+        const size_t c = (size_t)z * nT + j;
+        double s = 0.0;
+        for (int k = 0; k < ad; k++) {
+            const double v = gauss[(size_t)k * n + c];
+            s += v * v;
+        }
+        m = fmax(m, sqrt(s));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mx[z] = red[0];
+}
+
+__device__ __forceinline__ int wrap(int i, int n)
+{
+    while (i < 0) i += n;
+    while (i >= n) i -= n;
+    return i;
+}
+
+// Charges of the super particles of one block and their refinement along the beam
+// (LightConeNGPSuperParticleCreator.java:197-245, refine2/refine4 :248-307). One thread per
+// (component, transverse cell) walks its list of bw * ppc charges in `work` (element l at
+// work[l * ad * nT + thread]), then writes each charge to its super particle: subdivision j of the
+// block covers planes [P_j, P_j + w_j), particle k of it holds w_j * nT charges starting at
+// q_base + ppc nT P_j + k w_j nT.
+template <typename T>
+__global__ __launch_bounds__(kB) void k_sp_charges(uint32_t nT, uint32_t n, uint32_t Lz, int ad, SpBlock b,
+                                                   const double* __restrict__ gauss, double* __restrict__ work,
+                                                   T* __restrict__ q, int64_t q_total, int64_t q_base)
+{
+    const uint32_t g = blockIdx.x * kB + threadIdx.x;
+    if (g >= (uint32_t)ad * nT) return;
+    const uint32_t comp = g / nT, t = g - comp * nT;
+    const int ppc = b.ppc, nn = b.bw * ppc;
+    const size_t st = (size_t)ad * nT;
+    double* col = work + g;
+    const double inv = 1.0 / ppc;
+    for (int p = 0; p < b.bw; p++) {
+        const uint32_t z = (uint32_t)(b.zs + p), zn = (z + 1) % Lz;
+        const double qa = gauss[(size_t)comp * n + (size_t)z * nT + t] * inv;
+        const double qb = gauss[(size_t)comp * n + (size_t)zn * nT + t] * inv;
+        for (int k = 0; k < ppc; k++) col[(size_t)(p * ppc + k) * st] = k < ppc / 2 ? qa : qb;
+    }
+    // 2nd order: the last charge of an NGP cell is not moved
+#pragma unroll 1
+    for (int it = 0; it < 100; it++)
+#pragma unroll 1
+        for (int i = 0; i < nn; i++) {
+            const int jm = (i + ppc / 2) % ppc;
+            if (jm >= ppc - 1) continue;
+            double* Q0 = col + (size_t)wrap(i - 1, nn) * st;
+            double* Q1 = col + (size_t)wrap(i, nn) * st;
+            double* Q2 = col + (size_t)wrap(i + 1, nn) * st;
+            double* Q3 = col + (size_t)wrap(i + 2, nn) * st;
+            double dq = *Q0 * -1;
+            dq += *Q1 * 3;
+            dq += *Q2 * -3;
+            dq += *Q3 * 1;
+            dq *= 1.0 / 4.0;
+            *Q1 += dq * -1.0;
+            *Q2 += dq * 1.0;
+        }
+    // 4th order
+#pragma unroll 1
+    for (int it = 0; it < 100; it++)
+#pragma unroll 1
+        for (int i = 0; i < nn; i++) {
+            const int jm = (i + ppc / 2) % ppc;
+            if (jm >= ppc - 1) continue;
+            double* Q[6];
+#pragma unroll
+            for (int m = 0; m < 6; m++) Q[m] = col + (size_t)wrap(i - 2 + m, nn) * st;
+            double dq = *Q[0] * +1;
+            dq += *Q[1] * -5;
+            dq += *Q[2] * +10;
+            dq += *Q[3] * -10;
+            dq += *Q[4] * +5;
+            dq += *Q[5] * -1;
+            dq *= 1.0 / 12.0;
+            *Q[2] += dq * -1.0;
+            *Q[3] += dq * 1.0;
+        }
+    const int wl = b.ws + 1;
+    for (int p = 0; p < b.bw; p++) {
+        int P, w;
+        if (p < b.nl * wl) {
+            P = (p / wl) * wl;
+            w = wl;
+        } else {
+            P = b.nl * wl + ((p - b.nl * wl) / b.ws) * b.ws;
+            w = b.ws;
+        }
+        for (int k = 0; k < ppc; k++) {
+            const int64_t at = q_base + (int64_t)ppc * nT * P + (int64_t)k * w * nT + (int64_t)(p - P) * nT + t;
+            q[(size_t)comp * q_total + at] = (T)col[(size_t)(p * ppc + k) * st];
+        }
+    }
+}
+
+inline unsigned blocks(uint64_t m) { return (unsigned)((m + kB - 1) / kB); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+hipError_t cgc_twiddles(uint32_t n, double2* tw, hipStream_t st)
+{
+    k_twiddles<<<blocks(n), kB, 0, st>>>(n, tw);
+    return hipGetLastError();
+}
+
+bool cgc_fft_supported(uint32_t n) { return n >= 1 && n <= kCgcMaxFft; }
+
+hipError_t cgc_fft_axis(double2* x, uint64_t total, uint32_t n, uint32_t stride, const double2* tw, int sign,
+                        hipStream_t st)
+{
+    if (n == 1) return hipSuccess;
+    if (!cgc_fft_supported(n)) return hipErrorInvalidValue;
+    const bool pow2 = (n & (n - 1)) == 0;
+    uint32_t logn = 0;
+    while ((1u << logn) < n) logn++;
+    // lines per workgroup: a power of two with B (n + 1) complex values within 64 KiB of LDS; for
+    // strided axes at most the largest power of two dividing the stride
+    uint32_t B = 1;
+    while (B < 16 && (2 * B) * (n + 1) <= 4096u) B *= 2;
+    if (stride > 1)
+        while (B > 1 && stride % B != 0) B /= 2;
+    const uint32_t pitch = B == 1 ? n : n + 1;
+    const uint64_t lines = total / n;
+    const unsigned grid = (unsigned)((lines + B - 1) / B);
+    const size_t lds = (size_t)B * pitch * sizeof(double2);
+    if (pow2)
+        k_fft<true><<<grid, kFftThreads, lds, st>>>(x, lines, n, stride, B, pitch, logn, tw, (double)sign);
+    else
+        k_fft<false><<<grid, kFftThreads, lds, st>>>(x, lines, n, stride, B, pitch, logn, tw, (double)sign);
+    return hipGetLastError();
+}
+
+hipError_t cgc_mv_fill(const Lattice& L, uint64_t m, const double* noise, double rcw, double location, double norm,
+                       double i2s2, double2* x, hipStream_t st)
+{
+    k_mv_fill<<<blocks(m), kB, 0, st>>>(L, m, noise, rcw, location, norm, i2s2, x);
+    return hipGetLastError();
+}
+
+hipError_t cgc_mv_regulate(const Lattice& L, uint64_t m, double uvt2, double long_width, double ir2, double inv_n,
+                           double2* x, hipStream_t st)
+{
+    k_mv_regulate<<<blocks(m), kB, 0, st>>>(L, m, uvt2, long_width, ir2, inv_n, x);
+    return hipGetLastError();
+}
+
+hipError_t cgc_mv_store(const Lattice& L, const Box& active, uint64_t m, int lmin, int lmax, const double2* x,
+                        double* rho, hipStream_t st)
+{
+    k_mv_store<<<blocks(m), kB, 0, st>>>(L, active, m, lmin, lmax, x, rho);
+    return hipGetLastError();
+}
+
+hipError_t cgc_to_complex(uint64_t m, const double* a, double2* x, hipStream_t st)
+{
+    k_to_complex<<<blocks(m), kB, 0, st>>>(m, a, x);
+    return hipGetLastError();
+}
+
+hipError_t cgc_real_part(uint64_t m, const double2* x, double* a, hipStream_t st)
+{
+    k_real_part<<<blocks(m), kB, 0, st>>>(m, x, a);
+    return hipGetLastError();
+}
+
+hipError_t cgc_poisson_scale(const Lattice& L, uint64_t m, double inv_n, double2* x, hipStream_t st)
+{
+    k_poisson_scale<<<blocks(m), kB, 0, st>>>(L, m, inv_n, x);
+    return hipGetLastError();
+}
+
+hipError_t cgc_wilson_lines(int nc, const Lattice& L, int orientation, int mfrac, const double* phi, double* Wz,
+                            double* Vm, double* Vp, hipStream_t st)
+{
+    if (nc == 2) {
+        k_wilson_steps<2><<<blocks(L.n), kB, 0, st>>>(L, orientation, phi, Wz);
+        k_wilson_scan<2><<<blocks(L.s0), kB, 0, st>>>(L, orientation, Wz, Vm);
+        k_wilson_extra<2><<<blocks(L.n), kB, 0, st>>>(L, orientation, mfrac, phi, Vm, Vp);
+    } else {
+        k_wilson_steps<3><<<blocks(L.n), kB, 0, st>>>(L, orientation, phi, Wz);
+        k_wilson_scan<3><<<blocks(L.s0), kB, 0, st>>>(L, orientation, Wz, Vm);
+        k_wilson_extra<3><<<blocks(L.n), kB, 0, st>>>(L, orientation, mfrac, phi, Vm, Vp);
+    }
+    return hipGetLastError();
+}
+
+hipError_t cgc_links(int nc, int prec, const Lattice& L, const double* Vm, const double* Vp, void* U, void* Unext,
+                     void* E, hipStream_t st)
+{
+#define OPX_CGC_LINKS(NC, T) \
+    k_cgc_links<NC, T><<<blocks(L.n), kB, 0, st>>>(L, Vm, Vp, (T*)U, (T*)Unext, (T*)E)
+    if (nc == 2 && prec == 0) OPX_CGC_LINKS(2, double);
+    else if (nc == 2) OPX_CGC_LINKS(2, float);
+    else if (prec == 0) OPX_CGC_LINKS(3, double);
+    else OPX_CGC_LINKS(3, float);
+#undef OPX_CGC_LINKS
+    return hipGetLastError();
+}
+
+hipError_t cgc_gauss(int nc, int prec, const Lattice& L, const Box& active, const double* Vm, const double* Vp,
+                     const void* U, const void* Unext, const void* rho, double* gauss, hipStream_t st)
+{
+#define OPX_CGC_GAUSS(NC, T)                                                                                  \
+    k_cgc_gauss<NC, T><<<blocks(L.n), kB, 0, st>>>(L, active, Vm, Vp, (const T*)U, (const T*)Unext, (const T*)rho, \
+                                                   gauss)
+    if (nc == 2 && prec == 0) OPX_CGC_GAUSS(2, double);
+    else if (nc == 2) OPX_CGC_GAUSS(2, float);
+    else if (prec == 0) OPX_CGC_GAUSS(3, double);
+    else OPX_CGC_GAUSS(3, float);
+#undef OPX_CGC_GAUSS
+    return hipGetLastError();
+}
+
+hipError_t cgc_plane_max(int nc, const Lattice& L, const double* gauss, double* mx, hipStream_t st)
+{
+    k_plane_max<<<L.d0, kB, 0, st>>>(L.s0, L.n, nc == 2 ? 3 : 8, gauss, mx);
+    return hipGetLastError();
+}
+
+hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, const double* gauss, double* work,
+                          void* q, int64_t q_total, int64_t q_base, hipStream_t st)
+{
+    const int ad = nc == 2 ? 3 : 8;
+    const unsigned grid = blocks((uint64_t)ad * L.s0);
+    if (prec == 0)
+        k_sp_charges<double><<<grid, kB, 0, st>>>(L.s0, L.n, L.d0, ad, b, gauss, work, (double*)q, q_total, q_base);
+    else
+        k_sp_charges<float><<<grid, kB, 0, st>>>(L.s0, L.n, L.d0, ad, b, gauss, work, (float*)q, q_total, q_base);
+    return hipGetLastError();
+}
+
+}  // namespace opx

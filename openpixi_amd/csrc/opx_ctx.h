@@ -12,6 +12,7 @@
 
 namespace opx_host {
 struct AdjointWork;
+struct CgcWork;
 }
 
 struct opx_ctx {
@@ -75,6 +76,7 @@ struct opx_ctx {
     int num_cus = 256;
     bool batch_view = false;        // event of an opx_batch: memory borrowed, stepping through the batch only
     opx_host::AdjointWork* adj = nullptr;   // checkpoint and cotangent buffers of opx_adjoint_gradient
+    opx_host::CgcWork* cgc = nullptr;       // charge density and Gauss violation of the CGC initial condition
     std::string err;
 };
 
@@ -90,5 +92,10 @@ opx::Lattice lattice(const opx_ctx* c, double at);
 hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E);
 // frees the adjoint workspace (opx_adjoint_capi.cpp)
 void adjoint_release(opx_ctx* c);
+// frees the CGC initial-condition buffers (opx_cgc_capi.cpp)
+void cgc_release(opx_ctx* c);
+// one algebra element: reference storage (3 or 9 reals) <-> device coefficients (3 or 8)
+void storage_to_coeffs(int nc, const double* v, double* c);
+void coeffs_to_storage(int nc, const double* c, double* v);
 
 }  // namespace opx_host

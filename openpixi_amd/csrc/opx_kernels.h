@@ -181,4 +181,38 @@ hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperPart
 hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
                                    const void* q, void* rho, const CellRanges& cr, hipStream_t st);
 
+// CGC initial condition with the beam along x0 (opx_cgc.hip). Work arrays are double,
+// [component][cell]; complex ones double2; group elements [real][cell].
+constexpr uint32_t kCgcMaxFft = 4096;   // longest transform axis
+// A block of bw planes from zs carrying super-particle charges, ppc charges per cell, split into
+// subdivisions of ws + 1 planes (the first nl of them) and ws planes.
+struct SpBlock {
+    int zs, bw, ppc, ws, nl;
+};
+bool cgc_fft_supported(uint32_t n);
+hipError_t cgc_twiddles(uint32_t n, double2* tw, hipStream_t st);
+// in-place transform along one axis of `total` complex values: lines of n elements `stride` apart
+hipError_t cgc_fft_axis(double2* x, uint64_t total, uint32_t n, uint32_t stride, const double2* tw, int sign,
+                        hipStream_t st);
+hipError_t cgc_mv_fill(const Lattice& L, uint64_t m, const double* noise, double rcw, double location, double norm,
+                       double i2s2, double2* x, hipStream_t st);
+hipError_t cgc_mv_regulate(const Lattice& L, uint64_t m, double uvt2, double long_width, double ir2, double inv_n,
+                           double2* x, hipStream_t st);
+hipError_t cgc_mv_store(const Lattice& L, const Box& active, uint64_t m, int lmin, int lmax, const double2* x,
+                        double* rho, hipStream_t st);
+hipError_t cgc_to_complex(uint64_t m, const double* a, double2* x, hipStream_t st);
+hipError_t cgc_real_part(uint64_t m, const double2* x, double* a, hipStream_t st);
+hipError_t cgc_poisson_scale(const Lattice& L, uint64_t m, double inv_n, double2* x, hipStream_t st);
+// Wilson-line steps (Wz), the lines at -at/2 (Vm) and at +at/2 (Vp), each nc-colour [real][cell]
+hipError_t cgc_wilson_lines(int nc, const Lattice& L, int orientation, int mfrac, const double* phi, double* Wz,
+                            double* Vm, double* Vp, hipStream_t st);
+hipError_t cgc_links(int nc, int prec, const Lattice& L, const double* Vm, const double* Vp, void* U, void* Unext,
+                     void* E, hipStream_t st);
+hipError_t cgc_gauss(int nc, int prec, const Lattice& L, const Box& active, const double* Vm, const double* Vp,
+                     const void* U, const void* Unext, const void* rho, double* gauss, hipStream_t st);
+hipError_t cgc_plane_max(int nc, const Lattice& L, const double* gauss, double* mx, hipStream_t st);
+// work: bw * ppc * AD * (N1 N2) doubles; q: [coefficient][q_total] in the context precision
+hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, const double* gauss, double* work,
+                          void* q, int64_t q_total, int64_t q_base, hipStream_t st);
+
 }  // namespace opx

@@ -396,7 +396,6 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
     const BatchEvent* __restrict__ ev, int nev, uint32_t total_items, const T* __restrict__ Ubase,
     T* __restrict__ Unbase, T* __restrict__ Ebase)
 {
-    using GR = Group<NC, T>;
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
     const uint32_t wg_on_xcd = q8 + (xcd < r8 ? 1u : 0u);

@@ -67,6 +67,8 @@ class Settings:
     diagnostics: list = field(default_factory=list)   # Settings.addDiagnostics (YamlOutput.applyTo)
     implicit_solver_iterations: int = 0               # physics/Settings.java:79
     implicit_solver_damping: float = 0.0              # physics/Settings.java:80
+    number_of_threads: int = 1                        # physics/Settings.java:100 (super-particle subdivisions)
+    initial_conditions: list = field(default_factory=list)   # Settings.addInitialConditions (YamlInitialConditions)
 
     @property
     def iterations(self) -> int:
@@ -114,6 +116,15 @@ class Settings:
             s.implicit_solver_iterations = int(y["implicitSolverIterations"])
         if y.get("implicitSolverDamping") is not None:
             s.implicit_solver_damping = float(y["implicitSolverDamping"])
+        if y.get("numberOfThreads") is not None:
+            s.number_of_threads = int(y["numberOfThreads"])
+        ic = y.get("initialConditions") or {}
+        if ic.get("YM"):
+            raise NotImplementedError("Yang-Mills initial conditions (YamlYM) are not built")
+        if ic.get("CGC"):
+            from .cgc import from_yaml_cgc
+
+            s.initial_conditions = from_yaml_cgc(ic["CGC"])
         if y.get("output"):
             from .diagnostics import from_yaml_output
 
@@ -180,6 +191,9 @@ class Simulation:
         self.particles = None
         if particles is not None:
             self.set_particles(particles)
+        # physics/Simulation.java:274-276: initial conditions after the regions and particle lists
+        for ic in settings.initial_conditions:
+            ic.applyInitialCondition(self)
 
     def set_particles(self, p: ParticleSet):
         self.grid.setParticles(p.direction, p.pos0, p.vel, p.charges)
@@ -196,10 +210,26 @@ class Simulation:
         self.superparticles = sp
 
     def superparticle_charges(self):
-        sp = self.superparticles
-        out = np.zeros(int(np.sum(sp.number)) * sp.charges.shape[-1])
+        """The charges of the device's super-particle list, (total, A) in reference storage."""
+        import ctypes
+
+        count, total = ctypes.c_int(), ctypes.c_int64()
+        self.grid._check(_lib.load().opx_superparticles_count(self.grid.handle, ctypes.byref(count), ctypes.byref(total)))
+        a = 3 if self.grid.num_colors == 2 else 9
+        out = np.zeros(total.value * a)
         self.grid._check(_lib.load().opx_superparticles_download_charges(self.grid.handle, _lib.dptr(out), out.size))
-        return out.reshape(-1, sp.charges.shape[-1])
+        return out.reshape(-1, a)
+
+    def superparticle_descriptors(self) -> SuperParticleSet:
+        """The device's super-particle list (descriptors and charges)."""
+        import ctypes
+
+        count, total = ctypes.c_int(), ctypes.c_int64()
+        self.grid._check(_lib.load().opx_superparticles_count(self.grid.handle, ctypes.byref(count), ctypes.byref(total)))
+        arrs = [np.zeros(count.value, np.int32) for _ in range(6)]
+        self.grid._check(_lib.load().opx_superparticles_download_descriptors(self.grid.handle, count.value,
+                                                                            *[_lib.iptr(x) for x in arrs]))
+        return SuperParticleSet(*arrs, charges=self.superparticle_charges())
 
     def initialize(self):
         """physics/Simulation.java:318-352."""

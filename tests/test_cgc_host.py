@@ -1,0 +1,120 @@
+"""Host side of the CGC initial condition (no GPU):
+
+* JavaRandom against values java.util.Random returns in the JDK (new Random(42).nextInt(),
+  new Random(0 | 42).nextDouble(), new Random(0 | 42).nextGaussian()); the last needs
+  StrictMath.log, whose fdlibm result is one ulp off the correctly rounded logarithm there;
+* the vectorized draw equals the scalar nextGaussian sequence, including the kept second
+  value of a pair and the generator state afterwards;
+* strict_log within one ulp of the exact logarithm (decimal arithmetic) over normal and
+  subnormal arguments;
+* YAML: initialConditions.CGC builds one CGCInitialCondition per MVModel with the
+  reference's defaults, numberOfThreads is read, and what is not built is refused.
+"""
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+from openpixi_amd.cgc import CGCInitialCondition, JavaRandom, MVModel, strict_log
+from openpixi_amd.simulation import Settings
+
+
+def test_java_random_known_values():
+    assert JavaRandom(42).nextInt() == -1170105035
+    assert JavaRandom(0).nextDouble() == 0.730967787376657
+    assert JavaRandom(42).nextDouble() == 0.7275636800328681
+    assert JavaRandom(0).nextGaussian() == 0.8025330637390305
+    assert JavaRandom(42).nextGaussian() == 1.1419053154730547
+
+
+def test_strict_log_is_not_the_platform_log_where_java_differs():
+    # the first accepted pair of new Random(42): s = v1^2 + v2^2
+    r = JavaRandom(42)
+    v1, v2 = 2 * r.nextDouble() - 1, 2 * r.nextDouble() - 1
+    s = v1 * v1 + v2 * v2
+    assert float(strict_log(s)[0]) == -1.0746293630791768
+    assert float(np.log(s)) == -1.074629363079177
+
+
+@pytest.mark.parametrize("seed,count,chunks", [(12345, 1001, [1001]), (-7, 999, [1, 2, 333, 663]), (2**40 + 3, 64, [5, 59])])
+def test_vectorized_gaussians_equal_scalar_sequence(seed, count, chunks):
+    a = JavaRandom(seed)
+    want = np.array([a.nextGaussian() for _ in range(count)])
+    b = JavaRandom(seed)
+    got = np.concatenate([b.gaussians(c) for c in chunks])
+    assert np.array_equal(got, want)
+    # the generators continue identically (state and the kept second value)
+    assert [a.nextGaussian() for _ in range(3)] == list(b.gaussians(3))
+    assert a.nextInt() == b.nextInt()
+
+
+def test_strict_log_within_one_ulp_of_exact():
+    getcontext().prec = 50
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.random(300), np.exp(rng.uniform(-700, 700, 200)), [1e-310, 5e-324, 1.0, 2.0, 0.5,
+                                                                              1 + 2.0 ** -30, 1 - 2.0 ** -30]])
+    got = strict_log(x)
+    for xi, gi in zip(x, got):
+        exact = Decimal(float(xi)).ln()
+        ulp = Decimal(float(np.spacing(abs(gi)))) if gi != 0 else Decimal(0)
+        assert abs(Decimal(float(gi)) - exact) <= ulp, xi
+
+
+YAML = """
+simulationType: temporal optimized cgc ngp
+gridStep: 1
+timeStep: 0.25
+numberOfColors: 2
+numberOfThreads: 4
+gridCells: [32, 8, 8]
+initialConditions:
+  CGC:
+    poissonSolver: Improved Full
+    MVModel:
+      - direction: 0
+        orientation: 1
+        longitudinalLocation: 8.0
+        longitudinalWidth: 2.0
+        mu: 0.5
+        randomSeed: 3
+      - direction: 0
+        orientation: -1
+        longitudinalLocation: 24.0
+        longitudinalWidth: 2.0
+        mu: 0.5
+        ultravioletCutoffTransverse: 2.0
+        longitudinalCoherenceLength: 0.5
+        infraredCoefficient: 0.2
+"""
+
+
+def test_yaml_cgc_initial_conditions():
+    s = Settings.from_yaml(YAML)
+    assert s.number_of_threads == 4
+    assert len(s.initial_conditions) == 2
+    a, b = (ic.initialChargeDensity for ic in s.initial_conditions)
+    assert all(isinstance(ic, CGCInitialCondition) for ic in s.initial_conditions)
+    assert isinstance(a, MVModel) and isinstance(b, MVModel)
+    assert (a.orientation, a.location, a.useSeed, a.seed) == (1, 8.0, True, 3)
+    # YamlMVModel defaults (ui/util/yaml/initial/YamlMVModel.java:37-47)
+    assert (a.ultravioletCutoffTransverse, a.longitudinalCoherenceLength, a.infraredCoefficient) == (4.44, 0.0, 0.0)
+    assert (b.useSeed, b.ultravioletCutoffTransverse, b.longitudinalCoherenceLength, b.infraredCoefficient) == \
+        (False, 2.0, 0.5, 0.2)
+
+
+@pytest.mark.parametrize("patch", ["poissonSolver: improved", "poissonSolver: regular", "computeTadpole: true"])
+def test_yaml_cgc_refuses_what_is_not_built(patch):
+    text = YAML.replace("    poissonSolver: Improved Full", "    " + patch if "poisson" in patch else
+                        "    poissonSolver: Improved Full\n    " + patch)
+    if "computeTadpole" in patch:
+        s = Settings.from_yaml(text)
+        with pytest.raises(NotImplementedError):
+            s.initial_conditions[0].applyInitialCondition(None)
+    else:
+        with pytest.raises(NotImplementedError):
+            Settings.from_yaml(text)
+
+
+def test_mv_model_beam_direction():
+    with pytest.raises(NotImplementedError):
+        MVModel(1, 1, 8.0, 2.0, 0.5, True, 0, 4.44, 0.0, 0.0)

@@ -1,0 +1,262 @@
+"""The CGC initial condition on the device against the oracle (beam along x0):
+
+* MV charges from given noise, on power-of-two and other axis lengths (radix-2 and direct
+  transforms) and with an active region;
+* the light-cone solve on non-trivial links with a charge density already on the grid:
+  U, Unext, E and the Gauss violation of the Wilson-line copy, both orientations, FP64 and FP32;
+* the super particles: descriptors, refined charges, uneven subdivisions, a second nucleus
+  appended to the list;
+* a seeded YAML run of "temporal optimized cgc ngp" (JavaRandom noise, two nuclei, particles,
+  initialize and steps) against the same run of the oracle.
+
+FP64 bars are relative to the largest value: 1e-12 for the charges, fields and Gauss
+violation, 1e-11 for the refined charges (200 sweeps of a 4- and a 6-point stencil each).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from openpixi_amd import Grid, _lib
+from openpixi_amd.cgc import (CGCInitialCondition, ChargeDensity, JavaRandom, LightConeNGPSuperParticleCreator,
+                              LightConePoissonSolverImprovedFull, MVModel)
+from openpixi_amd.simulation import Settings, Simulation
+
+from .helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+SPACING = (1.0, 0.9, 0.9)
+AT = 0.25
+G = 1.3
+
+
+def grids(nc, dims, precision="fp64", spacing=SPACING, at=AT):
+    og = O.OracleGrid(nc, dims, spacing, at, G)
+    dg = Grid(nc, dims, list(spacing), at, G, precision)
+    return og, dg
+
+
+def mv_params(dims):
+    L = dims[0] * SPACING[0]
+    return dict(location=0.45 * L, width=0.1 * L, mu=0.8, uvt=2.5, long_width=0.6, ir=0.4)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+@pytest.mark.parametrize("dims,active", [((16, 8, 8), None), ((12, 6, 10), None), ((9, 5, 7), ((1, 0, 1), (7, 4, 5)))])
+def test_mv_charge_matches_oracle(gpu, nc, dims, active):
+    print(f"MV charge nc={nc} dims={dims} active={active}", flush=True)
+    og, dg = grids(nc, dims)
+    if active:
+        og.set_active_region(*active)
+        dg.setActiveRegion(*active)
+    noise = np.random.default_rng(3).normal(size=(nc * nc - 1) * og.n)
+    p = mv_params(dims)
+    want = og.mv_charge(p["location"], p["width"], p["mu"], p["uvt"], p["long_width"], p["ir"], noise)
+    m = MVModel(0, 1, p["location"], p["width"], p["mu"], True, 0, p["uvt"], p["long_width"], p["ir"])
+    m.initialize(dg, noise=noise)
+    got = m.getChargeDensity()
+    assert np.max(np.abs(want)) > 0
+    assert rel_err(got, want) < 1e-12
+
+
+def solve_setup(nc, dims, precision, seed=7):
+    """Both lattices with the same random links at both time levels and a grid charge density;
+    returns (og, dg, rho) with rho an MV charge density of the oracle."""
+    og, dg = grids(nc, dims, precision)
+    rng = np.random.default_rng(seed)
+    n = og.n
+    U = O.links_from_coeffs(nc, rng.normal(0, 0.3, (n, 3, nc * nc - 1)))
+    Un = O.links_from_coeffs(nc, rng.normal(0, 0.3, (n, 3, nc * nc - 1)))
+    grid_rho = O.algs_from_coeffs(nc, rng.normal(0, 0.05, (n, nc * nc - 1)))
+    og.U[:] = U
+    og.Unext[:] = Un
+    og.rho[:] = grid_rho
+    dg.setU(U)
+    dg.setUnext(Un)
+    dg.setRho(grid_rho)
+    p = mv_params(dims)
+    rho = og.mv_charge(p["location"], p["width"], p["mu"], p["uvt"], p["long_width"], p["ir"],
+                       rng.normal(size=(nc * nc - 1) * n))
+    return og, dg, rho
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+@pytest.mark.parametrize("orientation", [1, -1])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_lightcone_solve_matches_oracle(gpu, nc, orientation, precision, tol):
+    dims = (10, 6, 8)
+    print(f"light-cone solve nc={nc} orientation={orientation} {precision}", flush=True)
+    og, dg, rho = solve_setup(nc, dims, precision)
+    gauss = og.lightcone_solve(rho, orientation)
+    density = ChargeDensity(rho, 0, orientation)
+    density.initialize(dg)
+    solver = LightConePoissonSolverImprovedFull()
+    solver.initialize(dg)
+    solver.solve(density)
+    assert rel_err(dg.getU(), og.U) < tol
+    assert rel_err(dg.getUnext(), og.Unext) < tol
+    assert rel_err(dg.getE(), og.E) < tol
+    # the FP32 Gauss violation is evaluated in double from float-stored longitudinal links and rho
+    assert rel_err(solver.getGaussViolation(), gauss) < (tol if precision == "fp64" else 1e-4)
+
+
+def sp_lists_equal(sim_sp, arrs, charges, tol):
+    assert list(sim_sp.orientation) == list(arrs["orientation"])
+    assert list(sim_sp.number) == list(arrs["num"])
+    assert list(sim_sp.index_offset) == list(arrs["index_offset"])
+    assert list(sim_sp.per_plane) == list(arrs["per_plane"])
+    assert list(sim_sp.sub_lattice_shift) == list(arrs["shift"])
+    assert list(sim_sp.particles_per_cell) == list(arrs["ppc"])
+    want = np.concatenate(charges)
+    assert sim_sp.charges.shape == want.shape
+    assert rel_err(sim_sp.charges, want) < tol
+
+
+class _Sim:
+    """The parts of a Simulation the super-particle creator reads."""
+
+    def __init__(self, grid, threads):
+        self.grid = grid
+
+        class _S:
+            number_of_threads = threads
+            simulation_type = "TemporalOptimizedCGCNGP"
+
+        self.settings = _S()
+
+
+def device_superparticles(dg):
+    sim = Simulation.__new__(Simulation)
+    sim.grid = dg
+    return Simulation.superparticle_descriptors(sim)
+
+
+@pytest.mark.parametrize("nc,orientation,threads", [(2, 1, 3), (2, -1, 1), (3, 1, 4), (3, -1, 2)])
+def test_superparticles_match_oracle(gpu, nc, orientation, threads):
+    dims = (16, 4, 6)
+    print(f"super particles nc={nc} orientation={orientation} threads={threads}", flush=True)
+    og, dg, rho = solve_setup(nc, dims, "fp64", seed=11)
+    gauss = og.lightcone_solve(rho, orientation)
+    arrs, charges, (zs, bw) = og.lightcone_superparticles(gauss, orientation, threads)
+    density = ChargeDensity(rho, 0, orientation)
+    density.initialize(dg)
+    solver = LightConePoissonSolverImprovedFull()
+    solver.initialize(dg)
+    solver.solve(density)
+    creator = LightConeNGPSuperParticleCreator()
+    creator.initialize(_Sim(dg, threads), 0, orientation)
+    assert (creator.zStart, creator.blockWidth) == (zs, bw)
+    assert creator.added == len(arrs["num"])
+    sp_lists_equal(device_superparticles(dg), arrs, charges, 1e-11)
+
+
+def test_superparticles_of_a_narrow_block(gpu):
+    """A charge on a few planes only: the block is found from the plane maxima, and ws = 0
+    subdivisions (more threads than planes) take one plane each."""
+    nc, dims, orientation = 2, (20, 4, 4), 1
+    og, dg = grids(nc, dims)
+    rng = np.random.default_rng(5)
+    rho = np.zeros((og.n, 3))
+    T = dims[1] * dims[2]
+    rho[8 * T:11 * T] = rng.normal(0, 1.0, (3 * T, 3))
+    gauss = og.lightcone_solve(rho, orientation)
+    arrs, charges, (zs, bw) = og.lightcone_superparticles(gauss, orientation, 16)
+    assert 0 < bw < dims[0] - 2
+    ChargeDensity(rho, 0, orientation).initialize(dg)
+    solver = LightConePoissonSolverImprovedFull()
+    solver.initialize(dg)
+    solver.solve(ChargeDensity(rho, 0, orientation))
+    creator = LightConeNGPSuperParticleCreator()
+    creator.initialize(_Sim(dg, 16), 0, orientation)
+    assert (creator.zStart, creator.blockWidth) == (zs, bw)
+    sp_lists_equal(device_superparticles(dg), arrs, charges, 1e-11)
+
+
+YAML = """
+simulationType: temporal optimized cgc ngp
+gridStep: 1
+timeStep: 0.25
+duration: 1.0
+numberOfDimensions: 3
+numberOfColors: {nc}
+numberOfThreads: 3
+couplingConstant: 1.0
+gridCells: [24, 6, 6]
+initialConditions:
+  CGC:
+    poissonSolver: improved full
+    MVModel:
+      - direction: 0
+        orientation: 1
+        longitudinalLocation: 7.0
+        longitudinalWidth: 1.2
+        mu: 0.5
+        randomSeed: 11
+        ultravioletCutoffTransverse: 3.0
+        longitudinalCoherenceLength: 0.4
+        infraredCoefficient: 0.3
+      - direction: 0
+        orientation: -1
+        longitudinalLocation: 17.0
+        longitudinalWidth: 1.2
+        mu: 0.5
+        randomSeed: 12
+        ultravioletCutoffTransverse: 3.0
+        longitudinalCoherenceLength: 0.4
+        infraredCoefficient: 0.3
+"""
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_seeded_yaml_collision_matches_oracle(gpu, nc):
+    """Two nuclei from a reference-style YAML file: the device run (java.util.Random noise drawn
+    by JavaRandom, charges, fields, particle list, initialize and four steps) against the oracle
+    fed the same noise."""
+    print(f"YAML CGC collision nc={nc}", flush=True)
+    settings = Settings.from_yaml(YAML.format(nc=nc))
+    sim = Simulation(settings)
+    dims = tuple(settings.grid_cells)
+    og = O.OracleGrid(nc, dims, (1.0, 1.0, 1.0), 0.25, 1.0)
+    n = og.n
+    all_arrs, all_charges = None, []
+    for seed, orient, loc in ((11, 1, 7.0), (12, -1, 17.0)):
+        noise = JavaRandom(seed).gaussians((nc * nc - 1) * n)
+        rho = og.mv_charge(loc, 1.2, 0.5, 3.0, 0.4, 0.3, noise)
+        gauss = og.lightcone_solve(rho, orient)
+        arrs, charges, _ = og.lightcone_superparticles(gauss, orient, 3)
+        all_charges += charges
+        all_arrs = arrs if all_arrs is None else {k: np.concatenate([all_arrs[k], arrs[k]]) for k in arrs}
+    sp_lists_equal(sim.superparticle_descriptors(), all_arrs, all_charges, 1e-11)
+    assert rel_err(sim.grid.getU(), og.U) < 1e-12
+    assert rel_err(sim.grid.getUnext(), og.Unext) < 1e-12
+    assert rel_err(sim.grid.getE(), og.E) < 1e-12
+    og.set_superparticles(all_arrs["orientation"], all_arrs["num"], all_arrs["index_offset"],
+                          all_arrs["per_plane"], all_arrs["shift"], all_arrs["ppc"], all_charges)
+    og.initialize()
+    sim.initialize()
+    og.sim_step(4)
+    sim.step(4)
+    assert rel_err(sim.grid.getE(), og.E) < 1e-11
+    assert rel_err(sim.grid.getU(), og.U) < 1e-11
+    assert rel_err(sim.grid.getRho(), og.rho) < 1e-11
+    assert rel_err(sim.superparticle_charges(), og.superparticle_charges()) < 1e-11
+
+
+def test_cgc_entry_points_reject_bad_calls(gpu):
+    dg = Grid(2, (8, 4, 4), [1.0, 1.0, 1.0], 0.25, 1.0)
+    L = _lib.load()
+    assert L.opx_cgc_lightcone_solve(dg.handle, 1, None, 0) == _lib.OPX_E_STATE      # no charge density yet
+    assert L.opx_cgc_create_superparticles(dg.handle, 1, 1, None, None, None) == _lib.OPX_E_STATE
+    ChargeDensity(np.zeros((dg.total_cells, 3))).initialize(dg)
+    assert L.opx_cgc_lightcone_solve(dg.handle, 0, None, 0) == _lib.OPX_E_ARG         # orientation 0
+    assert L.opx_cgc_lightcone_solve(dg.handle, 1, None, 0) == 0
+    # no plane exceeds the cutoff of a zero violation: the reference's block is then every plane but
+    # the last, with zero charges
+    zs, bw = ctypes.c_int(), ctypes.c_int()
+    assert L.opx_cgc_create_superparticles(dg.handle, 1, 2, None, ctypes.byref(zs), ctypes.byref(bw)) == 0
+    assert (zs.value, bw.value) == (0, 7)
+    assert L.opx_cgc_create_superparticles(dg.handle, 1, 0, None, None, None) == _lib.OPX_E_ARG
+    z = np.zeros(5)
+    assert L.opx_cgc_mv_charge(dg.handle, 1.0, 1.0, 1.0, 1.0, 0.0, 0.0, _lib.dptr(z), z.size, None, 0) == _lib.OPX_E_ARG

@@ -12,8 +12,8 @@ of its code:
 * the MV charge profile and box: charge only inside the box, zero transverse mean.
 
 The reference draws its random charges with java.util.Random and transforms them with
-JTransforms: neither can be matched here, so the noise is an input (parity unpinned against
-the reference's random stream, as documented).
+JTransforms. Here the noise is an input: the package draws it with JavaRandom, pinned to
+the JDK's outputs in test_cgc_host.py, and the transforms are pinned to numpy's FFT above.
 """
 import numpy as np
 import pytest
