@@ -65,6 +65,7 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint line")
     p.add_argument("--no-implicit", action="store_true", help="skip the implicit-solver line")
     p.add_argument("--implicit-steps", type=int, default=20)
+    p.add_argument("--no-cgc", action="store_true", help="skip the CGC initial-condition line")
     p.add_argument("--adjoint-steps", type=int, default=200)
     p.add_argument("--fd-directions", type=int, default=256)
     p.add_argument("--probe-kernels", default="auto",
@@ -451,6 +452,114 @@ def time_implicit(steps, iterations, damping, stream):
     return out
 
 
+# Two MV nuclei of input/CGC/MV model/MVModel_new_optimized.yaml: (orientation, location as a
+# fraction of the lattice, seed); width 2, mu 0.2, UV cutoff 2, coherence 4, IR 0.2, g = 2, dt/a = 0.5
+CGC_NUCLEI = ((1, 0.25, 5), (-1, 0.75, 6))
+CGC_PARAMS = dict(width=2.0, mu=0.2, uvt=2.0, coherence=4.0, ir=0.2)
+
+
+def cgc_run(dims, precision, threads, stream=None, oracle=False):
+    """The reference's CGC initialization (two nuclei, each MVModel -> LightConePoissonSolverImprovedFull
+    -> LightConeNGPSuperParticleCreator) on `dims`, coupling 2 and dt/a = 0.5 as in the YAML file.
+    Returns per-stage seconds (wall clock; every device call synchronizes) and counts."""
+    from openpixi_amd.cgc import JavaRandom
+
+    n = int(np.prod(dims))
+    p = CGC_PARAMS
+    st = {"noise_host_s": 0.0, "mv_s": 0.0, "solve_s": 0.0, "particles_s": 0.0}
+    if oracle:
+        import oracle as O
+
+        og = O.OracleGrid(2, dims, (1.0, 1.0, 1.0), 0.5, 2.0)
+        charges = 0
+        for orient, frac, seed in CGC_NUCLEI:
+            t = time.perf_counter()
+            noise = JavaRandom(seed).gaussians(3 * n)
+            t1 = time.perf_counter()
+            rho = og.mv_charge(frac * dims[0], p["width"], p["mu"], p["uvt"], p["coherence"], p["ir"], noise)
+            t2 = time.perf_counter()
+            gauss = og.lightcone_solve(rho, orient)
+            t3 = time.perf_counter()
+            arrs, _, _ = og.lightcone_superparticles(gauss, orient, threads)
+            t4 = time.perf_counter()
+            charges += int(arrs["num"].sum())
+            for k, a, b in (("noise_host_s", t, t1), ("mv_s", t1, t2), ("solve_s", t2, t3), ("particles_s", t3, t4)):
+                st[k] += b - a
+        st["charges"] = charges
+        return st
+    from openpixi_amd import Grid, _lib
+    from openpixi_amd.cgc import LightConeNGPSuperParticleCreator, LightConePoissonSolverImprovedFull, MVModel
+
+    g = Grid(2, dims, 1.0, 0.5, 2.0, precision)
+    if stream is not None:
+        g._check(_lib.load().opx_set_stream(g.handle, ctypes_void(stream.cuda_stream)))
+
+    class _Sim:
+        grid = g
+
+        class settings:
+            number_of_threads = threads
+
+    blocks = []
+    for orient, frac, seed in CGC_NUCLEI:
+        t = time.perf_counter()
+        noise = JavaRandom(seed).gaussians(3 * n)
+        t1 = time.perf_counter()
+        m = MVModel(0, orient, frac * dims[0], p["width"], p["mu"], True, seed, p["uvt"], p["coherence"], p["ir"])
+        m.initialize(g, noise=noise)
+        t2 = time.perf_counter()
+        solver = LightConePoissonSolverImprovedFull()
+        solver.initialize(g)
+        solver.solve(m)
+        t3 = time.perf_counter()
+        c = LightConeNGPSuperParticleCreator()
+        c.initialize(_Sim, 0, orient)
+        t4 = time.perf_counter()
+        blocks.append((c.zStart, c.blockWidth))
+        for k, a, b in (("noise_host_s", t, t1), ("mv_s", t1, t2), ("solve_s", t2, t3), ("particles_s", t3, t4)):
+            st[k] += b - a
+    import ctypes
+
+    cnt, tot = ctypes.c_int(), ctypes.c_int64()
+    g._check(_lib.load().opx_superparticles_count(g.handle, ctypes.byref(cnt), ctypes.byref(tot)))
+    st.update({"super_particles": cnt.value, "charges": tot.value, "blocks": blocks})
+    # the charge density the particles deposit against the fields' Gauss law (block edges aside)
+    g.simulationInitialize()
+    from openpixi_amd import FieldMeasurements
+
+    m0 = FieldMeasurements().measure(g)
+    st["gauss_residual"] = float(m0[6])
+    st["charge_norm"] = float(m0[7])
+    g.close()
+    return st
+
+
+def time_cgc(stream, cpu=True):
+    """CGC initial condition of the reference's input/CGC/MV model/MVModel_new_optimized.yaml
+    (SU(2), 128 x 32 x 32, two MV nuclei, 8 subdivisions) and of the CPIC collision lattice
+    512 x 64 x 64, FP32 fields: wall-clock seconds per stage on the device, the host's
+    java.util.Random noise draw apart, and the FP64 oracle on the YAML lattice beside it."""
+    import torch
+
+    cgc_run((16, 8, 8), "fp32", 8, stream)   # loads the kernels
+    torch.cuda.synchronize()
+    out = {"workload": "CGC initial condition (MVModel x 2, LightConePoissonSolverImprovedFull, "
+                       "LightConeNGPSuperParticleCreator), SU(2), FP32 fields, coupling 2, dt/a = 0.5"}
+    for name, dims in (("yaml_128x32x32", (128, 32, 32)), ("cpic_512x64x64", (512, 64, 64))):
+        r = cgc_run(dims, "fp32", 8, stream)
+        r["device_s"] = r["mv_s"] + r["solve_s"] + r["particles_s"]
+        out[name] = r
+        print(f"# cgc {name}: {json.dumps(r)}", file=sys.stderr, flush=True)
+    if cpu:
+        r = cgc_run((128, 32, 32), "fp64", 8, oracle=True)
+        r["device_s"] = r["mv_s"] + r["solve_s"] + r["particles_s"]
+        r["cores"] = 1
+        r["kind"] = "port"
+        out["cpu_oracle_yaml_128x32x32"] = r
+        out["speedup_yaml_128x32x32"] = r["device_s"] / out["yaml_128x32x32"]["device_s"]
+    return out
+
+
 def ctypes_void(p):
     import ctypes
 
@@ -512,6 +621,10 @@ def main():
     if not args.no_implicit:
         implicit = time_implicit(args.implicit_steps, 2, 0.25, stream)
         print(f"# implicit: {json.dumps(implicit)}", file=sys.stderr, flush=True)
+    cgc = None
+    if not args.no_cgc:
+        cgc = time_cgc(stream, cpu=not args.no_cpu_baseline)
+        print(f"# cgc: {json.dumps(cgc)}", file=sys.stderr, flush=True)
     adjoint = None
     if not args.no_adjoint:
         adjoint = time_adjoint(args.adjoint_steps, args.fd_directions, stream)
@@ -568,6 +681,8 @@ def main():
         out["adjoint"] = adjoint
     if implicit is not None:
         out["implicit"] = implicit
+    if cgc is not None:
+        out["cgc_init"] = cgc
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     out["host"] = platform.node()

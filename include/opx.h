@@ -257,6 +257,15 @@ int opx_cgc_create_superparticles(opx_ctx *ctx, int orientation, int num_threads
  * last one (LightConePoissonSolverImprovedFull.getGaussViolation), cells * A reals */
 enum { OPX_CGC_CHARGE = 0, OPX_CGC_GAUSS = 1 };
 int opx_cgc_download(opx_ctx *ctx, int which, double *out, int64_t n_values);
+/* WilsonLineObservables (physics/initial/CGC/WilsonLineObservables.java:36-131) on the Wilson
+ * lines the last solve left on the transverse plane behind the nucleus (getV: x0 = 0 for
+ * orientation +1, N0 - 1 for -1). Tadpole: the mean of Re tr V. Dipole: tr(V_x V_y^+) averaged
+ * over the pairs in each of num_bins = (int)sqrt(N1 N2) / 2 bins of periodic transverse distance,
+ * bin_width = min(N1 a1, N2 a2) / 2 / num_bins; corr and counts hold `capacity` >= num_bins
+ * entries (both NULL: only num_bins and bin_width are returned). An empty bin reads 0/0. */
+int opx_cgc_wilson_tadpole(opx_ctx *ctx, double *value);
+int opx_cgc_wilson_dipole(opx_ctx *ctx, int capacity, int *num_bins, double *bin_width, double *corr,
+                          int64_t *counts);
 /* frees the charge density and Gauss violation (IInitialChargeDensity.clear) */
 int opx_cgc_clear(opx_ctx *ctx);
 

@@ -15,6 +15,7 @@ from __future__ import annotations
 import ctypes
 import math
 import os
+from decimal import ROUND_HALF_UP, Decimal
 
 import numpy as np
 
@@ -233,6 +234,12 @@ class MVModel:
         """(cells, A) in reference algebra storage."""
         return _download(self._grid, _lib.CGC_CHARGE)
 
+    def getInfo(self) -> str:
+        """MVModel.getInfo (MVModel.java:175-185)."""
+        f = java_format_f
+        return (f"MV, mu: {f(self.mu)}, w: {f(self.longitudinalWidth)}, UVT: {f(self.ultravioletCutoffTransverse)}, "
+                f"c: {f(self.longitudinalCoherenceLength)}, m: {f(self.infraredCoefficient)}")
+
     def getDirection(self):
         return self.direction
 
@@ -263,6 +270,9 @@ class ChargeDensity:
     def getChargeDensity(self):
         return self.rho.reshape(-1, _alg_size(self._grid.getNumberOfColors()))
 
+    def getInfo(self) -> str:
+        return "charge density"
+
     def getDirection(self):
         return self.direction
 
@@ -278,6 +288,62 @@ def _download(g, which):
     out = np.zeros(g.getTotalNumberOfCells() * _alg_size(g.getNumberOfColors()))
     g._check(_lib.load().opx_cgc_download(g.handle, which, _lib.dptr(out), out.size))
     return out.reshape(g.getTotalNumberOfCells(), -1)
+
+
+def java_format_f(x: float, precision: int = 6) -> str:
+    """String.format("%f") of a double (java.util.Formatter): the digits of Double.toString
+    rounded half up to `precision` decimals."""
+    x = float(x)
+    if math.isnan(x):
+        return "NaN"
+    if math.isinf(x):
+        return "Infinity" if x > 0 else "-Infinity"
+    q = Decimal(repr(x)).quantize(Decimal(1).scaleb(-precision), rounding=ROUND_HALF_UP)
+    return format(q, "f")
+
+
+class WilsonLineObservables:
+    """physics/initial/CGC/WilsonLineObservables.java: tadpole and dipole of the Wilson lines the
+    light-cone solve leaves on the transverse plane behind the nucleus, appended to text files
+    in the reference's format (FileFunctions.getFile paths)."""
+
+    def __init__(self, s, solver, density):
+        self.grid = _grid(s)
+        self.solver = solver
+        self.density = density
+
+    def tadpole(self) -> float:
+        v = ctypes.c_double()
+        self.grid._check(_lib.load().opx_cgc_wilson_tadpole(self.grid.handle, ctypes.byref(v)))
+        return v.value
+
+    def dipole(self):
+        """(bin distances bin * ds, mean tr(V_x V_y^+) per bin, pair counts)."""
+        L = _lib.load()
+        nb, ds = ctypes.c_int(), ctypes.c_double()
+        self.grid._check(L.opx_cgc_wilson_dipole(self.grid.handle, 0, ctypes.byref(nb), ctypes.byref(ds), None, None))
+        corr = np.zeros(max(nb.value, 1))
+        counts = np.zeros(max(nb.value, 1), np.int64)
+        self.grid._check(L.opx_cgc_wilson_dipole(self.grid.handle, len(corr), None, None, _lib.dptr(corr),
+                                                 counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        n = nb.value
+        return np.arange(n) * ds.value, corr[:n], counts[:n]
+
+    def computeTadpole(self, filename: str):
+        from .diagnostics import java_sci, output_path
+
+        with open(output_path(filename), "a") as f:
+            f.write(self.density.getInfo() + ", V: " + java_sci(self.tadpole()) + "\n")
+
+    def computeDipole(self, filename: str):
+        from .diagnostics import java_sci, output_path
+
+        d, corr, _ = self.dipole()
+        with open(output_path(filename), "a") as f:
+            f.write(self.density.getInfo() + "\n")
+            f.write("d\ttr(V V^t)\n")
+            for b in range(len(corr)):
+                f.write(java_sci(d[b]) + "\t" + java_sci(corr[b]) + "\n")
 
 
 class LightConePoissonSolverImprovedFull:
@@ -335,7 +401,9 @@ class CGCInitialCondition:
         self.solver = LightConePoissonSolverImprovedFull()
         self.initialParticleCreator = None
         self.computeTadpole = False
+        self.tadpoleFilename = "tadpole.txt"
         self.computeDipole = False
+        self.dipoleFilename = "dipole.txt"
 
     def setInitialChargeDensity(self, rho):
         self.initialChargeDensity = rho
@@ -346,8 +414,6 @@ class CGCInitialCondition:
         self.solver = solver
 
     def applyInitialCondition(self, s):
-        if self.computeTadpole or self.computeDipole:
-            raise NotImplementedError("Wilson-line observables (tadpole, dipole) are not built")
         sim_type = getattr(getattr(s, "settings", None), "simulation_type", "TemporalOptimizedCGCNGP")
         if sim_type not in ("TemporalOptimizedCGCNGP", "TemporalOptimizedImplicitCGCNGP"):
             raise NotImplementedError(f"CGC particles for {sim_type}: only the super-particle (optimized) types are built")
@@ -356,6 +422,11 @@ class CGCInitialCondition:
         orientation = self.initialChargeDensity.getOrientation()
         self.solver.initialize(s)
         self.solver.solve(self.initialChargeDensity)
+        wilson = WilsonLineObservables(s, self.solver, self.initialChargeDensity)
+        if self.computeTadpole:
+            wilson.computeTadpole(self.tadpoleFilename)
+        if self.computeDipole:
+            wilson.computeDipole(self.dipoleFilename)
         self.initialParticleCreator = LightConeNGPSuperParticleCreator()
         self.initialParticleCreator.setGaussConstraint()
         self.initialParticleCreator.initialize(s, direction, orientation)
@@ -387,5 +458,7 @@ def from_yaml_cgc(y: dict) -> list:
         ic.setPoissonSolver(_POISSON_SOLVERS[key]())
         ic.computeTadpole = bool(y.get("computeTadpole", False))
         ic.computeDipole = bool(y.get("computeDipole", False))
+        ic.tadpoleFilename = str(y.get("tadpoleFilename", "tadpole.txt"))
+        ic.dipoleFilename = str(y.get("dipoleFilename", "dipole.txt"))
         out.append(ic)
     return out

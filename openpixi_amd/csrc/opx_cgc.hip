@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <utility>
+
 #include "opx_group.h"
 #include "opx_kernels.h"
 
@@ -515,37 +517,18 @@ __device__ __forceinline__ int wrap(int i, int n)
     return i;
 }
 
-// Charges of the super particles of one block and their refinement along the beam
-// (LightConeNGPSuperParticleCreator.java:197-245, refine2/refine4 :248-307). One thread per
-// (component, transverse cell) walks its list of bw * ppc charges in `work` (element l at
-// work[l * ad * nT + thread]), then writes each charge to its super particle: subdivision j of the
-// block covers planes [P_j, P_j + w_j), particle k of it holds w_j * nT charges starting at
-// q_base + ppc nT P_j + k w_j nT.
-template <typename T>
-__global__ __launch_bounds__(kB) void k_sp_charges(uint32_t nT, uint32_t n, uint32_t Lz, int ad, SpBlock b,
-                                                   const double* __restrict__ gauss, double* __restrict__ work,
-                                                   T* __restrict__ q, int64_t q_total, int64_t q_base)
+// One sweep of the 2nd- or 4th-order charge refinement over a periodic list of nn charges
+// (refine2/refine4, LightConeNGPSuperParticleCreator.java:248-307), element l at col[l * st];
+// the last charge of each NGP cell is not moved. Written through memory as the reference does,
+// so that short lists whose stencil wraps onto itself behave the same.
+template <int ORDER>
+__device__ __forceinline__ void refine_sweep(double* col, size_t st, int nn, int ppc)
 {
-    const uint32_t g = blockIdx.x * kB + threadIdx.x;
-    if (g >= (uint32_t)ad * nT) return;
-    const uint32_t comp = g / nT, t = g - comp * nT;
-    const int ppc = b.ppc, nn = b.bw * ppc;
-    const size_t st = (size_t)ad * nT;
-    double* col = work + g;
-    const double inv = 1.0 / ppc;
-    for (int p = 0; p < b.bw; p++) {
-        const uint32_t z = (uint32_t)(b.zs + p), zn = (z + 1) % Lz;
-        const double qa = gauss[(size_t)comp * n + (size_t)z * nT + t] * inv;
-        const double qb = gauss[(size_t)comp * n + (size_t)zn * nT + t] * inv;
-        for (int k = 0; k < ppc; k++) col[(size_t)(p * ppc + k) * st] = k < ppc / 2 ? qa : qb;
-    }
-    // 2nd order: the last charge of an NGP cell is not moved
 #pragma unroll 1
-    for (int it = 0; it < 100; it++)
-#pragma unroll 1
-        for (int i = 0; i < nn; i++) {
-            const int jm = (i + ppc / 2) % ppc;
-            if (jm >= ppc - 1) continue;
+    for (int i = 0; i < nn; i++) {
+        const int jm = (i + ppc / 2) % ppc;
+        if (jm >= ppc - 1) continue;
+        if constexpr (ORDER == 2) {
             double* Q0 = col + (size_t)wrap(i - 1, nn) * st;
             double* Q1 = col + (size_t)wrap(i, nn) * st;
             double* Q2 = col + (size_t)wrap(i + 1, nn) * st;
@@ -557,14 +540,7 @@ __global__ __launch_bounds__(kB) void k_sp_charges(uint32_t nT, uint32_t n, uint
             dq *= 1.0 / 4.0;
             *Q1 += dq * -1.0;
             *Q2 += dq * 1.0;
-        }
-    // 4th order
-#pragma unroll 1
-    for (int it = 0; it < 100; it++)
-#pragma unroll 1
-        for (int i = 0; i < nn; i++) {
-            const int jm = (i + ppc / 2) % ppc;
-            if (jm >= ppc - 1) continue;
+        } else {
             double* Q[6];
 #pragma unroll
             for (int m = 0; m < 6; m++) Q[m] = col + (size_t)wrap(i - 2 + m, nn) * st;
@@ -578,7 +554,121 @@ __global__ __launch_bounds__(kB) void k_sp_charges(uint32_t nT, uint32_t n, uint
             *Q[2] += dq * -1.0;
             *Q[3] += dq * 1.0;
         }
-    const int wl = b.ws + 1;
+    }
+}
+
+// The initial charges of a block (LightConeNGPSuperParticleCreator.java:197-215): one thread per
+// (component, transverse cell) g fills its list of bw * ppc charges, work[l * ad * nT + g] =
+// G(ngp) / ppc, with ngp the cell itself for the first ppc / 2 charges of a cell and its +x0
+// neighbour for the rest.
+__global__ __launch_bounds__(kB) void k_sp_init(uint32_t nT, uint32_t n, uint32_t Lz, int ad, SpBlock b,
+                                                const double* __restrict__ gauss, double* __restrict__ work)
+{
+    const uint32_t g = blockIdx.x * kB + threadIdx.x;
+    if (g >= (uint32_t)ad * nT) return;
+    const uint32_t comp = g / nT, t = g - comp * nT;
+    const int ppc = b.ppc;
+    const size_t st = (size_t)ad * nT;
+    const double inv = 1.0 / ppc;
+    for (int p = 0; p < b.bw; p++) {
+        const uint32_t z = (uint32_t)(b.zs + p), zn = (z + 1) % Lz;
+        const double qa = gauss[(size_t)comp * n + (size_t)z * nT + t] * inv;
+        const double qb = gauss[(size_t)comp * n + (size_t)zn * nT + t] * inv;
+        for (int k = 0; k < ppc; k++) work[(size_t)(p * ppc + k) * st + g] = k < ppc / 2 ? qa : qb;
+    }
+}
+
+// 100 sweeps of each order on every list, one thread per list (long lists)
+__global__ __launch_bounds__(kB) void k_sp_refine(uint32_t cols, int nn, int ppc, double* __restrict__ work)
+{
+    const uint32_t g = blockIdx.x * kB + threadIdx.x;
+    if (g >= cols) return;
+#pragma unroll 1
+    for (int it = 0; it < 100; it++) refine_sweep<2>(work + g, cols, nn, ppc);
+#pragma unroll 1
+    for (int it = 0; it < 100; it++) refine_sweep<4>(work + g, cols, nn, ppc);
+}
+
+// The matrix of one sweep, S[l * nn + c] = (sweep e_c)[l]: thread c runs the sweep on the unit vector c.
+template <int ORDER>
+__global__ __launch_bounds__(kB) void k_sweep_matrix(int nn, int ppc, double* __restrict__ S)
+{
+    const int c = blockIdx.x * kB + threadIdx.x;
+    if (c >= nn) return;
+    for (int l = 0; l < nn; l++) S[(size_t)l * nn + c] = l == c ? 1.0 : 0.0;
+    refine_sweep<ORDER>(S + c, (size_t)nn, nn, ppc);
+}
+
+__global__ __launch_bounds__(kB) void k_identity(int nn, double* __restrict__ R)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < (uint64_t)nn * nn) R[i] = (i / nn == i % nn) ? 1.0 : 0.0;
+}
+
+// C[M x N] = A[M x K] B[K x N], row-major doubles, fixed summation order over K. 64 x 64 tiles,
+// 256 threads of 4 x 4 outputs (rows ty + 16 i, columns tx + 16 j), K in LDS slabs of 16.
+constexpr int kGemmTile = 64, kGemmK = 16;
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const double* __restrict__ A,
+                                              const double* __restrict__ B, double* __restrict__ C)
+{
+    __shared__ double As[kGemmK][kGemmTile + 1];
+    __shared__ double Bs[kGemmK][kGemmTile];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int row0 = blockIdx.y * kGemmTile, col0 = blockIdx.x * kGemmTile;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int k0 = 0; k0 < K; k0 += kGemmK) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int f = tid + 256 * e;
+            // A slab: 64 rows x 16 k, k fastest in memory
+            const int ar = f / kGemmK, ak = f % kGemmK;
+            const int gr = row0 + ar, gk = k0 + ak;
+            As[ak][ar] = (gr < M && gk < K) ? A[(size_t)gr * K + gk] : 0.0;
+            // B slab: 16 k x 64 columns, columns fastest
+            const int bk = f / kGemmTile, bc = f % kGemmTile;
+            const int hk = k0 + bk, hc = col0 + bc;
+            Bs[bk][bc] = (hk < K && hc < N) ? B[(size_t)hk * N + hc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kGemmK; k++) {
+            double a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = As[k][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bb[j] = Bs[k][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * bb[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = row0 + ty + 16 * i, c = col0 + tx + 16 * j;
+            if (r < M && c < N) C[(size_t)r * N + c] = acc[i][j];
+        }
+}
+
+// Each refined charge to its super particle (LightConeNGPSuperParticleCreator.java:177-215):
+// subdivision j of the block covers planes [P_j, P_j + w_j), particle k of it holds w_j * nT
+// charges starting at q_base + ppc nT P_j + k w_j nT.
+template <typename T>
+__global__ __launch_bounds__(kB) void k_sp_scatter(uint32_t nT, int ad, SpBlock b, const double* __restrict__ work,
+                                                   T* __restrict__ q, int64_t q_total, int64_t q_base)
+{
+    const uint32_t g = blockIdx.x * kB + threadIdx.x;
+    if (g >= (uint32_t)ad * nT) return;
+    const uint32_t comp = g / nT, t = g - comp * nT;
+    const int ppc = b.ppc, wl = b.ws + 1;
+    const size_t st = (size_t)ad * nT;
     for (int p = 0; p < b.bw; p++) {
         int P, w;
         if (p < b.nl * wl) {
@@ -590,8 +680,102 @@ __global__ __launch_bounds__(kB) void k_sp_charges(uint32_t nT, uint32_t n, uint
         }
         for (int k = 0; k < ppc; k++) {
             const int64_t at = q_base + (int64_t)ppc * nT * P + (int64_t)k * w * nT + (int64_t)(p - P) * nT + t;
-            q[(size_t)comp * q_total + at] = (T)col[(size_t)(p * ppc + k) * st];
+            q[(size_t)comp * q_total + at] = (T)work[(size_t)(p * ppc + k) * st + g];
         }
+    }
+}
+
+// ------------------------------------------------------------------ Wilson-line observables
+// Re tr(V_i V_j^+) of two elements of a [real][T] array (GroupElement.mult(adj()).getRealTrace())
+template <int NC>
+__device__ __forceinline__ double re_tr_adj(const double* __restrict__ V, uint32_t T, uint32_t i, uint32_t j)
+{
+    using GD = Group<NC, double>;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < GD::G; k++) s += V[(size_t)k * T + i] * V[(size_t)k * T + j];
+    return NC == 2 ? 2.0 * s : s;
+}
+
+// sum over the plane of Re tr V (GroupElement.getRealTrace): one workgroup, fixed order
+template <int NC>
+__global__ __launch_bounds__(kB) void k_tadpole(uint32_t T, const double* __restrict__ V, double* __restrict__ out)
+{
+    __shared__ double red[kB];
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < T; i += kB)
+        s += NC == 2 ? 2.0 * V[i] : V[i] + V[(size_t)4 * T + i] + V[(size_t)8 * T + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// binned tr(V_x V_y^+) over all pairs (WilsonLineObservables.computeDipole,
+// WilsonLineObservables.java:86-115): thread x walks every y in order and adds into its own
+// column of the [bin][T] partial sums and counts
+template <int NC>
+__global__ __launch_bounds__(kB) void k_dipole_pairs(uint32_t n1, uint32_t n2, double a1, double a2, double ds,
+                                                     int nbins, const double* __restrict__ V, double* __restrict__ psum,
+                                                     unsigned long long* __restrict__ pcnt)
+{
+    const uint32_t T = n1 * n2;
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= T) return;
+    for (int b = 0; b < nbins; b++) {
+        psum[(size_t)b * T + i] = 0.0;
+        pcnt[(size_t)b * T + i] = 0ull;
+    }
+    const int x0 = (int)(i / n2), x1 = (int)(i % n2);
+    const int h1 = (int)n1 / 2, h2 = (int)n2 / 2;
+#pragma unroll 1
+    for (uint32_t j = 0; j < T; j++) {
+        const int y0 = (int)(j / n2), y1 = (int)(j % n2);
+        double d0 = fabs((double)(x0 - y0)), d1 = fabs((double)(x1 - y1));
+        if (d0 > h1) d0 -= n1;
+        if (d1 > h2) d1 -= n2;
+        double dist = 0.0;
+        dist += d0 * d0 * a1 * a1;
+        dist += d1 * d1 * a2 * a2;
+        dist = sqrt(dist);
+        const int bin = (int)(dist / ds);
+        if (bin < nbins) {
+            psum[(size_t)bin * T + i] += re_tr_adj<NC>(V, T, i, j);
+            pcnt[(size_t)bin * T + i] += 1ull;
+        }
+    }
+}
+
+// per bin: the mean over all pairs (0/0 = NaN for an empty bin, as the reference divides)
+__global__ __launch_bounds__(kB) void k_dipole_bins(uint32_t T, const double* __restrict__ psum,
+                                                    const unsigned long long* __restrict__ pcnt,
+                                                    double* __restrict__ corr, unsigned long long* __restrict__ cnt)
+{
+    __shared__ double rs[kB];
+    __shared__ unsigned long long rc[kB];
+    const uint32_t b = blockIdx.x;
+    double s = 0.0;
+    unsigned long long c = 0;
+    for (uint32_t i = threadIdx.x; i < T; i += kB) {
+        s += psum[(size_t)b * T + i];
+        c += pcnt[(size_t)b * T + i];
+    }
+    rs[threadIdx.x] = s;
+    rc[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = kB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            rs[threadIdx.x] += rs[threadIdx.x + w];
+            rc[threadIdx.x] += rc[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        corr[b] = rs[0] / (double)rc[0];
+        cnt[b] = rc[0];
     }
 }
 
@@ -720,15 +904,102 @@ hipError_t cgc_plane_max(int nc, const Lattice& L, const double* gauss, double* 
     return hipGetLastError();
 }
 
+size_t cgc_sp_work(const SpBlock& b, uint32_t nT, int ad)
+{
+    const size_t nn = (size_t)b.bw * b.ppc, cols = (size_t)ad * nT;
+    return nn * cols * (nn <= (size_t)kRefineMatrixMax ? 2 : 1) + (nn <= (size_t)kRefineMatrixMax ? 5 * nn * nn : 0);
+}
+
+namespace {
+
+hipError_t gemm(int M, int N, int K, const double* A, const double* B, double* C, hipStream_t st)
+{
+    dim3 grid((unsigned)((N + kGemmTile - 1) / kGemmTile), (unsigned)((M + kGemmTile - 1) / kGemmTile));
+    k_gemm<<<grid, 256, 0, st>>>(M, N, K, A, B, C);
+    return hipGetLastError();
+}
+
+// R = S^e by binary powering; P and tmp are scratch matrices; returns the buffer holding R
+double* matrix_power(int nn, double* S, int e, double* R, double* P, double* tmp, hipStream_t st, hipError_t* err)
+{
+    const size_t m = (size_t)nn * nn;
+    k_identity<<<blocks(m), kB, 0, st>>>(nn, R);
+    hipMemcpyAsync(P, S, m * sizeof(double), hipMemcpyDeviceToDevice, st);
+    while (e) {
+        if (e & 1) {
+            gemm(nn, nn, nn, R, P, tmp, st);
+            std::swap(R, tmp);
+        }
+        e >>= 1;
+        if (e) {
+            gemm(nn, nn, nn, P, P, tmp, st);
+            std::swap(P, tmp);
+        }
+    }
+    *err = hipGetLastError();
+    return R;
+}
+
+}  // namespace
+
 hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, const double* gauss, double* work,
                           void* q, int64_t q_total, int64_t q_base, hipStream_t st)
 {
     const int ad = nc == 2 ? 3 : 8;
-    const unsigned grid = blocks((uint64_t)ad * L.s0);
+    const uint32_t cols = (uint32_t)ad * L.s0;
+    const int nn = b.bw * b.ppc;
+    const unsigned grid = blocks(cols);
+    k_sp_init<<<grid, kB, 0, st>>>(L.s0, L.n, L.d0, ad, b, gauss, work);
+    const double* refined = work;
+    if (nn <= kRefineMatrixMax) {
+        // every list goes through the same linear map: one sweep's matrix per order, raised to the
+        // 100th power, and (R4 R2) applied to all lists at once
+        double* out = work + (size_t)nn * cols;
+        double* m = out + (size_t)nn * cols;
+        const size_t mm = (size_t)nn * nn;
+        double *S = m, *X = m + mm, *Y = m + 2 * mm, *Z = m + 3 * mm, *W = m + 4 * mm;
+        hipError_t e = hipSuccess;
+        const unsigned sg = blocks((uint64_t)nn);
+        k_sweep_matrix<2><<<sg, kB, 0, st>>>(nn, b.ppc, S);
+        double* R2 = matrix_power(nn, S, 100, X, Y, Z, st, &e);
+        if (e != hipSuccess) return e;
+        // keep R2 in W while the 4th-order power runs in the other buffers
+        hipMemcpyAsync(W, R2, mm * sizeof(double), hipMemcpyDeviceToDevice, st);
+        k_sweep_matrix<4><<<sg, kB, 0, st>>>(nn, b.ppc, S);
+        double* R4 = matrix_power(nn, S, 100, X, Y, Z, st, &e);
+        if (e != hipSuccess) return e;
+        double* Rt = R4 == X ? Y : X;
+        gemm(nn, nn, nn, R4, W, Rt, st);
+        gemm(nn, (int)cols, nn, Rt, work, out, st);
+        refined = out;
+    } else {
+        k_sp_refine<<<grid, kB, 0, st>>>(cols, nn, b.ppc, work);
+    }
     if (prec == 0)
-        k_sp_charges<double><<<grid, kB, 0, st>>>(L.s0, L.n, L.d0, ad, b, gauss, work, (double*)q, q_total, q_base);
+        k_sp_scatter<double><<<grid, kB, 0, st>>>(L.s0, ad, b, refined, (double*)q, q_total, q_base);
     else
-        k_sp_charges<float><<<grid, kB, 0, st>>>(L.s0, L.n, L.d0, ad, b, gauss, work, (float*)q, q_total, q_base);
+        k_sp_scatter<float><<<grid, kB, 0, st>>>(L.s0, ad, b, refined, (float*)q, q_total, q_base);
+    return hipGetLastError();
+}
+
+hipError_t cgc_tadpole(int nc, uint32_t T, const double* V, double* out, hipStream_t st)
+{
+    if (nc == 2)
+        k_tadpole<2><<<1, kB, 0, st>>>(T, V, out);
+    else
+        k_tadpole<3><<<1, kB, 0, st>>>(T, V, out);
+    return hipGetLastError();
+}
+
+hipError_t cgc_dipole(int nc, uint32_t n1, uint32_t n2, double a1, double a2, double ds, int nbins, const double* V,
+                      double* psum, unsigned long long* pcnt, double* corr, unsigned long long* cnt, hipStream_t st)
+{
+    const uint32_t T = n1 * n2;
+    if (nc == 2)
+        k_dipole_pairs<2><<<blocks(T), kB, 0, st>>>(n1, n2, a1, a2, ds, nbins, V, psum, pcnt);
+    else
+        k_dipole_pairs<3><<<blocks(T), kB, 0, st>>>(n1, n2, a1, a2, ds, nbins, V, psum, pcnt);
+    if (nbins > 0) k_dipole_bins<<<(unsigned)nbins, kB, 0, st>>>(T, psum, pcnt, corr, cnt);
     return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@ namespace opx_host {
 struct CgcWork {
     double* rho = nullptr;     // charge density of the next light-cone solve
     double* gauss = nullptr;   // Gauss violation of the last light-cone solve
+    double* VT = nullptr;      // [real][N1 N2] Wilson lines of the last solve on the plane behind the nucleus
 };
 
 void cgc_release(opx_ctx* c)
@@ -27,6 +28,7 @@ void cgc_release(opx_ctx* c)
     if (!c || !c->cgc) return;
     if (c->cgc->rho) (void)hipFree(c->cgc->rho);
     if (c->cgc->gauss) (void)hipFree(c->cgc->gauss);
+    if (c->cgc->VT) (void)hipFree(c->cgc->VT);
     delete c->cgc;
     c->cgc = nullptr;
 }
@@ -211,6 +213,14 @@ int opx_cgc_lightcone_solve(opx_ctx* ctx, int orientation, double* gauss_out, in
     // Wilson lines at -at/2 and +at/2, the fields and the Gauss violation of the Wilson-line copy
     const int mfrac = (int)(32 * ctx->at / ctx->as[0] * 0.5);
     CGC_TRY(opx::cgc_wilson_lines(ctx->nc, L, orientation, mfrac, phi, Wz, Vm, Vp, ctx->stream), "Wilson lines");
+    // the lines on the plane "far behind" the nucleus (LightConePoissonSolverImprovedFull.java:157-167)
+    const size_t T = (size_t)ctx->dims[1] * ctx->dims[2];
+    if (!ctx->cgc->VT) CGC_TRY(hipMalloc((void**)&ctx->cgc->VT, (size_t)ctx->G * T * sizeof(double)), "alloc VT");
+    const size_t zb = orientation > 0 ? 0 : (size_t)ctx->dims[0] - 1;
+    for (int k = 0; k < ctx->G; k++)
+        CGC_TRY(hipMemcpyAsync(ctx->cgc->VT + k * T, Vm + (size_t)k * ctx->n + zb * T, T * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream),
+                "copy VT");
     CGC_TRY(opx::cgc_links(ctx->nc, ctx->prec, L, Vm, Vp, ctx->U, ctx->Unext, ctx->E, ctx->stream), "light-cone links");
     CGC_TRY(opx::cgc_gauss(ctx->nc, ctx->prec, L, ctx->active, Vm, Vp, ctx->U, ctx->Unext, ctx->rho, ctx->cgc->gauss,
                            ctx->stream),
@@ -317,7 +327,7 @@ int opx_cgc_create_superparticles(opx_ctx* ctx, int orientation, int num_threads
         e = hipMemcpyAsync((char*)dq + (size_t)k * total * ctx->tsz, (const char*)ctx->sp_q + (size_t)k * old_total * ctx->tsz,
                            (size_t)old_total * ctx->tsz, hipMemcpyDeviceToDevice, ctx->stream);
     double* work = nullptr;
-    if (e == hipSuccess) e = s.alloc(&work, (size_t)bw * ppc * ctx->AD * nT);
+    if (e == hipSuccess) e = s.alloc(&work, opx::cgc_sp_work(b, (uint32_t)nT, ctx->AD));
     if (e == hipSuccess)
         e = opx::cgc_sp_charges(ctx->nc, ctx->prec, L, b, ctx->cgc->gauss, work, dq, total, old_total, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -350,6 +360,62 @@ int opx_cgc_download(opx_ctx* ctx, int which, double* out, int64_t n_values)
     const double* src = !ctx->cgc ? nullptr : (which == OPX_CGC_CHARGE ? ctx->cgc->rho : ctx->cgc->gauss);
     if (!src) return fail(ctx, OPX_E_STATE, "CGC download: not computed");
     return download_alg(ctx, src, out);
+}
+
+int opx_cgc_wilson_tadpole(opx_ctx* ctx, double* value)
+{
+    using namespace opx_host;
+    if (!ctx) return OPX_E_ARG;
+    if (!value) return fail(ctx, OPX_E_ARG, "tadpole: null output");
+    if (!ctx->cgc || !ctx->cgc->VT) return fail(ctx, OPX_E_STATE, "tadpole: no light-cone solve yet");
+    const uint32_t T = (uint32_t)(ctx->dims[1] * ctx->dims[2]);
+    Scratch s(ctx);
+    double* d = nullptr;
+    CGC_TRY(s.alloc(&d, 1), "alloc tadpole");
+    CGC_TRY(opx::cgc_tadpole(ctx->nc, T, ctx->cgc->VT, d, ctx->stream), "tadpole");
+    double sum = 0.0;
+    CGC_TRY(hipMemcpyAsync(&sum, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "download tadpole");
+    CGC_TRY(hipStreamSynchronize(ctx->stream), "tadpole sync");
+    *value = sum / T;
+    return OPX_OK;
+}
+
+int opx_cgc_wilson_dipole(opx_ctx* ctx, int capacity, int* num_bins, double* bin_width, double* corr, int64_t* counts)
+{
+    using namespace opx_host;
+    if (!ctx) return OPX_E_ARG;
+    if (!ctx->cgc || !ctx->cgc->VT) return fail(ctx, OPX_E_STATE, "dipole: no light-cone solve yet");
+    const int n1 = ctx->dims[1], n2 = ctx->dims[2];
+    const int T = n1 * n2;
+    const int nbins = (int)std::sqrt((double)T) / 2;
+    const double ds = std::min(n1 * ctx->as[1], n2 * ctx->as[2]) * 0.5 / (double)nbins;
+    if (num_bins) *num_bins = nbins;
+    if (bin_width) *bin_width = ds;
+    if (!corr && !counts) return OPX_OK;   // size query
+    if (capacity < nbins) return fail(ctx, OPX_E_ARG, "dipole: output holds fewer than num_bins entries");
+    if (nbins == 0) return OPX_OK;
+    Scratch s(ctx);
+    double *psum = nullptr, *dcorr = nullptr;
+    unsigned long long *pcnt = nullptr, *dcnt = nullptr;
+    CGC_TRY(s.alloc(&psum, (size_t)nbins * T), "alloc dipole partials");
+    CGC_TRY(s.alloc(&pcnt, (size_t)nbins * T), "alloc dipole counts");
+    CGC_TRY(s.alloc(&dcorr, (size_t)nbins), "alloc dipole");
+    CGC_TRY(s.alloc(&dcnt, (size_t)nbins), "alloc dipole");
+    CGC_TRY(opx::cgc_dipole(ctx->nc, (uint32_t)n1, (uint32_t)n2, ctx->as[1], ctx->as[2], ds, nbins, ctx->cgc->VT, psum,
+                            pcnt, dcorr, dcnt, ctx->stream),
+            "dipole");
+    std::vector<double> hc((size_t)nbins);
+    std::vector<unsigned long long> hn((size_t)nbins);
+    CGC_TRY(hipMemcpyAsync(hc.data(), dcorr, hc.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+            "download dipole");
+    CGC_TRY(hipMemcpyAsync(hn.data(), dcnt, hn.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream),
+            "download dipole");
+    CGC_TRY(hipStreamSynchronize(ctx->stream), "dipole sync");
+    for (int b = 0; b < nbins; b++) {
+        if (corr) corr[b] = hc[b];
+        if (counts) counts[b] = (int64_t)hn[b];
+    }
+    return OPX_OK;
 }
 
 int opx_cgc_clear(opx_ctx* ctx)

@@ -211,8 +211,17 @@ hipError_t cgc_links(int nc, int prec, const Lattice& L, const double* Vm, const
 hipError_t cgc_gauss(int nc, int prec, const Lattice& L, const Box& active, const double* Vm, const double* Vp,
                      const void* U, const void* Unext, const void* rho, double* gauss, hipStream_t st);
 hipError_t cgc_plane_max(int nc, const Lattice& L, const double* gauss, double* mx, hipStream_t st);
-// work: bw * ppc * AD * (N1 N2) doubles; q: [coefficient][q_total] in the context precision
+// Lists of at most kRefineMatrixMax charges are refined by the 100th powers of the sweep
+// matrices; longer ones sweep by sweep. work: cgc_sp_work doubles; q: [coefficient][q_total]
+// in the context precision.
+constexpr int kRefineMatrixMax = 2048;
+size_t cgc_sp_work(const SpBlock& b, uint32_t nT, int ad);
 hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, const double* gauss, double* work,
                           void* q, int64_t q_total, int64_t q_base, hipStream_t st);
+// Wilson-line observables of the plane behind a nucleus, V [real][N1 N2]: the sum of Re tr V,
+// and tr(V_x V_y^+) binned by distance (psum, pcnt: nbins * N1 N2 partials)
+hipError_t cgc_tadpole(int nc, uint32_t T, const double* V, double* out, hipStream_t st);
+hipError_t cgc_dipole(int nc, uint32_t n1, uint32_t n2, double a1, double a2, double ds, int nbins, const double* V,
+                      double* psum, unsigned long long* pcnt, double* corr, unsigned long long* cnt, hipStream_t st);
 
 }  // namespace opx

@@ -130,6 +130,10 @@ def lib():
             "orc_mv_charge": (None, [gp] + [ctypes.c_double] * 6 + [_dp, _dp]),
             "orc_wilson_segment": (None, [gp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]),
             "orc_lightcone_solve": (None, [gp, _dp, ctypes.c_int, _dp]),
+            "orc_lightcone_solve_vt": (None, [gp, _dp, ctypes.c_int, _dp, _dp]),
+            "orc_wilson_tadpole": (ctypes.c_double, [ctypes.c_int, ctypes.c_long, _dp]),
+            "orc_wilson_dipole": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         _dp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
             "orc_lightcone_block": (None, [gp, _dp, _ip, _ip]),
             "orc_lightcone_superparticles": (None, [gp, _dp] + [ctypes.c_int] * 5 + [_ip] * 6 + [_dp]),
             "orc_group_log": (None, [ctypes.c_int, _dp, _dp]),
@@ -215,6 +219,22 @@ def group_pow(nc, u, x):
     o = np.zeros(gsize(nc))
     lib().orc_group_pow(nc, _ptr(np.ascontiguousarray(u, np.float64)), float(x), _ptr(o))
     return o
+
+
+def wilson_tadpole(nc, VT):
+    """WilsonLineObservables.computeTadpole's value: mean Re tr V over the transverse plane."""
+    VT = np.ascontiguousarray(VT, np.float64)
+    return float(lib().orc_wilson_tadpole(nc, VT.shape[0], _ptr(VT)))
+
+
+def wilson_dipole(nc, n1, n2, a1, a2, VT, nbins):
+    """WilsonLineObservables.computeDipole's bins: (mean tr(V_x V_y^+) per bin, pair counts)."""
+    VT = np.ascontiguousarray(VT, np.float64)
+    corr = np.zeros(nbins)
+    counts = np.zeros(nbins, np.int64)
+    lib().orc_wilson_dipole(nc, n1, n2, a1, a2, _ptr(VT), nbins, _ptr(corr),
+                            counts.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+    return corr, counts
 
 
 def regulate_charge_gaussian(dims, uvt, long_width, ir, aT, aL, rho):
@@ -398,10 +418,12 @@ class OracleGrid:
         return rho
 
     def lightcone_solve(self, rho, orientation):
-        """LightConePoissonSolverImprovedFull.solve: sets U, Unext, E; returns the Gauss violation (cells, A)."""
+        """LightConePoissonSolverImprovedFull.solve: sets U, Unext, E; returns the Gauss violation (cells, A).
+        The Wilson lines on the plane behind the nucleus (getV) are kept in self.VT."""
         rho = np.ascontiguousarray(rho, np.float64)
         gauss = np.zeros((self.n, asize(self.nc)))
-        lib().orc_lightcone_solve(self._g, _ptr(rho), int(orientation), _ptr(gauss))
+        self.VT = np.zeros((self.dims[1] * self.dims[2], gsize(self.nc)))
+        lib().orc_lightcone_solve_vt(self._g, _ptr(rho), int(orientation), _ptr(gauss), _ptr(self.VT))
         return gauss
 
     def lightcone_superparticles(self, gauss, orientation, subdivisions):

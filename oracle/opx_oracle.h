@@ -85,6 +85,12 @@ void orc_mv_charge(const orc_grid *gr, double location, double width, double mu,
                    double ir, const double *noise, double *rho);
 void orc_wilson_segment(const orc_grid *gr, double z1, double z2, int M, const double *P1, const double *P2, double *W);
 void orc_lightcone_solve(orc_grid *gr, const double *rho, int orientation, double *gauss);
+/* as orc_lightcone_solve; VT (N1 N2 group elements, may be NULL) receives the Wilson lines at -at/2
+ * on the plane behind the nucleus (x0 = 0 for orientation +1, N0 - 1 for -1) */
+void orc_lightcone_solve_vt(orc_grid *gr, const double *rho, int orientation, double *gauss, double *VT);
+double orc_wilson_tadpole(int nc, long T, const double *VT);
+void orc_wilson_dipole(int nc, int n1, int n2, double a1, double a2, const double *VT, int nbins, double *corr,
+                       long *counts);
 void orc_lightcone_block(const orc_grid *gr, const double *gauss, int *z_start, int *block_width);
 void orc_lightcone_superparticles(const orc_grid *gr, const double *gauss, int orientation, int z_start,
                                   int block_width, int subdivisions, int ppc, int *sp_orientation, int *sp_num,

@@ -249,6 +249,11 @@ void orc_wilson_segment(const orc_grid *gr, double z1, double z2, int M, const d
 /* LightConePoissonSolverImprovedFull.solve (:40-263), beam along x0 */
 void orc_lightcone_solve(orc_grid *gr, const double *rho, int orientation, double *gauss)
 {
+    orc_lightcone_solve_vt(gr, rho, orientation, gauss, NULL);
+}
+
+void orc_lightcone_solve_vt(orc_grid *gr, const double *rho, int orientation, double *gauss, double *VT)
+{
     const int nc = gr->nc, A = gr->asz, G = gr->gsz, ncomp = nc * nc - 1;
     const int L = gr->dims[0], n1 = gr->dims[1], n2 = gr->dims[2];
     const long T = (long)n1 * n2, n = gr->n;
@@ -291,6 +296,11 @@ void orc_lightcone_solve(orc_grid *gr, const double *rho, int orientation, doubl
             orc_group_mult(nc, W, W2, W);
             orc_group_mult(nc, V + indexL * G, W, V + index * G);
         }
+    }
+    /* the Wilson lines "far behind" the nucleus (:157-167) */
+    if (VT) {
+        const int zb = orientation > 0 ? 0 : L - 1;
+        memcpy(VT, V + (long)zb * T * G, sizeof(double) * T * G);
     }
     /* links at t = -at/2: U -> V_x U V_{x+i}^dagger; the copy holds V_x V_{x+i}^dagger */
     double *cU = (double *)malloc(sizeof(double) * n * 3 * G), *cUn = (double *)malloc(sizeof(double) * n * 3 * G);
@@ -482,4 +492,60 @@ void orc_lightcone_superparticles(const orc_grid *gr, const double *gauss, int o
     }
     (void)nc;
     free(list);
+}
+
+/* ===================================================================== */
+/* Wilson-line observables                                                */
+/* ===================================================================== */
+
+static double real_trace(int nc, const double *g)
+{
+    return nc == 2 ? 2 * g[0] : g[0] + g[4] + g[8];
+}
+
+/* WilsonLineObservables.computeTadpole (WilsonLineObservables.java:36-63): mean real trace */
+double orc_wilson_tadpole(int nc, long T, const double *VT)
+{
+    const int G = orc_group_size(nc);
+    double s = 0.0;
+    for (long i = 0; i < T; i++) s += real_trace(nc, VT + i * G);
+    return s / T;
+}
+
+/* WilsonLineObservables.computeDipole (WilsonLineObservables.java:71-131): tr(V_x V_y^dagger)
+ * binned by the periodic transverse distance, pairs in (x, y) order */
+void orc_wilson_dipole(int nc, int n1, int n2, double a1, double a2, const double *VT, int nbins, double *corr,
+                       long *counts)
+{
+    const int G = orc_group_size(nc);
+    const long T = (long)n1 * n2;
+    const int nc_[2] = {n1, n2};
+    const double as[2] = {a1, a2};
+    const double maxd = fmin(n1 * a1, n2 * a2) * 0.5, ds = maxd / ((double)nbins);
+    for (int b = 0; b < nbins; b++) {
+        corr[b] = 0.0;
+        counts[b] = 0;
+    }
+    for (long i = 0; i < T; i++) {
+        const int x[2] = {(int)(i / n2), (int)(i % n2)};
+        for (long j = 0; j < T; j++) {
+            const int y[2] = {(int)(j / n2), (int)(j % n2)};
+            double dist = 0.0;
+            for (int k = 0; k < 2; k++) {
+                double d = fabs((double)(x[k] - y[k]));
+                if (d > nc_[k] / 2) d -= nc_[k];
+                dist += d * d * as[k] * as[k];
+            }
+            dist = sqrt(dist);
+            const int bin = (int)(dist / ds);
+            if (bin < nbins) {
+                double a[18], m[18];
+                orc_group_adj(nc, VT + j * G, a);
+                orc_group_mult(nc, VT + i * G, a, m);
+                corr[bin] += real_trace(nc, m);
+                counts[bin]++;
+            }
+        }
+    }
+    for (int b = 0; b < nbins; b++) corr[b] /= (double)counts[b];
 }

@@ -15,7 +15,7 @@ from decimal import Decimal, getcontext
 import numpy as np
 import pytest
 
-from openpixi_amd.cgc import CGCInitialCondition, JavaRandom, MVModel, strict_log
+from openpixi_amd.cgc import CGCInitialCondition, JavaRandom, MVModel, java_format_f, strict_log
 from openpixi_amd.simulation import Settings
 
 
@@ -102,17 +102,32 @@ def test_yaml_cgc_initial_conditions():
         (False, 2.0, 0.5, 0.2)
 
 
-@pytest.mark.parametrize("patch", ["poissonSolver: improved", "poissonSolver: regular", "computeTadpole: true"])
+@pytest.mark.parametrize("patch", ["poissonSolver: improved", "poissonSolver: regular", "Nucleus: [{}]"])
 def test_yaml_cgc_refuses_what_is_not_built(patch):
-    text = YAML.replace("    poissonSolver: Improved Full", "    " + patch if "poisson" in patch else
-                        "    poissonSolver: Improved Full\n    " + patch)
-    if "computeTadpole" in patch:
-        s = Settings.from_yaml(text)
-        with pytest.raises(NotImplementedError):
-            s.initial_conditions[0].applyInitialCondition(None)
-    else:
-        with pytest.raises(NotImplementedError):
-            Settings.from_yaml(text)
+    key = "    poissonSolver: Improved Full"
+    text = YAML.replace(key, "    " + patch if "poisson" in patch else key + "\n    " + patch)
+    with pytest.raises(NotImplementedError):
+        Settings.from_yaml(text)
+
+
+def test_yaml_wilson_observables_options():
+    text = YAML.replace("    poissonSolver: Improved Full", "    poissonSolver: Improved Full\n    computeTadpole: true\n"
+                        "    tadpoleFilename: t.dat\n    computeDipole: true")
+    ic = Settings.from_yaml(text).initial_conditions[1]
+    assert (ic.computeTadpole, ic.tadpoleFilename, ic.computeDipole, ic.dipoleFilename) == (True, "t.dat", True, "dipole.txt")
+
+
+def test_java_percent_f():
+    """String.format("%f"): Double.toString's digits rounded half up (0.0078125 is an exact
+    binary tie, 1.0000005 is stored below its decimal digits)."""
+    assert java_format_f(0.2) == "0.200000"
+    assert java_format_f(0.0078125) == "0.007813"
+    assert java_format_f(1.0000005) == "1.000001"
+    assert java_format_f(-1e-7) == "-0.000000"
+    assert java_format_f(4.44) == "4.440000"
+    assert java_format_f(123456.5) == "123456.500000"
+    m = MVModel(0, 1, 32.0, 2.0, 0.2, True, 5, 2.0, 4.0, 0.2)
+    assert m.getInfo() == "MV, mu: 0.200000, w: 2.000000, UVT: 2.000000, c: 4.000000, m: 0.200000"
 
 
 def test_mv_model_beam_direction():

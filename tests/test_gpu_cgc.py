@@ -174,6 +174,27 @@ def test_superparticles_of_a_narrow_block(gpu):
     sp_lists_equal(device_superparticles(dg), arrs, charges, 1e-11)
 
 
+def test_superparticles_of_a_long_block(gpu):
+    """A block of 598 planes at 4 charges per cell: lists of 2392 charges, above the length up to
+    which the device refines through the sweep matrices' powers, so the lists are swept one by
+    one as the reference does."""
+    nc, dims, orientation = 2, (600, 2, 2), -1
+    print("super particles, long block", flush=True)
+    og, dg = grids(nc, dims)
+    rho = np.random.default_rng(9).normal(0, 1.0, (og.n, 3))
+    gauss = og.lightcone_solve(rho, orientation)
+    arrs, charges, (zs, bw) = og.lightcone_superparticles(gauss, orientation, 5)
+    assert bw * 4 > 2048
+    ChargeDensity(rho, 0, orientation).initialize(dg)
+    solver = LightConePoissonSolverImprovedFull()
+    solver.initialize(dg)
+    solver.solve(ChargeDensity(rho, 0, orientation))
+    creator = LightConeNGPSuperParticleCreator()
+    creator.initialize(_Sim(dg, 5), 0, orientation)
+    assert (creator.zStart, creator.blockWidth) == (zs, bw)
+    sp_lists_equal(device_superparticles(dg), arrs, charges, 1e-11)
+
+
 YAML = """
 simulationType: temporal optimized cgc ngp
 gridStep: 1
@@ -260,3 +281,41 @@ def test_cgc_entry_points_reject_bad_calls(gpu):
     assert L.opx_cgc_create_superparticles(dg.handle, 1, 0, None, None, None) == _lib.OPX_E_ARG
     z = np.zeros(5)
     assert L.opx_cgc_mv_charge(dg.handle, 1.0, 1.0, 1.0, 1.0, 0.0, 0.0, _lib.dptr(z), z.size, None, 0) == _lib.OPX_E_ARG
+
+
+@pytest.mark.parametrize("nc,orientation", [(2, 1), (3, -1)])
+def test_wilson_observables_match_oracle(gpu, nc, orientation, tmp_path, monkeypatch):
+    """Tadpole and dipole of the device's lines behind the nucleus against the oracle's, and the
+    files WilsonLineObservables appends (info line, header, one line per bin)."""
+    from openpixi_amd.cgc import WilsonLineObservables
+    from openpixi_amd.diagnostics import java_sci
+
+    dims = (12, 6, 8)
+    print(f"Wilson observables nc={nc} orientation={orientation}", flush=True)
+    og, dg, rho = solve_setup(nc, dims, "fp64", seed=23)
+    og.lightcone_solve(rho, orientation)
+    density = ChargeDensity(rho, 0, orientation)
+    density.initialize(dg)
+    solver = LightConePoissonSolverImprovedFull()
+    solver.initialize(dg)
+    solver.solve(density)
+    w = WilsonLineObservables(dg, solver, density)
+    want_t = O.wilson_tadpole(nc, og.VT)
+    assert abs(w.tadpole() - want_t) < 1e-12 * nc
+    d, corr, counts = w.dipole()
+    nbins = int(np.sqrt(dims[1] * dims[2])) // 2
+    want_c, want_n = O.wilson_dipole(nc, dims[1], dims[2], SPACING[1], SPACING[2], og.VT, nbins)
+    assert len(corr) == nbins and list(counts) == list(want_n)
+    assert np.max(np.abs(corr - want_c)) < 1e-12 * nc
+    monkeypatch.chdir(tmp_path)
+    w.computeTadpole("t.dat")
+    w.computeDipole("d.dat")
+    t = (tmp_path / "output" / "t.dat").read_text().splitlines()
+    assert len(t) == 1 and t[0].startswith("charge density, V: ")
+    assert float(t[0].split("V: ")[1]) == pytest.approx(want_t, abs=1e-12 * nc)
+    lines = (tmp_path / "output" / "d.dat").read_text().splitlines()
+    assert lines[:2] == ["charge density", "d\ttr(V V^t)"] and len(lines) == 2 + nbins
+    for b, line in enumerate(lines[2:]):
+        x, v = line.split("\t")
+        assert x == java_sci(d[b])
+        assert float(v) == pytest.approx(want_c[b], abs=1e-12 * nc)

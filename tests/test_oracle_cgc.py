@@ -10,6 +10,8 @@ of its code:
   particles deposit exactly the Gauss violation of the Poisson solution, and the complete
   initialization (fields, particles, charge density) satisfies Gauss's law;
 * the MV charge profile and box: charge only inside the box, zero transverse mean.
+* the Wilson-line observables: tadpole and dipole against dense numpy brute force, and the
+  lines they use are those of the plane behind the nucleus.
 
 The reference draws its random charges with java.util.Random and transforms them with
 JTransforms. Here the noise is an input: the package draws it with JavaRandom, pinned to
@@ -198,3 +200,48 @@ def test_refinement_conserves_the_charge_of_each_transverse_column():
             want[idx % T] += gauss[ngp] / ppc
     assert np.allclose(total, want, atol=1e-12 * np.max(np.abs(gauss)))
     assert q.shape[0] == bw * T * ppc
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_wilson_observables_match_dense_brute_force(nc):
+    """Tadpole: the mean real trace; dipole: tr(V_x V_y^+) over all ordered pairs binned by the
+    periodic transverse distance (a coordinate difference above half the extent wraps), both
+    from dense matrices with numpy."""
+    n1, n2, a1, a2 = 7, 6, 0.9, 1.1
+    T = n1 * n2
+    VT = O.links_from_coeffs(nc, RNG.normal(0, 0.8, (T, nc * nc - 1)))
+    M = np.array([group_to_matrix(nc, v) for v in VT])
+    assert O.wilson_tadpole(nc, VT) == pytest.approx(np.mean(np.trace(M, axis1=1, axis2=2).real), abs=1e-14)
+    nbins = int(np.sqrt(T)) // 2
+    ds = min(n1 * a1, n2 * a2) * 0.5 / nbins
+    x = np.stack([np.arange(T) // n2, np.arange(T) % n2], axis=1)
+    d = np.abs(x[:, None, :] - x[None, :, :]).astype(float)
+    half = np.array([n1 // 2, n2 // 2])
+    d = np.where(d > half, d - np.array([n1, n2]), d)
+    dist = np.sqrt((d[..., 0] * a1) ** 2 + (d[..., 1] * a2) ** 2)
+    bins = (dist / ds).astype(int)
+    tr = np.einsum("iab,jab->ij", M, M.conj()).real
+    corr, counts = O.wilson_dipole(nc, n1, n2, a1, a2, VT, nbins)
+    for b in range(nbins):
+        sel = bins == b
+        assert counts[b] == sel.sum()
+        assert corr[b] == pytest.approx(tr[sel].mean(), abs=1e-13)
+    # the nearest bin holds the x = y pairs, tr(V V^+) = N
+    assert counts[0] >= T
+
+
+@pytest.mark.parametrize("orientation", [1, -1])
+def test_wilson_lines_kept_from_the_plane_behind_the_nucleus(orientation):
+    """A charge on planes 8-10 of 20: the lines start as identities on the plane the path begins
+    at and pick the charge up on their way, so the plane behind the nucleus (x0 = 0 for +1,
+    N0 - 1 for -1) holds non-trivial lines, and a zero charge leaves identities."""
+    dims = (20, 4, 4)
+    og = O.OracleGrid(2, dims, (1.0, 1.0, 1.0), 0.25, 1.0)
+    T = dims[1] * dims[2]
+    rho = np.zeros((og.n, 3))
+    rho[8 * T:11 * T] = RNG.normal(0, 1.0, (3 * T, 3))
+    og.lightcone_solve(rho, orientation)
+    assert np.max(np.abs(og.VT - np.array([1.0, 0, 0, 0]))) > 1e-2
+    og2 = O.OracleGrid(2, dims, (1.0, 1.0, 1.0), 0.25, 1.0)
+    og2.lightcone_solve(np.zeros((og2.n, 3)), orientation)
+    assert np.max(np.abs(og2.VT - np.array([1.0, 0, 0, 0]))) == 0.0
