@@ -235,6 +235,14 @@ int opx_superparticles_download_descriptors(opx_ctx *ctx, int count, int *orient
 int opx_cgc_mv_charge(opx_ctx *ctx, double location, double width, double mu, double uv_cutoff,
                       double coherence_length, double ir_coefficient, const double *noise, int64_t n_noise,
                       double *rho_out, int64_t n_rho);
+/* as opx_cgc_mv_charge with the noise drawn on the device from java.util.Random(seed): the
+ * values new Random(seed).nextGaussian() returns, in order (MVModel with useSeed). */
+int opx_cgc_mv_charge_seeded(opx_ctx *ctx, double location, double width, double mu, double uv_cutoff,
+                             double coherence_length, double ir_coefficient, int64_t seed, double *rho_out,
+                             int64_t n_rho);
+/* the first `count` values of new java.util.Random(seed).nextGaussian() (the generator and polar
+ * method the JDK specifies, with StrictMath.log), computed on the device into a host buffer */
+int opx_java_gaussians(int64_t seed, int64_t count, double *out);
 /* an arbitrary charge density for the next solve (IInitialChargeDensity.getChargeDensity),
  * cells * A reals in reference storage */
 int opx_cgc_set_charge(opx_ctx *ctx, const double *rho, int64_t n_values);

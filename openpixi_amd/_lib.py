@@ -76,6 +76,10 @@ SIGNATURES = {
     "opx_superparticles_download_descriptors": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip, _ip]),
     "opx_cgc_mv_charge": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int64, _dp, ctypes.c_int64]),
+    "opx_cgc_mv_charge_seeded": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int64,
+                                                _dp, ctypes.c_int64]),
+    "opx_java_gaussians": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _dp]),
     "opx_cgc_set_charge": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     "opx_cgc_lightcone_solve": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64]),
     "opx_cgc_create_superparticles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip]),

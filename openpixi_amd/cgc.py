@@ -222,12 +222,18 @@ class MVModel:
         return rand.gaussians(_components(grid.getNumberOfColors()) * grid.getTotalNumberOfCells())
 
     def initialize(self, s, noise=None):
-        """MVModel.initialize; `noise` replaces the generator's draws when given."""
+        """MVModel.initialize. The noise is drawn on the device from java.util.Random(seed), or
+        from a random seed without useSeed (as `new Random()`); `noise` replaces it when given."""
         g = _grid(s)
-        z = self.noise(g) if noise is None else np.ascontiguousarray(noise, np.float64).reshape(-1)
-        g._check(_lib.load().opx_cgc_mv_charge(g.handle, self.location, self.longitudinalWidth, self.mu,
-                                               self.ultravioletCutoffTransverse, self.longitudinalCoherenceLength,
-                                               self.infraredCoefficient, _lib.dptr(z), z.size, None, 0))
+        L = _lib.load()
+        args = (self.location, self.longitudinalWidth, self.mu, self.ultravioletCutoffTransverse,
+                self.longitudinalCoherenceLength, self.infraredCoefficient)
+        if noise is None:
+            seed = self.seed if self.useSeed else int.from_bytes(os.urandom(6), "little")
+            g._check(L.opx_cgc_mv_charge_seeded(g.handle, *args, seed, None, 0))
+        else:
+            z = np.ascontiguousarray(noise, np.float64).reshape(-1)
+            g._check(L.opx_cgc_mv_charge(g.handle, *args, _lib.dptr(z), z.size, None, 0))
         self._grid = g
 
     def getChargeDensity(self) -> np.ndarray:

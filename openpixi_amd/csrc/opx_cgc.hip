@@ -685,6 +685,165 @@ __global__ __launch_bounds__(kB) void k_sp_scatter(uint32_t nT, int ad, SpBlock 
     }
 }
 
+// ------------------------------------------------------------------ java.util.Random
+// The generator java.util.Random specifies: seed' = seed * 0x5DEECE66D + 0xB (mod 2^48),
+// next(bits) = the top `bits` of the 48, nextDouble = (next(26) 2^27 + next(27)) 2^-53, and
+// nextGaussian by the polar method, which consumes two doubles per trial and returns both
+// values of an accepted pair in turn. Trial p consumes LCG steps 4p+1 .. 4p+4, reached by a
+// jump (the affine step composed with itself); accepted trials are then compacted in order.
+constexpr uint64_t kJMult = 0x5DEECE66DULL, kJAdd = 0xBULL, kJMask = (1ULL << 48) - 1;
+
+__device__ __forceinline__ uint64_t lcg_jump(uint64_t s, uint64_t k)
+{
+    uint64_t ra = 1, rc = 0, ba = kJMult, bc = kJAdd;   // arithmetic mod 2^64; 2^48 divides it
+    while (k) {
+        if (k & 1) {
+            ra = ra * ba;
+            rc = rc * ba + bc;
+        }
+        bc = bc * ba + bc;
+        ba = ba * ba;
+        k >>= 1;
+    }
+    return (ra * s + rc) & kJMask;
+}
+
+// StrictMath.log for positive finite x: fdlibm's e_log.c (Sun Microsystems, freely
+// distributable), x = 2^k (1 + f), log(1 + f) = f - s (f - R(s^2)) with s = f / (2 + f), k ln 2
+// added in two parts; evaluated without contractions, as Java does.
+__device__ double strict_log(double x)
+{
+#pragma clang fp contract(off)
+    const double Lg1 = 6.666666666666735130e-01, Lg2 = 3.999999999940941908e-01, Lg3 = 2.857142874366239149e-01,
+                 Lg4 = 2.222219843214978396e-01, Lg5 = 1.818357216161805012e-01, Lg6 = 1.531383769920937332e-01,
+                 Lg7 = 1.479819860511658591e-01;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    int k = 0;
+    long long bits = __double_as_longlong(x);
+    int hx = (int)(bits >> 32);
+    if (hx < 0x00100000) {   // subnormal
+        k -= 54;
+        x *= 18014398509481984.0;
+        bits = __double_as_longlong(x);
+        hx = (int)(bits >> 32);
+    }
+    k += (hx >> 20) - 1023;
+    hx &= 0x000fffff;
+    const int i = (hx + 0x95f64) & 0x100000;
+    x = __longlong_as_double(((long long)(hx | (i ^ 0x3ff00000)) << 32) | (bits & 0xffffffffLL));
+    k += i >> 20;
+    const double f = x - 1.0;
+    const double dk = (double)k;
+    if ((0x000fffff & (2 + hx)) < 3) {   // |f| < 2^-20
+        if (f == 0.0) return k == 0 ? 0.0 : dk * ln2_hi + dk * ln2_lo;
+        const double R = f * f * (0.5 - 0.33333333333333333 * f);
+        return k == 0 ? f - R : dk * ln2_hi - ((R - dk * ln2_lo) - f);
+    }
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    const double w = z * z;
+    const double t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
+    const double t2 = z * (Lg1 + w * (Lg3 + w * (Lg5 + w * Lg7)));
+    const double R = t2 + t1;
+    if (((hx - 0x6147a) | (0x6b851 - hx)) > 0) {
+        const double hfsq = 0.5 * f * f;
+        return k == 0 ? f - (hfsq - s * (hfsq + R)) : dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+    }
+    return k == 0 ? f - s * (f - R) : dk * ln2_hi - ((s * (f - R) - dk * ln2_lo) - f);
+}
+
+__device__ __forceinline__ double java_double(uint64_t s26, uint64_t s27)
+{
+    return (double)(((s26 >> 22) << 27) + (s27 >> 21)) * 0x1.0p-53;
+}
+
+// trial p: its two values when accepted (flag 1)
+__global__ __launch_bounds__(kB) void k_jr_trials(uint64_t seed, uint64_t trials, double2* __restrict__ vals,
+                                                  uint32_t* __restrict__ flag)
+{
+#pragma clang fp contract(off)
+    const uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (p >= trials) return;
+    const uint64_t s1 = lcg_jump(seed, 4 * p + 1);
+    const uint64_t s2 = (s1 * kJMult + kJAdd) & kJMask;
+    const uint64_t s3 = (s2 * kJMult + kJAdd) & kJMask;
+    const uint64_t s4 = (s3 * kJMult + kJAdd) & kJMask;
+    const double v1 = 2 * java_double(s1, s2) - 1;
+    const double v2 = 2 * java_double(s3, s4) - 1;
+    const double sq = v1 * v1 + v2 * v2;
+    const bool ok = sq < 1 && sq != 0;
+    flag[p] = ok ? 1u : 0u;
+    if (ok) {
+        const double m = sqrt(-2 * strict_log(sq) / sq);
+        vals[p] = make_double2(v1 * m, v2 * m);
+    }
+}
+
+// exclusive prefix sums of `flag` within blocks of kScan, block totals to `sums`
+constexpr int kScan = 1024;
+__global__ __launch_bounds__(kB) void k_scan_blocks(uint64_t m, const uint32_t* __restrict__ flag,
+                                                    uint32_t* __restrict__ pos, uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t sh[kScan];
+    const uint64_t base = (uint64_t)blockIdx.x * kScan;
+    for (int e = threadIdx.x; e < kScan; e += kB) sh[e] = base + e < m ? flag[base + e] : 0u;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over kScan values, four per thread
+    for (int off = 1; off < kScan; off <<= 1) {
+        uint32_t v[kScan / kB];
+#pragma unroll
+        for (int r = 0; r < kScan / kB; r++) {
+            const int e = threadIdx.x + r * kB;
+            v[r] = sh[e] + (e >= off ? sh[e - off] : 0u);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kScan / kB; r++) sh[threadIdx.x + r * kB] = v[r];
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < kScan; e += kB)
+        if (base + e < m) pos[base + e] = sh[e] - flag[base + e];
+    if (threadIdx.x == 0) sums[blockIdx.x] = sh[kScan - 1];
+}
+
+// block offsets: one workgroup scans the block totals chunk by chunk (exclusive, carried over the
+// chunks), the grand total last
+__global__ __launch_bounds__(kScan) void k_scan_sums(uint32_t nb, uint32_t* __restrict__ sums,
+                                                     uint64_t* __restrict__ total)
+{
+    __shared__ uint64_t sh[kScan];
+    uint64_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += kScan) {
+        const uint32_t e = c0 + threadIdx.x;
+        const uint64_t v = e < nb ? sums[e] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < kScan; off <<= 1) {
+            const uint64_t a = sh[threadIdx.x] + (threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0);
+            __syncthreads();
+            sh[threadIdx.x] = a;
+            __syncthreads();
+        }
+        if (e < nb) sums[e] = (uint32_t)(carry + sh[threadIdx.x] - v);
+        carry += sh[kScan - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+// out[2 r], out[2 r + 1] of the r-th accepted trial, r < count / 2 rounded up
+__global__ __launch_bounds__(kB) void k_jr_scatter(uint64_t trials, uint64_t count, const double2* __restrict__ vals,
+                                                   const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                                   const uint32_t* __restrict__ sums, double* __restrict__ out)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (p >= trials || !flag[p]) return;
+    const uint64_t r = (uint64_t)pos[p] + sums[p / kScan];
+    const double2 v = vals[p];
+    if (2 * r < count) out[2 * r] = v.x;
+    if (2 * r + 1 < count) out[2 * r + 1] = v.y;
+}
+
 // ------------------------------------------------------------------ Wilson-line observables
 // Re tr(V_i V_j^+) of two elements of a [real][T] array (GroupElement.mult(adj()).getRealTrace())
 template <int NC>
@@ -924,19 +1083,19 @@ double* matrix_power(int nn, double* S, int e, double* R, double* P, double* tmp
 {
     const size_t m = (size_t)nn * nn;
     k_identity<<<blocks(m), kB, 0, st>>>(nn, R);
-    hipMemcpyAsync(P, S, m * sizeof(double), hipMemcpyDeviceToDevice, st);
-    while (e) {
+    hipError_t r = hipMemcpyAsync(P, S, m * sizeof(double), hipMemcpyDeviceToDevice, st);
+    while (e && r == hipSuccess) {
         if (e & 1) {
-            gemm(nn, nn, nn, R, P, tmp, st);
+            r = gemm(nn, nn, nn, R, P, tmp, st);
             std::swap(R, tmp);
         }
         e >>= 1;
-        if (e) {
-            gemm(nn, nn, nn, P, P, tmp, st);
+        if (e && r == hipSuccess) {
+            r = gemm(nn, nn, nn, P, P, tmp, st);
             std::swap(P, tmp);
         }
     }
-    *err = hipGetLastError();
+    *err = r == hipSuccess ? hipGetLastError() : r;
     return R;
 }
 
@@ -964,13 +1123,13 @@ hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, 
         double* R2 = matrix_power(nn, S, 100, X, Y, Z, st, &e);
         if (e != hipSuccess) return e;
         // keep R2 in W while the 4th-order power runs in the other buffers
-        hipMemcpyAsync(W, R2, mm * sizeof(double), hipMemcpyDeviceToDevice, st);
+        if ((e = hipMemcpyAsync(W, R2, mm * sizeof(double), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
         k_sweep_matrix<4><<<sg, kB, 0, st>>>(nn, b.ppc, S);
         double* R4 = matrix_power(nn, S, 100, X, Y, Z, st, &e);
         if (e != hipSuccess) return e;
         double* Rt = R4 == X ? Y : X;
-        gemm(nn, nn, nn, R4, W, Rt, st);
-        gemm(nn, (int)cols, nn, Rt, work, out, st);
+        if ((e = gemm(nn, nn, nn, R4, W, Rt, st)) != hipSuccess) return e;
+        if ((e = gemm(nn, (int)cols, nn, Rt, work, out, st)) != hipSuccess) return e;
         refined = out;
     } else {
         k_sp_refine<<<grid, kB, 0, st>>>(cols, nn, b.ppc, work);
@@ -980,6 +1139,37 @@ hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, 
     else
         k_sp_scatter<float><<<grid, kB, 0, st>>>(L.s0, ad, b, refined, (float*)q, q_total, q_base);
     return hipGetLastError();
+}
+
+size_t java_gaussian_work(uint64_t trials)
+{
+    const uint64_t nb = (trials + kScan - 1) / kScan;
+    return trials * (sizeof(double2) + 2 * sizeof(uint32_t)) + nb * sizeof(uint32_t) + sizeof(uint64_t) + 64;
+}
+
+hipError_t java_gaussians(uint64_t scrambled_seed, uint64_t count, uint64_t trials, void* work, double* out,
+                          uint64_t* accepted, hipStream_t st)
+{
+    const uint64_t nb = (trials + kScan - 1) / kScan;
+    char* w = (char*)work;
+    double2* vals = (double2*)w;
+    w += trials * sizeof(double2);
+    uint32_t* flag = (uint32_t*)w;
+    w += trials * sizeof(uint32_t);
+    uint32_t* pos = (uint32_t*)w;
+    w += trials * sizeof(uint32_t);
+    uint32_t* sums = (uint32_t*)w;
+    w += nb * sizeof(uint32_t);
+    uint64_t* total = (uint64_t*)(((uintptr_t)w + 7) & ~(uintptr_t)7);
+    k_jr_trials<<<blocks(trials), kB, 0, st>>>(scrambled_seed, trials, vals, flag);
+    k_scan_blocks<<<(unsigned)nb, kB, 0, st>>>(trials, flag, pos, sums);
+    k_scan_sums<<<1, kScan, 0, st>>>((uint32_t)nb, sums, total);
+    k_jr_scatter<<<blocks(trials), kB, 0, st>>>(trials, count, vals, flag, pos, sums, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(accepted, total, sizeof(uint64_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
 }
 
 hipError_t cgc_tadpole(int nc, uint32_t T, const double* V, double* out, hipStream_t st)

@@ -116,21 +116,33 @@ int download_alg(opx_ctx* ctx, const double* dev, double* host)
     return OPX_OK;
 }
 
-}  // namespace
-}  // namespace opx_host
-
-extern "C" {
-
-int opx_cgc_mv_charge(opx_ctx* ctx, double location, double width, double mu, double uv_cutoff,
-                      double coherence_length, double ir_coefficient, const double* noise, int64_t n_noise,
-                      double* rho_out, int64_t n_rho)
+// the unit Gaussians of java.util.Random(seed) into dev[0, count) on the context's stream
+int java_noise(opx_ctx* ctx, int64_t seed, uint64_t count, double* dev)
 {
-    using namespace opx_host;
+    const uint64_t scrambled = ((uint64_t)seed ^ 0x5DEECE66DULL) & ((1ULL << 48) - 1);
+    const uint64_t need = (count + 1) / 2;
+    // about pi/4 of the trials are accepted; more are drawn until enough are
+    for (uint64_t trials = need + need / 4 + need / 32 + 4096;; trials *= 2) {
+        if (trials >= (1ULL << 32)) return fail(ctx, OPX_E_UNSUPPORTED, "java.util.Random: too many values");
+        void* w = nullptr;
+        CGC_TRY(hipMalloc(&w, opx::java_gaussian_work(trials)), "alloc java.util.Random work");
+        uint64_t accepted = 0;
+        const hipError_t e = opx::java_gaussians(scrambled, count, trials, w, dev, &accepted, ctx->stream);
+        (void)hipFree(w);
+        CGC_TRY(e, "java.util.Random Gaussians");
+        if (accepted >= need) return OPX_OK;
+    }
+}
+
+int mv_charge(opx_ctx* ctx, double location, double width, double mu, double uv_cutoff, double coherence_length,
+              double ir_coefficient, const double* noise, int64_t n_noise, bool seeded, int64_t seed, double* rho_out,
+              int64_t n_rho)
+{
     if (!ctx) return OPX_E_ARG;
     int rc = cgc_check(ctx);
     if (rc != OPX_OK) return rc;
-    if (!noise) return fail(ctx, OPX_E_ARG, "MV model: null noise buffer");
-    if (n_noise != (int64_t)ctx->AD * ctx->n)
+    if (!seeded && !noise) return fail(ctx, OPX_E_ARG, "MV model: null noise buffer");
+    if (!seeded && n_noise != (int64_t)ctx->AD * ctx->n)
         return fail(ctx, OPX_E_ARG, "MV model: noise must hold (colours^2 - 1) values per cell");
     if (rho_out && n_rho != ctx->n * ctx->AS) return fail(ctx, OPX_E_ARG, "MV model: charge buffer size mismatch");
     if (!(width > 0)) return fail(ctx, OPX_E_ARG, "MV model: the longitudinal width must be positive");
@@ -141,7 +153,11 @@ int opx_cgc_mv_charge(opx_ctx* ctx, double location, double width, double mu, do
     double2* x = nullptr;
     CGC_TRY(s.alloc(&dn, m), "alloc noise");
     CGC_TRY(s.alloc(&x, m), "alloc transform buffer");
-    CGC_TRY(hipMemcpyAsync(dn, noise, m * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload noise");
+    if (seeded) {
+        if ((rc = java_noise(ctx, seed, m, dn)) != OPX_OK) return rc;
+    } else {
+        CGC_TRY(hipMemcpyAsync(dn, noise, m * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload noise");
+    }
     const opx::Lattice L = lattice(ctx, ctx->at);
     const double aL = ctx->as[0], aT = ctx->as[1];
     const double norm = 1.0 / (width * std::sqrt(2 * M_PI)), i2s2 = 0.5 / (width * width);
@@ -159,6 +175,41 @@ int opx_cgc_mv_charge(opx_ctx* ctx, double location, double width, double mu, do
     if (rho_out) return download_alg(ctx, ctx->cgc->rho, rho_out);
     CGC_TRY(hipStreamSynchronize(ctx->stream), "MV sync");
     return OPX_OK;
+}
+
+}  // namespace
+}  // namespace opx_host
+
+extern "C" {
+
+int opx_cgc_mv_charge(opx_ctx* ctx, double location, double width, double mu, double uv_cutoff,
+                      double coherence_length, double ir_coefficient, const double* noise, int64_t n_noise,
+                      double* rho_out, int64_t n_rho)
+{
+    return opx_host::mv_charge(ctx, location, width, mu, uv_cutoff, coherence_length, ir_coefficient, noise, n_noise,
+                               false, 0, rho_out, n_rho);
+}
+
+int opx_cgc_mv_charge_seeded(opx_ctx* ctx, double location, double width, double mu, double uv_cutoff,
+                             double coherence_length, double ir_coefficient, int64_t seed, double* rho_out,
+                             int64_t n_rho)
+{
+    return opx_host::mv_charge(ctx, location, width, mu, uv_cutoff, coherence_length, ir_coefficient, nullptr, 0,
+                               true, seed, rho_out, n_rho);
+}
+
+int opx_java_gaussians(int64_t seed, int64_t count, double* out)
+{
+    if (count < 0 || (count > 0 && !out)) return OPX_E_ARG;
+    if (count == 0) return OPX_OK;
+    opx_ctx tmp;   // stream and error slot only
+    void* dev = nullptr;
+    if (hipMalloc(&dev, (size_t)count * sizeof(double)) != hipSuccess) return OPX_E_DEVICE;
+    int rc = opx_host::java_noise(&tmp, seed, (uint64_t)count, (double*)dev);
+    if (rc == OPX_OK && hipMemcpy(out, dev, (size_t)count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = OPX_E_DEVICE;
+    (void)hipFree(dev);
+    return rc;
 }
 
 int opx_cgc_set_charge(opx_ctx* ctx, const double* rho, int64_t n_values)

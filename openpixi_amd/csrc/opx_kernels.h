@@ -218,6 +218,12 @@ constexpr int kRefineMatrixMax = 2048;
 size_t cgc_sp_work(const SpBlock& b, uint32_t nT, int ad);
 hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, const double* gauss, double* work,
                           void* q, int64_t q_total, int64_t q_base, hipStream_t st);
+// java.util.Random(seed).nextGaussian() x count on the device (seed already scrambled with
+// 0x5DEECE66D), from `trials` polar-method trials; *accepted (host) receives the number of
+// accepted trials, which must reach count / 2 rounded up. work: java_gaussian_work(trials) bytes.
+size_t java_gaussian_work(uint64_t trials);
+hipError_t java_gaussians(uint64_t scrambled_seed, uint64_t count, uint64_t trials, void* work, double* out,
+                          uint64_t* accepted, hipStream_t st);
 // Wilson-line observables of the plane behind a nucleus, V [real][N1 N2]: the sum of Re tr V,
 // and tr(V_x V_y^+) binned by distance (psum, pcnt: nbins * N1 N2 partials)
 hipError_t cgc_tadpole(int nc, uint32_t T, const double* V, double* out, hipStream_t st);

@@ -319,3 +319,29 @@ def test_wilson_observables_match_oracle(gpu, nc, orientation, tmp_path, monkeyp
         x, v = line.split("\t")
         assert x == java_sci(d[b])
         assert float(v) == pytest.approx(want_c[b], abs=1e-12 * nc)
+
+
+@pytest.mark.parametrize("seed,count", [(42, 1), (0, 2), (-7, 1001), (12345, 100000), (2**40 + 3, 30001)])
+def test_device_java_gaussians_equal_the_host_sequence(gpu, seed, count):
+    """java.util.Random(seed).nextGaussian() on the device: bit for bit the host JavaRandom
+    sequence (whose first values the JDK's are), odd and even counts."""
+    got = np.zeros(count)
+    assert _lib.load().opx_java_gaussians(seed, count, _lib.dptr(got)) == 0
+    assert np.array_equal(got, JavaRandom(seed).gaussians(count))
+    if seed == 42:
+        assert got[0] == 1.1419053154730547
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_seeded_mv_charge_equals_host_noise(gpu, nc):
+    dims = (12, 6, 10)
+    og, dg = grids(nc, dims)
+    p = mv_params(dims)
+    m = MVModel(0, 1, p["location"], p["width"], p["mu"], True, 77, p["uvt"], p["long_width"], p["ir"])
+    m.initialize(dg)
+    seeded = m.getChargeDensity()
+    m.initialize(dg, noise=JavaRandom(77).gaussians((nc * nc - 1) * og.n))
+    assert np.array_equal(seeded, m.getChargeDensity())
+    want = og.mv_charge(p["location"], p["width"], p["mu"], p["uvt"], p["long_width"], p["ir"],
+                        JavaRandom(77).gaussians((nc * nc - 1) * og.n))
+    assert rel_err(seeded, want) < 1e-12
