@@ -489,6 +489,81 @@ OPX_HD void su3_third_row(Su3<T>& m)
     }
 }
 
+// Products of SU(3) elements through their first two rows: rows 0 and 1 are computed and row 2,
+// where wanted, is rebuilt as conj(row0 x row1), exact for group elements. The left factor of a
+// * b and a * b^dagger is read in its first two rows only.
+// rows 0 and 1 of a * b^dagger (row 2 of the result left unset)
+template <typename T>
+OPX_HD Su3<T> mul_adj_r01(const Su3<T>& a, const Su3<T>& b)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) mul_adj_entry(a, b, r, c, o.r[3 * r + c], o.i[3 * r + c]);
+#pragma unroll
+    for (int c = 6; c < 9; c++) o.r[c] = o.i[c] = 0;
+    return o;
+}
+
+// rows 0 and 1 of a * b (row 2 of the result left unset)
+template <typename T>
+OPX_HD Su3<T> mul_r01(const Su3<T>& a, const Su3<T>& b)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            T re = a.r[3 * r] * b.r[c];
+            re -= a.i[3 * r] * b.i[c];
+            T im = a.r[3 * r] * b.i[c];
+            im += a.i[3 * r] * b.r[c];
+#pragma unroll
+            for (int k = 1; k < 3; k++) {
+                re += a.r[3 * r + k] * b.r[3 * k + c];
+                re -= a.i[3 * r + k] * b.i[3 * k + c];
+                im += a.r[3 * r + k] * b.i[3 * k + c];
+                im += a.i[3 * r + k] * b.r[3 * k + c];
+            }
+            o.r[3 * r + c] = re;
+            o.i[3 * r + c] = im;
+        }
+    }
+#pragma unroll
+    for (int c = 6; c < 9; c++) o.r[c] = o.i[c] = 0;
+    return o;
+}
+
+// rows 0 and 1 of a^dagger * b (reads columns 0 and 1 of a, all of b; row 2 left unset)
+template <typename T>
+OPX_HD Su3<T> adj_mul_r01(const Su3<T>& a, const Su3<T>& b)
+{
+    Su3<T> o;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            T re = a.r[r] * b.r[c];
+            re += a.i[r] * b.i[c];
+            T im = a.r[r] * b.i[c];
+            im -= a.i[r] * b.r[c];
+#pragma unroll
+            for (int k = 1; k < 3; k++) {
+                re += a.r[3 * k + r] * b.r[3 * k + c];
+                re += a.i[3 * k + r] * b.i[3 * k + c];
+                im += a.r[3 * k + r] * b.i[3 * k + c];
+                im -= a.i[3 * k + r] * b.r[3 * k + c];
+            }
+            o.r[3 * r + c] = re;
+            o.i[3 * r + c] = im;
+        }
+    }
+#pragma unroll
+    for (int c = 6; c < 9; c++) o.r[c] = o.i[c] = 0;
+    return o;
+}
+
 // Reprojection onto the group in double precision: SU(2) normalised; SU(3) rows 0 and 1
 // Gram-Schmidt orthonormalised, row 2 = conj(row0 x row1).
 OPX_HD Su2<double> reunit(Su2<double> u)

@@ -143,6 +143,16 @@ __device__ __forceinline__ typename Group<NC, T>::M fload_link(const F& f, uint3
     GR::complete(m);
     return m;
 }
+// the stored reals only (two rows of an FP32 SU(3) link), without rebuilding row 2
+template <int NC, typename T, typename F>
+__device__ __forceinline__ typename Group<NC, T>::M fload_link_r01(const F& f, uint32_t site, int dir)
+{
+    using GR = Group<NC, T>;
+    typename GR::M m = GR::zero();
+#pragma unroll
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = f.ld((uint32_t)(dir * GR::S + k), site);
+    return m;
+}
 template <int NC, typename T, typename F>
 __device__ __forceinline__ void fstore_link(const F& f, uint32_t site, int dir, const typename Group<NC, T>::M& m)
 {
@@ -221,10 +231,24 @@ __device__ __forceinline__ void link_update(const Lattice& L, const Fld& U, cons
         const uint32_t xpj = fwd(L, site, cj, j);
         const uint32_t xmj = bwd(L, site, cj, j);
         const uint32_t xpd_mj = bwd(L, xpd, cj, j);
-        M u1 = mul_adj(mul_adj(fload_link<NC, T>(U, xpd, j), fload_link<NC, T>(U, xpj, d)),
-                       fload_link<NC, T>(U, site, j));
-        M u2 = adj_mul(mul(fload_link<NC, T>(U, xmj, d), fload_link<NC, T>(U, xpd_mj, j)),
-                       fload_link<NC, T>(U, xmj, j));
+        M u1, u2;
+        if constexpr (GR::S != GR::G) {
+            // two-row storage: every staple factor is a group element, so each product is
+            // formed in its first two rows and completed by the row-2 identity; the left
+            // factors are read in the two rows stored
+            u1 = mul_adj_r01(mul_adj_r01(fload_link_r01<NC, T>(U, xpd, j), fload_link<NC, T>(U, xpj, d)),
+                             fload_link<NC, T>(U, site, j));
+            su3_third_row(u1);
+            M y = mul_r01(fload_link_r01<NC, T>(U, xmj, d), fload_link<NC, T>(U, xpd_mj, j));
+            su3_third_row(y);
+            u2 = adj_mul_r01(y, fload_link<NC, T>(U, xmj, j));
+            su3_third_row(u2);
+        } else {
+            u1 = mul_adj(mul_adj(fload_link<NC, T>(U, xpd, j), fload_link<NC, T>(U, xpj, d)),
+                         fload_link<NC, T>(U, site, j));
+            u2 = adj_mul(mul(fload_link<NC, T>(U, xmj, d), fload_link<NC, T>(U, xpd_mj, j)),
+                         fload_link<NC, T>(U, xmj, j));
+        }
         acc(u1, u2);
         acc_scaled(S, u1, (T)L.area[j]);
     }
@@ -241,7 +265,10 @@ __device__ __forceinline__ void link_update(const Lattice& L, const Fld& U, cons
         for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(jj, k) * uf;
     }
     fstore_alg_once<NC, T>(E, site, d, e);
-    fstore_link_once<NC, T>(Unext, site, d, mul(exp_link(e, -at), Ud));
+    if constexpr (GR::S != GR::G)
+        fstore_link_once<NC, T>(Unext, site, d, mul_r01(exp_link(e, -at), Ud));   // the two stored rows
+    else
+        fstore_link_once<NC, T>(Unext, site, d, mul(exp_link(e, -at), Ud));
 }
 
 template <int NC, typename T, bool HAS_J, bool BUF>
