@@ -42,8 +42,9 @@ BYTES_PER_LINK = {"fp32": 48 + 32 + 32 + 48, "fp64": 144 + 64 + 64 + 144}
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over `bench.py --probe` (KiB per dispatch,
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy). Reported only
 # when the run uses that kernel on that lattice.
-MEASURED_TRAFFIC = {("fp32", "link", DIMS): (275846.0 * 2.0 + 491520.0) * 1024,
-                    ("fp64", "link", DIMS): (906119.8 * 2.0 + 1277952.0) * 1024}
+MEASURED_TRAFFIC = {("fp32", "link", DIMS): (283889.0 * 2.0 + 491520.0) * 1024,
+                    ("fp32", "cell", DIMS): (274979.0 * 2.0 + 491520.0) * 1024,
+                    ("fp64", "link", DIMS): (875904.0 * 2.0 + 1277952.0) * 1024}
 
 
 def parse():

@@ -144,6 +144,10 @@ int opx_reset_charge(opx_ctx *ctx);
  *                     LDS and the x0 +- 1 neighbours read from global memory;
  *                     OPX_OPT_TILE the tile shape (0: 8x16, 1: 8x8, 2: 4x16 cells
  *                     over (x1,x2)), one thread per link;
+ *   OPX_KERNEL_CELL   one thread per cell updates its three links, the two upper staples of
+ *                     a plane sharing one product (33 link reads and 21 staple products per
+ *                     cell instead of 39 and 24); OPX_OPT_TILE the row tile the blocks sweep
+ *                     along x0 (0: 32, 1: 16, 2: 8 rows of x1)
  *   OPX_KERNEL_AUTO   LINK until opx_autotune picks a form (default).
  * OPX_OPT_TILE selects the STREAM tile shape (0: 16x16, 1: 8x32, 2: 16x32,
  * 3: 8x64, 4: 8x16 threads over (x1,x2)); OPX_OPT_CHUNKS the number of x0
@@ -152,7 +156,7 @@ int opx_reset_charge(opx_ctx *ctx);
  * the fastest and returns its time per launch in *best_ms. */
 enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2 };
 enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2, OPX_KERNEL_PLAQ = 3, OPX_KERNEL_TILE = 4,
-       OPX_KERNEL_PLANE = 5 };
+       OPX_KERNEL_PLANE = 5, OPX_KERNEL_CELL = 6 };
 int opx_set_option(opx_ctx *ctx, int option, int value);
 int opx_get_option(const opx_ctx *ctx, int option, int *value);
 int opx_autotune(opx_ctx *ctx, int reps, double *best_ms);

@@ -234,6 +234,9 @@ hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void*
         }
         return opx::launch_plaq_update(c->nc, c->prec, L, c->active, U, c->P, Unext, E, c->J, c->j_nonzero, c->stream);
     }
+    if (c->kernel == OPX_KERNEL_CELL)
+        return opx::launch_cell_update(c->nc, c->prec, L, c->active, c->tile, U, Unext, E, c->J, c->j_nonzero,
+                                       c->stream);
     return opx::launch_combined_update(c->nc, c->prec, L, c->active, U, Unext, E, c->J, c->j_nonzero, c->stream);
 }
 
@@ -1161,7 +1164,7 @@ int opx_set_option(opx_ctx* ctx, int option, int value)
     if (!ctx) return OPX_E_ARG;
     switch (option) {
     case OPX_OPT_KERNEL:
-        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_PLANE) return fail(ctx, OPX_E_ARG, "option: bad kernel");
+        if (value < OPX_KERNEL_AUTO || value > OPX_KERNEL_CELL) return fail(ctx, OPX_E_ARG, "option: bad kernel");
         if (value == OPX_KERNEL_TILE && !opx::tile_supported(ctx->nc))
             return fail(ctx, OPX_E_UNSUPPORTED, "option: the tile form is SU(2) only");
         ctx->kernel = value;
@@ -1223,6 +1226,7 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
     std::vector<Cand> cands;
     cands.push_back({OPX_KERNEL_LINK, 0, 0});
     cands.push_back({OPX_KERNEL_PLAQ, 0, 0});
+    for (int t = 0; t < opx::kCellTiles; t++) cands.push_back({OPX_KERNEL_CELL, t, 0});
     if (opx::tile_supported(ctx->nc)) {
         const opx::TileGeom g = opx::tile_geom(ctx->prec, L, 1);
         const uint32_t tiles = g.tiles1 * g.tiles2;

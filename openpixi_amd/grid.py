@@ -183,13 +183,13 @@ class Grid:
 
     # ---- kernel form (no counterpart in the reference) ----
     def setKernel(self, kernel="auto", tile=None, chunks=None):
-        """kernel: "auto", "link", "stream", "plaq", "tile" (SU(2)) or "plane"; tile: stream tile index,
+        """kernel: "auto", "link", "stream", "plaq", "tile" (SU(2)), "plane" or "cell"; tile: stream tile index,
         tile store policy or plane tile shape;
         chunks: x0 chunk count of the stream and tile forms (0 = auto)."""
         L = _lib.load()
         k = {"auto": _lib.KERNEL_AUTO, "link": _lib.KERNEL_LINK, "stream": _lib.KERNEL_STREAM,
              "plaq": _lib.KERNEL_PLAQ, "tile": _lib.KERNEL_TILE,
-             "plane": _lib.KERNEL_PLANE}[kernel]
+             "plane": _lib.KERNEL_PLANE, "cell": _lib.KERNEL_CELL}[kernel]
         self._check(L.opx_set_option(self._h, _lib.OPT_KERNEL, k))
         if tile is not None:
             self._check(L.opx_set_option(self._h, _lib.OPT_TILE, int(tile)))
@@ -203,7 +203,8 @@ class Grid:
             v = ctypes.c_int()
             self._check(L.opx_get_option(self._h, opt, ctypes.byref(v)))
             out.append(v.value)
-        return {"kernel": ["auto", "link", "stream", "plaq", "tile", "plane"][out[0]], "tile": out[1], "chunks": out[2]}
+        return {"kernel": ["auto", "link", "stream", "plaq", "tile", "plane", "cell"][out[0]], "tile": out[1],
+                "chunks": out[2]}
 
     def autotune(self, reps=5):
         """Times every kernel form / tile / chunk candidate and keeps the fastest; returns ms per launch."""

@@ -17,8 +17,9 @@ FP64_TOL = 1e-12
 
 
 # kernel forms of the fused update: (kernel, tile, chunks)
-KERNELS = [("link", None, None), ("plaq", None, None), ("stream", 0, None), ("stream", 1, 3), ("stream", 4, 1)]
-KERNEL_IDS = ["link", "plaq", "stream16x16", "stream8x32c3", "stream8x16c1"]
+KERNELS = [("link", None, None), ("plaq", None, None), ("cell", None, None), ("stream", 0, None), ("stream", 1, 3),
+           ("stream", 4, 1)]
+KERNEL_IDS = ["link", "plaq", "cell", "stream16x16", "stream8x32c3", "stream8x16c1"]
 
 
 def run_both(nc, dims, steps, at=0.25, spacing=(1.0, 1.0, 1.0), sigma_a=0.1, sigma_e=0.05, seed=0, J=None,
@@ -137,7 +138,7 @@ def test_projected_energy_parity(gpu, nc):
             assert rel_err(a, b) < 1e-12
 
 
-@pytest.mark.parametrize("kernel", KERNELS[:2] + [("stream", 2, None)], ids=KERNEL_IDS[:2] + ["stream16x32"])
+@pytest.mark.parametrize("kernel", KERNELS[:3] + [("stream", 2, None)], ids=KERNEL_IDS[:3] + ["stream16x32"])
 @pytest.mark.parametrize("nc", [2, 3])
 def test_fp32_form_tracks_fp64(gpu, nc, kernel):
     """FP32 links/E with FP64 reunitarization every 10 steps stay within FP32
