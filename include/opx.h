@@ -278,6 +278,13 @@ int opx_cgc_download(opx_ctx *ctx, int which, double *out, int64_t n_values);
 int opx_cgc_wilson_tadpole(opx_ctx *ctx, double *value);
 int opx_cgc_wilson_dipole(opx_ctx *ctx, int capacity, int *num_bins, double *bin_width, double *corr,
                           int64_t *counts);
+/* LightConeNGPParticleCreator.initialize (physics/initial/CGC/LightConeNGPParticleCreator.java:82-234)
+ * from the last solve's Gauss violation, for the non-optimized CGC types: round(a_0/at) particles
+ * per cell of the planes z_start..z_end (the block of the super-particle creator, its last plane
+ * included), charges refined along every transverse cell's list, APPENDED to the context's
+ * per-particle list (opx_particles_*) with Q1 = Q0 for every particle of the list. added,
+ * z_start and z_end (each may be NULL) report the particles added and the planes. */
+int opx_cgc_create_particles(opx_ctx *ctx, int orientation, int64_t *added, int *z_start, int *z_end);
 /* frees the charge density and Gauss violation (IInitialChargeDensity.clear) */
 int opx_cgc_clear(opx_ctx *ctx);
 
@@ -292,6 +299,8 @@ int opx_particles_upload(opx_ctx *ctx, int count, const int *direction, const do
                          const double *charges, int64_t n_values);
 /* pos0, pos1 (3 per particle) and Q0, Q1 (reference storage); any output may be NULL */
 int opx_particles_download(opx_ctx *ctx, double *pos0, double *pos1, double *Q0, double *Q1, int64_t count);
+/* the number of particles in the context's per-particle list */
+int opx_particles_count(const opx_ctx *ctx, int64_t *count);
 /* interpolateChargedensity: rho(ngp(pos0)) += Q0 */
 int opx_particles_interpolate_chargedensity(opx_ctx *ctx);
 /* updatePosition (pos1 = pos0 + v dt), interpolateToParticle (the crossed link of Unext)

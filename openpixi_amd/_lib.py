@@ -86,6 +86,8 @@ SIGNATURES = {
     "opx_cgc_download": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64]),
     "opx_cgc_wilson_tadpole": (ctypes.c_int, [_vp, _dp]),
     "opx_cgc_wilson_dipole": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _i64p]),
+    "opx_cgc_create_particles": (ctypes.c_int, [_vp, ctypes.c_int, _i64p, _ip, _ip]),
+    "opx_particles_count": (ctypes.c_int, [_vp, _i64p]),
     "opx_cgc_clear": (ctypes.c_int, [_vp]),
     "opx_simulation_initialize": (ctypes.c_int, [_vp]),
     "opx_simulation_step": (ctypes.c_int, [_vp, ctypes.c_int]),

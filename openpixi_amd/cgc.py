@@ -1,8 +1,9 @@
 """CGC initial condition on the device, beam along x0: the host-side mirror of
 ``CGCInitialCondition`` (physics/initial/CGC/CGCInitialCondition.java:55-93) with an
 ``MVModel`` charge density (MVModel.java:101-157), ``LightConePoissonSolverImprovedFull``
-(LightConePoissonSolverImprovedFull.java:34-263) and ``LightConeNGPSuperParticleCreator``
-(LightConeNGPSuperParticleCreator.java:76-307).
+(LightConePoissonSolverImprovedFull.java:34-263), ``WilsonLineObservables``, and
+``LightConeNGPSuperParticleCreator`` (LightConeNGPSuperParticleCreator.java:76-307) or
+``LightConeNGPParticleCreator`` (LightConeNGPParticleCreator.java:82-297).
 
 The classes keep the reference's names, constructor arguments and call order; the work
 runs in ``libopx.so`` (``opx_cgc_*``), which keeps the charge density and the Gauss
@@ -398,6 +399,31 @@ class LightConeNGPSuperParticleCreator:
         self.added, self.zStart, self.blockWidth = added.value, zs.value, bw.value
 
 
+class LightConeNGPParticleCreator:
+    """physics/initial/CGC/LightConeNGPParticleCreator.java: round(a0/at) NGP particles per cell of
+    the block of planes that carries the solver's Gauss violation, their charges refined along
+    the beam, added to the simulation's per-particle list (the non-optimized CGC types)."""
+
+    def __init__(self):
+        self.particlesPerCell = 1
+        self.zStart = None
+        self.zEnd = None
+        self.added = 0
+
+    def setGaussConstraint(self, gaussConstraint=None):
+        """Kept for the reference's call sequence; the device holds the solver's violation."""
+
+    def initialize(self, s, direction, orientation):
+        _check_beam(direction)
+        g = _grid(s)
+        self.particlesPerCell = int(math.floor(g.getLatticeSpacing(0) / g.getTemporalSpacing() + 0.5))
+        added = ctypes.c_int64()
+        zs, ze = ctypes.c_int(), ctypes.c_int()
+        g._check(_lib.load().opx_cgc_create_particles(g.handle, int(orientation), ctypes.byref(added),
+                                                      ctypes.byref(zs), ctypes.byref(ze)))
+        self.added, self.zStart, self.zEnd = added.value, zs.value, ze.value
+
+
 class CGCInitialCondition:
     """physics/initial/CGC/CGCInitialCondition.java:55-93 for the optimized CPIC types
     (super particles)."""
@@ -421,8 +447,13 @@ class CGCInitialCondition:
 
     def applyInitialCondition(self, s):
         sim_type = getattr(getattr(s, "settings", None), "simulation_type", "TemporalOptimizedCGCNGP")
-        if sim_type not in ("TemporalOptimizedCGCNGP", "TemporalOptimizedImplicitCGCNGP"):
-            raise NotImplementedError(f"CGC particles for {sim_type}: only the super-particle (optimized) types are built")
+        creators = {"TemporalCGCNGP": LightConeNGPParticleCreator,
+                    "TemporalImplicitCGCNGP": LightConeNGPParticleCreator,
+                    "TemporalOptimizedCGCNGP": LightConeNGPSuperParticleCreator,
+                    "TemporalOptimizedImplicitCGCNGP": LightConeNGPSuperParticleCreator}
+        if sim_type not in creators:
+            # the reference throws a RuntimeException here (CGCInitialCondition.java:77-89)
+            raise RuntimeError(f"CGCInitialCondition: simulation type {sim_type} not supported!")
         self.initialChargeDensity.initialize(s)
         direction = self.initialChargeDensity.getDirection()
         orientation = self.initialChargeDensity.getOrientation()
@@ -433,7 +464,7 @@ class CGCInitialCondition:
             wilson.computeTadpole(self.tadpoleFilename)
         if self.computeDipole:
             wilson.computeDipole(self.dipoleFilename)
-        self.initialParticleCreator = LightConeNGPSuperParticleCreator()
+        self.initialParticleCreator = creators[sim_type]()
         self.initialParticleCreator.setGaussConstraint()
         self.initialParticleCreator.initialize(s, direction, orientation)
         self.initialChargeDensity.clear()

@@ -518,15 +518,16 @@ __device__ __forceinline__ int wrap(int i, int n)
 }
 
 // One sweep of the 2nd- or 4th-order charge refinement over a periodic list of nn charges
-// (refine2/refine4, LightConeNGPSuperParticleCreator.java:248-307), element l at col[l * st];
-// the last charge of each NGP cell is not moved. Written through memory as the reference does,
+// (refine2/refine4, LightConeNGPSuperParticleCreator.java:248-307 with shift = ppc / 2,
+// LightConeNGPParticleCreator.java:237-297 with shift = 0), element l at col[l * st]; the
+// charge at position i is not moved when (i + shift) % ppc = ppc - 1. Written through memory as the reference does,
 // so that short lists whose stencil wraps onto itself behave the same.
 template <int ORDER>
-__device__ __forceinline__ void refine_sweep(double* col, size_t st, int nn, int ppc)
+__device__ __forceinline__ void refine_sweep(double* col, size_t st, int nn, int ppc, int shift)
 {
 #pragma unroll 1
     for (int i = 0; i < nn; i++) {
-        const int jm = (i + ppc / 2) % ppc;
+        const int jm = (i + shift) % ppc;
         if (jm >= ppc - 1) continue;
         if constexpr (ORDER == 2) {
             double* Q0 = col + (size_t)wrap(i - 1, nn) * st;
@@ -579,24 +580,24 @@ __global__ __launch_bounds__(kB) void k_sp_init(uint32_t nT, uint32_t n, uint32_
 }
 
 // 100 sweeps of each order on every list, one thread per list (long lists)
-__global__ __launch_bounds__(kB) void k_sp_refine(uint32_t cols, int nn, int ppc, double* __restrict__ work)
+__global__ __launch_bounds__(kB) void k_sp_refine(uint32_t cols, int nn, int ppc, int shift, double* __restrict__ work)
 {
     const uint32_t g = blockIdx.x * kB + threadIdx.x;
     if (g >= cols) return;
 #pragma unroll 1
-    for (int it = 0; it < 100; it++) refine_sweep<2>(work + g, cols, nn, ppc);
+    for (int it = 0; it < 100; it++) refine_sweep<2>(work + g, cols, nn, ppc, shift);
 #pragma unroll 1
-    for (int it = 0; it < 100; it++) refine_sweep<4>(work + g, cols, nn, ppc);
+    for (int it = 0; it < 100; it++) refine_sweep<4>(work + g, cols, nn, ppc, shift);
 }
 
 // The matrix of one sweep, S[l * nn + c] = (sweep e_c)[l]: thread c runs the sweep on the unit vector c.
 template <int ORDER>
-__global__ __launch_bounds__(kB) void k_sweep_matrix(int nn, int ppc, double* __restrict__ S)
+__global__ __launch_bounds__(kB) void k_sweep_matrix(int nn, int ppc, int shift, double* __restrict__ S)
 {
     const int c = blockIdx.x * kB + threadIdx.x;
     if (c >= nn) return;
     for (int l = 0; l < nn; l++) S[(size_t)l * nn + c] = l == c ? 1.0 : 0.0;
-    refine_sweep<ORDER>(S + c, (size_t)nn, nn, ppc);
+    refine_sweep<ORDER>(S + c, (size_t)nn, nn, ppc, shift);
 }
 
 __global__ __launch_bounds__(kB) void k_identity(int nn, double* __restrict__ R)
@@ -682,6 +683,73 @@ __global__ __launch_bounds__(kB) void k_sp_scatter(uint32_t nT, int ad, SpBlock 
             const int64_t at = q_base + (int64_t)ppc * nT * P + (int64_t)k * w * nT + (int64_t)(p - P) * nT + t;
             q[(size_t)comp * q_total + at] = (T)work[(size_t)(p * ppc + k) * st + g];
         }
+    }
+}
+
+// The particles of LightConeNGPParticleCreator.initializeParticles (LightConeNGPParticleCreator.java:155-205)
+// for planes zs .. zs + bw - 1: particle p of the new range is (plane, transverse cell i, j) in
+// list order, at x0 = z a0 + 1e-11 a0 + (j - ppc/2)/ppc a0 (t = 0) and that plus orientation
+// at (t = at), transverse coordinates x a + 1e-11 a, velocity orientation along x0. Its charge
+// G(ngp(pos0)) / ppc goes to the refinement lists: work[(plane ppc + j) cols + comp nT + i].
+// Arrays [3][count] (positions, velocities) are written at `base + p`.
+__global__ __launch_bounds__(kB) void k_pp_init(Lattice L, int ad, SpBlock b, int orientation, const double* __restrict__ gauss,
+                                                double* __restrict__ work, int* __restrict__ dir, double* __restrict__ pos0,
+                                                double* __restrict__ pos1, double* __restrict__ vel, uint64_t count,
+                                                uint64_t base)
+{
+#pragma clang fp contract(off)
+    const uint32_t nT = L.s0, ppc = (uint32_t)b.ppc;
+    const uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (p >= (uint64_t)b.bw * nT * ppc) return;
+    const uint32_t plane = (uint32_t)(p / ((uint64_t)nT * ppc));
+    const uint32_t r = (uint32_t)(p - (uint64_t)plane * nT * ppc);
+    const uint32_t i = r / ppc, j = r - i * ppc;
+    const int gp[3] = {b.zs + (int)plane, (int)(i / L.d2), (int)(i % L.d2)};
+    const double x = (1.0 * (int)j - (int)(ppc / 2)) / (int)ppc;
+    const double dz = x * L.as[0];
+    const uint64_t q = base + p;
+    uint32_t ngp[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double fix = 10E-12 * L.as[k];
+        double a = gp[k] * L.as[k] + fix, c = gp[k] * L.as[k] + fix, v = 0.0;
+        if (k == 0) {
+            a += 0.0 * orientation + dz;
+            c += (0.0 + L.at) * orientation + dz;
+            v = 1.0 * orientation;
+        }
+        pos0[(size_t)k * count + q] = a;
+        pos1[(size_t)k * count + q] = c;
+        vel[(size_t)k * count + q] = v;
+        const uint32_t len = k == 0 ? L.d0 : (k == 1 ? L.d1 : L.d2);
+        long long g = (long long)rint(a / L.as[k]) % (long long)len;
+        if (g < 0) g += len;
+        ngp[k] = (uint32_t)g;
+    }
+    dir[q] = 0;
+    const uint32_t cell = ngp[0] * L.s0 + ngp[1] * L.s1 + ngp[2];
+    const size_t cols = (size_t)ad * nT;
+    const double inv = 1.0 / (int)ppc;
+    for (int comp = 0; comp < ad; comp++)
+        work[((size_t)plane * ppc + j) * cols + (size_t)comp * nT + i] = gauss[(size_t)comp * L.n + cell] * inv;
+}
+
+// the refined charges into Q0 and Q1 of the new particles ([coefficient][count], precision T)
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pp_scatter(uint32_t nT, int ad, SpBlock b, const double* __restrict__ work,
+                                                   T* __restrict__ Q0, T* __restrict__ Q1, uint64_t count, uint64_t base)
+{
+    const uint32_t ppc = (uint32_t)b.ppc;
+    const uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (p >= (uint64_t)b.bw * nT * ppc) return;
+    const uint32_t plane = (uint32_t)(p / ((uint64_t)nT * ppc));
+    const uint32_t r = (uint32_t)(p - (uint64_t)plane * nT * ppc);
+    const uint32_t i = r / ppc, j = r - i * ppc;
+    const size_t cols = (size_t)ad * nT;
+    for (int comp = 0; comp < ad; comp++) {
+        const T v = (T)work[((size_t)plane * ppc + j) * cols + (size_t)comp * nT + i];
+        Q0[(size_t)comp * count + base + p] = v;
+        Q1[(size_t)comp * count + base + p] = v;
     }
 }
 
@@ -1101,43 +1169,74 @@ double* matrix_power(int nn, double* S, int e, double* R, double* P, double* tmp
 
 }  // namespace
 
+namespace {
+
+// the 2nd- then 4th-order refinement of `cols` lists of nn = bw * ppc charges in work
+// ([position][list]); returns the buffer holding the result (work or the one after it)
+hipError_t refine(int nn, int ppc, int shift, uint32_t cols, double* work, const double** refined, hipStream_t st)
+{
+    *refined = work;
+    if (nn > kRefineMatrixMax) {
+        k_sp_refine<<<blocks(cols), kB, 0, st>>>(cols, nn, ppc, shift, work);
+        return hipGetLastError();
+    }
+    // every list goes through the same linear map: one sweep's matrix per order, raised to the
+    // 100th power, and (R4 R2) applied to all lists at once
+    double* out = work + (size_t)nn * cols;
+    double* m = out + (size_t)nn * cols;
+    const size_t mm = (size_t)nn * nn;
+    double *S = m, *X = m + mm, *Y = m + 2 * mm, *Z = m + 3 * mm, *W = m + 4 * mm;
+    hipError_t e = hipSuccess;
+    const unsigned sg = blocks((uint64_t)nn);
+    k_sweep_matrix<2><<<sg, kB, 0, st>>>(nn, ppc, shift, S);
+    double* R2 = matrix_power(nn, S, 100, X, Y, Z, st, &e);
+    if (e != hipSuccess) return e;
+    // keep R2 in W while the 4th-order power runs in the other buffers
+    if ((e = hipMemcpyAsync(W, R2, mm * sizeof(double), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
+    k_sweep_matrix<4><<<sg, kB, 0, st>>>(nn, ppc, shift, S);
+    double* R4 = matrix_power(nn, S, 100, X, Y, Z, st, &e);
+    if (e != hipSuccess) return e;
+    double* Rt = R4 == X ? Y : X;
+    if ((e = gemm(nn, nn, nn, R4, W, Rt, st)) != hipSuccess) return e;
+    if ((e = gemm(nn, (int)cols, nn, Rt, work, out, st)) != hipSuccess) return e;
+    *refined = out;
+    return hipSuccess;
+}
+
+}  // namespace
+
 hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, const double* gauss, double* work,
                           void* q, int64_t q_total, int64_t q_base, hipStream_t st)
 {
     const int ad = nc == 2 ? 3 : 8;
     const uint32_t cols = (uint32_t)ad * L.s0;
-    const int nn = b.bw * b.ppc;
     const unsigned grid = blocks(cols);
     k_sp_init<<<grid, kB, 0, st>>>(L.s0, L.n, L.d0, ad, b, gauss, work);
-    const double* refined = work;
-    if (nn <= kRefineMatrixMax) {
-        // every list goes through the same linear map: one sweep's matrix per order, raised to the
-        // 100th power, and (R4 R2) applied to all lists at once
-        double* out = work + (size_t)nn * cols;
-        double* m = out + (size_t)nn * cols;
-        const size_t mm = (size_t)nn * nn;
-        double *S = m, *X = m + mm, *Y = m + 2 * mm, *Z = m + 3 * mm, *W = m + 4 * mm;
-        hipError_t e = hipSuccess;
-        const unsigned sg = blocks((uint64_t)nn);
-        k_sweep_matrix<2><<<sg, kB, 0, st>>>(nn, b.ppc, S);
-        double* R2 = matrix_power(nn, S, 100, X, Y, Z, st, &e);
-        if (e != hipSuccess) return e;
-        // keep R2 in W while the 4th-order power runs in the other buffers
-        if ((e = hipMemcpyAsync(W, R2, mm * sizeof(double), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
-        k_sweep_matrix<4><<<sg, kB, 0, st>>>(nn, b.ppc, S);
-        double* R4 = matrix_power(nn, S, 100, X, Y, Z, st, &e);
-        if (e != hipSuccess) return e;
-        double* Rt = R4 == X ? Y : X;
-        if ((e = gemm(nn, nn, nn, R4, W, Rt, st)) != hipSuccess) return e;
-        if ((e = gemm(nn, (int)cols, nn, Rt, work, out, st)) != hipSuccess) return e;
-        refined = out;
-    } else {
-        k_sp_refine<<<grid, kB, 0, st>>>(cols, nn, b.ppc, work);
-    }
+    const double* refined = nullptr;
+    hipError_t e = refine(b.bw * b.ppc, b.ppc, b.ppc / 2, cols, work, &refined, st);
+    if (e != hipSuccess) return e;
     if (prec == 0)
         k_sp_scatter<double><<<grid, kB, 0, st>>>(L.s0, ad, b, refined, (double*)q, q_total, q_base);
     else
         k_sp_scatter<float><<<grid, kB, 0, st>>>(L.s0, ad, b, refined, (float*)q, q_total, q_base);
+    return hipGetLastError();
+}
+
+hipError_t cgc_particles(int nc, int prec, const Lattice& L, const SpBlock& b, int orientation, const double* gauss,
+                         double* work, int* dir, double* pos0, double* pos1, double* vel, void* Q0, void* Q1,
+                         uint64_t count, uint64_t base, hipStream_t st)
+{
+    const int ad = nc == 2 ? 3 : 8;
+    const uint32_t cols = (uint32_t)ad * L.s0;
+    const uint64_t np = (uint64_t)b.bw * L.s0 * b.ppc;
+    k_pp_init<<<blocks(np), kB, 0, st>>>(L, ad, b, orientation, gauss, work, dir, pos0, pos1, vel, count, base);
+    const double* refined = nullptr;
+    hipError_t e = refine(b.bw * b.ppc, b.ppc, 0, cols, work, &refined, st);
+    if (e != hipSuccess) return e;
+    if (prec == 0)
+        k_pp_scatter<double><<<blocks(np), kB, 0, st>>>(L.s0, ad, b, refined, (double*)Q0, (double*)Q1, count, base);
+    else
+        k_pp_scatter<float><<<blocks(np), kB, 0, st>>>(L.s0, ad, b, refined, (float*)Q0, (float*)Q1, count, base);
     return hipGetLastError();
 }
 

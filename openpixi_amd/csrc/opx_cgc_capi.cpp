@@ -281,23 +281,17 @@ int opx_cgc_lightcone_solve(opx_ctx* ctx, int orientation, double* gauss_out, in
     return OPX_OK;
 }
 
-int opx_cgc_create_superparticles(opx_ctx* ctx, int orientation, int num_threads, int* added, int* z_start,
-                                  int* block_width)
+}  // extern "C"
+
+namespace opx_host {
+namespace {
+
+// the planes that carry charge (LightConeNGPSuperParticleCreator.java:108-143,
+// LightConeNGPParticleCreator.java:116-150): the first plane whose largest |G| exceeds 1e-9 of
+// the global largest, and the last such plane below the final one
+int charge_block(opx_ctx* ctx, Scratch& s, const opx::Lattice& L, int* zs_out, int* ze_out)
 {
-    using namespace opx_host;
-    if (!ctx) return OPX_E_ARG;
-    int rc = cgc_check(ctx);
-    if (rc != OPX_OK) return rc;
-    if (orientation != 1 && orientation != -1) return fail(ctx, OPX_E_ARG, "super particles: orientation must be +1 or -1");
-    if (num_threads < 1) return fail(ctx, OPX_E_ARG, "super particles: numberOfThreads must be >= 1");
-    if (!ctx->cgc->gauss) return fail(ctx, OPX_E_STATE, "super particles: no light-cone solve yet");
-    const int ppc = (int)std::floor(ctx->as[0] / ctx->at + 0.5);
-    if (ppc < 1) return fail(ctx, OPX_E_ARG, "super particles: particlesPerCell = round(aL / at) must be >= 1");
-    const opx::Lattice L = lattice(ctx, ctx->at);
     const int Lz = ctx->dims[0];
-    const int64_t nT = (int64_t)ctx->dims[1] * ctx->dims[2];
-    Scratch s(ctx);
-    // the block of planes that carries charge
     double* dmx = nullptr;
     CGC_TRY(s.alloc(&dmx, (size_t)Lz), "alloc plane maxima");
     CGC_TRY(opx::cgc_plane_max(ctx->nc, L, ctx->cgc->gauss, dmx, ctx->stream), "plane maxima");
@@ -319,6 +313,121 @@ int opx_cgc_create_superparticles(opx_ctx* ctx, int orientation, int num_threads
             ze = z;
             break;
         }
+    *zs_out = zs;
+    *ze_out = ze;
+    return OPX_OK;
+}
+
+}  // namespace
+}  // namespace opx_host
+
+extern "C" {
+
+int opx_particles_count(const opx_ctx* ctx, int64_t* count)
+{
+    if (!ctx || !count) return OPX_E_ARG;
+    *count = ctx->p_count;
+    return OPX_OK;
+}
+
+int opx_cgc_create_particles(opx_ctx* ctx, int orientation, int64_t* added, int* z_start, int* z_end)
+{
+    using namespace opx_host;
+    if (!ctx) return OPX_E_ARG;
+    int rc = cgc_check(ctx);
+    if (rc != OPX_OK) return rc;
+    if (orientation != 1 && orientation != -1) return fail(ctx, OPX_E_ARG, "particles: orientation must be +1 or -1");
+    if (!ctx->cgc->gauss) return fail(ctx, OPX_E_STATE, "particles: no light-cone solve yet");
+    const int ppc = (int)std::floor(ctx->as[0] / ctx->at + 0.5);
+    if (ppc < 1) return fail(ctx, OPX_E_ARG, "particles: particlesPerCell = round(aL / at) must be >= 1");
+    const opx::Lattice L = lattice(ctx, ctx->at);
+    const int64_t nT = (int64_t)ctx->dims[1] * ctx->dims[2];
+    Scratch s(ctx);
+    int zs = 0, ze = 0;
+    if ((rc = charge_block(ctx, s, L, &zs, &ze)) != OPX_OK) return rc;
+    if (z_start) *z_start = zs;
+    if (z_end) *z_end = ze;
+    opx::SpBlock b;
+    b.zs = zs;
+    b.bw = ze - zs + 1;
+    b.ppc = ppc;
+    b.ws = 0;
+    b.nl = 0;
+    const uint64_t old = ctx->p_count, fresh = (uint64_t)b.bw * nT * ppc, total = old + fresh;
+    if (total >= (1ull << 32)) return fail(ctx, OPX_E_UNSUPPORTED, "particles: too many particles");
+    const size_t ad = (size_t)ctx->AD, tsz = ctx->tsz;
+    // new arrays: the existing particles' rows copied over, the new ones appended
+    int* dir = nullptr;
+    double *p0 = nullptr, *p1 = nullptr, *v = nullptr;
+    void *q0 = nullptr, *q1 = nullptr;
+    auto drop = [&]() {
+        void* bufs[] = {dir, p0, p1, v, q0, q1};
+        for (void* x : bufs)
+            if (x) (void)hipFree(x);
+    };
+    hipError_t e = hipMalloc((void**)&dir, total * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&p0, 3 * total * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&p1, 3 * total * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&v, 3 * total * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&q0, ad * total * tsz);
+    if (e == hipSuccess) e = hipMalloc(&q1, ad * total * tsz);
+    if (old > 0) {
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dir, ctx->p_dir, old * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream);
+        const double* src[3] = {ctx->p_pos0, ctx->p_pos1, ctx->p_vel};
+        double* dst[3] = {p0, p1, v};
+        for (int a = 0; a < 3; a++)
+            for (int k = 0; k < 3 && e == hipSuccess; k++)
+                e = hipMemcpyAsync(dst[a] + k * total, src[a] + k * old, old * sizeof(double), hipMemcpyDeviceToDevice,
+                                   ctx->stream);
+        for (size_t k = 0; k < ad && e == hipSuccess; k++)
+            e = hipMemcpyAsync((char*)q0 + k * total * tsz, (const char*)ctx->p_Q0 + k * old * tsz, old * tsz,
+                               hipMemcpyDeviceToDevice, ctx->stream);
+    }
+    double* work = nullptr;
+    if (e == hipSuccess) e = s.alloc(&work, opx::cgc_sp_work(b, (uint32_t)nT, ctx->AD));
+    if (e == hipSuccess)
+        e = opx::cgc_particles(ctx->nc, ctx->prec, L, b, orientation, ctx->cgc->gauss, work, dir, p0, p1, v, q0, q1,
+                               total, old, ctx->stream);
+    // every particle's Q1 is its Q0 (LightConeNGPParticleCreator.java:229-233)
+    if (e == hipSuccess) e = hipMemcpyAsync(q1, q0, ad * total * tsz, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(ctx->stream);
+        drop();
+        return hip_check(ctx, e, "particles");
+    }
+    void* olds[] = {ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel, ctx->p_Q0, ctx->p_Q1};
+    for (void* x : olds)
+        if (x) (void)hipFree(x);
+    ctx->p_dir = dir;
+    ctx->p_pos0 = p0;
+    ctx->p_pos1 = p1;
+    ctx->p_vel = v;
+    ctx->p_Q0 = q0;
+    ctx->p_Q1 = q1;
+    ctx->p_count = (uint32_t)total;
+    if (added) *added = (int64_t)fresh;
+    return OPX_OK;
+}
+
+int opx_cgc_create_superparticles(opx_ctx* ctx, int orientation, int num_threads, int* added, int* z_start,
+                                  int* block_width)
+{
+    using namespace opx_host;
+    if (!ctx) return OPX_E_ARG;
+    int rc = cgc_check(ctx);
+    if (rc != OPX_OK) return rc;
+    if (orientation != 1 && orientation != -1) return fail(ctx, OPX_E_ARG, "super particles: orientation must be +1 or -1");
+    if (num_threads < 1) return fail(ctx, OPX_E_ARG, "super particles: numberOfThreads must be >= 1");
+    if (!ctx->cgc->gauss) return fail(ctx, OPX_E_STATE, "super particles: no light-cone solve yet");
+    const int ppc = (int)std::floor(ctx->as[0] / ctx->at + 0.5);
+    if (ppc < 1) return fail(ctx, OPX_E_ARG, "super particles: particlesPerCell = round(aL / at) must be >= 1");
+    const opx::Lattice L = lattice(ctx, ctx->at);
+    const int64_t nT = (int64_t)ctx->dims[1] * ctx->dims[2];
+    Scratch s(ctx);
+    int zs = 0, ze = 0;
+    if ((rc = charge_block(ctx, s, L, &zs, &ze)) != OPX_OK) return rc;
     const int bw = ze - zs;
     if (z_start) *z_start = zs;
     if (block_width) *block_width = bw;

@@ -224,6 +224,11 @@ constexpr int kRefineMatrixMax = 2048;
 size_t cgc_sp_work(const SpBlock& b, uint32_t nT, int ad);
 hipError_t cgc_sp_charges(int nc, int prec, const Lattice& L, const SpBlock& b, const double* gauss, double* work,
                           void* q, int64_t q_total, int64_t q_base, hipStream_t st);
+// per-particle CGC particles of planes b.zs .. b.zs + b.bw - 1 (b.ppc per cell) written at
+// [base, base + b.bw N1 N2 b.ppc) of count-long particle arrays; work: cgc_sp_work doubles
+hipError_t cgc_particles(int nc, int prec, const Lattice& L, const SpBlock& b, int orientation, const double* gauss,
+                         double* work, int* dir, double* pos0, double* pos1, double* vel, void* Q0, void* Q1,
+                         uint64_t count, uint64_t base, hipStream_t st);
 // java.util.Random(seed).nextGaussian() x count on the device (seed already scrambled with
 // 0x5DEECE66D), from `trials` polar-method trials; *accepted (host) receives the number of
 // accepted trials, which must reach count / 2 rounded up. work: java_gaussian_work(trials) bytes.

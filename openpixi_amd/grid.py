@@ -271,9 +271,14 @@ class Grid:
                                                      _lib.dptr(q), q.size))
         self._p_count = len(d)
 
+    def particleCount(self):
+        c = ctypes.c_int64()
+        self._check(_lib.load().opx_particles_count(self._h, ctypes.byref(c)))
+        return c.value
+
     def particleState(self):
         """(pos0, pos1, Q0, Q1) of the per-particle list."""
-        c = getattr(self, "_p_count", 0)
+        c = self.particleCount()
         A = algebra_size(self.num_colors)
         p0, p1, q0, q1 = np.zeros(c * 3), np.zeros(c * 3), np.zeros(c * A), np.zeros(c * A)
         self._check(_lib.load().opx_particles_download(self._h, _lib.dptr(p0), _lib.dptr(p1), _lib.dptr(q0),

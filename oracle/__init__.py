@@ -131,6 +131,7 @@ def lib():
             "orc_wilson_segment": (None, [gp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]),
             "orc_lightcone_solve": (None, [gp, _dp, ctypes.c_int, _dp]),
             "orc_lightcone_solve_vt": (None, [gp, _dp, ctypes.c_int, _dp, _dp]),
+            "orc_lightcone_particles": (None, [gp, _dp] + [ctypes.c_int] * 4 + [_dp] * 4),
             "orc_wilson_tadpole": (ctypes.c_double, [ctypes.c_int, ctypes.c_long, _dp]),
             "orc_wilson_dipole": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          _dp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
@@ -446,6 +447,21 @@ class OracleGrid:
             charges.append(q[pos:pos + arrs["num"][s_]])
             pos += arrs["num"][s_]
         return arrs, charges, (zs.value, bw.value)
+
+    def lightcone_particles(self, gauss, orientation):
+        """LightConeNGPParticleCreator.initializeParticles: (direction, pos0, pos1, vel, charges) of the
+        per-particle CGC particles in list order, and the block (z_start, z_end)."""
+        gauss = np.ascontiguousarray(gauss, np.float64)
+        zs, bw = ctypes.c_int(), ctypes.c_int()
+        lib().orc_lightcone_block(self._g, _ptr(gauss), ctypes.byref(zs), ctypes.byref(bw))
+        ze = zs.value + bw.value
+        ppc = int(np.floor(self.spacing[0] / self.at + 0.5))
+        cnt = (ze - zs.value + 1) * self.dims[1] * self.dims[2] * ppc
+        p0, p1, v = np.zeros((cnt, 3)), np.zeros((cnt, 3)), np.zeros((cnt, 3))
+        q = np.zeros((cnt, asize(self.nc)))
+        lib().orc_lightcone_particles(self._g, _ptr(gauss), int(orientation), zs.value, ze, ppc, _ptr(p0), _ptr(p1),
+                                      _ptr(v), _ptr(q))
+        return np.zeros(cnt, np.int32), p0, p1, v, q, (zs.value, ze)
 
     def projected_poynting(self, direction):
         """Longitudinal Poynting vector per slice (averaged, time averaged); SU(2) only."""

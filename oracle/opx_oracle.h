@@ -88,6 +88,8 @@ void orc_lightcone_solve(orc_grid *gr, const double *rho, int orientation, doubl
 /* as orc_lightcone_solve; VT (N1 N2 group elements, may be NULL) receives the Wilson lines at -at/2
  * on the plane behind the nucleus (x0 = 0 for orientation +1, N0 - 1 for -1) */
 void orc_lightcone_solve_vt(orc_grid *gr, const double *rho, int orientation, double *gauss, double *VT);
+void orc_lightcone_particles(const orc_grid *gr, const double *gauss, int orientation, int z_start, int z_end, int ppc,
+                             double *pos0, double *pos1, double *vel, double *charges);
 double orc_wilson_tadpole(int nc, long T, const double *VT);
 void orc_wilson_dipole(int nc, int n1, int n2, double a1, double a2, const double *VT, int nbins, double *corr,
                        long *counts);

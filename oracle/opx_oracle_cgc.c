@@ -494,6 +494,93 @@ void orc_lightcone_superparticles(const orc_grid *gr, const double *gauss, int o
     free(list);
 }
 
+/* LightConeNGPParticleCreator.initializeParticles (LightConeNGPParticleCreator.java:104-234):
+ * ppc particles per cell of the planes z_start..z_end (inclusive), in (z, transverse cell, j)
+ * order, at x0 = z a0 + 1e-11 a0 + (j - ppc/2)/ppc a0 (t = 0) and the same plus orientation at
+ * (t = at), the transverse coordinates at x a + 1e-11 a, v = orientation along x0; charge
+ * G(ngp(pos0)) / ppc, refined along each transverse cell's list (2nd then 4th order, 100
+ * sweeps each; the last particle of a cell is not moved). pos0/pos1/vel are [count][3],
+ * charges [count][A]; count = (z_end - z_start + 1) * N1 N2 * ppc. */
+void orc_lightcone_particles(const orc_grid *gr, const double *gauss, int orientation, int z_start, int z_end, int ppc,
+                             double *pos0, double *pos1, double *vel, double *charges)
+{
+    const int A = gr->asz, n1 = gr->dims[1], n2 = gr->dims[2];
+    const long T = (long)n1 * n2;
+    const int planes = z_end - z_start + 1;
+    const long nlong = (long)planes * ppc;
+    long *list = (long *)malloc(sizeof(long) * T * nlong);
+    long cnt = 0;
+    for (int z = z_start; z <= z_end; z++)
+        for (long i = 0; i < T; i++) {
+            const int gp[3] = {z, (int)(i / n2), (int)(i % n2)};
+            for (int j = 0; j < ppc; j++) {
+                const double x = (1.0 * j - ppc / 2) / ppc;
+                const double dz = x * gr->as[0];
+                double *p0 = pos0 + cnt * 3, *p1 = pos1 + cnt * 3, *v = vel + cnt * 3;
+                for (int k = 0; k < 3; k++) {
+                    const double fix = 10E-12 * gr->as[k];
+                    p0[k] = gp[k] * gr->as[k] + fix;
+                    p1[k] = gp[k] * gr->as[k] + fix;
+                    v[k] = 0.0;
+                    if (k == 0) {
+                        p0[k] += 0.0 * orientation + dz;
+                        p1[k] += (0.0 + gr->at) * orientation + dz;
+                        v[k] = 1.0 * orientation;
+                    }
+                }
+                long ngp[3];
+                for (int k = 0; k < 3; k++) {
+                    long c = (long)rint(p0[k] / gr->as[k]);
+                    c %= gr->dims[k];
+                    if (c < 0) c += gr->dims[k];
+                    ngp[k] = c;
+                }
+                const long index = (ngp[0] * n1 + ngp[1]) * n2 + ngp[2];
+                for (int q = 0; q < A; q++) charges[cnt * A + q] = gauss[index * A + q] * (1.0 / ppc);
+                list[i * nlong + (long)(z - z_start) * ppc + j] = cnt;
+                cnt++;
+            }
+        }
+    /* refinement along each transverse cell's list; refine2/refine4 use jmod = i % ppc here */
+    for (long t = 0; t < T; t++) {
+        long *lst = list + t * nlong;
+        const int nn = (int)nlong;
+        for (int it = 0; it < 100; it++)
+            for (int i = 0; i < nn; i++) {
+                if (!(i % ppc < ppc - 1)) continue;
+                double *Q0 = charges + lst[pmod(i - 1, nn)] * A, *Q1 = charges + lst[pmod(i, nn)] * A;
+                double *Q2 = charges + lst[pmod(i + 1, nn)] * A, *Q3 = charges + lst[pmod(i + 2, nn)] * A;
+                for (int q = 0; q < A; q++) {
+                    double dq = Q0[q] * -1;
+                    dq += Q1[q] * 3;
+                    dq += Q2[q] * -3;
+                    dq += Q3[q] * 1;
+                    dq *= 1.0 / 4.0;
+                    Q1[q] += dq * -1.0;
+                    Q2[q] += dq * 1.0;
+                }
+            }
+        for (int it = 0; it < 100; it++)
+            for (int i = 0; i < nn; i++) {
+                if (!(i % ppc < ppc - 1)) continue;
+                double *Q[6];
+                for (int m = 0; m < 6; m++) Q[m] = charges + lst[pmod(i - 2 + m, nn)] * A;
+                for (int q = 0; q < A; q++) {
+                    double dq = Q[0][q] * +1;
+                    dq += Q[1][q] * -5;
+                    dq += Q[2][q] * +10;
+                    dq += Q[3][q] * -10;
+                    dq += Q[4][q] * +5;
+                    dq += Q[5][q] * -1;
+                    dq *= 1.0 / 12.0;
+                    Q[2][q] += dq * -1.0;
+                    Q[3][q] += dq * 1.0;
+                }
+            }
+    }
+    free(list);
+}
+
 /* ===================================================================== */
 /* Wilson-line observables                                                */
 /* ===================================================================== */

@@ -345,3 +345,63 @@ def test_seeded_mv_charge_equals_host_noise(gpu, nc):
     want = og.mv_charge(p["location"], p["width"], p["mu"], p["uvt"], p["long_width"], p["ir"],
                         JavaRandom(77).gaussians((nc * nc - 1) * og.n))
     assert rel_err(seeded, want) < 1e-12
+
+
+@pytest.mark.parametrize("nc,orientation", [(2, 1), (2, -1), (3, 1), (3, -1)])
+def test_particles_match_oracle(gpu, nc, orientation):
+    """LightConeNGPParticleCreator on the device: positions at t = 0 and t = at bit for bit,
+    charges (Q0 = Q1) against the oracle's refined list."""
+    from openpixi_amd.cgc import LightConeNGPParticleCreator
+
+    dims = (16, 4, 6)
+    print(f"particles nc={nc} orientation={orientation}", flush=True)
+    og, dg, rho = solve_setup(nc, dims, "fp64", seed=21)
+    gauss = og.lightcone_solve(rho, orientation)
+    _, p0, p1, _, q, (zs, ze) = og.lightcone_particles(gauss, orientation)
+    density = ChargeDensity(rho, 0, orientation)
+    density.initialize(dg)
+    solver = LightConePoissonSolverImprovedFull()
+    solver.initialize(dg)
+    solver.solve(density)
+    creator = LightConeNGPParticleCreator()
+    creator.initialize(_Sim(dg, 1), 0, orientation)
+    assert (creator.zStart, creator.zEnd, creator.added) == (zs, ze, len(q))
+    d0, d1, dq0, dq1 = dg.particleState()
+    assert np.array_equal(d0, p0) and np.array_equal(d1, p1)
+    assert rel_err(dq0, q) < 1e-11 and np.array_equal(dq0, dq1)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_seeded_yaml_collision_per_particle_matches_oracle(gpu, nc):
+    """The YAML collision of "temporal cgc ngp": per-particle CGC particles for two nuclei,
+    initialize and four steps, against the oracle fed the same noise."""
+    print(f"YAML CGC per-particle collision nc={nc}", flush=True)
+    text = YAML.format(nc=nc).replace("temporal optimized cgc ngp", "temporal cgc ngp")
+    settings = Settings.from_yaml(text)
+    sim = Simulation(settings)
+    dims = tuple(settings.grid_cells)
+    og = O.OracleGrid(nc, dims, (1.0, 1.0, 1.0), 0.25, 1.0)
+    parts = []
+    for seed, orient, loc in ((11, 1, 7.0), (12, -1, 17.0)):
+        noise = JavaRandom(seed).gaussians((nc * nc - 1) * og.n)
+        rho = og.mv_charge(loc, 1.2, 0.5, 3.0, 0.4, 0.3, noise)
+        gauss = og.lightcone_solve(rho, orient)
+        parts.append(og.lightcone_particles(gauss, orient))
+    d = np.concatenate([p[0] for p in parts])
+    p0 = np.concatenate([p[1] for p in parts])
+    v = np.concatenate([p[3] for p in parts])
+    q = np.concatenate([p[4] for p in parts])
+    g0, _, gq0, _ = sim.grid.particleState()
+    assert np.array_equal(g0, p0) and rel_err(gq0, q) < 1e-11
+    assert rel_err(sim.grid.getE(), og.E) < 1e-12
+    og.set_particles(d, p0, v, q)
+    og.initialize()
+    sim.initialize()
+    og.sim_step(4)
+    sim.step(4)
+    assert rel_err(sim.grid.getE(), og.E) < 1e-11
+    assert rel_err(sim.grid.getU(), og.U) < 1e-11
+    assert rel_err(sim.grid.getRho(), og.rho) < 1e-11
+    pos0, _, _, Q1 = og.particle_state()
+    g0, _, _, gQ1 = sim.grid.particleState()
+    assert np.array_equal(g0, pos0) and rel_err(gQ1, Q1) < 1e-11

@@ -245,3 +245,40 @@ def test_wilson_lines_kept_from_the_plane_behind_the_nucleus(orientation):
     og2 = O.OracleGrid(2, dims, (1.0, 1.0, 1.0), 0.25, 1.0)
     og2.lightcone_solve(np.zeros((og2.n, 3)), orientation)
     assert np.max(np.abs(og2.VT - np.array([1.0, 0, 0, 0]))) == 0.0
+
+
+@pytest.mark.parametrize("nc,orientation", [(2, 1), (2, -1), (3, 1)])
+def test_per_particle_creator_deposits_the_gauss_violation(nc, orientation):
+    """LightConeNGPParticleCreator: every particle's nearest grid point is its own cell, ppc of
+    them per cell of the planes z_start..z_end, and the refinement (which never moves the last
+    particle of a cell) keeps each cell's total, so the particles deposit exactly the Gauss
+    violation on those planes and the initialized fields satisfy Gauss's law there."""
+    dims = (24, 4, 4)
+    og, rho, gauss = cgc_state(nc, dims, orientation, seed=7)
+    d, p0, p1, v, q, (zs, ze) = og.lightcone_particles(gauss, orientation)
+    T = dims[1] * dims[2]
+    ppc = int(round(1.0 / 0.25))
+    assert len(q) == (ze - zs + 1) * T * ppc and ze > zs
+    # layout: (z, transverse cell, j) order, NGP = own cell, v = orientation along x0
+    ngp = np.rint(p0).astype(int)
+    cells = (ngp[:, 0] * dims[1] + ngp[:, 1]) * dims[2] + ngp[:, 2]
+    assert np.array_equal(cells, np.repeat(np.arange(zs * T, (ze + 1) * T), ppc))
+    assert np.array_equal(v, np.tile([orientation, 0.0, 0.0], (len(q), 1)))
+    assert np.allclose(p1[:, 0] - p0[:, 0], 0.25 * orientation, atol=1e-12)
+    sub = p0[:ppc, 0] - zs
+    assert np.allclose(np.sort(sub), (np.arange(ppc) - ppc // 2) / ppc, atol=1e-10)
+    # the deposit is the Gauss violation on the block planes, nothing elsewhere
+    dep = np.zeros_like(gauss)
+    np.add.at(dep, cells, q)
+    scale = np.max(np.abs(gauss))
+    assert np.max(np.abs(dep[zs * T:(ze + 1) * T] - gauss[zs * T:(ze + 1) * T])) < 1e-12 * scale
+    assert np.all(dep[:zs * T] == 0) and np.all(dep[(ze + 1) * T:] == 0)
+    # Gauss's law with that charge density on the block planes; outside them the violation
+    # stays (the block never includes the last plane, whose violation the scan skips)
+    og.rho[:] = dep
+    g2 = np.array([O.lib().orc_gauss_constraint_squared(og._g, c) for c in range(og.n)])
+    q2 = np.array([O.alg_square(nc, x) for x in gauss])
+    assert np.max(g2[zs * T:(ze + 1) * T]) < 1e-20 * np.max(q2)
+    assert np.allclose(g2[(ze + 1) * T:], q2[(ze + 1) * T:], rtol=1e-9, atol=1e-30)
+    # the refinement moved charge between the particles of a cell
+    assert np.max(np.abs(q.reshape(-1, ppc, O.asize(nc)) - q.reshape(-1, ppc, O.asize(nc)).mean(axis=1, keepdims=True))) > 0
