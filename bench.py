@@ -316,6 +316,7 @@ def time_batch(count, steps, warmup, stream, autotune=True):
             "events_per_s": count / (ms * 1e-3),
             "link_updates_per_s": 3 * cells * steps / (ms * 1e-3),
             "grid_workgroups": info["grid"],
+            "form": b.form(),
             "autotune_ms": tune_ms,
             "max_gauss_residual_end": float(m1[:, 6].max()),
         }

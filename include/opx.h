@@ -327,7 +327,8 @@ typedef struct opx_batch opx_batch;
 enum {
     OPX_BATCH_OPT_GRID = 0,   /* workgroups of the persistent kernel */
     OPX_BATCH_OPT_FORM = 1    /* 0: per-link kernel; 1, 2: LDS-tile kernel with non-temporal / cached
-                                 stores of E and Unext (SU(2) only) */
+                                 stores of E and Unext (SU(2) only); 3: cell kernel (the three links
+                                 of a cell per thread, as OPX_KERNEL_CELL) */
 };
 int opx_batch_create(int num_colors, int count, const int *dims, const double *spacing, double time_step,
                      double coupling, int precision, opx_batch **out);

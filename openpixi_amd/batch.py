@@ -101,11 +101,12 @@ class Batch:
     def set_grid(self, workgroups):
         self._check(_lib.load().opx_batch_set_option(self._h, _lib.BATCH_OPT_GRID, int(workgroups)))
 
-    FORMS = ["link", "tile", "tile-cached"]
+    FORMS = ["link", "tile", "tile-cached", "cell"]
 
     def set_form(self, form):
         """"link" (per-link persistent kernel), "tile" or "tile-cached" (LDS-tile kernel with
-        non-temporal or cached stores of E and Unext; SU(2) only)."""
+        non-temporal or cached stores of E and Unext; SU(2) only), "cell" (the three links of
+        a cell per thread)."""
         self._check(_lib.load().opx_batch_set_option(self._h, _lib.BATCH_OPT_FORM, self.FORMS.index(form)))
 
     def form(self):

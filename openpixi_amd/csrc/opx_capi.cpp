@@ -1443,7 +1443,11 @@ int batch_step(opx_batch* b, int reunit_every)
     b->parity ^= 1;
     void* U = b->parity ? b->UB : b->UA;
     void* Un = b->parity ? b->UA : b->UB;
-    if (b->form >= 1)
+    if (b->form == 3)
+        OPX_BTRY(opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, U, Un, b->E,
+                                        b->stream),
+                 "batch cell update");
+    else if (b->form >= 1)
         OPX_BTRY(opx::launch_batch_tile(b->prec, b->form == 1, b->dev_ev, b->dev_items, b->nitems, b->grid, U, Un,
                                         b->E, b->stream),
                  "batch tile update");
@@ -1636,8 +1640,9 @@ int opx_batch_set_option(opx_batch* b, int option, int value)
         if (value < 1) return bfail(b, OPX_E_ARG, "batch option: grid must be >= 1");
         b->grid = (uint32_t)value;
     } else if (option == OPX_BATCH_OPT_FORM) {
-        if (value < 0 || value > 2) return bfail(b, OPX_E_ARG, "batch option: form must be 0, 1 or 2");
-        if (value >= 1 && !b->dev_items) return bfail(b, OPX_E_UNSUPPORTED, "batch option: the tile form is SU(2) only");
+        if (value < 0 || value > 3) return bfail(b, OPX_E_ARG, "batch option: form must be 0, 1, 2 or 3");
+        if ((value == 1 || value == 2) && !b->dev_items)
+            return bfail(b, OPX_E_UNSUPPORTED, "batch option: the tile form is SU(2) only");
         b->form = value;
     } else {
         return bfail(b, OPX_E_ARG, "batch option: unknown option");
@@ -1751,8 +1756,12 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
     if (b->dev_items)
         for (int m : {1, 2, 3, 4, 6, 8})
             for (int f : {1, 2}) cands.push_back({f, (uint32_t)(m * b->num_cus)});
+    for (int m : {1, 2, 3, 4, 6, 8, 12, 16}) cands.push_back({3, (uint32_t)(m * b->num_cus)});
     // first launch from the batch's links, timed launches alternating between two scratch fields
     auto run = [&](const Cand& c, const void* src, void* dst) {
+        if (c.form == 3)
+            return opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, src, dst, e2,
+                                          b->stream);
         return c.form >= 1 ? opx::launch_batch_tile(b->prec, c.form == 1, b->dev_ev, b->dev_items, b->nitems, c.grid,
                                                     src, dst, e2, b->stream)
                            : opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, src,

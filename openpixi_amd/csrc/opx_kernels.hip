@@ -579,6 +579,36 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
 }
 
 
+// Persistent cell form over a batch: work item i of event e covers cells
+// [256 (i - item_start/3), +256) of that event, all three links of each (the per-link items of
+// an event number 3 ceil(n/256), so item_start/3 numbers the cell items); same XCD-aware
+// static schedule as k_batch_update.
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock, OPX_CELL_WAVES) void k_batch_cell(const BatchEvent* __restrict__ ev, int nev,
+                                                                       uint32_t total_items, const T* __restrict__ Ubase,
+                                                                       T* __restrict__ Unbase, T* __restrict__ Ebase)
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
+    const uint32_t wg_on_xcd = q8 + (xcd < r8 ? 1u : 0u);
+    const uint32_t w = b >> 3;
+    const uint32_t i0 = (uint32_t)(((uint64_t)total_items * xcd) / 8), i1 = (uint32_t)(((uint64_t)total_items * (xcd + 1)) / 8);
+    for (uint32_t item = i0 + w; item < i1; item += wg_on_xcd) {
+        int lo = 0, hi = nev - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ev[mid].item_start / 3 <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchEvent& E = ev[lo];
+        const uint32_t site = (item - E.item_start / 3) * kBlock + threadIdx.x;
+        if (site >= E.L.n) continue;
+        const FieldG<T> U(Ubase + E.link_offset, E.L.n, 0), Un(Unbase + E.link_offset, E.L.n, 0);
+        const FieldG<T> Ef(Ebase + E.alg_offset, E.L.n, 0);
+        cell_update<NC, T, false>(E.L, U, Un, Ef, Ef, site);
+    }
+}
+
 // ---------------------------------------------------------------- kernel 2
 // FastTYMSolver.UpdateLinks.execute (physics/fields/FastTYMSolver.java:65-84)
 template <int NC, typename T>
@@ -1405,6 +1435,16 @@ hipError_t launch_batch_update(int nc, int prec, const BatchEvent* ev, int nev, 
     OPX_DISPATCH(nc, prec, {
         hipLaunchKernelGGL((k_batch_update<NC, T>), dim3(grid), dim3(kBlock), 0, st, ev, nev, total_items, (const T*)U,
                            (T*)Unext, (T*)E);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_cell(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items, uint32_t grid,
+                             const void* U, void* Unext, void* E, hipStream_t st)
+{
+    OPX_DISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_batch_cell<NC, T>), dim3(grid), dim3(kBlock), 0, st, ev, nev, total_items / 3,
+                           (const T*)U, (T*)Unext, (T*)E);
     });
     return hipGetLastError();
 }

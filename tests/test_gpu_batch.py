@@ -109,18 +109,20 @@ def test_stepping_an_event_view_is_rejected(gpu):
         FastTYMSolver().step(b.event(0), 0.25)
 
 
-@pytest.mark.parametrize("form", ["tile", "tile-cached"])
+@pytest.mark.parametrize("nc,form", [(2, "tile"), (2, "tile-cached"), (2, "cell"), (3, "cell")])
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_tile_form_matches_oracle(gpu, graph, form):
-    """The LDS-tile batch form (SU(2)) against per-event oracle runs."""
+def test_batch_tile_form_matches_oracle(gpu, graph, nc, form):
+    """The LDS-tile (SU(2)) and cell batch forms against per-event oracle runs."""
     dims_list = [(6, 6, 6), (8, 5, 7), (20, 17, 33), (1, 9, 6), (10, 18, 5)]
-    b, states = filled_batch(2, "fp64", dims_list)
+    if nc == 3:
+        dims_list = [(6, 6, 6), (8, 5, 7), (1, 9, 6), (10, 18, 5)]
+    b, states = filled_batch(nc, "fp64", dims_list)
     b.set_form(form)
     assert b.form() == form
     b.step_links(0.25)
     b.leapfrog(12, 0, graph)
     for e, dims in enumerate(dims_list):
-        og = oracle_grid(2, dims, *states[e])
+        og = oracle_grid(nc, dims, *states[e])
         og.step_links(0.25)
         for _ in range(12):
             og.store_fields()
