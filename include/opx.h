@@ -353,7 +353,9 @@ int opx_batch_autotune(opx_batch *batch, int reps, double *best_ms);
 /* ---- Simulation driver (physics/Simulation.java:318-452) -----------------
  * opx_simulation_initialize: stepLinks, rho, J deposit at t = 0.
  * opx_simulation_step: n times { clock++, storeFields, step, resetCharge,
- * interpolateChargedensity, resetCurrent, interpolateToGrid }.
+ * interpolateChargedensity, resetCurrent, interpolateToGrid }. Nothing inside a step reads
+ * the charge density, so it is gathered on the last of the n steps only; the state after the
+ * call is the same. Diagnostics that read rho between steps call with n = 1.
  * opx_simulation_steps_done returns the integer clock. */
 int opx_simulation_initialize(opx_ctx *ctx);
 int opx_simulation_step(opx_ctx *ctx, int n);

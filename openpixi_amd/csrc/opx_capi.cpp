@@ -1116,16 +1116,19 @@ int opx_simulation_step(opx_ctx* ctx, int n)
     for (int s = 0; s < n; s++) {
         ctx->steps++;
         const int t = ctx->steps;
+        // rho is written every step and read by nothing inside a step (the solver reads J), so
+        // within one call only the last step's charge density is observable: it alone is gathered
+        const bool last = s == n - 1;
         if ((rc = opx_particles_reassign(ctx)) != OPX_OK) return rc;
         if ((rc = opx_store_fields(ctx)) != OPX_OK) return rc;
         if ((rc = opx_step(ctx, ctx->at)) != OPX_OK) return rc;
         if (ctx->sp_count > 0 || ctx->p_count > 0) {
-            if ((rc = opx_reset_charge(ctx)) != OPX_OK) return rc;
-            if ((rc = opx_interpolate_chargedensity(ctx, t)) != OPX_OK) return rc;
-            if (ctx->p_count > 0) {
-                if ((rc = opx_particles_interpolate_chargedensity(ctx)) != OPX_OK) return rc;
-                if ((rc = opx_particles_push(ctx, ctx->at)) != OPX_OK) return rc;
+            if (last) {
+                if ((rc = opx_reset_charge(ctx)) != OPX_OK) return rc;
+                if ((rc = opx_interpolate_chargedensity(ctx, t)) != OPX_OK) return rc;
+                if (ctx->p_count > 0 && (rc = opx_particles_interpolate_chargedensity(ctx)) != OPX_OK) return rc;
             }
+            if (ctx->p_count > 0 && (rc = opx_particles_push(ctx, ctx->at)) != OPX_OK) return rc;
             if ((rc = opx_reset_current(ctx)) != OPX_OK) return rc;
             if ((rc = opx_interpolate_to_grid(ctx, t)) != OPX_OK) return rc;
             if ((rc = opx_particles_interpolate_to_grid(ctx)) != OPX_OK) return rc;
