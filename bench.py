@@ -582,12 +582,19 @@ def cpu_baseline(budget_s):
     t1 = og.time_steps(1, 1)
     steps = max(1, int(budget_s / max(t1, 1e-6)))
     t = og.time_steps(steps, 0)
+    # the same on one core, a third of the budget
+    og.set_threads(1)
+    s1 = og.time_steps(1, 0)
+    steps1 = max(1, int(budget_s / 3 / max(s1, 1e-6)))
+    t_one = og.time_steps(steps1, 0)
     return {
         "value": 3 * n * steps / t,
         "unit": "link-updates/s",
         "cores": cores,
         "kind": "port",
         "sample": f"FP64 oracle (C, -O2, OpenMP {cores} threads), SU(3) 32^3, {steps} steps, {t:.1f} s",
+        "single_core": {"value": 3 * n * steps1 / t_one, "unit": "link-updates/s", "cores": 1,
+                        "sample": f"the same oracle on one thread, {steps1} steps, {t_one:.1f} s"},
     }
 
 
