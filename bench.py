@@ -626,7 +626,8 @@ def main():
         print(f"# batch: {json.dumps(batch)}", file=sys.stderr, flush=True)
     implicit = None
     if not args.no_implicit:
-        implicit = time_implicit(args.implicit_steps, 2, 0.25, stream)
+        # the settings of input/CGC/MV model/MVModel_implicit_solver.yaml: 3 iterations, damping 0
+        implicit = time_implicit(args.implicit_steps, 3, 0.0, stream)
         print(f"# implicit: {json.dumps(implicit)}", file=sys.stderr, flush=True)
     cgc = None
     if not args.no_cgc:

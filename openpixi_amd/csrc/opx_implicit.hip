@@ -299,6 +299,9 @@ __device__ M damped_link(const M& u, const M& v, T damping)
         vd = reunit(vd);
     }
     const double dd = (double)damping;
+    // x^0 = getLink(0) is the identity exactly, and so is its product with the other factor
+    if (dd == 0.0) return from_double<M>(group_pow(vd, 1.0));
+    if (dd == 1.0) return from_double<M>(group_pow(ud, 1.0));
     return from_double<M>(mul(group_pow(ud, dd), group_pow(vd, 1.0 - dd)));
 }
 
