@@ -7,7 +7,8 @@
 * the super particles: descriptors, refined charges, uneven subdivisions, a second nucleus
   appended to the list;
 * a seeded YAML run of "temporal optimized cgc ngp" (JavaRandom noise, two nuclei, particles,
-  initialize and steps) against the same run of the oracle.
+  initialize and steps) against the same run of the oracle;
+* the same YAML run through Simulation.run() with the tadpole, dipole and bulk files written.
 
 FP64 bars are relative to the largest value: 1e-12 for the charges, fields and Gauss
 violation, 1e-11 for the refined charges (200 sweeps of a 4- and a 6-point stencil each).
@@ -405,3 +406,79 @@ def test_seeded_yaml_collision_per_particle_matches_oracle(gpu, nc):
     pos0, _, _, Q1 = og.particle_state()
     g0, _, _, gQ1 = sim.grid.particleState()
     assert np.array_equal(g0, pos0) and rel_err(gQ1, Q1) < 1e-11
+
+
+def test_yaml_collision_run_writes_reference_files(gpu, tmp_path, monkeypatch):
+    """The whole reference-style run: a YAML file with two seeded nuclei asking for the tadpole
+    and dipole of each nucleus's Wilson lines and for bulkQuantitiesInTime, through
+    Simulation(...), initialize() and run(). The three files under output/ hold what the oracle
+    run of the same noise gives: one tadpole line and one dipole block per nucleus (in the order
+    of the MVModel list), and the bulk rows of steps 0, 2 and 4."""
+    from openpixi_amd.diagnostics import java_sci
+
+    nc = 3
+    print(f"YAML CGC run with output files nc={nc}", flush=True)
+    text = YAML.format(nc=nc).replace("    poissonSolver: improved full",
+                                      "    poissonSolver: improved full\n    computeTadpole: true\n"
+                                      "    computeDipole: true\n    dipoleFilename: wilson/dipole.dat")
+    text += "output:\n  bulkQuantitiesInTime:\n    - path: bulk.dat\n      interval: 0.5\n"
+    settings = Settings.from_yaml(text)
+    monkeypatch.chdir(tmp_path)
+    sim = Simulation(settings)
+    sim.initialize()
+    sim.run()
+    assert sim.totalSimulationSteps == 5   # duration / timeStep + 1 steps
+
+    dims = tuple(settings.grid_cells)
+    og = O.OracleGrid(nc, dims, (1.0, 1.0, 1.0), 0.25, 1.0)
+    nbins = int(np.sqrt(dims[1] * dims[2])) // 2
+    all_arrs, all_charges, tadpoles, dipoles = None, [], [], []
+    for seed, orient, loc in ((11, 1, 7.0), (12, -1, 17.0)):
+        rho = og.mv_charge(loc, 1.2, 0.5, 3.0, 0.4, 0.3, JavaRandom(seed).gaussians((nc * nc - 1) * og.n))
+        gauss = og.lightcone_solve(rho, orient)
+        tadpoles.append(O.wilson_tadpole(nc, og.VT))
+        dipoles.append(O.wilson_dipole(nc, dims[1], dims[2], 1.0, 1.0, og.VT, nbins))
+        arrs, charges, _ = og.lightcone_superparticles(gauss, orient, 3)
+        all_charges += charges
+        all_arrs = arrs if all_arrs is None else {k: np.concatenate([all_arrs[k], arrs[k]]) for k in arrs}
+    og.set_superparticles(all_arrs["orientation"], all_arrs["num"], all_arrs["index_offset"],
+                          all_arrs["per_plane"], all_arrs["shift"], all_arrs["ppc"], all_charges)
+    og.initialize()
+    want_bulk = []
+    for step in range(5):
+        if step > 0:
+            og.sim_step(1)
+        if step % 2 == 0:
+            m = og.measure()
+            es, bs = m[0:3], m[3:6]
+            want_bulk.append([step * 0.25, es.sum(), bs.sum(), -es[0] + es[1] + es[2] - bs[0] + bs[1] + bs[2],
+                              es[0] - es[1] + es[2] + bs[0] - bs[1] + bs[2],
+                              es[0] + es[1] - es[2] + bs[0] + bs[1] - bs[2], m[6], m[7]])
+
+    info = "MV, mu: 0.500000, w: 1.200000, UVT: 3.000000, c: 0.400000, m: 0.300000"
+    out = tmp_path / "output"
+    t = (out / "tadpole.txt").read_text().splitlines()
+    assert len(t) == 2
+    for line, want in zip(t, tadpoles):
+        head, v = line.split(", V: ")
+        assert head == info
+        assert float(v) == pytest.approx(want, abs=1e-12 * nc)
+    lines = (out / "wilson" / "dipole.dat").read_text().splitlines()
+    assert len(lines) == 2 * (2 + nbins)
+    for k, (want_c, want_n) in enumerate(dipoles):
+        block = lines[k * (2 + nbins): (k + 1) * (2 + nbins)]
+        assert block[:2] == [info, "d\ttr(V V^t)"]
+        for b, line in enumerate(block[2:]):
+            x, v = line.split("\t")
+            assert x == java_sci(b * min(dims[1], dims[2]) * 0.5 / nbins)   # bin * ds, unit spacing
+            assert float(v) == pytest.approx(want_c[b], abs=1e-12 * nc)
+
+    lines = (out / "bulk.dat").read_text().splitlines()
+    assert lines[0] == "#time \t E^2 \t B^2 \t P_x \t P_y \t P_z \t G \t rho^2"
+    rows = [np.array([float(x) for x in l.split("\t")]) for l in lines[1:]]
+    assert len(rows) == len(want_bulk) == 3
+    for got, want in zip(rows, want_bulk):
+        assert got[0] == want[0]
+        assert np.allclose(got[1:6], want[1:6], rtol=1e-11, atol=0)
+        assert abs(got[6] - want[6]) < 1e-11 * want[7]   # Gauss violation: round-off of rho^2
+        assert got[7] == pytest.approx(want[7], rel=1e-11) and want[7] > 0
