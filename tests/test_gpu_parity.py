@@ -168,6 +168,49 @@ def test_fp32_form_tracks_fp64(gpu, nc, kernel):
     assert abs(m_d[0:6].sum() - e_o) <= 1e-5 * abs(e_o)
 
 
+@pytest.mark.parametrize("dims,active,with_j", [((4, 3, 64), None, False), ((3, 2, 128), None, True),
+                                                ((5, 3, 64), ((1, 1, 0), (3, 2, 63)), True),
+                                                ((3, 2, 128), ((0, 0, 1), (2, 1, 100)), False)],
+                         ids=["row64", "row128-J", "box-full-x2-J", "box-part-x2"])
+def test_fp32_cell_form_on_64_cell_rows(gpu, dims, active, with_j):
+    """FP32 SU(3) cell form on x2 rows a multiple of 64 long, where each wave holds 64 cells of one
+    row (a row of 64 wraps onto its own wave): against the FP64 oracle within FP32 rounding and
+    against the FP32 per-link form, with an active region spanning x2 (whole waves skip) or not,
+    with and without a current."""
+    nc, steps, at = 3, 12, 0.25
+    print(f"fp32 cell rows of 64 dims={dims} active={active} J={with_j}", flush=True)
+    n = int(np.prod(dims))
+    U, E = make_state(nc, dims, 0.1, 0.05, seed=21)
+    J = None
+    if with_j:
+        from openpixi_amd.inputs import su3_coeffs_to_storage
+
+        J = su3_coeffs_to_storage(np.random.default_rng(4).normal(0, 0.05, size=(n, 3, 8)))
+    og = oracle_grid(nc, dims, U, E, at, J=J)
+    grids = []
+    for kernel in ("cell", "link"):
+        dg = device_grid(nc, dims, U, E, at, precision="fp32", J=J)
+        dg.setKernel(kernel)
+        if active is not None:
+            dg.setActiveRegion(*active)
+        FastTYMSolver().stepLinks(dg, at)
+        for _ in range(steps):
+            dg.storeFields()
+            FastTYMSolver().step(dg, at)
+        grids.append(dg)
+    if active is not None:
+        og.set_active_region(*active)
+    og.step_links(at)
+    for _ in range(steps):
+        og.store_fields()
+        og.step(at)
+    cell, link = grids
+    assert rel_err(cell.getU(), og.U) < 2e-5 and rel_err(cell.getUnext(), og.Unext) < 2e-5
+    assert rel_err(cell.getE(), og.E) < 2e-4
+    assert rel_err(cell.getUnext(), link.getUnext()) < 1e-5
+    assert rel_err(cell.getE(), link.getE()) < 1e-4
+
+
 @pytest.mark.parametrize("nc", [2, 3])
 def test_reunitarize_and_health(gpu, nc):
     dims = (4, 4, 4)
