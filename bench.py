@@ -566,6 +566,19 @@ def ctypes_void(p):
     return ctypes.c_void_p(int(p))
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on, capped by
+    OMP_NUM_THREADS when it is set (the host's share for one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
 def cpu_baseline(budget_s):
     """The FP64 oracle (C restatement of FastTYMSolver, OpenMP over cells) on a
     bounded SU(3) sample: a 32^3 lattice, as many steps as fit the budget."""
@@ -574,7 +587,7 @@ def cpu_baseline(budget_s):
 
     dims = (32, 32, 32)
     n = int(np.prod(dims))
-    cores = os.cpu_count() or 1
+    cores = cpu_threads()
     og = O.OracleGrid(NC, dims, (1.0, 1.0, 1.0), DT, 1.0, threads=cores)
     A = inputs.random_links_coefficients(n, NC, 0.1, 0)
     og.U[:] = O.links_from_coeffs(NC, A)

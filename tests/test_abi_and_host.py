@@ -91,3 +91,25 @@ def test_settings_defaults_and_errors():
         Settings.from_yaml("numberOfDimensions: 3\ngridCells: [4, 4]\n")
     s = Settings.from_yaml("numberOfDimensions: 3\ngridSteps: [1.0, 0.5, 2.0]\n")
     assert s.grid_steps == [1.0, 0.5, 2.0]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, "bench.py", "--gpus", "2"], capture_output=True, text=True, timeout=120,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_bench_cpu_baseline_sample(monkeypatch):
+    """The CPU leg of bench.py on a short budget: the oracle's SU(3) 32^3 step loop, with the
+    thread count capped by OMP_NUM_THREADS."""
+    import bench
+
+    monkeypatch.setenv("OMP_NUM_THREADS", "2")
+    assert bench.cpu_threads() <= 2
+    r = bench.cpu_baseline(0.3)
+    assert r["kind"] == "port" and r["unit"] == "link-updates/s" and r["cores"] <= 2
+    assert r["value"] > 0 and r["single_core"]["value"] > 0 and r["single_core"]["cores"] == 1
