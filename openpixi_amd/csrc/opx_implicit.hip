@@ -39,6 +39,19 @@ __device__ __forceinline__ typename Group<NC, T>::M ld_link(const T* __restrict_
     return m;
 }
 
+// the stored reals only (two rows of an FP32 SU(3) link), row 2 left zero
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, T>::M ld_link_r01(const T* __restrict__ U, uint32_t n, uint32_t site,
+                                                                int dir)
+{
+    using GR = Group<NC, T>;
+    typename GR::M m = GR::zero();
+    const T* p = U + (size_t)dir * GR::S * n + site;
+#pragma unroll
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[(size_t)k * n];
+    return m;
+}
+
 template <int NC, typename T>
 __device__ __forceinline__ void st_link(T* __restrict__ U, uint32_t n, uint32_t site, int dir,
                                         const typename Group<NC, T>::M& m)
@@ -347,12 +360,13 @@ __global__ __launch_bounds__(kIB) void k_implicit_iteration(Lattice L, Box activ
     site_coords(L, site, x);
     if (!in_region(active, x)) return;
     const uint32_t n = L.n;
-    auto link = [&](int t, uint32_t s, int dir) -> M {
-        return ld_link<NC, T>(t < 0 ? Uold : (t == 0 ? U : iU), n, s, dir);
-    };
+    auto field = [&](int t) { return t < 0 ? Uold : (t == 0 ? U : iU); };
+    auto link = [&](int t, uint32_t s, int dir) -> M { return ld_link<NC, T>(field(t), n, s, dir); };
     const T at = (T)L.at;
     A e = ld_alg<NC, T>(E, n, site, d);
     const uint32_t xpd = step_site(L, x, d, 1);
+    uint32_t xp_d[3];
+    site_coords(L, xpd, xp_d);
     for (int t = 1; t >= -1; t -= 2) {
         M S = GR::zero();
         for (int i = 0; i < 3; i++) {
@@ -360,12 +374,24 @@ __global__ __launch_bounds__(kIB) void k_implicit_iteration(Lattice L, Box activ
             // implicit in the transverse plane; in planes with the beam only the beam link
             const int td = (d != 0) && (i != 0) ? t : (d == 0 ? t : 0);
             const int ti = (d != 0) && (i != 0) ? t : (i == 0 ? t : 0);
-            uint32_t xp_d[3];
-            site_coords(L, xpd, xp_d);
             const uint32_t xpi = step_site(L, x, i, 1), xmi = step_site(L, x, i, -1), xpdmi = step_site(L, xp_d, i, -1);
-            M u1 = mul_adj(mul_adj(link(ti, xpd, i), link(td, xpi, d)), link(ti, site, i));
-            const M u2 = adj_mul(mul(link(td, xmi, d), link(ti, xpdmi, i)), link(ti, xmi, i));
-            acc(u1, u2);
+            M u1;
+            if constexpr (GR::S != GR::G) {
+                // two-row storage: each product of group elements is formed in its first two rows
+                // and completed by the row-2 identity; left factors are read in the stored rows
+                u1 = mul_adj_r01(mul_adj_r01(ld_link_r01<NC, T>(field(ti), n, xpd, i), link(td, xpi, d)),
+                                 link(ti, site, i));
+                su3_third_row(u1);
+                M y = mul_r01(ld_link_r01<NC, T>(field(td), n, xmi, d), link(ti, xpdmi, i));
+                su3_third_row(y);
+                M u2 = adj_mul_r01(y, link(ti, xmi, i));
+                su3_third_row(u2);
+                acc(u1, u2);
+            } else {
+                u1 = mul_adj(mul_adj(link(ti, xpd, i), link(td, xpi, d)), link(ti, site, i));
+                const M u2 = adj_mul(mul(link(td, xmi, d), link(ti, xpdmi, i)), link(ti, xmi, i));
+                acc(u1, u2);
+            }
             acc_scaled(S, mul(link(td, site, d), u1), (T)L.area[i]);
         }
         const A pr = proj(S);
