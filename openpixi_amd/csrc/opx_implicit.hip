@@ -369,6 +369,9 @@ __global__ __launch_bounds__(kIB) void k_implicit_iteration(Lattice L, Box activ
     site_coords(L, xpd, xp_d);
     for (int t = 1; t >= -1; t -= 2) {
         M S = GR::zero();
+        // both planes' loads issued together where the registers allow it (FP32)
+        constexpr int kPlaneUnroll = sizeof(T) == 4 ? 3 : 1;
+#pragma unroll kPlaneUnroll
         for (int i = 0; i < 3; i++) {
             if (i == d) continue;
             // implicit in the transverse plane; in planes with the beam only the beam link
