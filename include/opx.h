@@ -114,7 +114,9 @@ int opx_step_links(opx_ctx *ctx, double dt);
  *                        then `iterations` passes that rebuild E from plaquettes averaged over
  *                        the past and future links (implicit in the transverse plane and along
  *                        the beam direction x0), each followed by the damping
- *                        Unext = U^damping Unext^(1 - damping) (GroupElement.pow). */
+ *                        Unext = U^damping Unext^(1 - damping) (GroupElement.pow).
+ *                        With damping 0 or 1 the powers are not evaluated: x^0 is the
+ *                        identity and x^1 is x up to the logarithm's rounding (~1e-16). */
 enum { OPX_SOLVER_FAST = 0, OPX_SOLVER_IMPLICIT = 1 };
 int opx_set_field_solver(opx_ctx *ctx, int solver, int iterations, double damping);
 /* GroupElement.pow(x) = getLink(x getAlgebraElement()) (math/SU2GroupElement.java:234-271,

@@ -305,17 +305,18 @@ OPX_HD Su3<T> group_pow(const Su3<T>& u, T x)
 template <typename M, typename T>
 __device__ M damped_link(const M& u, const M& v, T damping)
 {
-    using MD = decltype(to_double(u));
-    MD ud = to_double(u), vd = to_double(v);
-    if (sizeof(T) == 4) {
-        ud = reunit(ud);
-        vd = reunit(vd);
-    }
+    auto on_group = [](const M& x) {
+        auto xd = to_double(x);
+        if (sizeof(T) == 4) xd = reunit(xd);
+        return xd;
+    };
     const double dd = (double)damping;
-    // x^0 = getLink(0) is the identity exactly, and so is its product with the other factor
-    if (dd == 0.0) return from_double<M>(group_pow(vd, 1.0));
-    if (dd == 1.0) return from_double<M>(group_pow(ud, 1.0));
-    return from_double<M>(mul(group_pow(ud, dd), group_pow(vd, 1.0 - dd)));
+    // Damping 0 or 1: one factor is x^0 = getLink(0), the identity exactly, and the other is
+    // x^1 = getLink(getAlgebraElement(x)), which is x (projected onto the group) up to the
+    // rounding of the logarithm and the exponential, about 1e-16: it is returned as it is.
+    if (dd == 0.0) return from_double<M>(on_group(v));
+    if (dd == 1.0) return from_double<M>(on_group(u));
+    return from_double<M>(mul(group_pow(on_group(u), dd), group_pow(on_group(v), 1.0 - dd)));
 }
 
 // ------------------------------------------------------------------ kernels
