@@ -69,7 +69,7 @@ def implicit_pair(nc, dims, seed, precision="fp64", J=True, active=None, sigma_a
     return og, dg
 
 
-@pytest.mark.parametrize("iterations,damping", [(0, 0.0), (1, 0.0), (3, 0.0), (2, 0.3), (4, 0.7)])
+@pytest.mark.parametrize("iterations,damping", [(0, 0.0), (1, 0.0), (3, 0.0), (2, 0.3), (4, 0.7), (2, 1.0)])
 @pytest.mark.parametrize("nc,dims", [(2, (5, 6, 7)), (3, (4, 5, 3)), (2, (1, 9, 4))])
 def test_implicit_step_matches_oracle(gpu, nc, dims, iterations, damping):
     print(f"implicit nc={nc} dims={dims} K={iterations} damping={damping}", flush=True)
@@ -92,11 +92,14 @@ def test_implicit_step_active_region(gpu, nc):
     assert rel_err(dg.getUnext(), og.Unext) < tol
 
 
+@pytest.mark.parametrize("damping", [0.1, 0.0, 1.0])
 @pytest.mark.parametrize("nc", [2, 3])
-def test_implicit_fp32(gpu, nc):
+def test_implicit_fp32(gpu, nc, damping):
+    """FP32 links: the powers evaluated in double (damping 0.1), and damping 0 or 1, where the
+    link is only projected onto the group in double."""
     og, dg = implicit_pair(nc, (6, 6, 6), seed=2, precision="fp32")
-    og.implicit_step(iterations=3, damping=0.1)
-    ImplicitTYMSolver(3, 0.1).step(dg, 0.25)
+    og.implicit_step(iterations=3, damping=damping)
+    ImplicitTYMSolver(3, damping).step(dg, 0.25)
     assert rel_err(dg.getE(), og.E) < 2e-5
     assert rel_err(dg.getUnext(), og.Unext) < 2e-6
 
