@@ -1389,6 +1389,7 @@ struct opx_batch {
     void* UB = nullptr;
     void* E = nullptr;
     uint64_t link_total = 0, alg_total = 0;
+    bool small_events = true;   // every event's link field below 4 GiB (buffer loads in the cell form)
     std::vector<opx_ctx*> views;
     std::vector<opx::BatchEvent> ev;
     opx::BatchEvent* dev_ev = nullptr;
@@ -1447,7 +1448,7 @@ int batch_step(opx_batch* b, int reunit_every)
     void* U = b->parity ? b->UB : b->UA;
     void* Un = b->parity ? b->UA : b->UB;
     if (b->form == 3)
-        OPX_BTRY(opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, U, Un, b->E,
+        OPX_BTRY(opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, b->small_events, U, Un, b->E,
                                         b->stream),
                  "batch cell update");
     else if (b->form >= 1)
@@ -1518,6 +1519,7 @@ int opx_batch_create(int num_colors, int count, const int* dims, const double* s
         b->ev.push_back(E);
         lo += (uint64_t)n * 3 * b->S;
         ao += (uint64_t)n * 3 * b->AD;
+        if ((uint64_t)n * 3 * b->S * b->tsz >= (1ull << 32)) b->small_events = false;
         const uint64_t it = 3 * (((uint64_t)n + 255) / 256);
         if ((uint64_t)items + it >= (1ull << 32)) rc = bfail(b, OPX_E_UNSUPPORTED, "batch: too many cells");
         items += (uint32_t)it;
@@ -1763,7 +1765,7 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
     // first launch from the batch's links, timed launches alternating between two scratch fields
     auto run = [&](const Cand& c, const void* src, void* dst) {
         if (c.form == 3)
-            return opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, src, dst, e2,
+            return opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, b->small_events, src, dst, e2,
                                           b->stream);
         return c.form >= 1 ? opx::launch_batch_tile(b->prec, c.form == 1, b->dev_ev, b->dev_items, b->nitems, c.grid,
                                                     src, dst, e2, b->stream)
