@@ -38,13 +38,27 @@ REUNIT_EVERY = 10
 # algorithmic HBM bytes per link update of the fused kernel (read U, read E, write E, write Unext)
 # in the stored formats: FP32 SU(3) links hold their first two rows (12 reals), FP64 links all 18
 BYTES_PER_LINK = {"fp32": 48 + 32 + 32 + 48, "fp64": 144 + 64 + 64 + 144}
-# HBM bytes per launch of the per-link kernel on the 128^3 lattice, counted with
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over `bench.py --probe` (KiB per dispatch,
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a 1 GiB device copy). Reported only
-# when the run uses that kernel on that lattice.
-MEASURED_TRAFFIC = {("fp32", "link", DIMS): (283889.0 * 2.0 + 491520.0) * 1024,
-                    ("fp32", "cell", DIMS): (274979.0 * 2.0 + 491520.0) * 1024,
-                    ("fp64", "link", DIMS): (875904.0 * 2.0 + 1277952.0) * 1024}
+# HBM bytes per launch of the fused kernel forms on the 128^3 lattice, from separate profiling
+# runs (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over `bench.py --probe`; KiB per dispatch).
+# FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the counters read the
+# known bytes of a 1 GiB device-to-device copy. These are recorded figures, not measured by this
+# run: the JSON line carries them as roofline.traffic with roofline.traffic_source naming the
+# profile summary. Keyed by (precision, kernel form, lattice): (FETCH_SIZE KiB, WRITE_SIZE KiB, file).
+PROFILED_TRAFFIC = {("fp32", "link", DIMS): (283889.0, 491520.0, "profiles/k_combined_update_su3_128cube.md"),
+                    ("fp32", "cell", DIMS): (274979.0, 491520.0, "profiles/k_cell_update_su3_128cube.md"),
+                    ("fp64", "link", DIMS): (875904.0, 1277952.0, "profiles/k_combined_update_su3_128cube.md")}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+
+def profiled_traffic(precision, kernel, dims):
+    """(bytes per launch, source description) of a recorded counter run, or (None, None)."""
+    rec = PROFILED_TRAFFIC.get((precision, kernel, tuple(dims)))
+    if rec is None:
+        return None, None
+    fetch, write, src = rec
+    return ((fetch * FETCH_SCALE + write * WRITE_SCALE) * 1024,
+            f"{src}: rocprofv3 --pmc FETCH_SIZE {fetch:.0f} KiB x {FETCH_SCALE} + WRITE_SIZE {write:.0f} KiB x "
+            f"{WRITE_SCALE} per dispatch (scales from a 1 GiB device copy); recorded, not measured by this run")
 
 
 def parse():
@@ -579,35 +593,38 @@ def cpu_threads():
     return max(1, n)
 
 
-def cpu_baseline(budget_s):
-    """The FP64 oracle (C restatement of FastTYMSolver, OpenMP over cells) on a
-    bounded SU(3) sample: a 32^3 lattice, as many steps as fit the budget."""
+def cpu_baseline(budget_s, dims=DIMS):
+    """The FP64 oracle (C restatement of FastTYMSolver, -O3 -ffp-contract=off, OpenMP over
+    cells) on the measured workload's lattice (SU(3) 128^3, the same seeded links and E before
+    the Gauss projection, which leaves the per-step work unchanged): as many steps as fit the
+    budget on every host thread this process may use, then the same on one thread."""
     import oracle as O
     from openpixi_amd import inputs
 
-    dims = (32, 32, 32)
     n = int(np.prod(dims))
     cores = cpu_threads()
     og = O.OracleGrid(NC, dims, (1.0, 1.0, 1.0), DT, 1.0, threads=cores)
-    A = inputs.random_links_coefficients(n, NC, 0.1, 0)
-    og.U[:] = O.links_from_coeffs(NC, A)
+    og.U[:] = O.links_from_coeffs(NC, inputs.random_links_coefficients(n, NC, 0.1, 0))
     og.E[:] = O.algs_from_coeffs(NC, inputs.random_efield_coefficients(n, NC, 0.05, 0))
     t1 = og.time_steps(1, 1)
     steps = max(1, int(budget_s / max(t1, 1e-6)))
     t = og.time_steps(steps, 0)
-    # the same on one core, a third of the budget
+    # one thread: a third of the budget, at least one step
     og.set_threads(1)
     s1 = og.time_steps(1, 0)
-    steps1 = max(1, int(budget_s / 3 / max(s1, 1e-6)))
-    t_one = og.time_steps(steps1, 0)
+    steps1 = max(0, int(budget_s / 3 / max(s1, 1e-6)) - 1)
+    t_one = s1 + (og.time_steps(steps1, 0) if steps1 > 0 else 0.0)
+    steps1 += 1
+    lat = "x".join(str(d) for d in dims)
     return {
         "value": 3 * n * steps / t,
         "unit": "link-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"FP64 oracle (C, -O2, OpenMP {cores} threads), SU(3) 32^3, {steps} steps, {t:.1f} s",
+        "sample": f"FP64 oracle (C, gcc -O3 -ffp-contract=off, OpenMP {cores} threads), SU(3) {lat}, "
+                  f"{steps} steps after 1 warm-up step, {t:.1f} s",
         "single_core": {"value": 3 * n * steps1 / t_one, "unit": "link-updates/s", "cores": 1,
-                        "sample": f"the same oracle on one thread, {steps1} steps, {t_one:.1f} s"},
+                        "sample": f"the same oracle and lattice on one thread, {steps1} steps, {t_one:.1f} s"},
     }
 
 
@@ -689,7 +706,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": MEASURED_TRAFFIC.get((main_form, r["kernel_config"]["kernel"], dims)),
+        "traffic": profiled_traffic(main_form, r["kernel_config"]["kernel"], dims)[0],
+        "traffic_source": profiled_traffic(main_form, r["kernel_config"]["kernel"], dims)[1],
         "kernel": r["kernel_config"],
         "kernel_ms": r["kernel_ms"],
         "bytes_per_link": BYTES_PER_LINK[main_form],

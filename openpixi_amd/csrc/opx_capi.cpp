@@ -380,7 +380,7 @@ void opx_destroy(opx_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->batch_view) {
         // borrowed field memory belongs to the batch; only the view's own buffers are freed
-        void* own[] = {ctx->partials, ctx->dout, ctx->counters};
+        void* own[] = {ctx->partials, ctx->dout, ctx->counters, ctx->diag};
         for (void* b : own)
             if (b) (void)hipFree(b);
         delete ctx;
@@ -389,7 +389,7 @@ void opx_destroy(opx_ctx* ctx)
     opx_host::adjoint_release(ctx);
     opx_host::cgc_release(ctx);
     void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->P, ctx->iU, ctx->iUn, ctx->iE,
-                    ctx->partials, ctx->dout, ctx->counters,
+                    ctx->partials, ctx->dout, ctx->counters, ctx->diag,
                     ctx->sp_desc, ctx->sp_qoff, ctx->sp_q, ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel,
                     ctx->p_Q0, ctx->p_Q1};
     for (void* b : bufs)
@@ -543,6 +543,8 @@ int opx_set_efield_from_coefficients(opx_ctx* ctx, const double* coeffs, int64_t
 int opx_set_active_region(opx_ctx* ctx, const int* lo, const int* hi)
 {
     if (!ctx) return OPX_E_ARG;
+    // the batch kernels update every cell of every event
+    if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "the events of a batch have no active region");
     int rc = check_box(ctx, lo, hi);
     if (rc != OPX_OK) return rc;
     for (int i = 0; i < 3; i++) {
@@ -694,6 +696,15 @@ int opx_measure(opx_ctx* ctx, double* out)
     return OPX_OK;
 }
 
+// the per-slice output buffer of the projected diagnostics, allocated on the first call and
+// kept: 4 values per slice along the longest axis
+static int diag_buffer(opx_ctx* ctx)
+{
+    if (ctx->diag) return OPX_OK;
+    const int nd = std::max(ctx->dims[0], std::max(ctx->dims[1], ctx->dims[2]));
+    return hip_check(ctx, hipMalloc(&ctx->diag, (size_t)4 * nd * sizeof(double)), "alloc projected diagnostics");
+}
+
 int opx_projected_energy_density(opx_ctx* ctx, int direction, double* eT_el, double* eT_mag, double* eL_el,
                                  double* eL_mag)
 {
@@ -701,18 +712,18 @@ int opx_projected_energy_density(opx_ctx* ctx, int direction, double* eT_el, dou
     if (direction < 0 || direction > 2) return fail(ctx, OPX_E_ARG, "projected energy: bad direction");
     if (!eT_el || !eT_mag || !eL_el || !eL_mag) return fail(ctx, OPX_E_ARG, "projected energy: null output");
     const int nd = ctx->dims[direction];
-    double* dbuf = nullptr;
-    OPX_TRY(hipMalloc(&dbuf, (size_t)4 * nd * sizeof(double)), "alloc projected energy");
+    int rc = diag_buffer(ctx);
+    if (rc != OPX_OK) return rc;
     opx::Lattice L = lattice(ctx, ctx->at);
-    int rc = hip_check(ctx, opx::launch_projected_energy(ctx->nc, ctx->prec, L, ctx->eval, direction, ctx->U,
-                                                         ctx->Unext, ctx->E, dbuf, ctx->stream),
-                       "projected energy");
+    rc = hip_check(ctx, opx::launch_projected_energy(ctx->nc, ctx->prec, L, ctx->eval, direction, ctx->U, ctx->Unext,
+                                                     ctx->E, ctx->diag, ctx->stream),
+                   "projected energy");
     std::vector<double> h((size_t)4 * nd);
     if (rc == OPX_OK)
-        rc = hip_check(ctx, hipMemcpyAsync(h.data(), dbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+        rc = hip_check(ctx, hipMemcpyAsync(h.data(), ctx->diag, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                           ctx->stream),
                        "projected energy copy");
     if (rc == OPX_OK) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "projected energy sync");
-    (void)hipFree(dbuf);
     if (rc != OPX_OK) return rc;
     std::memcpy(eT_el, h.data(), nd * sizeof(double));
     std::memcpy(eT_mag, h.data() + nd, nd * sizeof(double));
@@ -729,18 +740,18 @@ int opx_projected_poynting(opx_ctx* ctx, int direction, double* averaged, double
     if (ctx->nc != 2)
         return fail(ctx, OPX_E_UNSUPPORTED, "projected Poynting vector: SU(3) code has not been tested yet.");
     const int nd = ctx->dims[direction];
-    double* dbuf = nullptr;
-    OPX_TRY(hipMalloc(&dbuf, (size_t)2 * nd * sizeof(double)), "alloc projected Poynting vector");
+    int rc = diag_buffer(ctx);
+    if (rc != OPX_OK) return rc;
     opx::Lattice L = lattice(ctx, ctx->at);
-    int rc = hip_check(ctx, opx::launch_projected_poynting(ctx->nc, ctx->prec, L, direction, ctx->U, ctx->Unext,
-                                                           ctx->E, dbuf, ctx->stream),
-                       "projected Poynting vector");
+    rc = hip_check(ctx, opx::launch_projected_poynting(ctx->nc, ctx->prec, L, direction, ctx->U, ctx->Unext, ctx->E,
+                                                       ctx->diag, ctx->stream),
+                   "projected Poynting vector");
     std::vector<double> h((size_t)2 * nd);
     if (rc == OPX_OK)
-        rc = hip_check(ctx, hipMemcpyAsync(h.data(), dbuf, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+        rc = hip_check(ctx, hipMemcpyAsync(h.data(), ctx->diag, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                           ctx->stream),
                        "projected Poynting vector copy");
     if (rc == OPX_OK) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "projected Poynting vector sync");
-    (void)hipFree(dbuf);
     if (rc != OPX_OK) return rc;
     std::memcpy(averaged, h.data(), nd * sizeof(double));
     std::memcpy(time_averaged, h.data() + nd, nd * sizeof(double));
@@ -1219,6 +1230,15 @@ int opx_autotune(opx_ctx* ctx, int reps, double* best_ms)
     if (rc == OPX_OK) rc = hip_check(ctx, hipEventCreate(&ev1), "autotune event");
     if (rc == OPX_OK)
         rc = hip_check(ctx, hipMemcpyAsync(e2, ctx->E, alg_count(ctx) * ctx->tsz, hipMemcpyDeviceToDevice, ctx->stream),
+                       "autotune copy");
+    // with an active region a launch writes the active links only, and the staples at the region's
+    // edge read the rest: both scratch link fields start as the lattice's links
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemcpyAsync(u2, ctx->Unext, link_count(ctx) * ctx->tsz, hipMemcpyDeviceToDevice,
+                                           ctx->stream),
+                       "autotune copy");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemcpyAsync(u3, ctx->U, link_count(ctx) * ctx->tsz, hipMemcpyDeviceToDevice, ctx->stream),
                        "autotune copy");
     const int saved_kernel = ctx->kernel, saved_tile = ctx->tile, saved_chunks = ctx->nchunks;
     double best = 1e30;

@@ -49,6 +49,7 @@ struct opx_ctx {
     hipStream_t stream = nullptr;
     double* partials = nullptr;
     double* dout = nullptr;
+    double* diag = nullptr;         // per-slice sums of the projected diagnostics (4 x the longest axis)
     unsigned long long* counters = nullptr;
     // super particles
     int sp_count = 0;

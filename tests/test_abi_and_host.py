@@ -104,12 +104,15 @@ def test_bench_refuses_more_than_one_gpu():
 
 
 def test_bench_cpu_baseline_sample(monkeypatch):
-    """The CPU leg of bench.py on a short budget: the oracle's SU(3) 32^3 step loop, with the
-    thread count capped by OMP_NUM_THREADS."""
+    """The CPU leg of bench.py on a short budget (the bench runs it on SU(3) 128^3; a 16^3 lattice
+    here): the oracle's step loop, with the thread count capped by OMP_NUM_THREADS, and the sample
+    naming the lattice and the compiler flags."""
     import bench
 
     monkeypatch.setenv("OMP_NUM_THREADS", "2")
     assert bench.cpu_threads() <= 2
-    r = bench.cpu_baseline(0.3)
+    r = bench.cpu_baseline(0.3, dims=(16, 16, 16))
     assert r["kind"] == "port" and r["unit"] == "link-updates/s" and r["cores"] <= 2
     assert r["value"] > 0 and r["single_core"]["value"] > 0 and r["single_core"]["cores"] == 1
+    assert "16x16x16" in r["sample"] and "-O3 -ffp-contract=off" in r["sample"]
+    assert bench.DIMS == (128, 128, 128)

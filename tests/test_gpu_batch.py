@@ -107,6 +107,11 @@ def test_stepping_an_event_view_is_rejected(gpu):
     b, _ = filled_batch(2, "fp64", [(4, 4, 4), (5, 4, 3)])
     with pytest.raises(OpxError):
         FastTYMSolver().step(b.event(0), 0.25)
+    # the batch kernels update every cell of every event: an event has no active region
+    with pytest.raises(OpxError, match="active region"):
+        b.event(1).setActiveRegion((1, 1, 0), (3, 2, 2))
+    # the evaluation region only selects measured cells and stays allowed
+    b.event(1).setEvaluationRegion((1, 1, 0), (3, 2, 2))
 
 
 @pytest.mark.parametrize("nc,form", [(2, "tile"), (2, "tile-cached"), (2, "cell"), (3, "cell")])

@@ -26,7 +26,7 @@ def setup_pair(nc, dims, ppc, seed, precision="fp64", sigma_a=0.3, subdivisions=
     return og, dg
 
 
-@pytest.mark.parametrize("kernel", ["link", "plaq", "stream", "tile"])
+@pytest.mark.parametrize("kernel", ["link", "plaq", "stream", "tile", "cell"])
 @pytest.mark.parametrize("nc,ppc", [(2, 4), (3, 4), (2, 8), (2, 1), (3, 2)])
 def test_cpic_simulation_parity(gpu, nc, ppc, kernel):
     if kernel == "tile" and nc == 3:
@@ -57,11 +57,13 @@ def test_cpic_simulation_parity(gpu, nc, ppc, kernel):
         assert abs(m_d[k] - m_o[k]) <= 1e-11 * max(abs(m_o[k]), 1e-30), (k, m_d[k], m_o[k])
 
 
+@pytest.mark.parametrize("kernel", ["link", "cell"])
 @pytest.mark.parametrize("nc", [2, 3])
-def test_cpic_fp32_conserves_gauss(gpu, nc):
+def test_cpic_fp32_conserves_gauss(gpu, nc, kernel):
     """FP32 CPIC with reunitarization: G(x) stays at its initial value to FP32 rounding."""
     dims = (16, 6, 5)
     og, dg = setup_pair(nc, dims, 4, seed=5, precision="fp32")
+    dg.setKernel(kernel)
     dg.simulationInitialize()
     m0 = FieldMeasurements().measure(dg)
     for _ in range(10):
@@ -87,7 +89,7 @@ def setup_particle_pair(nc, dims, ppc, seed, precision="fp64"):
     return og, dg
 
 
-@pytest.mark.parametrize("kernel", ["link", "plaq", "stream", "tile"])
+@pytest.mark.parametrize("kernel", ["link", "plaq", "stream", "tile", "cell"])
 @pytest.mark.parametrize("nc,ppc", [(2, 4), (3, 4), (2, 1), (3, 2)])
 def test_per_particle_simulation_parity(gpu, nc, ppc, kernel):
     if kernel == "tile" and nc == 3:

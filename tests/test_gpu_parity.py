@@ -18,8 +18,8 @@ FP64_TOL = 1e-12
 
 # kernel forms of the fused update: (kernel, tile, chunks)
 KERNELS = [("link", None, None), ("plaq", None, None), ("cell", None, None), ("stream", 0, None), ("stream", 1, 3),
-           ("stream", 4, 1)]
-KERNEL_IDS = ["link", "plaq", "cell", "stream16x16", "stream8x32c3", "stream8x16c1"]
+           ("stream", 4, 1), ("cell", 1, None), ("cell", 2, None)]
+KERNEL_IDS = ["link", "plaq", "cell", "stream16x16", "stream8x32c3", "stream8x16c1", "cell16rows", "cell8rows"]
 
 
 def run_both(nc, dims, steps, at=0.25, spacing=(1.0, 1.0, 1.0), sigma_a=0.1, sigma_e=0.05, seed=0, J=None,
@@ -209,6 +209,144 @@ def test_fp32_cell_form_on_64_cell_rows(gpu, dims, active, with_j):
     assert rel_err(cell.getE(), og.E) < 2e-4
     assert rel_err(cell.getUnext(), link.getUnext()) < 1e-5
     assert rel_err(cell.getE(), link.getE()) < 1e-4
+
+
+# Row tiles of the cell form (32, 16 and 8 x1 rows per tile, swept along x0): lattices whose x1
+# extent spans several tiles with a partial last one (80 = 2 x 32 + 16, 40 = 32 + 8 = 2 x 16 + 8,
+# 45 = 32 + 13 = 2 x 16 + 13 = 5 x 8 + 5), and x2 rows of 70 (waves straddle rows), 128 (two waves
+# per row) and 33 cells.
+CELL_TILE_LATTICES = [((5, 80, 70), 6), ((4, 40, 128), 6), ((3, 45, 33), 8)]
+CELL_TILE_IDS = ["5x80x70", "4x40x128", "3x45x33"]
+
+
+def su3_current(n, seed, sigma=0.05):
+    from openpixi_amd.inputs import su3_coeffs_to_storage
+
+    return su3_coeffs_to_storage(np.random.default_rng(seed).normal(0, sigma, size=(n, 3, 8)))
+
+
+def cell_tile_region(dims):
+    return (1, 3, 2), (dims[0] - 2, dims[1] - 4, dims[2] - 3)
+
+
+@pytest.mark.parametrize("with_region", [False, True], ids=["all", "J-region"])
+@pytest.mark.parametrize("tile", [0, 1, 2], ids=["rows32", "rows16", "rows8"])
+@pytest.mark.parametrize("dims,steps", CELL_TILE_LATTICES, ids=CELL_TILE_IDS)
+def test_cell_form_row_tiles_fp64(gpu, dims, steps, tile, with_region):
+    """SU(3) FP64 cell form with every row tile on lattices of several tiles and a partial last
+    tile, bare and with a current plus an active region: within 1e-12 of the oracle."""
+    print(f"cell row tiles fp64 dims={dims} tile={tile} region={with_region}", flush=True)
+    n = int(np.prod(dims))
+    J = su3_current(n, 17) if with_region else None
+    active = cell_tile_region(dims) if with_region else None
+    og, dg = run_both(3, dims, steps, J=J, active=active, seed=19, kernel=("cell", tile, None))
+    assert dg.kernelConfig()["kernel"] == "cell" and dg.kernelConfig()["tile"] == tile
+    assert rel_err(dg.getU(), og.U) < FP64_TOL
+    assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
+    assert rel_err(dg.getE(), og.E) < FP64_TOL
+
+
+@pytest.mark.parametrize("with_region", [False, True], ids=["all", "J-region"])
+@pytest.mark.parametrize("tile", [0, 1, 2], ids=["rows32", "rows16", "rows8"])
+@pytest.mark.parametrize("dims,steps", CELL_TILE_LATTICES, ids=CELL_TILE_IDS)
+def test_cell_form_row_tiles_fp32(gpu, dims, steps, tile, with_region):
+    """SU(3) FP32 cell form (12-real links, two-row products) with every row tile, through
+    opx_leapfrog with FP64 reunitarization every 5 steps: against the FP64 oracle within FP32
+    rounding (links 2e-5, E 2e-4 of the largest entry) and against the FP32 per-link form
+    (1e-5, 1e-4)."""
+    from openpixi_amd import _lib
+
+    print(f"cell row tiles fp32 dims={dims} tile={tile} region={with_region}", flush=True)
+    nc, at = 3, 0.25
+    n = int(np.prod(dims))
+    U, E = make_state(nc, dims, 0.1, 0.05, seed=23)
+    J = su3_current(n, 29) if with_region else None
+    active = cell_tile_region(dims) if with_region else None
+    og = oracle_grid(nc, dims, U, E, at, J=J)
+    if active is not None:
+        og.set_active_region(*active)
+    og.step_links(at)
+    for _ in range(steps):
+        og.store_fields()
+        og.step(at)
+    L = _lib.load()
+    grids = []
+    for kernel in (("cell", tile, None), ("link", None, None)):
+        dg = device_grid(nc, dims, U, E, at, precision="fp32", J=J)
+        dg.setKernel(*kernel)
+        if active is not None:
+            dg.setActiveRegion(*active)
+        FastTYMSolver().stepLinks(dg, at)
+        dg._check(L.opx_leapfrog(dg.handle, steps, 5))
+        grids.append(dg)
+    cell, link = grids
+    assert rel_err(cell.getU(), og.U) < 2e-5 and rel_err(cell.getUnext(), og.Unext) < 2e-5
+    assert rel_err(cell.getE(), og.E) < 2e-4
+    assert rel_err(cell.getUnext(), link.getUnext()) < 1e-5
+    assert rel_err(cell.getE(), link.getE()) < 1e-4
+
+
+def test_su3_128cube_fp32_cell_against_fp64_link(gpu):
+    """The measured workload at its own size: SU(3) 128^3 built on the device from seeded
+    algebra coefficients (A ~ N(0, 0.1), E ~ N(0, 0.05) projected onto Gauss's law), the
+    initial half step, then 20 leapfrog steps. The FP32 cell form (32-row tiles, four full
+    tiles along x1, FP64 reunitarization every 10 steps) and the FP64 per-link form give the
+    same E^2 and B^2 per direction within 1e-7 relative, and both keep the Gauss residual below
+    1e-10."""
+    from openpixi_amd import Grid, _lib, inputs
+
+    dims, nc, at, steps = (128, 128, 128), 3, 0.25, 20
+    n = int(np.prod(dims))
+    A = inputs.random_links_coefficients(n, nc, 0.1, 0)
+    Ec = inputs.random_efield_coefficients(n, nc, 0.05, 0)
+    L = _lib.load()
+    meas = {}
+    for precision, kernel in (("fp32", ("cell", 0, None)), ("fp64", ("link", None, None))):
+        print(f"128^3 {precision} {kernel}", flush=True)
+        g = Grid(nc, dims, 1.0, at, 1.0, precision)
+        g.setLinksFromCoefficients(A)
+        g.setEFromCoefficients(Ec)
+        it, g0, g1 = g.projectGauss(tol=1e-10, max_iter=4000)
+        assert g0 > 1e-2 and g1 < 1e-15, (it, g0, g1)
+        g.setKernel(*kernel)
+        FastTYMSolver().stepLinks(g, at)
+        g._check(L.opx_leapfrog(g.handle, steps, 10 if precision == "fp32" else 0))
+        m = FieldMeasurements().measure(g)
+        assert m[6] < 1e-10, (precision, m[6])
+        meas[precision] = m
+        g.close()
+    m32, m64 = meas["fp32"], meas["fp64"]
+    for k in range(6):
+        assert abs(m32[k] - m64[k]) <= 1e-7 * abs(m64[k]), (k, m32[k], m64[k])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_fields_from_coefficients(gpu, nc, precision):
+    """opx_set_links_from_coefficients / opx_set_efield_from_coefficients (the device builds
+    U = exp(i A_a t_a) and stores E coefficients) against the oracle's getLink and algebra storage:
+    1e-13 in FP64 (float rounding in FP32), including coefficients with tr A^2 > 1 (the SU(3)
+    exponential's scaled branch), for U and Unext."""
+    dims = (5, 4, 6)
+    n = int(np.prod(dims))
+    ad = nc * nc - 1
+    rng = np.random.default_rng(31 + nc)
+    A = rng.normal(0, 0.1, size=(n, 3, ad))
+    A[: n // 2] *= 30.0            # half the links far from the identity (tr A^2 up to ~100)
+    Ec = rng.normal(0, 0.5, size=(n, 3, ad))
+    from openpixi_amd import Grid
+    from openpixi_amd._lib import FIELD_UNEXT
+
+    big = np.einsum("ndk,ndk->nd", A, A) / 2
+    assert big.max() > 1.0 and big.min() < 1.0
+    g = Grid(nc, dims, 1.0, 0.25, 1.0, precision)
+    g.setLinksFromCoefficients(A)
+    g.setLinksFromCoefficients(-A, field=FIELD_UNEXT)
+    g.setEFromCoefficients(Ec)
+    tol_u, tol_e = (1e-13, 1e-15) if precision == "fp64" else (2e-6, 1e-7)
+    assert rel_err(g.getU(), O.links_from_coeffs(nc, A)) < tol_u
+    assert rel_err(g.getUnext(), O.links_from_coeffs(nc, -A)) < tol_u
+    assert rel_err(g.getE(), O.algs_from_coeffs(nc, Ec)) < tol_e
 
 
 @pytest.mark.parametrize("nc", [2, 3])
