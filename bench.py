@@ -66,7 +66,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=500)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--kernel-reps", type=int, default=50, help="launches of the fused kernel for its own timing")
+    p.add_argument("--kernel-reps", type=int, default=50, help="back-to-back launches of the fused kernel after the timed region (kernel_ms_back_to_back)")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--dims", type=int, nargs=3, default=list(DIMS))
@@ -156,17 +156,30 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
     m0 = FieldMeasurements().measure(g)
     g._check(L.opx_leapfrog(g.handle, warmup, reunit))
     torch.cuda.synchronize()
+    # The timed region: `steps` calls of opx_leapfrog(1 step), HIP events on the kernels' stream
+    # around the whole region and around each step. A step is one launch of the fused kernel (the
+    # cell form reprojects the links inside the kernel on every 10th step); where a form runs the
+    # reprojection as a pass of its own, those steps are left out of the kernel's average.
+    fused = kcfg["kernel"] == "cell"
+    step_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
-    g._check(L.opx_leapfrog(g.handle, steps, reunit))
+    for i in range(steps):
+        step_ev[i][0].record(stream)
+        g._check(L.opx_leapfrog(g.handle, 1, reunit))
+        step_ev[i][1].record(stream)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = ev0.elapsed_time(ev1)
+    per_step = [a.elapsed_time(b) for a, b in step_ev]
+    counted = [t for i, t in enumerate(per_step)
+               if fused or reunit == 0 or (warmup + i + 1) % reunit != 0]
+    kms = sum(counted) / len(counted)
     m1 = FieldMeasurements().measure(g)
-    # the fused update kernel alone, back to back on the same stream
+    # the fused update kernel alone, back to back on the same stream, after the timed region
     k0 = torch.cuda.Event(enable_timing=True)
     k1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -176,7 +189,7 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
         g._check(L.opx_step(g.handle, DT))
     k1.record(stream)
     torch.cuda.synchronize()
-    kms = k0.elapsed_time(k1) / kernel_reps
+    kms_b2b = k0.elapsed_time(k1) / kernel_reps
     n = int(np.prod(dims))
     links = 3 * n
     energy0 = float(m0[0:6].sum())
@@ -186,7 +199,10 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
         "wall_ms_per_step": 1e3 * wall / steps,
         "link_updates_per_s": links * steps / (ms * 1e-3),
         "kernel_ms": kms,
+        "kernel_launches_timed": len(counted),
+        "kernel_ms_min_max": [min(counted), max(counted)],
         "kernel_gbs": links * BYTES_PER_LINK[precision] / (kms * 1e-3) / 1e9,
+        "kernel_ms_back_to_back": kms_b2b,
         "gauss_projection": {"cg_iterations": cg_iters, "before": gauss_raw, "after": gauss_proj},
         "gauss_residual_start": float(m0[6]),
         "gauss_residual_end": float(m1[6]),
@@ -723,7 +739,7 @@ def main():
     if cgc is not None:
         out["cgc_init"] = cgc
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, dims)
     out["host"] = platform.node()
     print(json.dumps(out), flush=True)
 

@@ -182,8 +182,10 @@ int opx_gauss_project(opx_ctx *ctx, double tol, int max_iter, int *iterations, d
                       double *gauss_after);
 
 /* `steps` leapfrog steps of pure Yang-Mills: for each, storeFields then step
- * (physics/Simulation.java:389-395 with no particles), reunitarizing U and
- * Unext after every `reunit_every`-th step (0 = never). */
+ * (physics/Simulation.java:389-395 with no particles). Every `reunit_every`-th
+ * step (counted over the context's life; 0 = never) reprojects the new links
+ * (Unext) onto the group in double precision, as opx_reunitarize does: inside
+ * the update kernel in the cell form, as a separate pass in the others. */
 int opx_leapfrog(opx_ctx *ctx, int steps, int reunit_every);
 
 /* ---- observables seam (FieldMeasurements, double precision) -------------

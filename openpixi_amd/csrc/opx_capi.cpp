@@ -208,9 +208,13 @@ uint32_t auto_tile_chunks(const opx_ctx* c, const opx::Lattice& L)
     return k;
 }
 
-// The fused E-field/link update with the configured kernel form.
-hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E)
+// The fused E-field/link update with the configured kernel form. With reunit, a form that can
+// reproject the new links onto the group as it stores them (the cell form) does so and sets
+// *fused; otherwise the caller reprojects them in a pass of its own.
+hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E,
+                         bool reunit = false, bool* fused = nullptr)
 {
+    if (fused) *fused = false;
     if (c->kernel == OPX_KERNEL_STREAM && opx::stream_supported(c->nc, c->prec, L, c->tile)) {
         const uint32_t k = c->nchunks > 0 ? (uint32_t)c->nchunks : auto_chunks(c, L, c->tile);
         return opx::launch_stream_update(c->nc, c->prec, L, c->active, c->tile, k, U, Unext, E, c->J, c->j_nonzero,
@@ -234,9 +238,11 @@ hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void*
         }
         return opx::launch_plaq_update(c->nc, c->prec, L, c->active, U, c->P, Unext, E, c->J, c->j_nonzero, c->stream);
     }
-    if (c->kernel == OPX_KERNEL_CELL)
+    if (c->kernel == OPX_KERNEL_CELL) {
+        if (fused) *fused = reunit;
         return opx::launch_cell_update(c->nc, c->prec, L, c->active, c->tile, U, Unext, E, c->J, c->j_nonzero,
-                                       c->stream);
+                                       reunit && fused, c->stream);
+    }
     return opx::launch_combined_update(c->nc, c->prec, L, c->active, U, Unext, E, c->J, c->j_nonzero, c->stream);
 }
 
@@ -267,11 +273,12 @@ hipError_t implicit_step(opx_ctx* c, const opx::Lattice& L)
     return opx::launch_implicit_end(c->nc, c->prec, L, c->active, c->iE, c->iUn, c->E, c->Unext, c->stream);
 }
 
-// FieldSolver.step with the context's field solver
-hipError_t field_step(opx_ctx* c, const opx::Lattice& L)
+// FieldSolver.step with the context's field solver; reunit and fused as in launch_update
+hipError_t field_step(opx_ctx* c, const opx::Lattice& L, bool reunit = false, bool* fused = nullptr)
 {
+    if (fused) *fused = false;
     if (c->solver == OPX_SOLVER_IMPLICIT) return implicit_step(c, L);
-    return launch_update(c, L, c->U, c->Unext, c->E);
+    return launch_update(c, L, c->U, c->Unext, c->E, reunit, fused);
 }
 
 }  // namespace
@@ -298,7 +305,7 @@ void coeffs_to_storage(int nc, const double* c, double* v)
 }
 hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void* Unext, void* E)
 {
-    return ::launch_update(c, L, U, Unext, E);
+    return ::launch_update(c, L, U, Unext, E, false, nullptr);
 }
 }  // namespace opx_host
 
@@ -671,9 +678,12 @@ int opx_leapfrog(opx_ctx* ctx, int steps, int reunit_every)
     opx::Lattice L = lattice(ctx, ctx->at);
     for (int s = 0; s < steps; s++) {
         std::swap(ctx->U, ctx->Unext);
-        OPX_TRY(field_step(ctx, L), "leapfrog step");
+        // every reunit_every-th step reprojects the new links, inside the update where the form can
+        const bool reunit = reunit_every > 0 && (ctx->steps + 1) % reunit_every == 0;
+        bool fused = false;
+        OPX_TRY(field_step(ctx, L, reunit, &fused), "leapfrog step");
         ctx->steps++;
-        if (reunit_every > 0 && ctx->steps % reunit_every == 0)
+        if (reunit && !fused)
             OPX_TRY(opx::launch_reunitarize(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->Unext, ctx->stream),
                     "leapfrog reunitarize");
     }

@@ -392,15 +392,23 @@ OPX_HD Su3<T> herm_square(const Su3<T>& q)
 // The series stops once a term is below the working precision (at most
 // ExpTerms<T>::n terms). When tr(Q^2) > 1 the argument is halved s times and the
 // result squared s times, so the series always runs with spectral radius <= 1.
+// In float, for tr(Q^2) <= kExpPolyTrQ2 the terms n <= 8 are summed in closed form:
+//   f0 = 1 - k0^2/720 + k0^2 k1/20160 + i (-k0/6 + k0 k1/120 - k0 k1^2/5040)
+//   f1 = k0/24 - k0 k1/360 + k0 k1^2/13440 + i (1 - k1/6 + k1^2/120 - (k0^2 + k1^3)/5040)
+//   f2 = -1/2 + k1/24 - k1^2/720 + (k0^2 + k1^3)/40320 + i (k0/120 - k0 k1/2520)
+// (the recursion above unrolled); the spectral radius is then at most sqrt(2/3 tr Q^2) = 0.41,
+// so the first omitted term is below 1e-9, under float rounding.
+constexpr double kExpPolyTrQ2 = 0.25;
 template <typename T>
 OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
 {
     Su3<T> q = herm(x, f);
     Su3<T> q2 = herm_square(q);
     T trq2 = q2.r[0] + q2.r[4] + q2.r[8];
+    const bool poly = sizeof(T) == 4 && trq2 <= T(kExpPolyTrQ2);
     int s = 0;
     T sc = T(1);
-    while (trq2 * sc * sc > T(1) && s < 30) {
+    while (!poly && trq2 * sc * sc > T(1) && s < 30) {
         sc *= T(0.5);
         s++;
     }
@@ -426,10 +434,21 @@ OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
         }
     }
     const T k0 = trq3 / 3;
+    T f0r = 1, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
+    if (poly) {
+        const T k00 = k0 * k0, k11 = k1 * k1;
+        const T k0k1 = k0 * k1;
+        const T h = k00 + k11 * k1;
+        f0r = T(1) - k00 * (T(1.0 / 720) - k1 * T(1.0 / 20160));
+        f0i = k0 * (T(-1.0 / 6) + k1 * T(1.0 / 120) - k11 * T(1.0 / 5040));
+        f1r = k0 * (T(1.0 / 24) - k1 * T(1.0 / 360) + k11 * T(1.0 / 13440));
+        f1i = T(1) - k1 * T(1.0 / 6) + k11 * T(1.0 / 120) - h * T(1.0 / 5040);
+        f2r = T(-0.5) + k1 * T(1.0 / 24) - k11 * T(1.0 / 720) + h * T(1.0 / 40320);
+        f2i = k0 * T(1.0 / 120) - k0k1 * T(1.0 / 2520);
+    } else {
     // bound on the size of the terms: |a| + |b| rho + |c| rho^2 with rho^2 <= tr Q^2
     const T rho = fsqrt(trq2);
     T ar = 1, ai = 0, br = 0, bi = 0, cr = 0, ci = 0;
-    T f0r = 1, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
 #pragma unroll
     for (int n = 0; n < ExpTerms<T>::n; n++) {
         const T inv = T(1) / T(n + 1);
@@ -451,6 +470,7 @@ OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
         f2r += cr; f2i += ci;
         const T mag = (fabs(ar) + fabs(ai)) + (fabs(br) + fabs(bi)) * rho + (fabs(cr) + fabs(ci)) * trq2;
         if (mag < T(ExpTerms<T>::eps)) break;
+    }
     }
     // e = f0 + f1 Q + f2 Q^2 (Q and Q^2 Hermitian: real diagonals)
     Su3<T> e;
@@ -476,16 +496,25 @@ OPX_HD Su3<T> exp_link(const Alg3<T>& x, T f)
 
 // Third row of an SU(3) matrix from its first two: u_2j = conj(u_0a u_1b - u_0b u_1a),
 // (j, a, b) cyclic. Exact for det = 1 unitary matrices.
+// Each real and imaginary part is one multiply and three fused multiply-adds.
 template <typename T>
 OPX_HD void su3_third_row(Su3<T>& m)
 {
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const int a = (j + 1) % 3, b = (j + 2) % 3;
-        const T cr = (m.r[a] * m.r[3 + b] - m.i[a] * m.i[3 + b]) - (m.r[b] * m.r[3 + a] - m.i[b] * m.i[3 + a]);
-        const T ci = (m.r[a] * m.i[3 + b] + m.i[a] * m.r[3 + b]) - (m.r[b] * m.i[3 + a] + m.i[b] * m.r[3 + a]);
+        // Re(u_0a u_1b - u_0b u_1a)
+        T cr = m.r[a] * m.r[3 + b];
+        cr -= m.i[a] * m.i[3 + b];
+        cr -= m.r[b] * m.r[3 + a];
+        cr += m.i[b] * m.i[3 + a];
+        // -Im(u_0a u_1b - u_0b u_1a)
+        T ci = m.r[b] * m.i[3 + a];
+        ci += m.i[b] * m.r[3 + a];
+        ci -= m.r[a] * m.i[3 + b];
+        ci -= m.i[a] * m.r[3 + b];
         m.r[6 + j] = cr;
-        m.i[6 + j] = -ci;
+        m.i[6 + j] = ci;
     }
 }
 

@@ -110,11 +110,13 @@ hipError_t launch_batch_cell(int nc, int prec, const BatchEvent* ev, int nev, ui
 hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U,
                                   void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
 // cell form: the three link updates of a cell in one thread, the upper staples of a plane
-// sharing one product; blocks sweep row tiles of kCellRows[tile] x1 rows along x0
+// sharing one product; blocks sweep row tiles of kCellRows[tile] x1 rows along x0.
+// reunit: the new links are reprojected onto the group in double (as launch_reunitarize) before
+// they are stored
 constexpr int kCellTiles = 3;
 constexpr uint32_t kCellRows[kCellTiles] = {32, 16, 8};
 hipError_t launch_cell_update(int nc, int prec, const Lattice& L, const Box& active, int tile, const void* U,
-                              void* Unext, void* E, const void* J, bool has_j, hipStream_t st);
+                              void* Unext, void* E, const void* J, bool has_j, bool reunit, hipStream_t st);
 // SU(2) LDS-tile form of the fused update (opx_tile.hip): blocks of TY x TX cells
 // march along x0 over nchunks chunks of planes.
 constexpr int kTileTY32 = 16, kTileTY64 = 8, kTileTX = 16;

@@ -315,7 +315,24 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
 #ifndef OPX_CELL_WAVES
 #define OPX_CELL_WAVES 2
 #endif
-template <int NC, typename T, bool HAS_J, typename Fld>
+// The new link reprojected onto the group in double precision (reunit), as k_reunitarize does
+// to a stored link: only the rows the storage keeps enter.
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, T>::M reunit_link(const typename Group<NC, T>::M& u)
+{
+    using GR = Group<NC, T>;
+    using GD = Group<NC, double>;
+    typename GD::M ud;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) GD::gref(ud, k) = (double)GR::gget(u, k);
+    ud = reunit(ud);
+    typename GR::M o;
+#pragma unroll
+    for (int k = 0; k < GR::G; k++) GR::gref(o, k) = (T)GD::gget(ud, k);
+    return o;
+}
+
+template <int NC, typename T, bool HAS_J, bool REUNIT, typename Fld>
 __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, const Fld& Unext, const Fld& E,
                                             const Fld& J, uint32_t site)
 {
@@ -393,10 +410,13 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
             for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(jj, k) * uf;
         }
         fstore_alg_once<NC, T>(E, site, d, e);
+        M nu;
         if constexpr (TWO_ROWS)
-            fstore_link_once<NC, T>(Unext, site, d, mul_r01(exp_link(e, -at), Ud));
+            nu = mul_r01(exp_link(e, -at), Ud);
         else
-            fstore_link_once<NC, T>(Unext, site, d, mul(exp_link(e, -at), Ud));
+            nu = mul(exp_link(e, -at), Ud);
+        if constexpr (REUNIT) nu = reunit_link<NC, T>(nu);
+        fstore_link_once<NC, T>(Unext, site, d, nu);
     };
     M S0 = GR::zero(), S1 = GR::zero(), S2 = GR::zero();
     {
@@ -452,8 +472,9 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nb)
     return xcd * q8 + (xcd < r8 ? xcd : r8) + (b >> 3);
 }
 
-// one thread per cell, blocks in the per-link kernel's tile order without the direction
-template <int NC, typename T, bool HAS_J, bool BUF>
+// one thread per cell, blocks in the per-link kernel's tile order without the direction;
+// REUNIT: the new links are reprojected onto the group in double before they are stored
+template <int NC, typename T, bool HAS_J, bool BUF, bool REUNIT>
 __global__ __launch_bounds__(kBlock, OPX_CELL_WAVES) void k_cell_update(Lattice L, LinkOrder order, Box active,
                                                                         const T* __restrict__ Up, T* __restrict__ Unextp,
                                                                         T* __restrict__ Ep, const T* __restrict__ Jp)
@@ -465,7 +486,7 @@ __global__ __launch_bounds__(kBlock, OPX_CELL_WAVES) void k_cell_update(Lattice 
     if (!tile_site(L, order, xcd_logical(blockIdx.x, gridDim.x), site)) return;
     const Coord c = coords(L, site);
     if (!in_box(active, c.x0, c.x1, c.x2)) return;
-    cell_update<NC, T, HAS_J>(L, U, Unext, E, J, site);
+    cell_update<NC, T, HAS_J, REUNIT>(L, U, Unext, E, J, site);
 }
 
 #ifndef OPX_PLAQ_WAVES
@@ -608,7 +629,7 @@ __global__ __launch_bounds__(kBlock, OPX_CELL_WAVES) void k_batch_cell(const Bat
         using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
         const Fld U(Ubase + E.link_offset, E.L.n, 3 * GR::S), Un(Unbase + E.link_offset, E.L.n, 3 * GR::S);
         const Fld Ef(Ebase + E.alg_offset, E.L.n, 3 * GR::AD);
-        cell_update<NC, T, false>(E.L, U, Un, Ef, Ef, site);
+        cell_update<NC, T, false, false>(E.L, U, Un, Ef, Ef, site);
     }
 }
 
@@ -1258,8 +1279,26 @@ hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box&
     return hipGetLastError();
 }
 
+template <int NC, typename T, bool REUNIT>
+static void launch_cell_variant(const Lattice& L, const Box& active, const LinkOrder& order, dim3 g1, bool buf,
+                                const void* U, void* Unext, void* E, const void* J, bool has_j, hipStream_t st)
+{
+    if (buf && has_j)
+        hipLaunchKernelGGL((k_cell_update<NC, T, true, true, REUNIT>), g1, dim3(kBlock), 0, st, L, order, active,
+                           (const T*)U, (T*)Unext, (T*)E, (const T*)J);
+    else if (buf)
+        hipLaunchKernelGGL((k_cell_update<NC, T, false, true, REUNIT>), g1, dim3(kBlock), 0, st, L, order, active,
+                           (const T*)U, (T*)Unext, (T*)E, (const T*)E);
+    else if (has_j)
+        hipLaunchKernelGGL((k_cell_update<NC, T, true, false, REUNIT>), g1, dim3(kBlock), 0, st, L, order, active,
+                           (const T*)U, (T*)Unext, (T*)E, (const T*)J);
+    else
+        hipLaunchKernelGGL((k_cell_update<NC, T, false, false, REUNIT>), g1, dim3(kBlock), 0, st, L, order, active,
+                           (const T*)U, (T*)Unext, (T*)E, (const T*)E);
+}
+
 hipError_t launch_cell_update(int nc, int prec, const Lattice& L, const Box& active, int tile, const void* U,
-                              void* Unext, void* E, const void* J, bool has_j, hipStream_t st)
+                              void* Unext, void* E, const void* J, bool has_j, bool reunit, hipStream_t st)
 {
     const uint32_t rows = kCellRows[tile >= 0 && tile < kCellTiles ? tile : 0];
     OPX_DISPATCH(nc, prec, {
@@ -1269,18 +1308,10 @@ hipError_t launch_cell_update(int nc, int prec, const Lattice& L, const Box& act
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const dim3 g1(order.blocks_per_tile_plane * L.d0 * tiles);
         const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
-        if (buf && has_j)
-            hipLaunchKernelGGL((k_cell_update<NC, T, true, true>), g1, dim3(kBlock), 0, st, L, order, active,
-                               (const T*)U, (T*)Unext, (T*)E, (const T*)J);
-        else if (buf)
-            hipLaunchKernelGGL((k_cell_update<NC, T, false, true>), g1, dim3(kBlock), 0, st, L, order, active,
-                               (const T*)U, (T*)Unext, (T*)E, (const T*)E);
-        else if (has_j)
-            hipLaunchKernelGGL((k_cell_update<NC, T, true, false>), g1, dim3(kBlock), 0, st, L, order, active,
-                               (const T*)U, (T*)Unext, (T*)E, (const T*)J);
+        if (reunit)
+            launch_cell_variant<NC, T, true>(L, active, order, g1, buf, U, Unext, E, J, has_j, st);
         else
-            hipLaunchKernelGGL((k_cell_update<NC, T, false, false>), g1, dim3(kBlock), 0, st, L, order, active,
-                               (const T*)U, (T*)Unext, (T*)E, (const T*)E);
+            launch_cell_variant<NC, T, false>(L, active, order, g1, buf, U, Unext, E, J, has_j, st);
     });
     return hipGetLastError();
 }

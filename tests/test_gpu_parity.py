@@ -110,6 +110,23 @@ def test_step_links_parity(gpu, nc):
     assert rel_err(dg.getUnext(), og.Unext) < FP64_TOL
 
 
+def test_step_links_fp32_exponential_branches(gpu):
+    """FP32 SU(3) exponential on both sides of its closed-form range: the field is drawn so that
+    tr(Q^2) of Q = -dt E spans 0.03 to 1.7 (the closed form below 0.25, the series above,
+    scaling and squaring above 1). The new links agree with the FP64 oracle's getLink (Taylor-15 /
+    spectral) to FP32 rounding."""
+    dims = (4, 6, 8)
+    U, E = make_state(3, dims, 0.3, 1.4, seed=41)
+    og = oracle_grid(3, dims, U, E)
+    dg = device_grid(3, dims, U, E, precision="fp32")
+    coeffs = np.array([[O.alg_coeffs(3, E[i, d]) for d in range(3)] for i in range(E.shape[0])])
+    trq2 = 0.25 ** 2 / 2 * np.sum(coeffs ** 2, axis=2)
+    assert trq2.min() < 0.1 and trq2.max() > 1.0 and np.mean(trq2 <= 0.25) > 0.1
+    og.step_links(0.25)
+    FastTYMSolver().stepLinks(dg, 0.25)
+    assert rel_err(dg.getUnext(), og.Unext) < 2e-6
+
+
 @pytest.mark.parametrize("nc", [2, 3])
 def test_measurements_parity(gpu, nc):
     dims = (6, 5, 7)
@@ -284,6 +301,49 @@ def test_cell_form_row_tiles_fp32(gpu, dims, steps, tile, with_region):
     assert rel_err(cell.getE(), og.E) < 2e-4
     assert rel_err(cell.getUnext(), link.getUnext()) < 1e-5
     assert rel_err(cell.getE(), link.getE()) < 1e-4
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_cell_form_fused_reunitarization(gpu, nc):
+    """opx_leapfrog with the cell form reprojects the new links inside the update kernel on
+    every 10th step. The result equals the kernel without it followed by the separate
+    reprojection pass (to FP32 rounding: the two are different compilations of the same
+    arithmetic), the step count carries across calls, and the reprojected links are closer to the
+    group as they are stored."""
+    from openpixi_amd import _lib
+
+    dims = (6, 40, 36)
+    L = _lib.load()
+    U, E = make_state(nc, dims, 0.3, 0.2, seed=37)
+
+    def grid():
+        dg = device_grid(nc, dims, U, E, precision="fp32")
+        dg.setKernel("cell", 0)
+        FastTYMSolver().stepLinks(dg, 0.25)
+        return dg
+
+    fused = grid()
+    fused._check(L.opx_leapfrog(fused.handle, 4, 10))
+    fused._check(L.opx_leapfrog(fused.handle, 6, 10))      # the 10th step reprojects
+    plain = grid()
+    plain._check(L.opx_leapfrog(plain.handle, 10, 0))
+    plain.reunitarize()                                     # U and Unext reprojected in a pass
+    assert rel_err(fused.getUnext(), plain.getUnext()) < 1e-6
+    assert rel_err(fused.getE(), plain.getE()) < 1e-6
+    assert fused.checkHealth(1e-5) == (0, 0)
+    # the pass and the fused kernel apply the same double-precision projection: starting from the
+    # same float links (plain's Unext, copied into a fresh grid as U), one more step of each form
+    # agrees to rounding as well
+    once = device_grid(nc, dims, plain.getUnext(), plain.getE(), precision="fp32")
+    once.setKernel("cell", 0)
+    once.setUnext(plain.getUnext())
+    twice = device_grid(nc, dims, plain.getUnext(), plain.getE(), precision="fp32")
+    twice.setKernel("cell", 0)
+    twice.setUnext(plain.getUnext())
+    once._check(L.opx_leapfrog(once.handle, 10, 10))
+    twice._check(L.opx_leapfrog(twice.handle, 10, 0))
+    twice.reunitarize()
+    assert rel_err(once.getUnext(), twice.getUnext()) < 1e-6
 
 
 def test_su3_128cube_fp32_cell_against_fp64_link(gpu):
