@@ -397,18 +397,22 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
         return u;
     };
     const T at = (T)L.at;
-    auto finish = [&](int d, const M& S) {
-        const M Ud = fload_link<NC, T>(U, site, d);
-        const A pr = proj(mul(Ud, S));
+    // E(x) of direction d plus the current term, read before any store of the cell
+    auto e_start = [&](int d) -> A {
         A e = fload_alg_once<NC, T>(E, site, d);
-#pragma unroll
-        for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * at;
         if (HAS_J) {
             const A jj = fload_alg<NC, T>(J, site, d);
             const T uf = (T)L.uf[d];
 #pragma unroll
             for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(jj, k) * uf;
         }
+        return e;
+    };
+    auto finish = [&](int d, const M& S, A e) {
+        const M Ud = fload_link<NC, T>(U, site, d);
+        const A pr = proj(mul(Ud, S));
+#pragma unroll
+        for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * at;
         fstore_alg_once<NC, T>(E, site, d, e);
         M nu;
         if constexpr (TWO_ROWS)
@@ -419,6 +423,9 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
         fstore_link_once<NC, T>(Unext, site, d, nu);
     };
     M S0 = GR::zero(), S1 = GR::zero(), S2 = GR::zero();
+    // E of all three directions (and the current term) is read before the first store to E: a load
+    // of E cannot move above a store to it, and read late its latency is exposed at every update
+    const A e0 = e_start(0), e1 = e_start(1), e2 = e_start(2);
     {
         const M P = upper(0, 1);
         acc_scaled(S0, staple_a(0, 1, P), (T)L.area[1]);
@@ -427,15 +434,15 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
     {
         const M P = upper(0, 2);
         acc_scaled(S0, staple_a(0, 2, P), (T)L.area[2]);
-        finish(0, S0);
+        finish(0, S0, e0);
         acc_scaled(S2, staple_b(0, 2, P), (T)L.area[0]);
     }
     {
         const M P = upper(1, 2);
         acc_scaled(S1, staple_a(1, 2, P), (T)L.area[2]);
-        finish(1, S1);
+        finish(1, S1, e1);
         acc_scaled(S2, staple_b(1, 2, P), (T)L.area[1]);
-        finish(2, S2);
+        finish(2, S2, e2);
     }
 }
 
