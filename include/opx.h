@@ -199,6 +199,17 @@ int opx_measure(opx_ctx *ctx, double *out);
  * longitudinal electric/magnetic energy per slice. */
 int opx_projected_energy_density(opx_ctx *ctx, int direction, double *eT_el, double *eT_mag,
                                  double *eL_el, double *eL_mag);
+/* opx_projected_energy_density without any device allocation. No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * per-slice sums go to a static device buffer of 4 x 4096 doubles (128 KiB, one
+ * per GPU, held until the process ends) instead of the context's own buffer, so
+ * `direction` may have at most 4096 slices (more: OPX_E_ARG). Results are those
+ * of opx_projected_energy_density bit for bit. Every call on a GPU shares that
+ * GPU's buffer: calls that overlap on one GPU race. The context's GPU must be
+ * current on the first call. */
+int opx_projected_energy_density_noalloc(opx_ctx *ctx, int direction, double *eT_el, double *eT_mag,
+                                         double *eL_el, double *eL_mag);
 /* Longitudinal Poynting vector along `direction` (PoyntingComputation,
  * diagnostics/methods/ProjectedEnergyDensity.java:178-262): two arrays of
  * num_cells[direction] doubles, spatially and time averaged and only time averaged,

@@ -66,6 +66,7 @@ SIGNATURES = {
     "opx_leapfrog": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "opx_measure": (ctypes.c_int, [_vp, _dp]),
     "opx_projected_energy_density": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "opx_projected_energy_density_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
     "opx_projected_poynting": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "opx_superparticles_upload": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                                                  ctypes.c_int64]),

@@ -344,8 +344,10 @@ int opx_create(int num_colors, int num_dims, const int* num_cells, const double*
     int rc = hip_check(ctx, hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking), "stream");
     if (rc == OPX_OK) {
         int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+        rc = hip_check(ctx, hipGetDevice(&dev), "current device");
+        ctx->device = dev;
+        if (rc == OPX_OK && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            cus > 0)
             ctx->num_cus = cus;
     }
     if (rc == OPX_OK) {
@@ -739,6 +741,32 @@ int opx_projected_energy_density(opx_ctx* ctx, int direction, double* eT_el, dou
     std::memcpy(eT_mag, h.data() + nd, nd * sizeof(double));
     std::memcpy(eL_el, h.data() + 2 * (size_t)nd, nd * sizeof(double));
     std::memcpy(eL_mag, h.data() + 3 * (size_t)nd, nd * sizeof(double));
+    return OPX_OK;
+}
+
+int opx_projected_energy_density_noalloc(opx_ctx* ctx, int direction, double* eT_el, double* eT_mag, double* eL_el,
+                                         double* eL_mag)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (direction < 0 || direction > 2) return fail(ctx, OPX_E_ARG, "projected energy: bad direction");
+    if (!eT_el || !eT_mag || !eL_el || !eL_mag) return fail(ctx, OPX_E_ARG, "projected energy: null output");
+    const int nd = ctx->dims[direction];
+    if (nd > opx::kNoAllocMaxSlices)
+        return fail(ctx, OPX_E_ARG, "projected energy (no allocation): " + std::to_string(nd) +
+                                        " slices along the direction, more than the static buffer's " +
+                                        std::to_string(opx::kNoAllocMaxSlices));
+    double* buf = nullptr;
+    OPX_TRY(opx::noalloc_scratch(ctx->device, &buf), "projected energy (no allocation): static buffer address");
+    opx::Lattice L = lattice(ctx, ctx->at);
+    OPX_TRY(opx::launch_projected_energy(ctx->nc, ctx->prec, L, ctx->eval, direction, ctx->U, ctx->Unext, ctx->E, buf,
+                                         ctx->stream),
+            "projected energy");
+    double* outs[4] = {eT_el, eT_mag, eL_el, eL_mag};
+    for (int q = 0; q < 4; q++)
+        OPX_TRY(hipMemcpyAsync(outs[q], buf + (size_t)q * nd, (size_t)nd * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream),
+                "projected energy copy");
+    OPX_TRY(hipStreamSynchronize(ctx->stream), "projected energy sync");
     return OPX_OK;
 }
 

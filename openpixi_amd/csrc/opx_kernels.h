@@ -183,6 +183,12 @@ hipError_t launch_projected_poynting(int nc, int prec, const Lattice& L, int dir
                                      const void* E, double* out2, hipStream_t st);
 hipError_t launch_projected_energy(int nc, int prec, const Lattice& L, const Box& eval, int dir, const void* U,
                                    const void* Unext, const void* E, double* out4, hipStream_t st);
+// Static per-slice buffer of the allocation-free projected energy: 4 x kNoAllocMaxSlices doubles
+// (128 KiB) on every GPU the library runs on. noalloc_scratch returns its address on `device`,
+// which must be the current device the first time it is asked for (looked up once per device).
+constexpr int kNoAllocMaxSlices = 4096;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t noalloc_scratch(int device, double** out);
 hipError_t launch_health(int nc, int prec, uint32_t n, const void* U, double tol,
                          unsigned long long* counters, hipStream_t st);
 hipError_t launch_health_alg(int nc, int prec, uint32_t n_values, const void* X,

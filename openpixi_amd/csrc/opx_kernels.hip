@@ -5,6 +5,7 @@
 // [dir][coefficient][cell], the charge density [coefficient][cell]. Consecutive
 // lanes of a wavefront touch consecutive cells, so every load and store is a
 // fully coalesced 256-byte (fp32) or 512-byte (fp64) wavefront access.
+#include <mutex>
 #include <type_traits>
 
 #include "opx_buf.h"
@@ -1417,6 +1418,32 @@ hipError_t launch_projected_poynting(int nc, int prec, const Lattice& L, int dir
         hipLaunchKernelGGL(k_projected_poynting<float>, dim3(nd), dim3(kBlock), 0, st, L, dir, (const float*)U,
                            (const float*)Unext, (const float*)E, out2);
     return hipGetLastError();
+}
+
+// The per-slice sums of opx_projected_energy_density_noalloc: a static device array, one copy per
+// GPU, addressed through the pointer hipGetSymbolAddress returns on that GPU (its name is only a
+// host-side handle). Every block of k_projected_energy writes its four sums, so nothing in it
+// needs clearing; calls that overlap on one GPU race on it.
+static __device__ __align__(256) double scratch[4 * kNoAllocMaxSlices];
+
+hipError_t noalloc_scratch(int device, double** out)
+{
+    static std::mutex lock;
+    static double* ptr[kNoAllocMaxDevices] = {};
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(lock);
+    if (!ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;   // the address is looked up on the current GPU
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptr[device] = static_cast<double*>(p);
+    }
+    *out = ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_projected_energy(int nc, int prec, const Lattice& L, const Box& eval, int dir, const void* U,
