@@ -377,8 +377,31 @@ int opx_simulation_step(opx_ctx *ctx, int n);
 int opx_simulation_steps_done(const opx_ctx *ctx);
 
 /* Health monitor: counts non-finite values and links with |U U^dagger - 1|
- * above tol (UnitarityTester, diagnostics/methods/UnitarityTester.java:31-46). */
+ * above tol (a stricter relative of UnitarityTester, below). */
 int opx_check_health(opx_ctx *ctx, double tol, int64_t *non_finite, int64_t *non_unitary);
+
+/* UnitarityTester.calculate (diagnostics/methods/UnitarityTester.java:31-46,
+ * GroupElement.getRealTrace): the links (cell, dir) where |Re tr U| or |Re tr Unext|
+ * is not below num_colors + tol (the reference uses tol = 1e-8; NaN counts as not
+ * below). *non_unitary receives their number; cells (up to max_cells entries, may
+ * be NULL when max_cells is 0) the cell index of each, in the reference's loop order
+ * (cell, then direction; a cell appears once per flagged direction). */
+int opx_unitarity_test(opx_ctx *ctx, double tol, int64_t *non_unitary, int64_t *cells, int64_t max_cells);
+
+/* GaussConstraintRestoration.iterateRestorationAlgorithm
+ * (diagnostics/methods/GaussConstraintRestoration.java:79-163, after hep-ph/9603384):
+ * the violation C(x) = sum_j [Unext_j(x) U_j(x)^+ - U_j(x-j)^+ Unext_j(x-j)] / (a_t a)
+ * with total (1/N) sum_x |proj C(x) - rho(x)|^2; each iteration backs up E and Unext,
+ * applies E_j(x) += -a gamma/2 proj(U_j(x) C(x+j) U_j(x)^+ - C(x)) on every cell, then
+ * stepLinks (Unext = exp(-i a_t E) U on the active cells). It stops when the total is
+ * below absolute_value (*reason 0), when a correction made it larger, which is then
+ * undone (1), after max_iterations corrections (2), or at once when max_iterations
+ * <= 0 (3). *iterations: corrections kept; violation_before/after: the total before
+ * and for the state left behind. Evenly spaced lattices only (the reference's
+ * Grid.getLatticeSpacing throws otherwise): OPX_E_UNSUPPORTED. In FP32 contexts the
+ * violation and the correction are computed in double and E is stored in float. */
+int opx_gauss_restore(opx_ctx *ctx, double gamma, int max_iterations, double absolute_value, int *iterations,
+                      int *reason, double *violation_before, double *violation_after);
 
 /* ---- SU(2) adjoint of the leapfrog (differentiation; no counterpart in the reference) ----
  *

@@ -67,6 +67,10 @@ SIGNATURES = {
     "opx_measure": (ctypes.c_int, [_vp, _dp]),
     "opx_projected_energy_density": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
     "opx_projected_energy_density_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "opx_unitarity_test": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]),
+    "opx_gauss_restore": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp, _dp]),
     "opx_projected_poynting": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "opx_superparticles_upload": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                                                  ctypes.c_int64]),

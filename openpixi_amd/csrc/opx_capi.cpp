@@ -1436,6 +1436,125 @@ int opx_gauss_project(opx_ctx* ctx, double tol, int max_iter, int* iterations, d
     return rc;
 }
 
+int opx_gauss_restore(opx_ctx* ctx, double gamma, int max_iterations, double absolute_value, int* iterations,
+                      int* reason, double* violation_before, double* violation_after)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (ctx->batch_view) return fail(ctx, OPX_E_STATE, "restore the events of a batch through the batch");
+    if (!std::isfinite(gamma) || !std::isfinite(absolute_value))
+        return fail(ctx, OPX_E_ARG, "gauss_restore: gamma and the absolute value must be finite");
+    // the reference reads one lattice spacing (Grid.getLatticeSpacing throws on uneven grids)
+    if (ctx->as[0] != ctx->as[1] || ctx->as[0] != ctx->as[2])
+        return fail(ctx, OPX_E_UNSUPPORTED, "gauss_restore: uneven lattice spacings (Use getGridStep(int direction).)");
+    int rc = materialize_rho(ctx);
+    if (rc != OPX_OK) return rc;
+    opx::Lattice L = lattice(ctx, ctx->at);
+    const uint64_t ms = (uint64_t)ctx->n * ctx->AD;
+    const size_t eb = alg_count(ctx) * ctx->tsz, lb = link_count(ctx) * ctx->tsz;
+    double *G = nullptr, *R = nullptr, *scr = nullptr;
+    void *E0 = nullptr, *Un0 = nullptr;
+    rc = hip_check(ctx, hipMalloc(&G, ms * sizeof(double)), "restore alloc");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&R, ms * sizeof(double)), "restore alloc");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&scr, (opx::gauss_dot_scratch() + 1) * sizeof(double)), "restore alloc");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&E0, eb), "restore alloc");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&Un0, lb), "restore alloc");
+    double* total_dev = scr + opx::gauss_dot_scratch();
+    const double factor = 1.0 / (ctx->at * ctx->as[0]);
+    const double f2 = -0.5 * ctx->as[0] * gamma;
+    // computeGaussViolation: G, and the mean violation read back to the host
+    auto violation = [&](double* out) -> int {
+        int r2 = hip_check(ctx, opx::restore_violation(ctx->nc, ctx->prec, L, ctx->U, ctx->Unext, ctx->rho, factor, G,
+                                                       R, ctx->stream),
+                           "restore violation");
+        if (r2 == OPX_OK) r2 = hip_check(ctx, opx::gauss_dot(ms, R, R, scr, total_dev, ctx->stream), "restore sum");
+        double v = 0;
+        if (r2 == OPX_OK)
+            r2 = hip_check(ctx, hipMemcpyAsync(&v, total_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                           "restore copy");
+        if (r2 == OPX_OK) r2 = hip_check(ctx, hipStreamSynchronize(ctx->stream), "restore sync");
+        *out = v / (double)ctx->n;
+        return r2;
+    };
+    double total = 0, old = 0;
+    int kept = 0, why = 3;
+    if (rc == OPX_OK) rc = violation(&total);
+    old = total;
+    if (violation_before) *violation_before = total;
+    // iterateRestorationAlgorithm (GaussConstraintRestoration.java:79-112)
+    for (int i = 0; rc == OPX_OK && i < max_iterations; i++) {
+        if (total < absolute_value) {
+            why = 0;
+            break;
+        }
+        // backupGrid: a correction changes E and, through updateLinks, Unext
+        rc = hip_check(ctx, hipMemcpyAsync(E0, ctx->E, eb, hipMemcpyDeviceToDevice, ctx->stream), "restore backup");
+        if (rc == OPX_OK)
+            rc = hip_check(ctx, hipMemcpyAsync(Un0, ctx->Unext, lb, hipMemcpyDeviceToDevice, ctx->stream),
+                           "restore backup");
+        if (rc == OPX_OK)
+            rc = hip_check(ctx, opx::restore_correct(ctx->nc, ctx->prec, L, ctx->U, G, f2, ctx->E, ctx->stream),
+                           "restore correction");
+        if (rc == OPX_OK)
+            rc = hip_check(ctx, opx::launch_step_links(ctx->nc, ctx->prec, L, ctx->active, ctx->U, ctx->Unext, ctx->E,
+                                                       ctx->stream),
+                           "restore updateLinks");
+        if (rc == OPX_OK) rc = violation(&total);
+        if (rc != OPX_OK) break;
+        if (old - total < 0) {
+            // restoreGrid: the last correction made the violation worse
+            rc = hip_check(ctx, hipMemcpyAsync(ctx->E, E0, eb, hipMemcpyDeviceToDevice, ctx->stream), "restore undo");
+            if (rc == OPX_OK)
+                rc = hip_check(ctx, hipMemcpyAsync(ctx->Unext, Un0, lb, hipMemcpyDeviceToDevice, ctx->stream),
+                               "restore undo");
+            if (rc == OPX_OK) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "restore sync");
+            why = 1;
+            break;
+        }
+        old = total;
+        kept = i + 1;
+        if (i == max_iterations - 1) why = 2;
+    }
+    if (violation_after) *violation_after = old;
+    if (iterations) *iterations = kept;
+    if (reason) *reason = why;
+    for (void* b : {(void*)G, (void*)R, (void*)scr, E0, Un0})
+        if (b) (void)hipFree(b);
+    return rc;
+}
+
+int opx_unitarity_test(opx_ctx* ctx, double tol, int64_t* non_unitary, int64_t* cells, int64_t max_cells)
+{
+    if (!ctx) return OPX_E_ARG;
+    if (!non_unitary) return fail(ctx, OPX_E_ARG, "unitarity test: null count");
+    if (max_cells < 0 || (max_cells > 0 && !cells)) return fail(ctx, OPX_E_ARG, "unitarity test: bad cell list");
+    const size_t nl = (size_t)ctx->n * 3;
+    unsigned char* flags = nullptr;
+    int rc = hip_check(ctx, hipMalloc(&flags, nl), "unitarity alloc");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemsetAsync(ctx->counters + 2, 0, sizeof(unsigned long long), ctx->stream),
+                       "unitarity reset");
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, opx::unitarity_flags(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->U, ctx->Unext, tol, flags,
+                                                 ctx->counters + 2, ctx->stream),
+                       "unitarity test");
+    unsigned long long cnt = 0;
+    if (rc == OPX_OK)
+        rc = hip_check(ctx, hipMemcpyAsync(&cnt, ctx->counters + 2, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream),
+                       "unitarity copy");
+    if (rc == OPX_OK) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "unitarity sync");
+    if (rc == OPX_OK && cnt > 0 && max_cells > 0) {
+        // the flagged links in the reference's loop order (cell, then direction)
+        std::vector<unsigned char> h(nl);
+        rc = hip_check(ctx, hipMemcpy(h.data(), flags, nl, hipMemcpyDeviceToHost), "unitarity flags copy");
+        int64_t k = 0;
+        for (size_t i = 0; rc == OPX_OK && i < nl && k < max_cells; i++)
+            if (h[i]) cells[k++] = (int64_t)(i / 3);
+    }
+    if (flags) (void)hipFree(flags);
+    *non_unitary = (int64_t)cnt;
+    return rc;
+}
+
 }  // extern "C"
 
 // ============================================================== batched events

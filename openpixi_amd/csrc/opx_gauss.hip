@@ -112,6 +112,86 @@ __global__ __launch_bounds__(kB) void k_apply_Dt(Lattice L, const T* __restrict_
 }
 
 // E (precision T) -> double copy, and back with E -= F
+// GaussConstraintRestoration.computeGaussViolation (diagnostics/methods/GaussConstraintRestoration.java:
+// 117-135): the violation C = sum_j [Unext_j(x) U_j(x)^+ - U_j(x-j)^+ Unext_j(x-j)] / (a_t a) enters
+// only through its projection (the projection is linear, so it is the sum of the terms'
+// projections): G = proj C, and R = G - rho, whose mean square is the total violation.
+template <int NC, typename T>
+__global__ __launch_bounds__(kB) void k_restore_violation(Lattice L, const T* __restrict__ U,
+                                                          const T* __restrict__ Unext, const T* __restrict__ rho,
+                                                          double factor, double* __restrict__ G,
+                                                          double* __restrict__ R)
+{
+    using GD = Group<NC, double>;
+    const uint32_t site = blockIdx.x * kB + threadIdx.x;
+    if (site >= L.n) return;
+    uint32_t x[3];
+    coords3(L, site, x);
+    typename GD::A g = GD::azero();
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const uint32_t xm = nb_bwd(L, site, x[j], j);
+        const auto c1 = proj(mul_adj(link_d<NC, T>(Unext, L.n, site, j), link_d<NC, T>(U, L.n, site, j)));
+        const auto c2 = proj(adj_mul(link_d<NC, T>(U, L.n, xm, j), link_d<NC, T>(Unext, L.n, xm, j)));
+#pragma unroll
+        for (int k = 0; k < GD::AD; k++) aref(g, k) += aget(c1, k) - aget(c2, k);
+    }
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) {
+        const double v = aget(g, k) * factor;
+        G[(size_t)k * L.n + site] = v;
+        R[(size_t)k * L.n + site] = v - (rho ? (double)rho[(size_t)k * L.n + site] : 0.0);
+    }
+}
+
+// GaussConstraintRestoration.applyCorrection (:142-158), every cell and direction:
+//   E_j(x) += f2 proj(U_j(x) C(x+j) U_j(x)^+ - C(x)) = f2 [act(G(x+j), U_j(x)) - G(x)]
+template <int NC, typename T>
+__global__ __launch_bounds__(kB) void k_restore_correct(Lattice L, const T* __restrict__ U,
+                                                        const double* __restrict__ G, double f2, T* __restrict__ E)
+{
+    using GD = Group<NC, double>;
+    const uint32_t site = blockIdx.x * kB + threadIdx.x;
+    const int d = blockIdx.y;
+    if (site >= L.n) return;
+    uint32_t x[3];
+    coords3(L, site, x);
+    const uint32_t xp = nb_fwd(L, site, x[d], d);
+    const auto t = act(alg_at<NC>(G, L.n, xp, 0), link_d<NC, T>(U, L.n, site, d));
+    const auto gi = alg_at<NC>(G, L.n, site, 0);
+    T* e = E + (size_t)d * GD::AD * L.n + site;
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) {
+        const double v = (double)e[(size_t)k * L.n] + (aget(t, k) - aget(gi, k)) * f2;
+        e[(size_t)k * L.n] = (T)v;
+    }
+}
+
+// UnitarityTester.calculate (diagnostics/methods/UnitarityTester.java:31-46): flag (cell, dir) when
+// |Re tr U| or |Re tr Unext| is not below N + tol (NaN included)
+template <int NC, typename T>
+__global__ __launch_bounds__(kB) void k_trace_flags(uint32_t n, const T* __restrict__ U, const T* __restrict__ Unext,
+                                                    double tol, unsigned char* __restrict__ flags,
+                                                    unsigned long long* __restrict__ count)
+{
+    const uint32_t site = blockIdx.x * kB + threadIdx.x;
+    const int d = blockIdx.y;
+    if (site >= n) return;
+    bool bad = false;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const auto u = link_d<NC, T>(t ? Unext : U, n, site, d);
+        double tr;
+        if constexpr (NC == 2)
+            tr = 2.0 * u.e0;
+        else
+            tr = u.r[0] + u.r[4] + u.r[8];
+        if (!(fabs(tr) < NC + tol)) bad = true;
+    }
+    flags[(size_t)site * 3 + d] = bad ? 1 : 0;
+    if (bad) atomicAdd(count, 1ull);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kB) void k_to_double(uint64_t m, const T* __restrict__ a, double* __restrict__ b)
 {
@@ -263,6 +343,38 @@ hipError_t gauss_axpy(uint64_t m, double alpha, const double* x, double* y, hipS
 hipError_t gauss_xpby(uint64_t m, const double* x, double beta, double* y, hipStream_t st)
 {
     hipLaunchKernelGGL(k_xpby, g1(m), dim3(kB), 0, st, m, x, beta, y);
+    return hipGetLastError();
+}
+
+hipError_t restore_violation(int nc, int prec, const Lattice& L, const void* U, const void* Unext, const void* rho,
+                             double factor, double* G, double* R, hipStream_t st)
+{
+    const dim3 g((L.n + kB - 1) / kB);
+    OPX_GDISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_restore_violation<NC, T>), g, dim3(kB), 0, st, L, (const T*)U, (const T*)Unext,
+                           (const T*)rho, factor, G, R);
+    });
+    return hipGetLastError();
+}
+
+hipError_t restore_correct(int nc, int prec, const Lattice& L, const void* U, const double* G, double f2, void* E,
+                           hipStream_t st)
+{
+    const dim3 g((L.n + kB - 1) / kB, 3);
+    OPX_GDISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_restore_correct<NC, T>), g, dim3(kB), 0, st, L, (const T*)U, G, f2, (T*)E);
+    });
+    return hipGetLastError();
+}
+
+hipError_t unitarity_flags(int nc, int prec, uint32_t n, const void* U, const void* Unext, double tol,
+                           unsigned char* flags, unsigned long long* count, hipStream_t st)
+{
+    const dim3 g((n + kB - 1) / kB, 3);
+    OPX_GDISPATCH(nc, prec, {
+        hipLaunchKernelGGL((k_trace_flags<NC, T>), g, dim3(kB), 0, st, n, (const T*)U, (const T*)Unext, tol, flags,
+                           count);
+    });
     return hipGetLastError();
 }
 

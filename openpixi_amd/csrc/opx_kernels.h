@@ -91,6 +91,17 @@ hipError_t gauss_dot(uint64_t m, const double* a, const double* b, double* scrat
 hipError_t gauss_axpy(uint64_t m, double alpha, const double* x, double* y, hipStream_t st);
 hipError_t gauss_xpby(uint64_t m, const double* x, double beta, double* y, hipStream_t st);
 
+// GaussConstraintRestoration building blocks: G = proj of the violation times factor, R = G - rho
+// ([coefficient][cell] doubles; rho may be null), and E_j(x) += f2 [act(G(x+j), U_j(x)) - G(x)]
+hipError_t restore_violation(int nc, int prec, const Lattice& L, const void* U, const void* Unext, const void* rho,
+                             double factor, double* G, double* R, hipStream_t st);
+hipError_t restore_correct(int nc, int prec, const Lattice& L, const void* U, const double* G, double f2, void* E,
+                           hipStream_t st);
+// UnitarityTester: flags[cell * 3 + dir] = 1 where |Re tr U| or |Re tr Unext| >= N + tol; *count
+// (device) is incremented once per flagged link
+hipError_t unitarity_flags(int nc, int prec, uint32_t n, const void* U, const void* Unext, double tol,
+                           unsigned char* flags, unsigned long long* count, hipStream_t st);
+
 // Batched events: persistent grid of `grid` workgroups over total_items work items.
 hipError_t launch_p_chargedensity(int nc, int prec, const Lattice& L, const ParticleArrays& p, void* rho,
                                   hipStream_t st);

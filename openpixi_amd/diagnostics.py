@@ -1,6 +1,7 @@
-"""Output files of the FastTYMSolver path: the reference's BulkQuantitiesInTime and
-ProjectedEnergyDensity diagnostics, computed by the device reductions and written in
-the reference's file formats.
+"""Diagnostics of the FastTYMSolver path: the reference's BulkQuantitiesInTime and
+ProjectedEnergyDensity output files, computed by the device reductions and written in
+the reference's file formats, and the UnitarityTester and GaussConstraintRestoration
+diagnostics run on the device with the reference's console messages.
 
 Number formats follow the Java library calls the reference uses:
 
@@ -22,6 +23,7 @@ from __future__ import annotations
 
 import math
 import os
+import sys
 from decimal import Decimal
 from typing import List
 
@@ -175,14 +177,89 @@ class ProjectedEnergyDensity:
             f.write("\n".join(lines) + "\n")
 
 
+class UnitarityTester:
+    """diagnostics/methods/UnitarityTester.java:13-46: every stepInterval steps, one line
+    "UnitarityTester: non-unitary link found @ i" per link (cell i, any direction) whose U or
+    Unext has |Re tr| not below N + 1e-8."""
+
+    TOLERANCE = 1e-8
+
+    def __init__(self, time_interval: float, out=None):
+        self.time_interval = time_interval
+        self.out = out if out is not None else sys.stdout
+
+    def initialize(self, sim) -> None:
+        self.stepInterval = step_interval(self.time_interval, sim.tstep)
+
+    def calculate(self, grid: Grid, steps: int) -> None:
+        if steps % self.stepInterval != 0:
+            return
+        count, cells = grid.unitarityTest(self.TOLERANCE)
+        for i in cells:
+            print(f"UnitarityTester: non-unitary link found @ {int(i)}", file=self.out)
+
+
+class GaussConstraintRestoration:
+    """diagnostics/methods/GaussConstraintRestoration.java:17-112: every stepInterval steps from
+    stepOffset on (or once at stepOffset with applyOnlyOnce), the restoration iteration on the
+    device, with the reference's messages."""
+
+    def __init__(self, time_interval: float, time_offset: float, gamma: float, max_iterations: int,
+                 absolute_value: float, apply_only_once: bool, out=None):
+        self.time_interval = time_interval
+        self.time_offset = time_offset
+        self.gamma = gamma
+        self.max_iterations = max_iterations
+        self.absolute_value = absolute_value
+        self.apply_only_once = apply_only_once
+        self.already_applied = False
+        self.out = out if out is not None else sys.stdout
+        self.last = None   # (kept, reason, before, after) of the last run
+
+    def initialize(self, sim) -> None:
+        self.stepInterval = step_interval(self.time_interval, sim.tstep)
+        self.stepOffset = int(self.time_offset / sim.tstep)   # (int) truncates towards zero
+
+    def calculate(self, grid: Grid, steps: int) -> None:
+        if not self.apply_only_once:
+            if (steps - self.stepOffset) % self.stepInterval == 0:
+                self.iterate(grid)
+        elif not self.already_applied and steps == self.stepOffset:
+            self.iterate(grid)
+            self.already_applied = True
+
+    def iterate(self, grid: Grid) -> None:
+        kept, reason, before, after = grid.restoreGaussConstraint(self.gamma, self.max_iterations,
+                                                                  self.absolute_value)
+        self.last = (kept, reason, before, after)
+        if reason == 0:
+            print(f"GaussConstraintRestoration: Reached absolute value at step #{kept}", file=self.out)
+        elif reason == 1:
+            print(f"GaussConstraintRestoration: Reached instability at step #{kept}", file=self.out)
+        elif reason == 2:
+            print(f"GaussConstraintRestoration: Reached maximum number of iterations at step #{kept - 1}",
+                  file=self.out)
+
+
 def from_yaml_output(section) -> list:
-    """YamlOutput.applyTo (ui/util/yaml/YamlOutput.java:45-105) for the writers of this path,
-    with the YAML defaults of YamlProjectedEnergyDensity (computeEnergyDensity and
+    """YamlOutput.applyTo (ui/util/yaml/YamlOutput.java:45-105) for the diagnostics of this path,
+    in the reference's order (bulkQuantitiesInTime, unitarityTester, gaussConstraintRestoration,
+    projectedEnergyDensity), with the YAML defaults of YamlUnitarityTester,
+    YamlGaussConstraintRestoration and YamlProjectedEnergyDensity (computeEnergyDensity and
     computePoyntingVector true). Other output types are not part of this path."""
     out = []
     section = section or {}
     for o in section.get("bulkQuantitiesInTime") or []:
         out.append(BulkQuantitiesInTime(str(o["path"]), float(o["interval"])))
+    # YamlUnitarityTester (interval has no default: an absent one is 0, every step)
+    for o in section.get("unitarityTester") or []:
+        out.append(UnitarityTester(float(o.get("interval", 0.0))))
+    # YamlGaussConstraintRestoration defaults; absoluteValue's is the expression 10-8 = 2
+    for o in section.get("gaussConstraintRestoration") or []:
+        out.append(GaussConstraintRestoration(float(o.get("interval", 1.0)), float(o.get("offset", 0.0)),
+                                              float(o.get("gamma", 0.25)), int(o.get("maxIterations", 100)),
+                                              float(o.get("absoluteValue", 10 - 8)),
+                                              bool(o.get("applyOnlyOnce", False))))
     for o in section.get("projectedEnergyDensity") or []:
         out.append(ProjectedEnergyDensity(str(o["path"]), float(o["interval"]), int(o.get("direction", 0)),
                                           bool(o.get("computeEnergyDensity", True)),

@@ -221,6 +221,30 @@ class Grid:
                                                   ctypes.byref(g0), ctypes.byref(g1)))
         return it.value, g0.value, g1.value
 
+    def restoreGaussConstraint(self, gamma, maxIterations, absoluteValue):
+        """GaussConstraintRestoration.iterateRestorationAlgorithm
+        (diagnostics/methods/GaussConstraintRestoration.java:79-112) on the device. Returns
+        (corrections kept, reason, violation before, violation after); reason 0: the absolute
+        value was reached, 1: a correction made the violation worse and was undone, 2: the
+        maximum number of iterations, 3: no iteration ran."""
+        it, why = ctypes.c_int(), ctypes.c_int()
+        v0, v1 = ctypes.c_double(), ctypes.c_double()
+        self._check(_lib.load().opx_gauss_restore(self._h, float(gamma), int(maxIterations), float(absoluteValue),
+                                                  ctypes.byref(it), ctypes.byref(why), ctypes.byref(v0),
+                                                  ctypes.byref(v1)))
+        return it.value, why.value, v0.value, v1.value
+
+    def unitarityTest(self, tol=1e-8, max_cells=1 << 20):
+        """UnitarityTester (diagnostics/methods/UnitarityTester.java:31-46): (number of links whose
+        |Re tr U| or |Re tr Unext| is not below N + tol, their cell indices in the reference's loop
+        order, at most max_cells of them)."""
+        cnt = ctypes.c_int64()
+        cells = np.zeros(max(int(max_cells), 1), np.int64)
+        self._check(_lib.load().opx_unitarity_test(self._h, float(tol), ctypes.byref(cnt),
+                                                   cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                   int(max_cells)))
+        return cnt.value, cells[: min(cnt.value, int(max_cells))]
+
     def reunitarize(self):
         self._check(_lib.load().opx_reunitarize(self._h))
 

@@ -3,7 +3,8 @@ FastTYMSolver path.
 
 ``Settings.from_yaml`` accepts the reference's YAML keys
 (ui/util/yaml/YamlSettings.java:14-191) that concern this path, including the
-``output`` writers of this path (bulkQuantitiesInTime, projectedEnergyDensity), and
+``output`` diagnostics of this path (bulkQuantitiesInTime, unitarityTester,
+gaussConstraintRestoration, projectedEnergyDensity), and
 ignores panel sections. ``Simulation`` keeps the reference's step ordering
 (physics/Simulation.java:318-452): ``initialize()`` once, then ``step()`` while
 ``continues()``, which runs ceil(duration/timeStep) + 1 steps like the reference;

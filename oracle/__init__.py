@@ -108,6 +108,11 @@ def lib():
             "orc_grid_set_threads": (None, [gp, ctypes.c_int]),
             "orc_shift": (ctypes.c_int64, [gp, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
             "orc_set_active_region": (None, [gp, _ip, _ip]),
+            "orc_restore_violation": (ctypes.c_double, [gp, _dp]),
+            "orc_restore_correct": (None, [gp, _dp, ctypes.c_double]),
+            "orc_gauss_restore": (ctypes.c_int, [gp, ctypes.c_double, ctypes.c_int, ctypes.c_double, _ip, _dp, _dp,
+                                                 _dp]),
+            "orc_unitarity_flags": (ctypes.c_int64, [gp, ctypes.c_double, _u8p]),
             "orc_set_evaluation_region": (None, [gp, _ip, _ip]),
             "orc_staple_sum": (None, [gp, ctypes.c_int64, ctypes.c_int, _dp]),
             "orc_step": (None, [gp, ctypes.c_double]),
@@ -470,6 +475,32 @@ class OracleGrid:
         if lib().orc_projected_poynting(self._g, direction, _ptr(avg), _ptr(tavg)) != 0:
             raise RuntimeError("SU(3) code has not been tested yet.")
         return avg, tavg
+
+    # ---- GaussConstraintRestoration, UnitarityTester (opx_oracle_gauss.c) ----
+    def restore_violation(self):
+        """(mean violation, gv as n group elements) of GaussConstraintRestoration.computeGaussViolation."""
+        gv = np.zeros((self.n, gsize(self.nc)))
+        v = lib().orc_restore_violation(self._g, _ptr(gv.reshape(-1)))
+        return v, gv
+
+    def restore_correct(self, gv, gamma):
+        lib().orc_restore_correct(self._g, _ptr(np.ascontiguousarray(gv, np.float64).reshape(-1)), float(gamma))
+
+    def gauss_restore(self, gamma, max_iterations, absolute_value):
+        """iterateRestorationAlgorithm: (kept corrections, reason, before, after, history)."""
+        reason = np.zeros(1, np.int32)
+        b, a = np.zeros(1), np.zeros(1)
+        hist = np.full(max(int(max_iterations), 0) + 1, np.nan)
+        it = lib().orc_gauss_restore(self._g, float(gamma), int(max_iterations), float(absolute_value),
+                                     reason.ctypes.data_as(_ip), _ptr(b), _ptr(a), _ptr(hist))
+        if it < 0:
+            raise RuntimeError("Use getGridStep(int direction).")
+        return it, int(reason[0]), float(b[0]), float(a[0]), hist[: it + 1]
+
+    def unitarity_flags(self, tol=1e-8):
+        flags = np.zeros(self.n * 3, np.uint8)
+        cnt = lib().orc_unitarity_flags(self._g, float(tol), flags.ctypes.data_as(_u8p))
+        return int(cnt), flags.reshape(self.n, 3)
 
     def gauge_transform(self, g_cells):
         g_cells = np.ascontiguousarray(g_cells, np.float64).reshape(-1)

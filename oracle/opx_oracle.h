@@ -158,6 +158,18 @@ double orc_adj_bsq(const orc_grid *gr, const double *U, int dir, double *Ubar);
 double orc_adj_loss(const orc_grid *gr, const double *Uprev, const double *Ulast, const double *E, int dir);
 int orc_adjoint_gradient(orc_grid *gr, int steps, int dir, double *loss, double *grad);
 
+/* ---- GaussConstraintRestoration and UnitarityTester (opx_oracle_gauss.c) ---- */
+/* gv: n group elements; returns the mean violation |proj gv - rho|^2 */
+double orc_restore_violation(const orc_grid *gr, double *gv);
+void orc_restore_correct(orc_grid *gr, const double *gv, double gamma);
+/* iterateRestorationAlgorithm: returns the corrections kept (-1: uneven spacings); reason 0 = the
+ * absolute value was reached, 1 = a correction made the violation worse and was undone, 2 = the
+ * maximum number of iterations, 3 = no iteration ran; history (max_iterations + 1 doubles, may be
+ * NULL) receives the violation before and after each kept correction */
+int orc_gauss_restore(orc_grid *gr, double gamma, int max_iterations, double absolute_value, int *reason,
+                      double *before, double *after, double *history);
+int64_t orc_unitarity_flags(const orc_grid *gr, double tol, unsigned char *flags);
+
 /* ---- timing helper for the CPU baseline ---- */
 double orc_time_steps(orc_grid *gr, int steps, int warmup);   /* seconds for `steps` leapfrog steps */
 
