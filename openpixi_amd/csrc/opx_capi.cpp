@@ -400,7 +400,7 @@ void opx_destroy(opx_ctx* ctx)
     void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->P, ctx->iU, ctx->iUn, ctx->iE,
                     ctx->partials, ctx->dout, ctx->counters, ctx->diag,
                     ctx->sp_desc, ctx->sp_qoff, ctx->sp_q, ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel,
-                    ctx->p_Q0, ctx->p_Q1};
+                    ctx->p_Q0, ctx->p_Q1, ctx->ps_buf, ctx->ps_temp};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -997,13 +997,40 @@ int opx_particles_download(opx_ctx* ctx, double* pos0, double* pos1, double* Q0,
     return OPX_OK;
 }
 
+// the ordered deposits' work arrays for the context's particles (grown when the count grows)
+static int particle_sort_work(opx_ctx* ctx, opx::ParticleSort* w)
+{
+    if ((uint64_t)ctx->n * 3 >= (1ull << 32))
+        return fail(ctx, OPX_E_UNSUPPORTED, "particles: lattices of 2^32 / 3 cells or more");
+    if (ctx->ps_capacity < ctx->p_count) {
+        if (ctx->ps_buf) (void)hipFree(ctx->ps_buf);
+        if (ctx->ps_temp) (void)hipFree(ctx->ps_temp);
+        ctx->ps_buf = nullptr;
+        ctx->ps_temp = nullptr;
+        ctx->ps_capacity = 0;
+        const size_t tb = opx::particle_sort_temp_bytes(ctx->p_count);
+        if (tb == 0) return fail(ctx, OPX_E_DEVICE, "particles: radix sort storage query failed");
+        OPX_TRY(hipMalloc(&ctx->ps_buf, (size_t)4 * ctx->p_count * sizeof(uint32_t)), "alloc particle sort");
+        OPX_TRY(hipMalloc(&ctx->ps_temp, tb), "alloc particle sort");
+        ctx->ps_temp_bytes = tb;
+        ctx->ps_capacity = ctx->p_count;
+    }
+    const size_t c = ctx->ps_capacity;
+    *w = opx::ParticleSort{ctx->ps_buf, ctx->ps_buf + c, ctx->ps_buf + 2 * c, ctx->ps_buf + 3 * c, ctx->ps_temp,
+                           ctx->ps_temp_bytes};
+    return OPX_OK;
+}
+
 int opx_particles_interpolate_chargedensity(opx_ctx* ctx)
 {
     if (!ctx) return OPX_E_ARG;
     int rc = materialize_rho(ctx);
     if (rc != OPX_OK) return rc;
+    if (ctx->p_count == 0) return OPX_OK;
+    opx::ParticleSort w;
+    if ((rc = particle_sort_work(ctx, &w)) != OPX_OK) return rc;
     opx::Lattice L = lattice(ctx, ctx->at);
-    OPX_TRY(opx::launch_p_chargedensity(ctx->nc, ctx->prec, L, p_view(ctx), ctx->rho, ctx->stream),
+    OPX_TRY(opx::launch_p_chargedensity(ctx->nc, ctx->prec, L, p_view(ctx), w, ctx->rho, ctx->stream),
             "particle charge density");
     return OPX_OK;
 }
@@ -1020,8 +1047,11 @@ int opx_particles_interpolate_to_grid(opx_ctx* ctx)
 {
     if (!ctx) return OPX_E_ARG;
     if (ctx->p_count == 0) return OPX_OK;
+    opx::ParticleSort w;
+    int rc = particle_sort_work(ctx, &w);
+    if (rc != OPX_OK) return rc;
     opx::Lattice L = lattice(ctx, ctx->at);
-    OPX_TRY(opx::launch_p_to_grid(ctx->nc, ctx->prec, L, p_view(ctx), ctx->J, ctx->stream), "particle current");
+    OPX_TRY(opx::launch_p_to_grid(ctx->nc, ctx->prec, L, p_view(ctx), w, ctx->J, ctx->stream), "particle current");
     ctx->j_nonzero = true;
     return OPX_OK;
 }

@@ -67,6 +67,11 @@ struct opx_ctx {
     double* p_vel = nullptr;
     void* p_Q0 = nullptr;
     void* p_Q1 = nullptr;
+    // work arrays of the ordered deposits (opx::ParticleSort), sized for ps_capacity particles
+    uint32_t* ps_buf = nullptr;     // keys, values and their sorted copies, 4 x ps_capacity
+    void* ps_temp = nullptr;
+    size_t ps_temp_bytes = 0;
+    uint32_t ps_capacity = 0;
     bool rho_pending = false;       // charge density deposit deferred until rho is read
     int rho_pending_t = 0;
     int steps = 0;                  // Simulation.totalSimulationSteps

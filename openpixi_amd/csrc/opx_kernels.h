@@ -103,11 +103,23 @@ hipError_t unitarity_flags(int nc, int prec, uint32_t n, const void* U, const vo
                            unsigned char* flags, unsigned long long* count, hipStream_t st);
 
 // Batched events: persistent grid of `grid` workgroups over total_items work items.
-hipError_t launch_p_chargedensity(int nc, int prec, const Lattice& L, const ParticleArrays& p, void* rho,
-                                  hipStream_t st);
+// Work arrays of the ordered per-particle deposits: (target, particle) keys and values, their
+// sorted copies (count entries each) and the radix sort's temporary storage
+struct ParticleSort {
+    uint32_t* keys;
+    uint32_t* vals;
+    uint32_t* keys_sorted;
+    uint32_t* vals_sorted;
+    void* temp;
+    size_t temp_bytes;
+};
+size_t particle_sort_temp_bytes(uint32_t count);
+hipError_t launch_p_chargedensity(int nc, int prec, const Lattice& L, const ParticleArrays& p, const ParticleSort& w,
+                                  void* rho, hipStream_t st);
 hipError_t launch_p_push(int nc, int prec, const Lattice& L, const ParticleArrays& p, double dt, const void* Unext,
                          hipStream_t st);
-hipError_t launch_p_to_grid(int nc, int prec, const Lattice& L, const ParticleArrays& p, void* J, hipStream_t st);
+hipError_t launch_p_to_grid(int nc, int prec, const Lattice& L, const ParticleArrays& p, const ParticleSort& w, void* J,
+                            hipStream_t st);
 hipError_t launch_batch_update(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items, uint32_t grid,
                                const void* U, void* Unext, void* E, hipStream_t st);
 hipError_t launch_batch_reunitarize(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items,

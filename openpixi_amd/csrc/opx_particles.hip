@@ -6,10 +6,15 @@
 // One thread per particle. Positions stay in double in every precision so that
 // the nearest grid point, (int) rint(pos / a) rounded half to even
 // (physics/util/GridFunctions.java:67-73), is bit-exact; charges are in the
-// context precision, [coefficient][particle]. Deposits onto rho and J are
-// atomic additions (several particles share a cell).
+// context precision, [coefficient][particle]. Deposits onto rho and J are ordered:
+// the (target, particle) pairs are sorted by target with a stable radix sort, so the
+// particles of one cell stay in list order, and the first entry of each run adds the
+// run's charges to the cell in that order, as the reference's sequential loop does
+// (bit for bit in FP64; no atomics, so identical runs give identical fields).
 #include "opx_group.h"
 #include "opx_kernels.h"
+
+#include <hipcub/device/device_radix_sort.hpp>
 
 namespace opx {
 
@@ -63,18 +68,34 @@ __device__ __forceinline__ void store_charge(T* __restrict__ Q, uint32_t count, 
     for (int m = 0; m < GR::AD; m++) Q[(size_t)m * count + k] = aget(a, m);
 }
 
-// interpolateChargedensity: rho(ngp(pos0)) += Q0
-template <int NC, typename T>
-__global__ __launch_bounds__(kPBlock) void k_p_chargedensity(Lattice L, uint32_t count,
-                                                             const double* __restrict__ pos0,
-                                                             const T* __restrict__ Q0, T* __restrict__ rho)
+// interpolateChargedensity, key pass: particle k deposits Q0 onto rho(ngp(pos0))
+__global__ __launch_bounds__(kPBlock) void k_p_rho_keys(Lattice L, uint32_t count, const double* __restrict__ pos0,
+                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
-    using GR = Group<NC, T>;
     const uint32_t k = blockIdx.x * kPBlock + threadIdx.x;
     if (k >= count) return;
-    const uint32_t cell = cell_of(L, nearest_grid_point(pos0, count, k, L));
+    keys[k] = cell_of(L, nearest_grid_point(pos0, count, k, L));
+    vals[k] = k;
+}
+
+// rho(cell) += Q0 of the run of sorted entries that starts here, in particle order
+template <int NC, typename T>
+__global__ __launch_bounds__(kPBlock) void k_p_rho_gather(Lattice L, uint32_t count, const uint32_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ vals, const T* __restrict__ Q0,
+                                                          T* __restrict__ rho)
+{
+#pragma clang fp contract(off)
+    using GR = Group<NC, T>;
+    const uint32_t i = blockIdx.x * kPBlock + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t cell = keys[i];
+    if (i > 0 && keys[i - 1] == cell) return;
 #pragma unroll
-    for (int m = 0; m < GR::AD; m++) atomicAdd(rho + (size_t)m * L.n + cell, Q0[(size_t)m * count + k]);
+    for (int m = 0; m < GR::AD; m++) {
+        T v = rho[(size_t)m * L.n + cell];
+        for (uint32_t j = i; j < count && keys[j] == cell; j++) v += Q0[(size_t)m * count + vals[j]];
+        rho[(size_t)m * L.n + cell] = v;
+    }
 }
 
 // updatePosition, interpolateToParticle and updateCharge of one particle:
@@ -117,28 +138,57 @@ __global__ __launch_bounds__(kPBlock) void k_p_push(Lattice L, uint32_t count, c
     }
 }
 
-// interpolateToGrid: on a crossing, J_d(old cell) += Q0 a_d / a_t (v > 0) or
-// J_d(new cell) += -Q1 a_d / a_t (v < 0)
-template <int NC, typename T>
-__global__ __launch_bounds__(kPBlock) void k_p_to_grid(Lattice L, uint32_t count, const int* __restrict__ dir,
-                                                       const double* __restrict__ pos0,
-                                                       const double* __restrict__ pos1,
-                                                       const double* __restrict__ vel, const T* __restrict__ Q0,
-                                                       const T* __restrict__ Q1, T* __restrict__ J)
+// interpolateToGrid, key pass: on a crossing, particle k deposits into J_d(old cell)
+// (v > 0) or J_d(new cell) (v < 0), key d n + cell; the others get the key 3 n (after
+// every deposit in the sorted order)
+__global__ __launch_bounds__(kPBlock) void k_p_j_keys(Lattice L, uint32_t count, const int* __restrict__ dir,
+                                                      const double* __restrict__ pos0, const double* __restrict__ pos1,
+                                                      const double* __restrict__ vel, uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals)
 {
-    using GR = Group<NC, T>;
     const uint32_t k = blockIdx.x * kPBlock + threadIdx.x;
     if (k >= count) return;
     const int d = dir[k];
     const Ngp no = nearest_grid_point(pos0, count, k, L), nn = nearest_grid_point(pos1, count, k, L);
-    if (no.c[d] == nn.c[d]) return;
-    const bool forward = vel[(size_t)d * count + k] > 0;
-    const uint32_t cell = cell_of(L, forward ? no : nn);
-    const T f = forward ? (T)(L.as[d] / L.at) : (T)(-L.as[d] / L.at);
-    const T* Q = forward ? Q0 : Q1;
+    uint32_t key = 3 * L.n;
+    if (no.c[d] != nn.c[d]) {
+        const bool forward = vel[(size_t)d * count + k] > 0;
+        key = (uint32_t)d * L.n + cell_of(L, forward ? no : nn);
+    }
+    keys[k] = key;
+    vals[k] = k;
+}
+
+// J_d(cell) += Q0 a_d / a_t (v > 0) or -Q1 a_d / a_t (v < 0) for the run of sorted entries that
+// starts here, in particle order
+template <int NC, typename T>
+__global__ __launch_bounds__(kPBlock) void k_p_j_gather(Lattice L, uint32_t count, const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ vals,
+                                                        const double* __restrict__ vel, const T* __restrict__ Q0,
+                                                        const T* __restrict__ Q1, T* __restrict__ J)
+{
+#pragma clang fp contract(off)
+    using GR = Group<NC, T>;
+    const uint32_t i = blockIdx.x * kPBlock + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t key = keys[i];
+    if (key >= 3 * L.n || (i > 0 && keys[i - 1] == key)) return;
+    const int d = (int)(key / L.n);
+    const uint32_t cell = key - (uint32_t)d * L.n;
+    const T fp = (T)(L.as[d] / L.at), fm = (T)(-L.as[d] / L.at);
     T* Jc = J + (size_t)d * GR::AD * L.n + cell;
 #pragma unroll
-    for (int m = 0; m < GR::AD; m++) atomicAdd(Jc + (size_t)m * L.n, Q[(size_t)m * count + k] * f);
+    for (int m = 0; m < GR::AD; m++) {
+        T v = Jc[(size_t)m * L.n];
+        for (uint32_t j = i; j < count && keys[j] == key; j++) {
+            const uint32_t k = vals[j];
+            const bool forward = vel[(size_t)d * count + k] > 0;
+            const T q = forward ? Q0[(size_t)m * count + k] : Q1[(size_t)m * count + k];
+            const T c = q * (forward ? fp : fm);
+            v += c;
+        }
+        Jc[(size_t)m * L.n] = v;
+    }
 }
 
 #define OPX_PDISPATCH(nc, prec, ...)                                                           \
@@ -153,13 +203,34 @@ inline dim3 pgrid(uint32_t count) { return dim3((count + kPBlock - 1) / kPBlock)
 
 }  // namespace
 
-hipError_t launch_p_chargedensity(int nc, int prec, const Lattice& L, const ParticleArrays& p, void* rho,
-                                  hipStream_t st)
+size_t particle_sort_temp_bytes(uint32_t count)
+{
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)count, 0, 32) !=
+        hipSuccess)
+        return 0;
+    return bytes;
+}
+
+static hipError_t sort_pairs(const ParticleSort& w, uint32_t count, hipStream_t st)
+{
+    size_t bytes = w.temp_bytes;
+    return hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, w.keys, w.keys_sorted, w.vals, w.vals_sorted, (int)count,
+                                              0, 32, st);
+}
+
+hipError_t launch_p_chargedensity(int nc, int prec, const Lattice& L, const ParticleArrays& p, const ParticleSort& w,
+                                  void* rho, hipStream_t st)
 {
     if (p.count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_p_rho_keys, pgrid(p.count), dim3(kPBlock), 0, st, L, p.count, p.pos0, w.keys, w.vals);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = sort_pairs(w, p.count, st);
+    if (e != hipSuccess) return e;
     OPX_PDISPATCH(nc, prec, {
-        hipLaunchKernelGGL((k_p_chargedensity<NC, T>), pgrid(p.count), dim3(kPBlock), 0, st, L, p.count, p.pos0,
-                           (const T*)p.Q0, (T*)rho);
+        hipLaunchKernelGGL((k_p_rho_gather<NC, T>), pgrid(p.count), dim3(kPBlock), 0, st, L, p.count, w.keys_sorted,
+                           w.vals_sorted, (const T*)p.Q0, (T*)rho);
     });
     return hipGetLastError();
 }
@@ -175,12 +246,18 @@ hipError_t launch_p_push(int nc, int prec, const Lattice& L, const ParticleArray
     return hipGetLastError();
 }
 
-hipError_t launch_p_to_grid(int nc, int prec, const Lattice& L, const ParticleArrays& p, void* J, hipStream_t st)
+hipError_t launch_p_to_grid(int nc, int prec, const Lattice& L, const ParticleArrays& p, const ParticleSort& w, void* J,
+                            hipStream_t st)
 {
     if (p.count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_p_j_keys, pgrid(p.count), dim3(kPBlock), 0, st, L, p.count, p.dir, p.pos0, p.pos1, p.vel,
+                       w.keys, w.vals);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = sort_pairs(w, p.count, st);
+    if (e != hipSuccess) return e;
     OPX_PDISPATCH(nc, prec, {
-        hipLaunchKernelGGL((k_p_to_grid<NC, T>), pgrid(p.count), dim3(kPBlock), 0, st, L, p.count, p.dir, p.pos0,
-                           p.pos1, p.vel, (const T*)p.Q0, (const T*)p.Q1, (T*)J);
+        hipLaunchKernelGGL((k_p_j_gather<NC, T>), pgrid(p.count), dim3(kPBlock), 0, st, L, p.count, w.keys_sorted,
+                           w.vals_sorted, p.vel, (const T*)p.Q0, (const T*)p.Q1, (T*)J);
     });
     return hipGetLastError();
 }

@@ -141,3 +141,27 @@ def test_per_particle_fp32(gpu, nc):
     assert rel_err(qd1, qo1) < 1e-5
     assert m0[6] > 1e-3
     assert abs(m1[6] - m0[6]) < 1e-5 * m0[6]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_per_particle_deposits_are_deterministic(gpu, nc, precision):
+    """The per-particle charge density and current are ordered gathers (a stable sort of the
+    particles by target cell, each cell summed in particle order): two runs from the same state
+    give bit-identical rho and J, with several particles per cell (ppc 4)."""
+    dims, ppc = (16, 6, 5), 4
+    runs = []
+    for _ in range(2):
+        og, dg = setup_particle_pair(nc, dims, ppc, seed=7, precision=precision)
+        dg.simulationInitialize()
+        dg.simulationStep(3 * ppc + 1)
+        runs.append((dg.getRho(), dg.getJ(), dg.getE()))
+    (r0, j0, e0), (r1, j1, e1) = runs
+    assert np.max(np.abs(r0)) > 0 and np.max(np.abs(j0)) > 0
+    assert np.array_equal(r0, r1) and np.array_equal(j0, j1) and np.array_equal(e0, e1)
+    # cells do hold several particles' charges
+    from .test_oracle_cpic import particle_nuclei
+
+    pos0 = particle_nuclei(nc, dims, ppc, 7)[1]
+    cells = [tuple(int(np.rint(x)) % n for x, n in zip(p, dims)) for p in np.asarray(pos0).reshape(-1, 3)]
+    assert max(cells.count(c) for c in set(cells)) >= ppc
