@@ -108,6 +108,8 @@ def lib():
             "orc_grid_set_threads": (None, [gp, ctypes.c_int]),
             "orc_shift": (ctypes.c_int64, [gp, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
             "orc_set_active_region": (None, [gp, _ip, _ip]),
+            "orc_refine_list": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int]),
             "orc_restore_violation": (ctypes.c_double, [gp, _dp]),
             "orc_restore_correct": (None, [gp, _dp, ctypes.c_double]),
             "orc_gauss_restore": (ctypes.c_int, [gp, ctypes.c_double, ctypes.c_int, ctypes.c_double, _ip, _dp, _dp,
@@ -272,6 +274,14 @@ def alg_from_coeffs(nc, c):
     o = np.zeros(asize(nc))
     lib().orc_alg_from_coeffs(nc, _ptr(np.ascontiguousarray(c, np.float64)), _ptr(o))
     return o
+
+
+def refine_list(q, ppc, skip, order, sweeps):
+    """The light-cone creators' refinement sweeps on one periodic list q[nn][A] (returns a copy)."""
+    q = np.array(q, np.float64, order="C", copy=True)
+    nn, A = q.shape
+    lib().orc_refine_list(_ptr(q.reshape(-1)), nn, A, int(ppc), int(skip), int(order), int(sweeps))
+    return q
 
 
 def links_from_coeffs(nc, coeffs):

@@ -94,6 +94,8 @@ double orc_wilson_tadpole(int nc, long T, const double *VT);
 void orc_wilson_dipole(int nc, int n1, int n2, double a1, double a2, const double *VT, int nbins, double *corr,
                        long *counts);
 void orc_lightcone_block(const orc_grid *gr, const double *gauss, int *z_start, int *block_width);
+/* one list's refinement sweeps (order 2 or 4), as both light-cone creators apply them */
+void orc_refine_list(double *q, int nn, int A, int ppc, int skip, int order, int sweeps);
 void orc_lightcone_superparticles(const orc_grid *gr, const double *gauss, int orientation, int z_start,
                                   int block_width, int subdivisions, int ppc, int *sp_orientation, int *sp_num,
                                   int *sp_index_offset, int *sp_per_plane, int *sp_shift, int *sp_ppc, double *charges);

@@ -407,6 +407,57 @@ void orc_lightcone_block(const orc_grid *gr, const double *gauss, int *z_start, 
 
 static int pmod(int i, int n) { return (i % n + n) % n; }
 
+/* The refinement sweeps of LightConeNGPSuperParticleCreator.refine2/refine4 (:245-300) and
+ * LightConeNGPParticleCreator over one periodic list of nn charges (q[i*A .. i*A+A)): `sweeps`
+ * passes of increasing i, each moving dq between entries i and i+1 where (i + skip) % ppc < ppc - 1
+ * (skip = ppc/2 for super particles, 0 for particles): order 2 with dq the (-1, 3, -3, 1)/4
+ * combination of entries i-1 .. i+2, order 4 with (1, -5, 10, -10, 5, -1)/12 of i-2 .. i+3. */
+void orc_refine_list(double *q, int nn, int A, int ppc, int skip, int order, int sweeps)
+{
+    for (int it = 0; it < sweeps; it++)
+        for (int i = 0; i < nn; i++) {
+            const int jm = (i + skip) % ppc;
+            if (!(jm >= 0 && jm < ppc - 1)) continue;
+            if (order == 2) {
+                double *Q0 = q + pmod(i - 1, nn) * A, *Q1 = q + pmod(i, nn) * A;
+                double *Q2 = q + pmod(i + 1, nn) * A, *Q3 = q + pmod(i + 2, nn) * A;
+                for (int c = 0; c < A; c++) {
+                    double dq = Q0[c] * -1;
+                    dq += Q1[c] * 3;
+                    dq += Q2[c] * -3;
+                    dq += Q3[c] * 1;
+                    dq *= 1.0 / 4.0;
+                    Q1[c] += dq * -1.0;
+                    Q2[c] += dq * 1.0;
+                }
+            } else {
+                double *Q[6];
+                for (int m = 0; m < 6; m++) Q[m] = q + pmod(i - 2 + m, nn) * A;
+                for (int c = 0; c < A; c++) {
+                    double dq = Q[0][c] * +1;
+                    dq += Q[1][c] * -5;
+                    dq += Q[2][c] * +10;
+                    dq += Q[3][c] * -10;
+                    dq += Q[4][c] * +5;
+                    dq += Q[5][c] * -1;
+                    dq *= 1.0 / 12.0;
+                    Q[2][c] += dq * -1.0;
+                    Q[3][c] += dq * 1.0;
+                }
+            }
+        }
+}
+
+/* 2nd- then 4th-order refinement, 100 sweeps each, of a list held at positions lst[0 .. nn) of
+ * `charges` (gathered into tmp and scattered back) */
+static void refine_positions(double *charges, const long *lst, int nn, int A, int ppc, int skip, double *tmp)
+{
+    for (int i = 0; i < nn; i++) memcpy(tmp + (size_t)i * A, charges + lst[i] * A, sizeof(double) * A);
+    orc_refine_list(tmp, nn, A, ppc, skip, 2, 100);
+    orc_refine_list(tmp, nn, A, ppc, skip, 4, 100);
+    for (int i = 0; i < nn; i++) memcpy(charges + lst[i] * A, tmp + (size_t)i * A, sizeof(double) * A);
+}
+
 /* LightConeNGPSuperParticleCreator.initializeParticles (:104-245) for a given block: `subdivisions`
  * groups of ppc super particles (descriptor arrays of subdivisions * ppc entries) and their charges
  * (block_width * T * ppc algebra elements in super-particle order), refined to 2nd then 4th order
@@ -452,44 +503,9 @@ void orc_lightcone_superparticles(const orc_grid *gr, const double *gauss, int o
         offset += num;
     }
     /* refinement along each transverse cell's list */
-    for (long t = 0; t < T; t++) {
-        long *lst = list + t * nlong;
-        const int nn = (int)nlong;
-        for (int it = 0; it < 100; it++)
-            for (int i = 0; i < nn; i++) {
-                const int jm = (i + ppc / 2) % ppc;
-                if (!(jm >= 0 && jm < ppc - 1)) continue;
-                double *Q0 = charges + lst[pmod(i - 1, nn)] * A, *Q1 = charges + lst[pmod(i, nn)] * A;
-                double *Q2 = charges + lst[pmod(i + 1, nn)] * A, *Q3 = charges + lst[pmod(i + 2, nn)] * A;
-                for (int q = 0; q < A; q++) {
-                    double dq = Q0[q] * -1;
-                    dq += Q1[q] * 3;
-                    dq += Q2[q] * -3;
-                    dq += Q3[q] * 1;
-                    dq *= 1.0 / 4.0;
-                    Q1[q] += dq * -1.0;
-                    Q2[q] += dq * 1.0;
-                }
-            }
-        for (int it = 0; it < 100; it++)
-            for (int i = 0; i < nn; i++) {
-                const int jm = (i + ppc / 2) % ppc;
-                if (!(jm >= 0 && jm < ppc - 1)) continue;
-                double *Q[6];
-                for (int m = 0; m < 6; m++) Q[m] = charges + lst[pmod(i - 2 + m, nn)] * A;
-                for (int q = 0; q < A; q++) {
-                    double dq = Q[0][q] * +1;
-                    dq += Q[1][q] * -5;
-                    dq += Q[2][q] * +10;
-                    dq += Q[3][q] * -10;
-                    dq += Q[4][q] * +5;
-                    dq += Q[5][q] * -1;
-                    dq *= 1.0 / 12.0;
-                    Q[2][q] += dq * -1.0;
-                    Q[3][q] += dq * 1.0;
-                }
-            }
-    }
+    double *tmp = (double *)malloc(sizeof(double) * nlong * A);
+    for (long t = 0; t < T; t++) refine_positions(charges, list + t * nlong, (int)nlong, A, ppc, ppc / 2, tmp);
+    free(tmp);
     (void)nc;
     free(list);
 }
@@ -542,42 +558,9 @@ void orc_lightcone_particles(const orc_grid *gr, const double *gauss, int orient
             }
         }
     /* refinement along each transverse cell's list; refine2/refine4 use jmod = i % ppc here */
-    for (long t = 0; t < T; t++) {
-        long *lst = list + t * nlong;
-        const int nn = (int)nlong;
-        for (int it = 0; it < 100; it++)
-            for (int i = 0; i < nn; i++) {
-                if (!(i % ppc < ppc - 1)) continue;
-                double *Q0 = charges + lst[pmod(i - 1, nn)] * A, *Q1 = charges + lst[pmod(i, nn)] * A;
-                double *Q2 = charges + lst[pmod(i + 1, nn)] * A, *Q3 = charges + lst[pmod(i + 2, nn)] * A;
-                for (int q = 0; q < A; q++) {
-                    double dq = Q0[q] * -1;
-                    dq += Q1[q] * 3;
-                    dq += Q2[q] * -3;
-                    dq += Q3[q] * 1;
-                    dq *= 1.0 / 4.0;
-                    Q1[q] += dq * -1.0;
-                    Q2[q] += dq * 1.0;
-                }
-            }
-        for (int it = 0; it < 100; it++)
-            for (int i = 0; i < nn; i++) {
-                if (!(i % ppc < ppc - 1)) continue;
-                double *Q[6];
-                for (int m = 0; m < 6; m++) Q[m] = charges + lst[pmod(i - 2 + m, nn)] * A;
-                for (int q = 0; q < A; q++) {
-                    double dq = Q[0][q] * +1;
-                    dq += Q[1][q] * -5;
-                    dq += Q[2][q] * +10;
-                    dq += Q[3][q] * -10;
-                    dq += Q[4][q] * +5;
-                    dq += Q[5][q] * -1;
-                    dq *= 1.0 / 12.0;
-                    Q[2][q] += dq * -1.0;
-                    Q[3][q] += dq * 1.0;
-                }
-            }
-    }
+    double *tmp = (double *)malloc(sizeof(double) * nlong * A);
+    for (long t = 0; t < T; t++) refine_positions(charges, list + t * nlong, (int)nlong, A, ppc, 0, tmp);
+    free(tmp);
     free(list);
 }
 

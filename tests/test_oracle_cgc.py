@@ -282,3 +282,60 @@ def test_per_particle_creator_deposits_the_gauss_violation(nc, orientation):
     assert np.allclose(g2[(ze + 1) * T:], q2[(ze + 1) * T:], rtol=1e-9, atol=1e-30)
     # the refinement moved charge between the particles of a cell
     assert np.max(np.abs(q.reshape(-1, ppc, O.asize(nc)) - q.reshape(-1, ppc, O.asize(nc)).mean(axis=1, keepdims=True))) > 0
+
+
+def window_off_polynomial(q, i, lo, width, degree):
+    """Distance of the last value of the periodic window q[i+lo .. i+lo+width) from the polynomial of
+    `degree` through the others (numpy interpolation), relative to the list's scale."""
+    nn = len(q)
+    xs = np.arange(width, dtype=float)
+    ys = np.array([q[(i + lo + m) % nn] for m in range(width)])
+    coef = np.polyfit(xs[:-1], ys[:-1], degree)
+    return abs(np.polyval(coef, xs[-1]) - ys[-1]) / np.max(np.abs(q))
+
+
+@pytest.mark.parametrize("ppc,skip", [(4, 2), (4, 0), (2, 1), (8, 4)])
+def test_refinement_sweeps_converge_to_polynomial_windows(ppc, skip):
+    """LightConeNGPSuperParticleCreator.refine2 / refine4 (:245-300): each sweep moves charge between
+    neighbours i, i+1 where (i + skip) % ppc < ppc - 1 so that, at the fixed point, every such
+    window of 4 charges (i-1 .. i+2) lies on a quadratic (2nd order) and every window of 6
+    (i-2 .. i+3) on a quartic (4th order), the charges summing to the same total. Checked with
+    numpy polynomial interpolation of the windows, after enough sweeps to converge; a wrong
+    coefficient in either stencil moves the fixed point off the polynomials."""
+    nn = 10 * ppc
+    q0 = np.random.default_rng(ppc + skip).normal(size=(nn, 1))
+    eligible = [i for i in range(nn) if (i + skip) % ppc < ppc - 1]
+    q2 = O.refine_list(q0, ppc, skip, 2, 4000)
+    assert abs(q2.sum() - q0.sum()) < 1e-12 * np.abs(q0).sum()
+    assert max(window_off_polynomial(q2[:, 0], i, -1, 4, 2) for i in eligible) < 1e-9
+    q4 = O.refine_list(q2, ppc, skip, 4, 40000)
+    assert abs(q4.sum() - q0.sum()) < 1e-12 * np.abs(q0).sum()
+    assert max(window_off_polynomial(q4[:, 0], i, -2, 6, 4) for i in eligible) < 1e-9
+    # the starting list is far from both
+    assert max(window_off_polynomial(q0[:, 0], i, -2, 6, 4) for i in eligible) > 1e-2
+    # one sweep moves a generic list but leaves a fixed point where it is
+    assert np.max(np.abs(O.refine_list(q0, ppc, skip, 2, 1) - q0)) > 1e-3
+    assert np.max(np.abs(O.refine_list(q4, ppc, skip, 4, 1) - q4)) < 1e-9
+
+
+def test_creator_refinement_reaches_the_quartic_fixed_point():
+    """The charges the super-particle creator returns, arranged as each transverse cell's list (plane,
+    then shift), have every eligible 6-window on a quartic to the accuracy of the reference's 100
+    sweeps, and the ineligible ones do not."""
+    nc, dims = 2, (20, 3, 3)
+    og, rho, gauss = cgc_state(nc, dims, 1, seed=5)
+    arrs, charges, (zs, bw) = og.lightcone_superparticles(gauss, 1, subdivisions=2)
+    T, ppc = dims[1] * dims[2], int(arrs["ppc"][0])
+    nn = bw * ppc
+    lists = np.zeros((T, nn, 3))
+    for s_, c in enumerate(charges):
+        off, k = arrs["index_offset"][s_], arrs["shift"][s_]
+        for i in range(arrs["num"][s_]):
+            idx = off + i
+            lists[idx % T, (idx // T - zs) * ppc + k] = c[i]
+    eligible = [i for i in range(nn) if (i + ppc // 2) % ppc < ppc - 1]
+    other = [i for i in range(nn) if i not in eligible]
+    worst = max(window_off_polynomial(lists[t, :, a], i, -2, 6, 4) for t in range(T) for a in range(3) for i in eligible)
+    assert worst < 1e-5
+    assert max(window_off_polynomial(lists[t, :, a], i, -2, 6, 4) for t in range(T) for a in range(3)
+               for i in other) > 1e-3
