@@ -156,7 +156,10 @@ int opx_reset_charge(opx_ctx *ctx);
  * chunks (0 = derived from the CU count). opx_autotune times every candidate
  * on scratch copies of E and Unext (the lattice state is not changed), keeps
  * the fastest and returns its time per launch in *best_ms. */
-enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2 };
+/* OPX_OPT_GLOBAL_POINTERS 1: the per-link, cell and plaquette forms read their fields through
+ * plain global pointers, as they do by themselves for fields of 4 GiB or more (0, the default:
+ * buffer instructions below 4 GiB); for testing that path on small lattices. */
+enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2, OPX_OPT_GLOBAL_POINTERS = 3 };
 enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2, OPX_KERNEL_PLAQ = 3, OPX_KERNEL_TILE = 4,
        OPX_KERNEL_PLANE = 5, OPX_KERNEL_CELL = 6 };
 int opx_set_option(opx_ctx *ctx, int option, int value);

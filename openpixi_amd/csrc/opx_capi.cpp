@@ -55,6 +55,7 @@ opx::Lattice lattice(const opx_ctx* c, double at)
         L.as[i] = c->as[i];
     }
     L.g = c->g;
+    L.force_global = c->force_global ? 1u : 0u;
     return L;
 }
 
@@ -1255,6 +1256,10 @@ int opx_set_option(opx_ctx* ctx, int option, int value)
         if (value < 0 || value >= opx::kStreamTiles) return fail(ctx, OPX_E_ARG, "option: bad tile shape");
         ctx->tile = value;
         return OPX_OK;
+    case OPX_OPT_GLOBAL_POINTERS:
+        if (value != 0 && value != 1) return fail(ctx, OPX_E_ARG, "option: global pointers is 0 or 1");
+        ctx->force_global = value == 1;
+        return OPX_OK;
     case OPX_OPT_CHUNKS:
         if (value < 0) return fail(ctx, OPX_E_ARG, "option: negative chunk count");
         ctx->nchunks = value;
@@ -1270,6 +1275,7 @@ int opx_get_option(const opx_ctx* ctx, int option, int* value)
     switch (option) {
     case OPX_OPT_KERNEL: *value = ctx->kernel; return OPX_OK;
     case OPX_OPT_TILE: *value = ctx->tile; return OPX_OK;
+    case OPX_OPT_GLOBAL_POINTERS: *value = ctx->force_global ? 1 : 0; return OPX_OK;
     case OPX_OPT_CHUNKS: {
         opx::Lattice L = lattice(ctx, ctx->at);
         *value = ctx->nchunks > 0 ? ctx->nchunks : (int)auto_chunks(ctx, L, ctx->tile);

@@ -23,6 +23,7 @@ struct Lattice {
     double area[3];    // 1/a_i^2 (Grid.getStapleSum areaFactor)
     double as[3];      // lattice spacings
     double g;          // coupling
+    uint32_t force_global = 0;   // 1: the fused kernels read through plain pointers below 4 GiB too
 };
 
 // One event of a batch: its lattice, the element offsets of its link block

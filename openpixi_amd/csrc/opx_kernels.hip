@@ -1275,7 +1275,7 @@ hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box&
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const dim3 g1(3 * order.blocks_per_tile_plane * L.d0 * tiles);
-        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
+        const bool buf = !L.force_global && (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
         if (buf && has_j)
             hipLaunchKernelGGL((k_combined_update<NC, T, true, true>), g1, dim3(kBlock), 0, st, L, order, active,
                                (const T*)U, (T*)Unext, (T*)E, (const T*)J);
@@ -1320,7 +1320,7 @@ hipError_t launch_cell_update(int nc, int prec, const Lattice& L, const Box& act
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const dim3 g1(order.blocks_per_tile_plane * L.d0 * tiles);
-        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
+        const bool buf = !L.force_global && (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
         if (reunit)
             launch_cell_variant<NC, T, true>(L, active, order, g1, buf, U, Unext, E, J, has_j, st);
         else
@@ -1338,7 +1338,7 @@ hipError_t launch_plaq_update(int nc, int prec, const Lattice& L, const Box& act
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const uint32_t cell_blocks = order.blocks_per_tile_plane * L.d0 * tiles;
-        const bool buf = (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
+        const bool buf = !L.force_global && (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
         const T* Jt = has_j ? (const T*)J : (const T*)E;
         if (buf) {
             hipLaunchKernelGGL((k_plaquettes<NC, T, true>), dim3(cell_blocks), dim3(kBlock), 0, st, L, order,

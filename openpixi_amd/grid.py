@@ -196,6 +196,11 @@ class Grid:
         if chunks is not None:
             self._check(L.opx_set_option(self._h, _lib.OPT_CHUNKS, int(chunks)))
 
+    def setGlobalPointers(self, on=True):
+        """Read the fields through plain global pointers in the per-link, cell and plaquette forms,
+        as they do for fields of 4 GiB or more (testing that path on small lattices)."""
+        self._check(_lib.load().opx_set_option(self._h, _lib.OPT_GLOBAL_POINTERS, 1 if on else 0))
+
     def kernelConfig(self):
         L = _lib.load()
         out = []

@@ -583,3 +583,51 @@ def test_plane_form_fp32_matches_link_form(gpu, nc, shape):
         out.append((dg.getU(), dg.getE()))
     assert rel_err(out[1][0], out[0][0]) < 1e-5
     assert rel_err(out[1][1], out[0][1]) < 1e-4
+
+
+@pytest.mark.parametrize("kernel", [("link", None, None), ("cell", 0, None), ("plaq", None, None)],
+                         ids=["link", "cell", "plaq"])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_global_pointer_variants(gpu, nc, kernel):
+    """The plain-pointer variants of the fused kernels (taken for fields of 4 GiB or more), forced
+    on a small lattice with a current and an active region: FP64 within 1e-12 of the oracle, and
+    FP32 equal to the buffer-instruction variants (the same arithmetic, other loads): bit for bit in
+    the per-link and plaquette forms; in the cell form the compiler schedules the two differently
+    and fuses a few multiply-adds differently, so within float rounding (1e-6)."""
+    dims = (6, 5, 7)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(47)
+    J = rng.normal(0, 0.1, size=(n, 3, O.asize(nc)))
+    if nc == 3:
+        J = su3_current(n, 48, 0.1)
+    active = ((1, 0, 1), (4, 4, 5))
+    U, E = make_state(nc, dims, 0.1, 0.05, 49)
+    og = oracle_grid(nc, dims, U, E, J=J)
+    og.set_active_region(*active)
+    dg = device_grid(nc, dims, U, E, J=J)
+    dg.setKernel(*kernel)
+    dg.setActiveRegion(*active)
+    dg.setGlobalPointers(True)
+    og.step_links(0.25)
+    FastTYMSolver().stepLinks(dg, 0.25)
+    for _ in range(8):
+        og.store_fields()
+        og.step(0.25)
+        dg.storeFields()
+        FastTYMSolver().step(dg, 0.25)
+    assert rel_err(dg.getU(), og.U) < FP64_TOL and rel_err(dg.getE(), og.E) < FP64_TOL
+    from openpixi_amd import _lib
+
+    out = []
+    for forced in (True, False):
+        g = device_grid(nc, dims, U, E, precision="fp32", J=J)
+        g.setKernel(*kernel)
+        g.setActiveRegion(*active)
+        g.setGlobalPointers(forced)
+        FastTYMSolver().stepLinks(g, 0.25)
+        g._check(_lib.load().opx_leapfrog(g.handle, 8, 0))
+        out.append((g.getUnext(), g.getE()))
+    if kernel[0] == "cell":
+        assert rel_err(out[0][0], out[1][0]) < 1e-6 and rel_err(out[0][1], out[1][1]) < 1e-6
+    else:
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
