@@ -43,22 +43,43 @@ BYTES_PER_LINK = {"fp32": 48 + 32 + 32 + 48, "fp64": 144 + 64 + 64 + 144}
 # FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the counters read the
 # known bytes of a 1 GiB device-to-device copy. These are recorded figures, not measured by this
 # run: the JSON line carries them as roofline.traffic with roofline.traffic_source naming the
-# profile summary. Keyed by (precision, kernel form, lattice): (FETCH_SIZE KiB, WRITE_SIZE KiB, file).
-PROFILED_TRAFFIC = {("fp32", "link", DIMS): (283889.0, 491520.0, "profiles/k_combined_update_su3_128cube.md"),
-                    ("fp32", "cell", DIMS): (274979.0, 491520.0, "profiles/k_cell_update_su3_128cube.md"),
-                    ("fp64", "link", DIMS): (875904.0, 1277952.0, "profiles/k_combined_update_su3_128cube.md")}
+# profile summary. Keyed by (precision, kernel form, tile, lattice): (FETCH_SIZE KiB, WRITE_SIZE KiB, file).
+_PROFILE = "profiles/k_cell_update_su3_128cube.md"
+PROFILED_TRAFFIC = {("fp32", "cell", 0, DIMS): (272439.3, 491520.0, _PROFILE),
+                    ("fp32", "cell", 1, DIMS): (383548.5, 491520.0, _PROFILE),
+                    ("fp32", "link", 0, DIMS): (287201.4, 491520.0, _PROFILE),
+                    ("fp64", "cell", 1, DIMS): (1282185.2, 1278147.4, _PROFILE),
+                    ("fp64", "cell", 0, DIMS): (1575067.0, 1277995.4, _PROFILE),
+                    ("fp64", "link", 0, DIMS): (796190.8, 1277952.0, _PROFILE)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+# VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, same probe runs): with the launch
+# time this run measures, the share of the launch the VALU needs to issue them at the 2.4 GHz peak
+# clock (4 cycles per wave64 instruction on each of 1024 SIMDs).
+PROFILED_VALU = {("fp32", "cell", 0, DIMS): (121307136.0, _PROFILE),
+                 ("fp64", "cell", 1, DIMS): (163380791.6, _PROFILE)}
+SIMDS, PEAK_CLOCK_GHZ = 1024, 2.4
 
 
-def profiled_traffic(precision, kernel, dims):
+def profiled_traffic(precision, kernel_config, dims):
     """(bytes per launch, source description) of a recorded counter run, or (None, None)."""
-    rec = PROFILED_TRAFFIC.get((precision, kernel, tuple(dims)))
+    rec = PROFILED_TRAFFIC.get((precision, kernel_config["kernel"], kernel_config["tile"], tuple(dims)))
     if rec is None:
         return None, None
     fetch, write, src = rec
     return ((fetch * FETCH_SCALE + write * WRITE_SCALE) * 1024,
             f"{src}: rocprofv3 --pmc FETCH_SIZE {fetch:.0f} KiB x {FETCH_SCALE} + WRITE_SIZE {write:.0f} KiB x "
             f"{WRITE_SCALE} per dispatch (scales from a 1 GiB device copy); recorded, not measured by this run")
+
+
+def valu_issue(precision, kernel_config, dims, kernel_ms):
+    """VALU issue time of the recorded instruction count at peak clock, against this run's launch."""
+    rec = PROFILED_VALU.get((precision, kernel_config["kernel"], kernel_config["tile"], tuple(dims)))
+    if rec is None:
+        return None
+    insts, src = rec
+    ms = insts * 4 / SIMDS / (PEAK_CLOCK_GHZ * 1e9) * 1e3
+    return {"instructions": insts, "issue_ms_at_peak_clock": ms, "frac_of_launch": ms / kernel_ms,
+            "source": f"{src}: rocprofv3 --pmc SQ_INSTS_VALU per dispatch; recorded, not measured by this run"}
 
 
 def parse():
@@ -722,10 +743,11 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": profiled_traffic(main_form, r["kernel_config"]["kernel"], dims)[0],
-        "traffic_source": profiled_traffic(main_form, r["kernel_config"]["kernel"], dims)[1],
+        "traffic": profiled_traffic(main_form, r["kernel_config"], dims)[0],
+        "traffic_source": profiled_traffic(main_form, r["kernel_config"], dims)[1],
         "kernel": r["kernel_config"],
         "kernel_ms": r["kernel_ms"],
+        "valu_issue": valu_issue(main_form, r["kernel_config"], dims, r["kernel_ms"]),
         "bytes_per_link": BYTES_PER_LINK[main_form],
     }
     if cpic is not None:
