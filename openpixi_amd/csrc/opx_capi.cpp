@@ -401,9 +401,13 @@ void opx_destroy(opx_ctx* ctx)
     void* bufs[] = {ctx->U, ctx->Unext, ctx->E, ctx->J, ctx->rho, ctx->P, ctx->iU, ctx->iUn, ctx->iE,
                     ctx->partials, ctx->dout, ctx->counters, ctx->diag,
                     ctx->sp_desc, ctx->sp_qoff, ctx->sp_q, ctx->p_dir, ctx->p_pos0, ctx->p_pos1, ctx->p_vel,
-                    ctx->p_Q0, ctx->p_Q1, ctx->ps_buf, ctx->ps_temp};
+                    ctx->p_Q0, ctx->p_Q1, ctx->ps_buf, ctx->ps_temp, ctx->rs_dev};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    for (int k = 0; k < opx_ctx::kSegSlots; k++) {
+        if (ctx->rs_host[k]) (void)hipHostFree(ctx->rs_host[k]);
+        if (ctx->rs_copied[k]) (void)hipEventDestroy(ctx->rs_copied[k]);
+    }
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -1103,41 +1107,111 @@ int host_ngp_offset(int o, int off, int pp, int k, int ppc, int t)
     return cur + ((((t - k + ppc + ppc / 2 - 1) % ppc) < ppc / 2) ? 0 : pp);
 }
 
+// The cells the super particles sit on at the deposit's time, cut into segments that
+// one fixed set of particles covers (opx::CellSegments), written into the pinned
+// staging buffer and copied to the device on the context stream. A particle spans at
+// most as many segments as it has charges, so the lists hold at most sp_total entries.
+int upload_cell_segments(opx_ctx* ctx, int t, opx::CellSegments* cs)
+{
+    const int cnt = ctx->sp_count;
+    const int* h = ctx->sp_host.data();
+    std::vector<int64_t> lo(cnt), hi(cnt), qoff(cnt);
+    std::vector<int64_t> bp;
+    bp.reserve(2 * (size_t)cnt);
+    int64_t qo = 0;
+    for (int p = 0; p < cnt; p++) {
+        const int off = host_ngp_offset(h[p], h[2 * cnt + p], h[3 * cnt + p], h[4 * cnt + p], h[5 * cnt + p], t);
+        qoff[p] = qo;
+        qo += h[cnt + p];
+        int64_t a = off, b = (int64_t)off + h[cnt + p];
+        if (a < 0) a = 0;
+        if (b > ctx->n) b = ctx->n;
+        lo[p] = a;
+        hi[p] = b;
+        if (b > a) {
+            bp.push_back(a);
+            bp.push_back(b);
+            qoff[p] -= off;   // charge of particle p on cell c: q[qoff[p] + c]
+        }
+    }
+    std::sort(bp.begin(), bp.end());
+    bp.erase(std::unique(bp.begin(), bp.end()), bp.end());
+    const size_t nb = bp.size() > 0 ? bp.size() - 1 : 0;   // elementary segments [bp[s], bp[s+1])
+    std::vector<uint32_t> covered(nb + 1, 0);               // particles per segment
+    auto seg_of = [&](int64_t c) { return (size_t)(std::lower_bound(bp.begin(), bp.end(), c) - bp.begin()); };
+    std::vector<size_t> s0(cnt), s1(cnt);
+    for (int p = 0; p < cnt; p++) {
+        if (hi[p] <= lo[p]) continue;
+        s0[p] = seg_of(lo[p]);
+        s1[p] = seg_of(hi[p]);
+        for (size_t s = s0[p]; s < s1[p]; s++) covered[s]++;
+    }
+    // keep the covered segments; their lists in particle-list order
+    std::vector<uint32_t> first, cells_prefix(1, 0), list_prefix(1, 0), kept(nb, UINT32_MAX);
+    for (size_t s = 0; s < nb; s++) {
+        if (covered[s] == 0) continue;
+        kept[s] = (uint32_t)first.size();
+        first.push_back((uint32_t)bp[s]);
+        cells_prefix.push_back(cells_prefix.back() + (uint32_t)(bp[s + 1] - bp[s]));
+        list_prefix.push_back(list_prefix.back() + covered[s]);
+    }
+    const size_t ns = first.size();
+    const size_t entries = list_prefix.back();
+    const size_t bytes = entries * sizeof(int64_t) + (3 * ns + 2) * sizeof(uint32_t);
+    *cs = opx::CellSegments{};
+    if (ns == 0) return OPX_OK;
+    if (bytes > ctx->rs_capacity) {
+        OPX_TRY(hipStreamSynchronize(ctx->stream), "charge density segments sync");
+        if (ctx->rs_dev) (void)hipFree(ctx->rs_dev);
+        ctx->rs_dev = nullptr;
+        for (int k = 0; k < opx_ctx::kSegSlots; k++) {
+            if (ctx->rs_host[k]) (void)hipHostFree(ctx->rs_host[k]);
+            ctx->rs_host[k] = nullptr;
+        }
+        ctx->rs_capacity = 0;
+        const size_t cap = bytes + bytes / 2 + 256;
+        OPX_TRY(hipMalloc(&ctx->rs_dev, cap), "alloc charge density segments");
+        for (int k = 0; k < opx_ctx::kSegSlots; k++)
+            OPX_TRY(hipHostMalloc((void**)&ctx->rs_host[k], cap, hipHostMallocDefault), "alloc charge density staging");
+        ctx->rs_capacity = cap;
+    }
+    // the slot's previous copy (kSegSlots deposits ago) must be done before it is rewritten
+    const int slot = ctx->rs_slot;
+    ctx->rs_slot = (slot + 1) % opx_ctx::kSegSlots;
+    if (ctx->rs_copied[slot]) OPX_TRY(hipEventSynchronize(ctx->rs_copied[slot]), "charge density segments wait");
+    else OPX_TRY(hipEventCreateWithFlags(&ctx->rs_copied[slot], hipEventDisableTiming), "charge density event");
+    unsigned char* stage = ctx->rs_host[slot];
+    int64_t* qb = (int64_t*)stage;
+    uint32_t* u = (uint32_t*)(stage + entries * sizeof(int64_t));
+    std::copy(first.begin(), first.end(), u);
+    std::copy(cells_prefix.begin(), cells_prefix.end(), u + ns);
+    std::copy(list_prefix.begin(), list_prefix.end(), u + 2 * ns + 1);
+    std::vector<uint32_t> cursor(list_prefix.begin(), list_prefix.end() - 1);
+    for (int p = 0; p < cnt; p++) {
+        if (hi[p] <= lo[p]) continue;
+        for (size_t s = s0[p]; s < s1[p]; s++) qb[cursor[kept[s]]++] = qoff[p];
+    }
+    OPX_TRY(hipMemcpyAsync(ctx->rs_dev, stage, bytes, hipMemcpyHostToDevice, ctx->stream),
+            "upload charge density segments");
+    OPX_TRY(hipEventRecord(ctx->rs_copied[slot], ctx->stream), "charge density event record");
+    const uint32_t* ud = (const uint32_t*)(ctx->rs_dev + entries * sizeof(int64_t));
+    cs->count = (uint32_t)ns;
+    cs->cells = cells_prefix.back();
+    cs->first = ud;
+    cs->cells_prefix = ud + ns;
+    cs->list_prefix = ud + 2 * ns + 1;
+    cs->qbase = (const int64_t*)ctx->rs_dev;
+    return OPX_OK;
+}
+
 int materialize_rho(opx_ctx* ctx)
 {
     if (!ctx->rho_pending) return OPX_OK;
     opx::Lattice L = lattice(ctx, ctx->at);
-    const int cnt = ctx->sp_count;
-    const int* h = ctx->sp_host.data();
-    // the cells the particles sit on now: union of their ranges, merged
-    std::vector<std::pair<int64_t, int64_t>> iv;
-    for (int p = 0; p < cnt; p++) {
-        const int off = host_ngp_offset(h[p], h[2 * cnt + p], h[3 * cnt + p], h[4 * cnt + p], h[5 * cnt + p],
-                                        ctx->rho_pending_t);
-        int64_t a = off, b = (int64_t)off + h[cnt + p];
-        if (a < 0) a = 0;
-        if (b > ctx->n) b = ctx->n;
-        if (b > a) iv.push_back({a, b});
-    }
-    std::sort(iv.begin(), iv.end());
-    std::vector<std::pair<int64_t, int64_t>> merged;
-    for (const auto& r : iv) {
-        if (!merged.empty() && r.first <= merged.back().second) merged.back().second = std::max(merged.back().second, r.second);
-        else merged.push_back(r);
-    }
-    if ((int)merged.size() > opx::kMaxCellRanges) {   // one covering range instead
-        const int64_t lo = merged.front().first, hi = merged.back().second;
-        merged.assign(1, {lo, hi});
-    }
-    opx::CellRanges cr{};
-    cr.count = (uint32_t)merged.size();
-    cr.prefix[0] = 0;
-    for (size_t r = 0; r < merged.size(); r++) {
-        cr.start[r] = (uint32_t)merged[r].first;
-        cr.prefix[r + 1] = cr.prefix[r] + (uint32_t)(merged[r].second - merged[r].first);
-    }
-    OPX_TRY(opx::launch_sp_chargedensity(ctx->nc, ctx->prec, L, sp_view(ctx), ctx->rho_pending_t, ctx->sp_q, ctx->rho,
-                                         cr, ctx->stream),
+    opx::CellSegments cs;
+    int rc = upload_cell_segments(ctx, ctx->rho_pending_t, &cs);
+    if (rc != OPX_OK) return rc;
+    OPX_TRY(opx::launch_sp_chargedensity(ctx->nc, ctx->prec, L, ctx->sp_total, ctx->sp_q, ctx->rho, cs, ctx->stream),
             "charge density");
     ctx->rho_pending = false;
     return OPX_OK;

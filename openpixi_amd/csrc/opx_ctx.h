@@ -72,6 +72,16 @@ struct opx_ctx {
     void* ps_temp = nullptr;
     size_t ps_temp_bytes = 0;
     uint32_t ps_capacity = 0;
+    // cell segments of the super-particle charge-density gather (opx::CellSegments), rebuilt
+    // per deposit: a ring of pinned host staging slots, each with the event that marks its
+    // copy done (the host runs up to kSegSlots deposits ahead), and one device copy (stream
+    // order keeps a copy behind the previous deposit's kernel)
+    static constexpr int kSegSlots = 4;
+    unsigned char* rs_host[kSegSlots] = {};
+    hipEvent_t rs_copied[kSegSlots] = {};
+    int rs_slot = 0;
+    unsigned char* rs_dev = nullptr;
+    size_t rs_capacity = 0;         // bytes of the device copy and of each slot
     bool rho_pending = false;       // charge density deposit deferred until rho is read
     int rho_pending_t = 0;
     int steps = 0;                  // Simulation.totalSimulationSteps

@@ -96,12 +96,23 @@ OPX_HD Alg2<T> proj(const Su2<T>& a)
     return Alg2<T>{{2 * a.e1, 2 * a.e2, 2 * a.e3}};
 }
 
-// exp(i * f * sum_a v_a sigma_a / 2) (SU2AlgebraElement.getLink applied to v*f)
+// exp(i * f * sum_a v_a sigma_a / 2) (SU2AlgebraElement.getLink applied to v*f): with
+// m = |w| / 2, e0 = cos m and e_a = w_a sin(m) / (2 m). In float, for m^2 <= kExpPolySu2M2
+// both are their Taylor series in m^2 through m^8 (no square root, divide or sine); the
+// first omitted terms, m^10 / 10! and m^10 / 11!, are below 3e-10 there.
+constexpr double kExpPolySu2M2 = 0.25;
 template <typename T>
 OPX_HD Su2<T> exp_link(const Alg2<T>& v, T f)
 {
     T w0 = v.v[0] * f, w1 = v.v[1] * f, w2 = v.v[2] * f;
     T sum = w0 * w0 + w1 * w1 + w2 * w2;
+    const T m2 = sum * T(0.25);
+    if (sizeof(T) == 4 && m2 <= T(kExpPolySu2M2)) {
+        const T c = T(1) + m2 * (T(-1.0 / 2) + m2 * (T(1.0 / 24) + m2 * (T(-1.0 / 720) + m2 * T(1.0 / 40320))));
+        const T h = T(0.5) + m2 * (T(-0.5 / 6) + m2 * (T(0.5 / 120) + m2 * (T(-0.5 / 5040) + m2 * T(0.5 / 362880))));
+        const T sinfakt = (m2 < T(1e-40)) ? T(0) : h;
+        return Su2<T>{c, w0 * sinfakt, w1 * sinfakt, w2 * sinfakt};
+    }
     T mod = fsqrt(sum) / 2;
     T sinfakt = (mod < T(1e-20)) ? T(0) : T(0.5) / mod * fsin(mod);
     return Su2<T>{fcos(mod), w0 * sinfakt, w1 * sinfakt, w2 * sinfakt};

@@ -13,6 +13,7 @@
 
 #include "opx_group.h"
 #include "opx_kernels.h"
+#include "opx_order.h"
 
 namespace opx {
 
@@ -343,20 +344,24 @@ __device__ __forceinline__ bool in_region(const Box& b, const uint32_t x[3])
            (int)x[2] >= b.lo[2] && (int)x[2] <= b.hi[2];
 }
 
-// ImplicitStep + ImplicitDamping for link (site, d = blockIdx.y). Time levels of the links:
-// -1 = Uold (the links before U), 0 = U, +1 = iU (the current implicit estimate).
+// ImplicitStep + ImplicitDamping for one link. Time levels of the links: -1 = Uold (the links
+// before U), 0 = U, +1 = iU (the current implicit estimate). Blocks run in the row-tile order
+// of opx_order.h with the direction fastest, so the neighbour links of the three fields that a
+// link reads are mostly still in L2 from the blocks just before it.
 template <int NC, typename T, bool HAS_J>
-__global__ __launch_bounds__(kIB) void k_implicit_iteration(Lattice L, Box active, const T* __restrict__ U,
-                                                            const T* __restrict__ Uold, const T* __restrict__ iU,
-                                                            const T* __restrict__ E, const T* __restrict__ J,
-                                                            T* __restrict__ iE, T* __restrict__ iUn, T damping)
+__global__ __launch_bounds__(kIB, 2) void k_implicit_iteration(Lattice L, LinkOrder order, Box active,
+                                                            const T* __restrict__ U, const T* __restrict__ Uold,
+                                                            const T* __restrict__ iU, const T* __restrict__ E,
+                                                            const T* __restrict__ J, T* __restrict__ iE,
+                                                            T* __restrict__ iUn, T damping)
 {
     using GR = Group<NC, T>;
     using M = typename GR::M;
     using A = typename GR::A;
-    const uint32_t site = blockIdx.x * kIB + threadIdx.x;
-    const int d = blockIdx.y;
-    if (site >= L.n) return;
+    const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
+    const int d = (int)(logical % 3);
+    uint32_t site;
+    if (!tile_site<kIB>(L, order, logical / 3, site)) return;
     uint32_t x[3];
     site_coords(L, site, x);
     if (!in_region(active, x)) return;
@@ -464,14 +469,19 @@ hipError_t launch_implicit_iteration(int nc, int prec, const Lattice& L, const B
                                      const void* Uold, const void* iU, const void* E, const void* J, bool has_j,
                                      void* iE, void* iUn, double damping, hipStream_t st)
 {
-    const dim3 grid((L.n + kIB - 1) / kIB, 3);
+    // A cell reads the links of three fields (U, Uold, iU) in its own and six neighbouring
+    // cells: 8-row tiles keep two tile-planes of that in one XCD's L2 in FP64, 16 rows in FP32.
+    const LinkOrder order = make_link_order(L, prec == 0 ? 8 : 16, kIB);
+    const dim3 grid(3 * link_order_blocks(L, order));
     OPX_IDISPATCH(nc, prec, {
         if (has_j)
-            hipLaunchKernelGGL((k_implicit_iteration<NC, T, true>), grid, dim3(kIB), 0, st, L, active, (const T*)U,
-                               (const T*)Uold, (const T*)iU, (const T*)E, (const T*)J, (T*)iE, (T*)iUn, (T)damping);
+            hipLaunchKernelGGL((k_implicit_iteration<NC, T, true>), grid, dim3(kIB), 0, st, L, order, active,
+                               (const T*)U, (const T*)Uold, (const T*)iU, (const T*)E, (const T*)J, (T*)iE, (T*)iUn,
+                               (T)damping);
         else
-            hipLaunchKernelGGL((k_implicit_iteration<NC, T, false>), grid, dim3(kIB), 0, st, L, active, (const T*)U,
-                               (const T*)Uold, (const T*)iU, (const T*)E, (const T*)E, (T*)iE, (T*)iUn, (T)damping);
+            hipLaunchKernelGGL((k_implicit_iteration<NC, T, false>), grid, dim3(kIB), 0, st, L, order, active,
+                               (const T*)U, (const T*)Uold, (const T*)iU, (const T*)E, (const T*)E, (T*)iE, (T*)iUn,
+                               (T)damping);
     });
     return hipGetLastError();
 }

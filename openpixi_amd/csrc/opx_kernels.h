@@ -37,13 +37,19 @@ struct BatchEvent {
     uint32_t pad;
 };
 
-// Disjoint cell ranges [start[r], start[r] + prefix[r+1] - prefix[r]) covered by a
-// kernel, passed by value; prefix[count] cells in total.
-constexpr int kMaxCellRanges = 16;
-struct CellRanges {
-    uint32_t count;
-    uint32_t start[kMaxCellRanges];
-    uint32_t prefix[kMaxCellRanges + 1];
+// Cell segments of the super-particle charge-density gather. The cells any particle
+// sits on at time t are cut into segments of consecutive cells that one fixed set of
+// particles covers: segment s holds cells [first[s], first[s] + cells_prefix[s+1] -
+// cells_prefix[s]) and the particles list[list_prefix[s] .. list_prefix[s+1]) in
+// particle-list order, each as qbase = (first charge index) - (first cell), so that the
+// charge of that particle on cell c sits at q[qbase + c]. Built on the host per deposit.
+struct CellSegments {
+    uint32_t count;                  // segments
+    uint32_t cells;                  // cells_prefix[count]
+    const uint32_t* first;           // [count]
+    const uint32_t* cells_prefix;    // [count + 1]
+    const uint32_t* list_prefix;     // [count + 1]
+    const int64_t* qbase;            // [list_prefix[count]]
 };
 
 // Super-particle descriptors (device arrays of `count` entries).
@@ -219,9 +225,10 @@ hipError_t launch_health_alg(int nc, int prec, uint32_t n_values, const void* X,
                              unsigned long long* counters, hipStream_t st);
 hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t, void* q,
                              const void* Unext, void* J, hipStream_t st);
-// gathers rho over cells [lo, hi) only (the cells the particles can occupy at t)
-hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
-                                   const void* q, void* rho, const CellRanges& cr, hipStream_t st);
+// gathers rho over the cells of `cs` only (the cells the particles occupy at t); q is
+// [coef][q_total]
+hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, int64_t q_total, const void* q, void* rho,
+                                   const CellSegments& cs, hipStream_t st);
 
 // CGC initial condition with the beam along x0 (opx_cgc.hip). Work arrays are double,
 // [component][cell]; complex ones double2; group elements [real][cell].

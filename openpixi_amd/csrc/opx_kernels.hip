@@ -11,6 +11,7 @@
 #include "opx_buf.h"
 #include "opx_group.h"
 #include "opx_kernels.h"
+#include "opx_order.h"
 
 namespace opx {
 
@@ -27,12 +28,6 @@ __device__ __forceinline__ bool in_box(const Box& b, uint32_t x0, uint32_t x1, u
 
 struct Coord {
     uint32_t x0, x1, x2;
-};
-
-// block order of the per-link kernel (row tiles swept along x0)
-struct LinkOrder {
-    uint32_t rows;                   // x1 rows per tile
-    uint32_t blocks_per_tile_plane;  // blocks covering rows * d2 cells
 };
 
 __device__ __forceinline__ Coord coords(const Lattice& L, uint32_t site)
@@ -280,26 +275,14 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
     using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
     const Fld U(Up, L.n, 3 * Group<NC, T>::S), Unext(Unextp, L.n, 3 * Group<NC, T>::S);
     const Fld E(Ep, L.n, 3 * Group<NC, T>::AD), J(Jp, L.n, 3 * Group<NC, T>::AD);
-    // Block order: each x0 plane is cut into row tiles of `rows` rows (x1) by the
-    // full x2 extent; logical blocks run direction-fastest, then over the blocks of
-    // one tile-plane, then along x0, then over tiles. XCD k (= blockIdx % 8) takes
-    // a contiguous range of logical blocks, so it sweeps its tiles plane by plane
-    // and the x0 +- 1 neighbours of a block were (or are about to be) read by
-    // blocks a few dozen positions away on the same XCD, i.e. from its L2.
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
-    const uint32_t logical = xcd * q8 + (xcd < r8 ? xcd : r8) + (b >> 3);
+    // Block order (opx_order.h) with the direction fastest: the three links of a cell
+    // are updated by consecutive logical blocks. XCD k (= blockIdx % 8) sweeps its
+    // tiles plane by plane, so the x0 +- 1 neighbours of a block were (or are about to
+    // be) read by blocks a few dozen positions away on the same XCD, i.e. from its L2.
+    const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
     const int d = (int)(logical % 3);
-    const uint32_t rest = logical / 3;
-    const uint32_t sb = rest % order.blocks_per_tile_plane;
-    const uint32_t rest2 = rest / order.blocks_per_tile_plane;
-    const uint32_t x0b = rest2 % L.d0;
-    const uint32_t tile = rest2 / L.d0;
-    const uint32_t row0 = tile * order.rows;
-    const uint32_t tile_rows = (L.d1 - row0 < order.rows) ? L.d1 - row0 : order.rows;
-    const uint32_t in_tile = sb * kBlock + threadIdx.x;
-    if (in_tile >= tile_rows * L.d2) return;
-    const uint32_t site = x0b * L.s0 + row0 * L.d2 + in_tile;
+    uint32_t site;
+    if (!tile_site<kBlock>(L, order, logical / 3, site)) return;
     const Coord c = coords(L, site);
     if (!in_box(active, c.x0, c.x1, c.x2)) return;
     link_update<NC, T, HAS_J>(L, U, Unext, E, J, site, d);
@@ -463,27 +446,6 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
 // E and link updates of FastTYMSolver.CombinedUpdate.
 
 __device__ __forceinline__ int plane_of(int a, int b) { return a + b - 1; }   // (0,1)->0, (0,2)->1, (1,2)->2
-
-// Block order of kernel A: as the per-link kernel's, without the direction.
-__device__ __forceinline__ bool tile_site(const Lattice& L, const LinkOrder& order, uint32_t logical, uint32_t& site)
-{
-    const uint32_t sb = logical % order.blocks_per_tile_plane;
-    const uint32_t rest2 = logical / order.blocks_per_tile_plane;
-    const uint32_t x0b = rest2 % L.d0;
-    const uint32_t tile = rest2 / L.d0;
-    const uint32_t row0 = tile * order.rows;
-    const uint32_t tile_rows = (L.d1 - row0 < order.rows) ? L.d1 - row0 : order.rows;
-    const uint32_t in_tile = sb * kBlock + threadIdx.x;
-    if (in_tile >= tile_rows * L.d2) return false;
-    site = x0b * L.s0 + row0 * L.d2 + in_tile;
-    return true;
-}
-
-__device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nb)
-{
-    const uint32_t q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
-    return xcd * q8 + (xcd < r8 ? xcd : r8) + (b >> 3);
-}
 
 // one thread per cell, blocks in the per-link kernel's tile order without the direction;
 // REUNIT: the new links are reprojected onto the group in double before they are stored
@@ -1147,8 +1109,9 @@ __global__ __launch_bounds__(kBlock) void k_health_alg(uint32_t nv, const T* __r
 }
 
 // ---------------------------------------------------------------- CPIC: super particles
-// CGCSuperParticle.getCurrentOffset / getCurrentNGPOffset / needsUpdate
-// (physics/particles/CGCSuperParticle.java:71-105)
+// CGCSuperParticle.getCurrentOffset / needsUpdate
+// (physics/particles/CGCSuperParticle.java:71-77, 100-105); getCurrentNGPOffset is
+// evaluated on the host, where the charge-density segments are built.
 __device__ __forceinline__ int floor_div(int a, int b)
 {
     int q = a / b;
@@ -1159,12 +1122,6 @@ __device__ __forceinline__ int sp_current_offset(int o, int off, int pp, int k, 
 {
     if (o > 0) return off + ((t + k) / ppc) * pp;
     return off + floor_div(-t + k, ppc) * pp;
-}
-__device__ __forceinline__ int sp_ngp_offset(int o, int off, int pp, int k, int ppc, int t)
-{
-    int cur = sp_current_offset(o, off, pp, k, ppc, t);
-    if (o > 0) return cur + ((((t + k) % ppc) < ppc / 2) ? 0 : pp);
-    return cur + ((((t - k + ppc + ppc / 2 - 1) % ppc) < ppc / 2) ? 0 : pp);
 }
 __device__ __forceinline__ bool sp_needs_update(int o, int k, int ppc, int t)
 {
@@ -1222,35 +1179,33 @@ __global__ __launch_bounds__(kBlock) void k_sp_to_grid(Lattice L, SuperParticles
 // interpolateChargedensity (physics/grid/CGCSuperParticleInterpolationNGP.java:75-93)
 // as a gather: each cell sums the charges that sit on it in particle-list order,
 // the order of the sequential reference loop, with no atomics. Threads cover the
-// cell ranges in `cr` (the union of the particles' current cells) back to back.
+// segments of `cs` back to back; a thread finds its segment by bisection and adds
+// only the particles that cover it (O(cells x particles per cell)).
 template <int NC, typename T>
-__global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, SuperParticles sp, int t,
-                                                             const T* __restrict__ q, T* __restrict__ rho,
-                                                             CellRanges cr)
+__global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, int64_t q_total, const T* __restrict__ q,
+                                                             T* __restrict__ rho, CellSegments cs)
 {
     using GR = Group<NC, T>;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= cr.prefix[cr.count]) return;
-    int r = 0;
-    while (r + 1 < (int)cr.count && i >= cr.prefix[r + 1]) r++;
-    const uint32_t cell = cr.start[r] + (i - cr.prefix[r]);
+    if (i >= cs.cells) return;
+    uint32_t lo = 0, hi = cs.count;   // cells_prefix[lo] <= i < cells_prefix[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cs.cells_prefix[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t cell = cs.first[lo] + (i - cs.cells_prefix[lo]);
     typename GR::A a;
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) aref(a, m) = rho[(size_t)m * L.n + cell];
-    bool touched = false;
-    for (int p = 0; p < sp.count; p++) {
-        const int off = sp_ngp_offset(sp.orientation[p], sp.index_offset[p], sp.per_plane[p], sp.shift[p], sp.ppc[p], t);
-        const long j = (long)cell - off;
-        if (j < 0 || j >= sp.num[p]) continue;
-        const size_t qi = (size_t)sp.q_offset[p] + j;
+    const uint32_t e1 = cs.list_prefix[lo + 1];
+    for (uint32_t e = cs.list_prefix[lo]; e < e1; e++) {
+        const size_t qi = (size_t)(cs.qbase[e] + (int64_t)cell);
 #pragma unroll
-        for (int m = 0; m < GR::AD; m++) aref(a, m) += q[(size_t)m * sp.total + qi];
-        touched = true;
+        for (int m = 0; m < GR::AD; m++) aref(a, m) += q[(size_t)m * q_total + qi];
     }
-    if (touched) {
 #pragma unroll
-        for (int m = 0; m < GR::AD; m++) rho[(size_t)m * L.n + cell] = aref(a, m);
-    }
+    for (int m = 0; m < GR::AD; m++) rho[(size_t)m * L.n + cell] = aref(a, m);
 }
 
 dim3 grid_links(uint32_t n) { return dim3((n + kBlock - 1) / kBlock, 3, 1); }
@@ -1490,14 +1445,14 @@ hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperPart
     return hipGetLastError();
 }
 
-hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, const SuperParticles& sp, int t,
-                                   const void* q, void* rho, const CellRanges& cr, hipStream_t st)
+hipError_t launch_sp_chargedensity(int nc, int prec, const Lattice& L, int64_t q_total, const void* q, void* rho,
+                                   const CellSegments& cs, hipStream_t st)
 {
-    const uint32_t total = cr.prefix[cr.count];
-    if (sp.count == 0 || total == 0) return hipSuccess;
-    dim3 g((total + kBlock - 1) / kBlock);
+    if (cs.count == 0 || cs.cells == 0) return hipSuccess;
+    dim3 g((cs.cells + kBlock - 1) / kBlock);
     OPX_DISPATCH(nc, prec, {
-        hipLaunchKernelGGL((k_sp_chargedensity<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (const T*)q, (T*)rho, cr);
+        hipLaunchKernelGGL((k_sp_chargedensity<NC, T>), g, dim3(kBlock), 0, st, L, q_total, (const T*)q, (T*)rho,
+                           cs);
     });
     return hipGetLastError();
 }

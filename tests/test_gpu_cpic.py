@@ -165,3 +165,48 @@ def test_per_particle_deposits_are_deterministic(gpu, nc, precision):
     pos0 = particle_nuclei(nc, dims, ppc, 7)[1]
     cells = [tuple(int(np.rint(x)) % n for x, n in zip(p, dims)) for p in np.asarray(pos0).reshape(-1, 3)]
     assert max(cells.count(c) for c in set(cells)) >= ppc
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_sp_chargedensity_ragged_layout(gpu, nc):
+    """Charge density of super particles whose cell ranges are not plane aligned: lengths that
+    are not multiples of a plane, offsets anywhere (ranges cut at both lattice ends, ranges
+    that nest, overlap in part or leave gaps), both beams, several ppc and shifts. The device
+    gathers per cell segment in particle-list order, as the oracle's sequential loop adds."""
+    dims = (12, 5, 3)
+    pp = dims[1] * dims[2]
+    n = dims[0] * pp
+    rng = np.random.default_rng(11)
+    cols = dict(orientation=[], num=[], index_offset=[], per_plane=[], shift=[], ppc=[])
+    charges = []
+    for p in range(40):
+        ppc = int(rng.choice([1, 2, 4]))
+        cols["orientation"].append(int(rng.choice([1, -1])))
+        num = int(rng.integers(1, 3 * pp))
+        cols["num"].append(num)
+        cols["index_offset"].append(int(rng.integers(-pp, n)))
+        cols["per_plane"].append(pp)
+        cols["shift"].append(int(rng.integers(0, ppc)))
+        cols["ppc"].append(ppc)
+        charges.append(O.algs_from_coeffs(nc, rng.normal(0, 1, (num, nc * nc - 1))))
+    args = (cols["orientation"], cols["num"], cols["index_offset"], cols["per_plane"], cols["shift"], cols["ppc"])
+    U, E = make_state(nc, dims, 0.3, 0.0, seed=2)
+    og = O.OracleGrid(nc, dims, (1.0, 1.0, 1.0), 0.5, 1.0)
+    og.U[:] = U
+    og.E[:] = E
+    og.set_superparticles(*args, charges)
+    dg = device_grid(nc, dims, U, E, 0.5)
+    dg.setSuperParticles(*args, np.concatenate(charges))
+    print(f"ragged charge density nc={nc}", flush=True)
+    for t in (0, 1, 3, 7, 12):
+        og.reset_charge()
+        dg.resetCharge()
+        og.sp_interpolate_chargedensity(t)
+        dg.interpolateChargedensity(t)
+        ro, rd = og.rho, dg.getRho()
+        assert np.max(np.abs(ro)) > 0
+        assert np.array_equal(np.abs(ro).sum(axis=1) > 0, np.abs(rd).sum(axis=1) > 0), t
+        if nc == 2:
+            assert np.array_equal(rd, ro), t
+        else:
+            assert rel_err(rd, ro) < 1e-14, t

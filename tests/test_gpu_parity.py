@@ -127,6 +127,29 @@ def test_step_links_fp32_exponential_branches(gpu):
     assert rel_err(dg.getUnext(), og.Unext) < 2e-6
 
 
+@pytest.mark.parametrize("kernel", ["link", "cell", "plaq", "tile"])
+def test_fp32_su2_exponential_branches(gpu, kernel):
+    """FP32 SU(2) exponential on both sides of its series range: m^2 = (dt |E| / 2)^2 spans
+    0.005 to 2.3 (the Taylor series up to 0.25, sine and cosine above). The links of stepLinks
+    and of two fused steps agree with the FP64 oracle's getLink (cos m, sin m / m) to FP32
+    rounding."""
+    dims = (4, 6, 8)
+    U, E = make_state(2, dims, 0.3, 3.0, seed=41)
+    coeffs = np.array([[O.alg_coeffs(2, E[i, d]) for d in range(3)] for i in range(E.shape[0])])
+    m2 = 0.25 ** 2 / 4 * np.sum(coeffs ** 2, axis=2)
+    assert m2.min() < 0.05 and m2.max() > 1.0 and 0.2 < np.mean(m2 <= 0.25) < 0.8
+    og, dg = run_both(2, dims, 0, sigma_a=0.3, sigma_e=3.0, seed=41, precision="fp32", kernel=(kernel, None, None))
+    assert rel_err(dg.getUnext(), og.Unext) < 2e-6
+    solver = FastTYMSolver()
+    for _ in range(2):
+        og.store_fields()
+        og.step(0.25)
+        dg.storeFields()
+        solver.step(dg, 0.25)
+    assert rel_err(dg.getUnext(), og.Unext) < 1e-5
+    assert rel_err(dg.getE(), og.E) < 1e-5
+
+
 @pytest.mark.parametrize("nc", [2, 3])
 def test_measurements_parity(gpu, nc):
     dims = (6, 5, 7)
