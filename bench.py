@@ -106,6 +106,8 @@ def parse():
     p.add_argument("--fd-directions", type=int, default=256)
     p.add_argument("--probe-kernels", default="auto",
                    help="kernel configurations for --probe: comma list of link | stream:TILE:CHUNKS | auto")
+    p.add_argument("--probe-colors", type=int, default=NC, choices=(2, 3),
+                   help="gauge group of the --probe lattices: SU(2) or SU(3)")
     p.add_argument("--probe", action="store_true",
                    help="counter-collection run: a few fused-kernel launches per form plus a 1 GiB device copy "
                         "(the calibration kernel for FETCH_SIZE/WRITE_SIZE); prints no JSON line")
@@ -121,15 +123,15 @@ def parse_kernels(spec):
     return out
 
 
-def probe(dims, forms, kernels, reps=5):
+def probe(dims, forms, kernels, reps=5, nc=NC):
     """Launches for hardware-counter runs: `reps` fused updates per form and kernel
-    configuration, then a 1 GiB float32 device-to-device copy of known size."""
+    configuration (SU(nc) links), then a 1 GiB float32 device-to-device copy of known size."""
     import torch
     from openpixi_amd import _lib
 
     L = _lib.load()
     for f in forms:
-        g = make_grid(f, dims)
+        g = make_grid(f, dims, nc=nc)
         for k in kernels:
             g.setKernel(*k)
             for _ in range(reps):
@@ -147,15 +149,15 @@ def probe(dims, forms, kernels, reps=5):
     print(f"# probe copy: {reps} x 1 GiB read + 1 GiB write", file=sys.stderr, flush=True)
 
 
-def make_grid(precision, dims, seed=0):
+def make_grid(precision, dims, seed=0, nc=NC):
     from openpixi_amd import Grid, inputs
 
     n = int(np.prod(dims))
-    g = Grid(NC, dims, 1.0, DT, 1.0, precision)
-    A = inputs.random_links_coefficients(n, NC, 0.1, seed)
+    g = Grid(nc, dims, 1.0, DT, 1.0, precision)
+    A = inputs.random_links_coefficients(n, nc, 0.1, seed)
     g.setLinksFromCoefficients(A)
     del A
-    Ec = inputs.random_efield_coefficients(n, NC, 0.05, seed)
+    Ec = inputs.random_efield_coefficients(n, nc, 0.05, seed)
     g.setEFromCoefficients(Ec)
     del Ec
     return g
@@ -676,7 +678,7 @@ def main():
     dims = tuple(args.dims)
     forms = args.forms.split(",")
     if args.probe:
-        probe(dims, forms, parse_kernels(args.probe_kernels))
+        probe(dims, forms, parse_kernels(args.probe_kernels), nc=args.probe_colors)
         return
     stream = torch.cuda.Stream()
     results = {}
