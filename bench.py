@@ -56,7 +56,9 @@ FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 # time this run measures, the share of the launch the VALU needs to issue them at the 2.4 GHz peak
 # clock (4 cycles per wave64 instruction on each of 1024 SIMDs).
 PROFILED_VALU = {("fp32", "cell", 0, DIMS): (121307136.0, _PROFILE),
-                 ("fp64", "cell", 1, DIMS): (163380791.6, _PROFILE)}
+                 ("fp32", "cell", 1, DIMS): (121307136.0, _PROFILE),
+                 ("fp64", "cell", 1, DIMS): (163380791.6, _PROFILE),
+                 ("fp64", "cell", 0, DIMS): (163072564.4, _PROFILE)}
 SIMDS, PEAK_CLOCK_GHZ = 1024, 2.4
 
 
@@ -171,38 +173,50 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
     g = make_grid(precision, dims)
     g._check(L.opx_set_stream(g.handle, ctypes_void(stream.cuda_stream)))
     cg_iters, gauss_raw, gauss_proj = g.projectGauss(tol=1e-10, max_iter=4000)
-    tune_ms = g.autotune(reps=3) if autotune else None
-    kcfg = g.kernelConfig()
     reunit = REUNIT_EVERY if precision == "fp32" else 0
     # initial half step U(-dt/2) -> U(+dt/2), as Simulation.initialize
     g._check(L.opx_step_links(g.handle, DT))
     m0 = FieldMeasurements().measure(g)
+    # the autotuner times its candidates on copies of the fields; it runs last before the warm-up
+    # steps, so that the GPU is under load from then until the timed region
+    tune_ms = g.autotune(reps=3) if autotune else None
+    kcfg = g.kernelConfig()
     g._check(L.opx_leapfrog(g.handle, warmup, reunit))
     torch.cuda.synchronize()
-    # The timed region: `steps` calls of opx_leapfrog(1 step), HIP events on the kernels' stream
-    # around the whole region and around each step. A step is one launch of the fused kernel (the
-    # cell form reprojects the links inside the kernel on every 10th step); where a form runs the
-    # reprojection as a pass of its own, those steps are left out of the kernel's average.
+    # The timed region: `steps` calls of opx_leapfrog(1 step), a HIP event pair on the kernels'
+    # stream around the whole region. A step is one launch of the fused kernel (the cell form
+    # reprojects the links inside the kernel on every 10th step), so the region's average is the
+    # kernel's average launch duration, gaps between launches included. An event recorded between
+    # two launches holds the second back until the first has finished and its timestamp is written
+    # (about 10 us per step in a kernel trace), so per-launch events are recorded only for a form
+    # that runs the reprojection as a pass of its own, whose steps then leave out of the average.
     fused = kcfg["kernel"] == "cell"
-    step_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    per_launch = not fused and reunit != 0
+    step_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps if per_launch else 0)]
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
     for i in range(steps):
-        step_ev[i][0].record(stream)
+        if per_launch:
+            step_ev[i][0].record(stream)
         g._check(L.opx_leapfrog(g.handle, 1, reunit))
-        step_ev[i][1].record(stream)
+        if per_launch:
+            step_ev[i][1].record(stream)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = ev0.elapsed_time(ev1)
-    per_step = [a.elapsed_time(b) for a, b in step_ev]
-    counted = [t for i, t in enumerate(per_step)
-               if fused or reunit == 0 or (warmup + i + 1) % reunit != 0]
-    kms = sum(counted) / len(counted)
+    if per_launch:
+        per_step = [a.elapsed_time(b) for a, b in step_ev]
+        counted = [t for i, t in enumerate(per_step) if (warmup + i + 1) % reunit != 0]
+        kms = sum(counted) / len(counted)
+    else:
+        kms = ms / steps
     m1 = FieldMeasurements().measure(g)
-    # the fused update kernel alone, back to back on the same stream, after the timed region
+    # the fused update kernel alone, back to back on the same stream, after the timed region;
+    # then the same launches one by one between event pairs, for their spread
     k0 = torch.cuda.Event(enable_timing=True)
     k1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -213,6 +227,14 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
     k1.record(stream)
     torch.cuda.synchronize()
     kms_b2b = k0.elapsed_time(k1) / kernel_reps
+    one = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(kernel_reps)]
+    for a, b in one:
+        g._check(L.opx_store_fields(g.handle))
+        a.record(stream)
+        g._check(L.opx_step(g.handle, DT))
+        b.record(stream)
+    torch.cuda.synchronize()
+    single = [a.elapsed_time(b) for a, b in one]
     n = int(np.prod(dims))
     links = 3 * n
     energy0 = float(m0[0:6].sum())
@@ -222,8 +244,10 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
         "wall_ms_per_step": 1e3 * wall / steps,
         "link_updates_per_s": links * steps / (ms * 1e-3),
         "kernel_ms": kms,
-        "kernel_launches_timed": len(counted),
-        "kernel_ms_min_max": [min(counted), max(counted)],
+        "kernel_ms_source": ("HIP events around each launch in the timed region, reprojection steps left out"
+                             if per_launch else "HIP events around the timed region / steps"),
+        "kernel_ms_single_launches": {"count": len(single), "mean": sum(single) / len(single),
+                                      "min": min(single), "max": max(single)},
         "kernel_gbs": links * BYTES_PER_LINK[precision] / (kms * 1e-3) / 1e9,
         "kernel_ms_back_to_back": kms_b2b,
         "gauss_projection": {"cg_iterations": cg_iters, "before": gauss_raw, "after": gauss_proj},
