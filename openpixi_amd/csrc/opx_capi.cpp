@@ -56,7 +56,25 @@ opx::Lattice lattice(const opx_ctx* c, double at)
     }
     L.g = c->g;
     L.force_global = c->force_global ? 1u : 0u;
+    L.j_lo = (uint32_t)c->j_lo;
+    L.j_hi = (uint32_t)c->j_hi;
     return L;
+}
+
+// J may now be nonzero on x0 planes [lo, hi) as well
+void extend_current_planes(opx_ctx* c, int lo, int hi)
+{
+    lo = std::max(lo, 0);
+    hi = std::min(hi, c->dims[0]);
+    if (lo >= hi) return;
+    if (c->j_lo >= c->j_hi) {
+        c->j_lo = lo;
+        c->j_hi = hi;
+    } else {
+        c->j_lo = std::min(c->j_lo, lo);
+        c->j_hi = std::max(c->j_hi, hi);
+    }
+    c->j_nonzero = true;
 }
 
 size_t link_count(const opx_ctx* c) { return (size_t)c->n * 3 * c->S; }
@@ -468,7 +486,7 @@ int opx_upload_field(opx_ctx* ctx, int field, const double* host, int64_t n_valu
                 }
                 for (int k = 0; k < ctx->AD; k++) soa[((size_t)d * ctx->AD + k) * n + s] = c[k];
             }
-        if (field == OPX_FIELD_J) ctx->j_nonzero = true;
+        if (field == OPX_FIELD_J) extend_current_planes(ctx, 0, ctx->dims[0]);
         if (field == OPX_FIELD_RHO) ctx->rho_pending = false;
     }
     return to_device(ctx, dst, soa);
@@ -655,7 +673,14 @@ int opx_store_fields(opx_ctx* ctx)
 int opx_reset_current(opx_ctx* ctx)
 {
     if (!ctx) return OPX_E_ARG;
-    OPX_TRY(hipMemsetAsync(ctx->J, 0, alg_count(ctx) * ctx->tsz, ctx->stream), "reset current");
+    // J is zero outside planes [j_lo, j_hi): only they are cleared
+    if (ctx->j_lo < ctx->j_hi) {
+        const uint32_t s0 = (uint32_t)ctx->dims[1] * (uint32_t)ctx->dims[2];
+        OPX_TRY(opx::launch_zero_cells(ctx->prec, ctx->J, (uint32_t)ctx->n, 3 * ctx->AD, (uint32_t)ctx->j_lo * s0,
+                                       (uint32_t)(ctx->j_hi - ctx->j_lo) * s0, ctx->stream),
+                "reset current");
+    }
+    ctx->j_lo = ctx->j_hi = 0;
     ctx->j_nonzero = false;
     return OPX_OK;
 }
@@ -1057,7 +1082,7 @@ int opx_particles_interpolate_to_grid(opx_ctx* ctx)
     if (rc != OPX_OK) return rc;
     opx::Lattice L = lattice(ctx, ctx->at);
     OPX_TRY(opx::launch_p_to_grid(ctx->nc, ctx->prec, L, p_view(ctx), w, ctx->J, ctx->stream), "particle current");
-    ctx->j_nonzero = true;
+    extend_current_planes(ctx, 0, ctx->dims[0]);
     return OPX_OK;
 }
 
@@ -1228,7 +1253,25 @@ int opx_interpolate_to_grid(opx_ctx* ctx, int t)
     opx::Lattice L = lattice(ctx, ctx->at);
     OPX_TRY(opx::launch_sp_to_grid(ctx->nc, ctx->prec, L, sp_view(ctx), t, ctx->sp_q, ctx->Unext, ctx->J, ctx->stream),
             "interpolate to grid");
-    ctx->j_nonzero = true;
+    // the planes the deposit can touch: every particle's current cells (the cells after the
+    // ppc = 1 shift by one plane included), one plane of margin on either side
+    const int cnt = ctx->sp_count;
+    const int* h = ctx->sp_host.data();
+    const int64_t s0 = (int64_t)ctx->dims[1] * ctx->dims[2];
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int p = 0; p < cnt; p++) {
+        const int o = h[p], num = h[cnt + p], off = h[2 * cnt + p], pp = h[3 * cnt + p], k = h[4 * cnt + p],
+                  ppc = h[5 * cnt + p];
+        const int64_t cur = o > 0 ? off + (int64_t)((t + k) / ppc) * pp : off + (int64_t)host_floor_div(-t + k, ppc) * pp;
+        const int64_t a = cur, b = cur + num + (ppc == 1 ? pp : 0);
+        lo = std::min(lo, a);
+        hi = std::max(hi, b);
+    }
+    if (lo < hi) {
+        const int64_t plo = (std::max<int64_t>(lo, 0)) / s0 - 1;
+        const int64_t phi = (std::min<int64_t>(hi, ctx->n) + s0 - 1) / s0 + 1;
+        extend_current_planes(ctx, (int)std::max<int64_t>(plo, 0), (int)std::min<int64_t>(phi, ctx->dims[0]));
+    }
     return OPX_OK;
 }
 

@@ -43,6 +43,7 @@ struct opx_ctx {
     void* iUn = nullptr;
     void* iE = nullptr;
     bool j_nonzero = false;
+    int j_lo = 0, j_hi = 0;         // x0 planes [j_lo, j_hi) outside which J is zero (j_nonzero: j_lo < j_hi)
     opx::Box active{{0, 0, 0}, {0, 0, 0}, 1};
     opx::Box eval{{0, 0, 0}, {0, 0, 0}, 1};
     hipStream_t own_stream = nullptr;

@@ -24,6 +24,9 @@ struct Lattice {
     double as[3];      // lattice spacings
     double g;          // coupling
     uint32_t force_global = 0;   // 1: the fused kernels read through plain pointers below 4 GiB too
+    // x0 planes [j_lo, j_hi) outside which the current J is zero: the cell and per-link forms
+    // read J only there (every other reader reads all of it)
+    uint32_t j_lo = 0, j_hi = 0xffffffffu;
 };
 
 // One event of a batch: its lattice, the element offsets of its link block
@@ -219,6 +222,9 @@ hipError_t launch_projected_energy(int nc, int prec, const Lattice& L, const Box
 constexpr int kNoAllocMaxSlices = 4096;
 constexpr int kNoAllocMaxDevices = 64;
 hipError_t noalloc_scratch(int device, double** out);
+// zero components [0, comps) of cells [first, first + count) of a [component][n] field
+hipError_t launch_zero_cells(int prec, void* f, uint32_t n, uint32_t comps, uint32_t first, uint32_t count,
+                             hipStream_t st);
 hipError_t launch_health(int nc, int prec, uint32_t n, const void* U, double tol,
                          unsigned long long* counters, hipStream_t st);
 hipError_t launch_health_alg(int nc, int prec, uint32_t n_values, const void* X,

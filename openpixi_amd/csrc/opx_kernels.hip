@@ -254,7 +254,7 @@ __device__ __forceinline__ void link_update(const Lattice& L, const Fld& U, cons
     const T at = (T)L.at;
 #pragma unroll
     for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * at;
-    if (HAS_J) {
+    if (HAS_J && c.x0 >= L.j_lo && c.x0 < L.j_hi) {
         const A jj = fload_alg<NC, T>(J, site, d);
         const T uf = (T)L.uf[d];
 #pragma unroll
@@ -387,9 +387,10 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
     };
     const T at = (T)L.at;
     // E(x) of direction d plus the current term, read before any store of the cell
+    const bool in_j = HAS_J && c.x0 >= L.j_lo && c.x0 < L.j_hi;
     auto e_start = [&](int d) -> A {
         A e = fload_alg_once<NC, T>(E, site, d);
-        if (HAS_J) {
+        if (in_j) {
             const A jj = fload_alg<NC, T>(J, site, d);
             const T uf = (T)L.uf[d];
 #pragma unroll
@@ -1210,6 +1211,16 @@ __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, int64_t 
 
 dim3 grid_links(uint32_t n) { return dim3((n + kBlock - 1) / kBlock, 3, 1); }
 
+// component blockIdx.y of cells [first, first + count) of a [component][n] field set to zero
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_cells(T* __restrict__ f, uint32_t n, uint32_t first,
+                                                       uint32_t count)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    f[(size_t)blockIdx.y * n + first + i] = T(0);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- dispatch
@@ -1442,6 +1453,18 @@ hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperPart
         hipLaunchKernelGGL((k_sp_to_grid<NC, T>), g, dim3(kBlock), 0, st, L, sp, t, (T*)q, (const T*)Unext,
                            (T*)J);
     });
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_cells(int prec, void* f, uint32_t n, uint32_t comps, uint32_t first, uint32_t count,
+                             hipStream_t st)
+{
+    if (count == 0 || comps == 0) return hipSuccess;
+    const dim3 g((count + kBlock - 1) / kBlock, comps);
+    if (prec == 0)
+        hipLaunchKernelGGL((k_zero_cells<double>), g, dim3(kBlock), 0, st, (double*)f, n, first, count);
+    else
+        hipLaunchKernelGGL((k_zero_cells<float>), g, dim3(kBlock), 0, st, (float*)f, n, first, count);
     return hipGetLastError();
 }
 
