@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from openpixi_amd import FieldMeasurements
+from openpixi_amd import FastTYMSolver, FieldMeasurements
 
 from .helpers import device_grid, make_state, rel_err
 from .test_oracle_cpic import two_nuclei
@@ -55,6 +55,36 @@ def test_cpic_simulation_parity(gpu, nc, ppc, kernel):
     m_o, m_d = og.measure(), FieldMeasurements().measure(dg)
     for k in (0, 1, 2, 3, 4, 5, 6, 7):
         assert abs(m_d[k] - m_o[k]) <= 1e-11 * max(abs(m_o[k]), 1e-30), (k, m_d[k], m_o[k])
+
+
+@pytest.mark.parametrize("kernel", ["link", "cell", "plaq"])
+def test_current_everywhere_then_reset_then_particles(gpu, kernel):
+    """The device keeps J zero outside the x0 planes it was written on and reads it only there
+    (cell and per-link forms). A current uploaded on every cell enters the update everywhere;
+    resetCurrent then clears all of it; particle deposits afterwards match the oracle."""
+    nc, dims, ppc = 2, (14, 4, 3), 4
+    og, dg = setup_pair(nc, dims, ppc, seed=7, subdivisions=2, width_cells=2)
+    dg.setKernel(kernel)
+    J = np.random.default_rng(1).normal(0, 0.1, size=og.J.shape)
+    og.J[:] = J
+    dg.setJ(J)
+    og.store_fields()
+    og.step(og.at)
+    dg.storeFields()
+    FastTYMSolver().step(dg, og.at)
+    assert rel_err(dg.getE(), og.E) < 1e-12
+    og.reset_current()
+    dg.resetCurrent()
+    assert not np.any(dg.getJ())
+    og.initialize()
+    dg.simulationInitialize()
+    steps = 2 * ppc + 1
+    og.sim_step(steps)
+    assert dg.simulationStep(steps) == steps
+    assert np.max(np.abs(og.J)) > 0
+    assert rel_err(dg.getJ(), og.J) < 1e-12
+    assert rel_err(dg.getE(), og.E) < 1e-12
+    assert rel_err(dg.getU(), og.U) < 1e-12
 
 
 @pytest.mark.parametrize("kernel", ["link", "cell"])
