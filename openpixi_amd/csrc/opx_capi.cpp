@@ -56,24 +56,38 @@ opx::Lattice lattice(const opx_ctx* c, double at)
     }
     L.g = c->g;
     L.force_global = c->force_global ? 1u : 0u;
-    L.j_lo = (uint32_t)c->j_lo;
-    L.j_hi = (uint32_t)c->j_hi;
+    L.j_slabs = (uint32_t)c->j_slabs.size();
+    for (size_t i = 0; i < c->j_slabs.size(); i++) {
+        L.j_lo[i] = (uint32_t)c->j_slabs[i].first;
+        L.j_hi[i] = (uint32_t)c->j_slabs[i].second;
+    }
     return L;
 }
 
-// J may now be nonzero on x0 planes [lo, hi) as well
+// J may now be nonzero on x0 planes [lo, hi) as well: the slab list stays sorted and disjoint
+// (touching or overlapping ranges merged), and over kMaxCurrentSlabs ranges the two with the
+// smallest gap between them are joined
 void extend_current_planes(opx_ctx* c, int lo, int hi)
 {
     lo = std::max(lo, 0);
     hi = std::min(hi, c->dims[0]);
     if (lo >= hi) return;
-    if (c->j_lo >= c->j_hi) {
-        c->j_lo = lo;
-        c->j_hi = hi;
-    } else {
-        c->j_lo = std::min(c->j_lo, lo);
-        c->j_hi = std::max(c->j_hi, hi);
+    auto& v = c->j_slabs;
+    v.push_back({lo, hi});
+    std::sort(v.begin(), v.end());
+    std::vector<std::pair<int, int>> m;
+    for (const auto& r : v) {
+        if (!m.empty() && r.first <= m.back().second) m.back().second = std::max(m.back().second, r.second);
+        else m.push_back(r);
     }
+    while ((int)m.size() > opx::kMaxCurrentSlabs) {
+        size_t best = 0;
+        for (size_t i = 1; i + 1 < m.size(); i++)
+            if (m[i + 1].first - m[i].second < m[best + 1].first - m[best].second) best = i;
+        m[best].second = m[best + 1].second;
+        m.erase(m.begin() + best + 1);
+    }
+    v = m;
     c->j_nonzero = true;
 }
 
@@ -673,14 +687,13 @@ int opx_store_fields(opx_ctx* ctx)
 int opx_reset_current(opx_ctx* ctx)
 {
     if (!ctx) return OPX_E_ARG;
-    // J is zero outside planes [j_lo, j_hi): only they are cleared
-    if (ctx->j_lo < ctx->j_hi) {
-        const uint32_t s0 = (uint32_t)ctx->dims[1] * (uint32_t)ctx->dims[2];
-        OPX_TRY(opx::launch_zero_cells(ctx->prec, ctx->J, (uint32_t)ctx->n, 3 * ctx->AD, (uint32_t)ctx->j_lo * s0,
-                                       (uint32_t)(ctx->j_hi - ctx->j_lo) * s0, ctx->stream),
+    // J is zero outside its slabs: only they are cleared
+    const uint32_t s0 = (uint32_t)ctx->dims[1] * (uint32_t)ctx->dims[2];
+    for (const auto& r : ctx->j_slabs)
+        OPX_TRY(opx::launch_zero_cells(ctx->prec, ctx->J, (uint32_t)ctx->n, 3 * ctx->AD, (uint32_t)r.first * s0,
+                                       (uint32_t)(r.second - r.first) * s0, ctx->stream),
                 "reset current");
-    }
-    ctx->j_lo = ctx->j_hi = 0;
+    ctx->j_slabs.clear();
     ctx->j_nonzero = false;
     return OPX_OK;
 }
@@ -1258,19 +1271,13 @@ int opx_interpolate_to_grid(opx_ctx* ctx, int t)
     const int cnt = ctx->sp_count;
     const int* h = ctx->sp_host.data();
     const int64_t s0 = (int64_t)ctx->dims[1] * ctx->dims[2];
-    int64_t lo = INT64_MAX, hi = INT64_MIN;
     for (int p = 0; p < cnt; p++) {
         const int o = h[p], num = h[cnt + p], off = h[2 * cnt + p], pp = h[3 * cnt + p], k = h[4 * cnt + p],
                   ppc = h[5 * cnt + p];
         const int64_t cur = o > 0 ? off + (int64_t)((t + k) / ppc) * pp : off + (int64_t)host_floor_div(-t + k, ppc) * pp;
-        const int64_t a = cur, b = cur + num + (ppc == 1 ? pp : 0);
-        lo = std::min(lo, a);
-        hi = std::max(hi, b);
-    }
-    if (lo < hi) {
-        const int64_t plo = (std::max<int64_t>(lo, 0)) / s0 - 1;
-        const int64_t phi = (std::min<int64_t>(hi, ctx->n) + s0 - 1) / s0 + 1;
-        extend_current_planes(ctx, (int)std::max<int64_t>(plo, 0), (int)std::min<int64_t>(phi, ctx->dims[0]));
+        const int64_t a = std::max<int64_t>(cur, 0), b = std::min<int64_t>(cur + num + (ppc == 1 ? pp : 0), ctx->n);
+        if (a >= b) continue;
+        extend_current_planes(ctx, (int)(a / s0 - 1), (int)((b + s0 - 1) / s0 + 1));
     }
     return OPX_OK;
 }

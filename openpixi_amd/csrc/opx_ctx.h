@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "opx.h"
@@ -43,7 +44,9 @@ struct opx_ctx {
     void* iUn = nullptr;
     void* iE = nullptr;
     bool j_nonzero = false;
-    int j_lo = 0, j_hi = 0;         // x0 planes [j_lo, j_hi) outside which J is zero (j_nonzero: j_lo < j_hi)
+    // disjoint x0 plane ranges, in order, outside which J is zero (at most opx::kMaxCurrentSlabs;
+    // j_nonzero: the list is not empty)
+    std::vector<std::pair<int, int>> j_slabs;
     opx::Box active{{0, 0, 0}, {0, 0, 0}, 1};
     opx::Box eval{{0, 0, 0}, {0, 0, 0}, 1};
     hipStream_t own_stream = nullptr;

@@ -24,10 +24,13 @@ struct Lattice {
     double as[3];      // lattice spacings
     double g;          // coupling
     uint32_t force_global = 0;   // 1: the fused kernels read through plain pointers below 4 GiB too
-    // x0 planes [j_lo, j_hi) outside which the current J is zero: the cell and per-link forms
-    // read J only there (every other reader reads all of it)
-    uint32_t j_lo = 0, j_hi = 0xffffffffu;
+    // x0 planes [j_lo[i], j_hi[i]), i < j_slabs, outside which the current J is zero: the cell and
+    // per-link forms read J only there (every other reader reads all of it)
+    uint32_t j_slabs = 1;
+    uint32_t j_lo[4] = {0, 0, 0, 0};
+    uint32_t j_hi[4] = {0xffffffffu, 0, 0, 0};
 };
+constexpr int kMaxCurrentSlabs = 4;
 
 // One event of a batch: its lattice, the element offsets of its link block
 // (3*G*n values) and E block (3*AD*n values) in the batch buffers, and the first

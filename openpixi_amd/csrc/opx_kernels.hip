@@ -30,6 +30,15 @@ struct Coord {
     uint32_t x0, x1, x2;
 };
 
+// whether plane x0 lies in one of the lattice's current slabs
+__device__ __forceinline__ bool in_current(const Lattice& L, uint32_t x0)
+{
+    bool r = false;
+#pragma unroll
+    for (int i = 0; i < kMaxCurrentSlabs; i++) r |= (i < (int)L.j_slabs) & (x0 >= L.j_lo[i]) & (x0 < L.j_hi[i]);
+    return r;
+}
+
 __device__ __forceinline__ Coord coords(const Lattice& L, uint32_t site)
 {
     Coord c;
@@ -254,7 +263,7 @@ __device__ __forceinline__ void link_update(const Lattice& L, const Fld& U, cons
     const T at = (T)L.at;
 #pragma unroll
     for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * at;
-    if (HAS_J && c.x0 >= L.j_lo && c.x0 < L.j_hi) {
+    if (HAS_J && in_current(L, c.x0)) {
         const A jj = fload_alg<NC, T>(J, site, d);
         const T uf = (T)L.uf[d];
 #pragma unroll
@@ -387,7 +396,7 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
     };
     const T at = (T)L.at;
     // E(x) of direction d plus the current term, read before any store of the cell
-    const bool in_j = HAS_J && c.x0 >= L.j_lo && c.x0 < L.j_hi;
+    const bool in_j = HAS_J && in_current(L, c.x0);
     auto e_start = [&](int d) -> A {
         A e = fload_alg_once<NC, T>(E, site, d);
         if (in_j) {
