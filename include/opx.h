@@ -159,7 +159,13 @@ int opx_reset_charge(opx_ctx *ctx);
 /* OPX_OPT_GLOBAL_POINTERS 1: the per-link, cell and plaquette forms read their fields through
  * plain global pointers, as they do by themselves for fields of 4 GiB or more (0, the default:
  * buffer instructions below 4 GiB); for testing that path on small lattices. */
-enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2, OPX_OPT_GLOBAL_POINTERS = 3 };
+/* OPX_OPT_REUNIT_EVERY N > 0: every N-th Simulation step of opx_simulation_step (counted by
+ * opx_simulation_steps_done) reprojects the new links (Unext) onto the group in double precision,
+ * as opx_leapfrog's reunit_every does: inside the update where the kernel form can, else by a pass
+ * after it. 0 (the default): never. No counterpart in the reference, whose double-precision links
+ * are not reprojected; for the FP32 form. */
+enum { OPX_OPT_KERNEL = 0, OPX_OPT_TILE = 1, OPX_OPT_CHUNKS = 2, OPX_OPT_GLOBAL_POINTERS = 3,
+       OPX_OPT_REUNIT_EVERY = 4 };
 enum { OPX_KERNEL_AUTO = 0, OPX_KERNEL_LINK = 1, OPX_KERNEL_STREAM = 2, OPX_KERNEL_PLAQ = 3, OPX_KERNEL_TILE = 4,
        OPX_KERNEL_PLANE = 5, OPX_KERNEL_CELL = 6 };
 int opx_set_option(opx_ctx *ctx, int option, int value);

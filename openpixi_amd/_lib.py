@@ -25,7 +25,7 @@ OPX_FP32 = 1
 
 FIELD_U, FIELD_UNEXT, FIELD_E, FIELD_J, FIELD_RHO = range(5)
 
-OPT_KERNEL, OPT_TILE, OPT_CHUNKS, OPT_GLOBAL_POINTERS = range(4)
+OPT_KERNEL, OPT_TILE, OPT_CHUNKS, OPT_GLOBAL_POINTERS, OPT_REUNIT_EVERY = range(5)
 KERNEL_AUTO, KERNEL_LINK, KERNEL_STREAM, KERNEL_PLAQ, KERNEL_TILE, KERNEL_PLANE, KERNEL_CELL = range(7)
 SOLVER_FAST, SOLVER_IMPLICIT = 0, 1
 CGC_CHARGE, CGC_GAUSS = 0, 1

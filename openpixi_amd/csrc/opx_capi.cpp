@@ -1325,7 +1325,16 @@ int opx_simulation_step(opx_ctx* ctx, int n)
         const bool last = s == n - 1;
         if ((rc = opx_particles_reassign(ctx)) != OPX_OK) return rc;
         if ((rc = opx_store_fields(ctx)) != OPX_OK) return rc;
-        if ((rc = opx_step(ctx, ctx->at)) != OPX_OK) return rc;
+        {
+            // every reunit_every-th step reprojects the new links, inside the update where the form can
+            const bool reunit = ctx->reunit_every > 0 && t % ctx->reunit_every == 0;
+            bool fused = false;
+            opx::Lattice L = lattice(ctx, ctx->at);
+            OPX_TRY(field_step(ctx, L, reunit, &fused), "step");
+            if (reunit && !fused)
+                OPX_TRY(opx::launch_reunitarize(ctx->nc, ctx->prec, (uint32_t)ctx->n, ctx->Unext, ctx->stream),
+                        "step reunitarize");
+        }
         if (ctx->sp_count > 0 || ctx->p_count > 0) {
             if (last) {
                 if ((rc = opx_reset_charge(ctx)) != OPX_OK) return rc;
@@ -1388,6 +1397,10 @@ int opx_set_option(opx_ctx* ctx, int option, int value)
         if (value < 0) return fail(ctx, OPX_E_ARG, "option: negative chunk count");
         ctx->nchunks = value;
         return OPX_OK;
+    case OPX_OPT_REUNIT_EVERY:
+        if (value < 0) return fail(ctx, OPX_E_ARG, "option: negative reprojection period");
+        ctx->reunit_every = value;
+        return OPX_OK;
     default:
         return fail(ctx, OPX_E_ARG, "option: unknown option");
     }
@@ -1400,6 +1413,7 @@ int opx_get_option(const opx_ctx* ctx, int option, int* value)
     case OPX_OPT_KERNEL: *value = ctx->kernel; return OPX_OK;
     case OPX_OPT_TILE: *value = ctx->tile; return OPX_OK;
     case OPX_OPT_GLOBAL_POINTERS: *value = ctx->force_global ? 1 : 0; return OPX_OK;
+    case OPX_OPT_REUNIT_EVERY: *value = ctx->reunit_every; return OPX_OK;
     case OPX_OPT_CHUNKS: {
         opx::Lattice L = lattice(ctx, ctx->at);
         *value = ctx->nchunks > 0 ? ctx->nchunks : (int)auto_chunks(ctx, L, ctx->tile);

@@ -95,6 +95,7 @@ struct opx_ctx {
     int nchunks = 0;                // 0: derived from the CU count
     int num_cus = 256;
     bool force_global = false;      // OPX_OPT_GLOBAL_POINTERS
+    int reunit_every = 0;           // OPX_OPT_REUNIT_EVERY
     int device = 0;                 // the GPU current when the context was created
     bool batch_view = false;        // event of an opx_batch: memory borrowed, stepping through the batch only
     opx_host::AdjointWork* adj = nullptr;   // checkpoint and cotangent buffers of opx_adjoint_gradient

@@ -201,6 +201,11 @@ class Grid:
         as they do for fields of 4 GiB or more (testing that path on small lattices)."""
         self._check(_lib.load().opx_set_option(self._h, _lib.OPT_GLOBAL_POINTERS, 1 if on else 0))
 
+    def setReunitarizeEvery(self, n):
+        """Every n-th Simulation step (simulationStep) reprojects the new links onto the group in
+        double precision, inside the update where the kernel form can (0: never)."""
+        self._check(_lib.load().opx_set_option(self._h, _lib.OPT_REUNIT_EVERY, int(n)))
+
     def kernelConfig(self):
         L = _lib.load()
         out = []

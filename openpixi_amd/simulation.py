@@ -182,6 +182,8 @@ class Simulation:
                                                               settings.implicit_solver_damping))
         else:
             self.fsolver = FastTYMSolver()
+        # FP64 reprojection of the new links every reunitarize_every steps, inside opx_simulation_step
+        self.grid.setReunitarizeEvery(settings.reunitarize_every)
         if settings.evaluation_region.enabled:
             self.grid.setEvaluationRegion(*settings.evaluation_region.resolved(settings.grid_cells))
         if settings.active_region.enabled:
@@ -245,7 +247,7 @@ class Simulation:
                 d.initialize(self)
             self.runDiagnostics()
             self._diagnostics_started = True
-        if diags or self.settings.reunitarize_every:
+        if diags:
             for _ in range(int(n)):
                 self._one_step(1)
                 self.runDiagnostics()
@@ -255,8 +257,6 @@ class Simulation:
     def _one_step(self, n: int):
         self.grid._check(_lib.load().opx_simulation_step(self.grid.handle, n))
         self.totalSimulationSteps = _lib.load().opx_simulation_steps_done(self.grid.handle)
-        if self.settings.reunitarize_every and self.totalSimulationSteps % self.settings.reunitarize_every == 0:
-            self.grid._check(_lib.load().opx_reunitarize(self.grid.handle))
 
     def runDiagnostics(self):
         """physics/Simulation.java:454-460."""

@@ -240,3 +240,38 @@ def test_sp_chargedensity_ragged_layout(gpu, nc):
             assert np.array_equal(rd, ro), t
         else:
             assert rel_err(rd, ro) < 1e-14, t
+
+
+@pytest.mark.parametrize("kernel", ["cell", "link"])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_simulation_reunit_option_equals_manual_reprojection(gpu, nc, kernel):
+    """opx_simulation_step with OPX_OPT_REUNIT_EVERY = 3 (the cell form reprojects the new links
+    inside its update, the per-link form by a pass after it) against single steps with
+    opx_reunitarize after every third: after the step that follows a reprojection both runs hold
+    the same U, Unext and E (the old links, which opx_reunitarize reprojects as well, are
+    overwritten by that step): bit for bit with the pass after the per-link form, to FP32 rounding
+    with the reprojection inside the cell form (a different compilation of the same arithmetic)."""
+    dims = (8, 6, 5)
+    U, E = make_state(nc, dims, 0.3, 0.2, seed=6)
+    a = device_grid(nc, dims, U, E, precision="fp32")
+    b = device_grid(nc, dims, U, E, precision="fp32")
+    for g in (a, b):
+        g.setKernel(kernel)
+        g.simulationInitialize()
+    a.setReunitarizeEvery(3)
+    assert a.simulationStep(7) == 7
+    for t in range(1, 8):
+        b.simulationStep(1)
+        if t % 3 == 0:
+            b.reunitarize()
+    for get in ("getU", "getUnext", "getE"):
+        if kernel == "link":
+            assert np.array_equal(getattr(a, get)(), getattr(b, get)()), get
+        else:
+            assert rel_err(getattr(a, get)(), getattr(b, get)()) < 1e-6, get
+    # and the reprojection did happen: without it the links drift from the group differently
+    c = device_grid(nc, dims, U, E, precision="fp32")
+    c.setKernel(kernel)
+    c.simulationInitialize()
+    c.simulationStep(7)
+    assert not np.array_equal(c.getU(), b.getU())
