@@ -1798,11 +1798,15 @@ int batch_step(opx_batch* b, int reunit_every)
     b->parity ^= 1;
     void* U = b->parity ? b->UB : b->UA;
     void* Un = b->parity ? b->UA : b->UB;
-    if (b->form == 3)
-        OPX_BTRY(opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, b->small_events, U, Un, b->E,
-                                        b->stream),
+    // every reunit_every-th step reprojects the new links: inside the cell form's update, else a pass
+    const bool reunit = reunit_every > 0 && (b->steps + 1) % reunit_every == 0;
+    bool fused = false;
+    if (b->form == 3) {
+        OPX_BTRY(opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, b->small_events,
+                                        U, Un, b->E, reunit, b->stream),
                  "batch cell update");
-    else if (b->form >= 1)
+        fused = reunit;
+    } else if (b->form >= 1)
         OPX_BTRY(opx::launch_batch_tile(b->prec, b->form == 1, b->dev_ev, b->dev_items, b->nitems, b->grid, U, Un,
                                         b->E, b->stream),
                  "batch tile update");
@@ -1811,7 +1815,7 @@ int batch_step(opx_batch* b, int reunit_every)
                                           b->stream),
                  "batch update");
     b->steps++;
-    if (reunit_every > 0 && b->steps % reunit_every == 0)
+    if (reunit && !fused)
         OPX_BTRY(opx::launch_batch_reunitarize(b->nc, b->prec, b->dev_ev, b->count, b->total_items, b->grid, Un,
                                                b->stream),
                  "batch reunitarize");
@@ -2117,7 +2121,7 @@ int opx_batch_autotune(opx_batch* b, int reps, double* best_ms)
     auto run = [&](const Cand& c, const void* src, void* dst) {
         if (c.form == 3)
             return opx::launch_batch_cell(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, b->small_events, src, dst, e2,
-                                          b->stream);
+                                          false, b->stream);
         return c.form >= 1 ? opx::launch_batch_tile(b->prec, c.form == 1, b->dev_ev, b->dev_items, b->nitems, c.grid,
                                                     src, dst, e2, b->stream)
                            : opx::launch_batch_update(b->nc, b->prec, b->dev_ev, b->count, b->total_items, c.grid, src,

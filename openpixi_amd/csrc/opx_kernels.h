@@ -138,9 +138,10 @@ hipError_t launch_batch_update(int nc, int prec, const BatchEvent* ev, int nev, 
 hipError_t launch_batch_reunitarize(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items,
                                     uint32_t grid, void* U, hipStream_t st);
 // the cell form over a batch (total_items: the per-link item count; a third of it are cell items);
-// buf: every event's link field is below 4 GiB, so the fields are read by buffer instructions
+// buf: every event's link field is below 4 GiB, so the fields are read by buffer instructions;
+// reunit: the new links are reprojected onto the group in double before they are stored
 hipError_t launch_batch_cell(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items, uint32_t grid,
-                             bool buf, const void* U, void* Unext, void* E, hipStream_t st);
+                             bool buf, const void* U, void* Unext, void* E, bool reunit, hipStream_t st);
 
 // All launchers enqueue on `st` and return hipGetLastError().
 hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box& active, const void* U,

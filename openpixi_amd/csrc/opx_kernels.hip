@@ -589,7 +589,7 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
 // [256 (i - item_start/3), +256) of that event, all three links of each (the per-link items of
 // an event number 3 ceil(n/256), so item_start/3 numbers the cell items); same XCD-aware
 // static schedule as k_batch_update.
-template <int NC, typename T, bool BUF>
+template <int NC, typename T, bool BUF, bool REUNIT>
 __global__ __launch_bounds__(kBlock, OPX_CELL_WAVES) void k_batch_cell(const BatchEvent* __restrict__ ev, int nev,
                                                                        uint32_t total_items, const T* __restrict__ Ubase,
                                                                        T* __restrict__ Unbase, T* __restrict__ Ebase)
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(kBlock, OPX_CELL_WAVES) void k_batch_cell(const Bat
         using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
         const Fld U(Ubase + E.link_offset, E.L.n, 3 * GR::S), Un(Unbase + E.link_offset, E.L.n, 3 * GR::S);
         const Fld Ef(Ebase + E.alg_offset, E.L.n, 3 * GR::AD);
-        cell_update<NC, T, false, false>(E.L, U, Un, Ef, Ef, site);
+        cell_update<NC, T, false, REUNIT>(E.L, U, Un, Ef, Ef, site);
     }
 }
 
@@ -1500,15 +1500,21 @@ hipError_t launch_batch_update(int nc, int prec, const BatchEvent* ev, int nev, 
 }
 
 hipError_t launch_batch_cell(int nc, int prec, const BatchEvent* ev, int nev, uint32_t total_items, uint32_t grid,
-                             bool buf, const void* U, void* Unext, void* E, hipStream_t st)
+                             bool buf, const void* U, void* Unext, void* E, bool reunit, hipStream_t st)
 {
     OPX_DISPATCH(nc, prec, {
-        if (buf)
-            hipLaunchKernelGGL((k_batch_cell<NC, T, true>), dim3(grid), dim3(kBlock), 0, st, ev, nev, total_items / 3,
-                               (const T*)U, (T*)Unext, (T*)E);
+        if (buf && reunit)
+            hipLaunchKernelGGL((k_batch_cell<NC, T, true, true>), dim3(grid), dim3(kBlock), 0, st, ev, nev,
+                               total_items / 3, (const T*)U, (T*)Unext, (T*)E);
+        else if (buf)
+            hipLaunchKernelGGL((k_batch_cell<NC, T, true, false>), dim3(grid), dim3(kBlock), 0, st, ev, nev,
+                               total_items / 3, (const T*)U, (T*)Unext, (T*)E);
+        else if (reunit)
+            hipLaunchKernelGGL((k_batch_cell<NC, T, false, true>), dim3(grid), dim3(kBlock), 0, st, ev, nev,
+                               total_items / 3, (const T*)U, (T*)Unext, (T*)E);
         else
-            hipLaunchKernelGGL((k_batch_cell<NC, T, false>), dim3(grid), dim3(kBlock), 0, st, ev, nev, total_items / 3,
-                               (const T*)U, (T*)Unext, (T*)E);
+            hipLaunchKernelGGL((k_batch_cell<NC, T, false, false>), dim3(grid), dim3(kBlock), 0, st, ev, nev,
+                               total_items / 3, (const T*)U, (T*)Unext, (T*)E);
     });
     return hipGetLastError();
 }

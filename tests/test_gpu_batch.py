@@ -45,13 +45,17 @@ def test_batch_matches_oracle_fp64(gpu, nc, graph):
         assert np.allclose(m[:7], mo[:7], rtol=1e-11, atol=1e-300), e
 
 
+@pytest.mark.parametrize("form", ["link", "cell"])
 @pytest.mark.parametrize("nc", [2, 3])
-def test_graph_replays_match_eager(gpu, nc):
-    """Replays of the captured period (10 steps incl. reunitarization) interleaved with
-    eager work agree exactly with eager stepping after every replay."""
-    print(f"graph replays nc={nc}", flush=True)
+def test_graph_replays_match_eager(gpu, nc, form):
+    """Replays of the captured period (10 steps incl. reunitarization, which the cell form does
+    inside its update) interleaved with eager work agree exactly with eager stepping after every
+    replay."""
+    print(f"graph replays nc={nc} form={form}", flush=True)
     bg, _ = filled_batch(nc, "fp32")
     be, _ = filled_batch(nc, "fp32")
+    bg.set_form(form)
+    be.set_form(form)
     bg.step_links(0.25)
     be.step_links(0.25)
     for r in range(4):
@@ -65,17 +69,21 @@ def test_graph_replays_match_eager(gpu, nc):
             assert np.array_equal(bg.event(e).getE(), be.event(e).getE()), (r, e)
 
 
+@pytest.mark.parametrize("form,kernel", [(None, "link"), ("cell", "cell")])
 @pytest.mark.parametrize("nc", [2, 3])
-def test_batch_matches_single_contexts(gpu, nc):
-    """An event of the batch evolves exactly as the same lattice in its own context."""
+def test_batch_matches_single_contexts(gpu, nc, form, kernel):
+    """An event of the batch evolves as the same lattice in its own context, FP32 with the
+    reprojection every 10 steps (inside the update in the cell forms)."""
     b, states = filled_batch(nc, "fp32")
+    if form is not None:
+        b.set_form(form)
     b.step_links(0.25)
     b.leapfrog(20, 10, graph=True)
     from openpixi_amd import Grid, _lib
 
     for e, dims in enumerate(EVENT_DIMS):
         g = Grid(nc, dims, 1.0, 0.25, 1.0, "fp32")
-        g.setKernel("link")
+        g.setKernel(kernel)
         g.setU(states[e][0])
         g.setE(states[e][1])
         FastTYMSolver().stepLinks(g, 0.25)
