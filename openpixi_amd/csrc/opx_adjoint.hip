@@ -61,37 +61,39 @@ __device__ __forceinline__ Site step(const Lattice& L, Site c, int d, int o)
     return c;
 }
 
+// link fields (U, Ubar: 12 components per cell) and algebra fields (E, Ebar: 9) in the device
+// block layout (fpos); the cell count argument is not needed to address them
 template <typename T>
-__device__ __forceinline__ Q<T> ld_q(const T* __restrict__ U, uint32_t n, uint32_t s, int d)
+__device__ __forceinline__ Q<T> ld_q(const T* __restrict__ U, uint32_t, uint32_t s, int d)
 {
-    const T* p = U + (size_t)d * 4 * n + s;
-    return Q<T>{p[0], p[(size_t)n], p[2 * (size_t)n], p[3 * (size_t)n]};
+    const T* p = fcell(U, s, 12) + 4 * d * 64;
+    return Q<T>{p[0], p[64], p[128], p[192]};
 }
 
 template <typename T>
-__device__ __forceinline__ void st_q(T* __restrict__ U, uint32_t n, uint32_t s, int d, const Q<T>& q)
+__device__ __forceinline__ void st_q(T* __restrict__ U, uint32_t, uint32_t s, int d, const Q<T>& q)
 {
-    T* p = U + (size_t)d * 4 * n + s;
+    T* p = fcell(U, s, 12) + 4 * d * 64;
     p[0] = q.e0;
-    p[(size_t)n] = q.e1;
-    p[2 * (size_t)n] = q.e2;
-    p[3 * (size_t)n] = q.e3;
+    p[64] = q.e1;
+    p[128] = q.e2;
+    p[192] = q.e3;
 }
 
 template <typename T>
-__device__ __forceinline__ V3<T> ld_v(const T* __restrict__ X, uint32_t n, uint32_t s, int d)
+__device__ __forceinline__ V3<T> ld_v(const T* __restrict__ X, uint32_t, uint32_t s, int d)
 {
-    const T* p = X + (size_t)d * 3 * n + s;
-    return V3<T>{{p[0], p[(size_t)n], p[2 * (size_t)n]}};
+    const T* p = fcell(X, s, 9) + 3 * d * 64;
+    return V3<T>{{p[0], p[64], p[128]}};
 }
 
 template <typename T>
-__device__ __forceinline__ void st_v(T* __restrict__ X, uint32_t n, uint32_t s, int d, const V3<T>& v)
+__device__ __forceinline__ void st_v(T* __restrict__ X, uint32_t, uint32_t s, int d, const V3<T>& v)
 {
-    T* p = X + (size_t)d * 3 * n + s;
+    T* p = fcell(X, s, 9) + 3 * d * 64;
     p[0] = v.v[0];
-    p[(size_t)n] = v.v[1];
-    p[2 * (size_t)n] = v.v[2];
+    p[64] = v.v[1];
+    p[128] = v.v[2];
 }
 
 template <typename T>

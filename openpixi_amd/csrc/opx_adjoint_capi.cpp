@@ -61,7 +61,8 @@ namespace {
 
 int ensure_work(opx_ctx* ctx, int ck_slots, int seg_slots)
 {
-    const size_t lb = (size_t)ctx->n * 3 * 4 * ctx->tsz, ab = (size_t)ctx->n * 3 * 3 * ctx->tsz;
+    const size_t cells = opx::padded_cells((uint64_t)ctx->n);   // fields in the device block layout
+    const size_t lb = cells * 3 * 4 * ctx->tsz, ab = cells * 3 * 3 * ctx->tsz;
     AdjointWork* w = ctx->adj;
     if (w && w->link_bytes == lb && w->ck_slots >= ck_slots && w->seg_slots >= seg_slots) return OPX_OK;
     adjoint_release(ctx);
@@ -188,7 +189,7 @@ extern "C" int opx_adjoint_gradient(opx_ctx* ctx, int steps, int reunit_every, i
     for (uint32_t c = 0; c < n; c++)
         for (int d = 0; d < 3; d++)
             for (int k = 0; k < 3; k++) {
-                const size_t i = ((size_t)d * 3 + k) * n + c;
+                const size_t i = opx::fpos(d * 3 + k, c, 9);
                 grad[((size_t)c * 3 + d) * 3 + k] =
                     ctx->prec == OPX_FP64 ? ((const double*)h.data())[i] : (double)((const float*)h.data())[i];
             }

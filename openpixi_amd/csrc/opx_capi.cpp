@@ -91,7 +91,8 @@ void extend_current_planes(opx_ctx* c, int lo, int hi)
     c->j_nonzero = true;
 }
 
-size_t link_count(const opx_ctx* c) { return (size_t)c->n * 3 * c->S; }
+// elements of a link / algebra / charge-density field in the device block layout (opx::fpos)
+size_t link_count(const opx_ctx* c) { return (size_t)opx::padded_cells((uint64_t)c->n) * 3 * c->S; }
 
 // Reals stored per link on the device: FP32 SU(3) links keep their first two rows
 // (real parts then imaginary parts of u00..u12); the third row is rebuilt on load.
@@ -113,22 +114,21 @@ void su3_rebuild_third_row(double* u)
     }
 }
 
-// identity links (physics/grid/Cell.java:63-68) in device SoA layout
+// identity links (physics/grid/Cell.java:63-68) in the device layout
 std::vector<double> identity_links(const opx_ctx* c)
 {
-    std::vector<double> soa(link_count(c), 0.0);
+    std::vector<double> dev(link_count(c), 0.0);
     for (int d = 0; d < 3; d++)
         for (int k = 0; k < c->S; k++) {
             const int e = stored_slot(c->S, k);
             const bool diag = c->nc == 2 ? e == 0 : (e == 0 || e == 4 || e == 8);
             if (!diag) continue;
-            double* p = soa.data() + ((size_t)d * c->S + k) * c->n;
-            for (int64_t s = 0; s < c->n; s++) p[s] = 1.0;
+            for (int64_t s = 0; s < c->n; s++) dev[opx::fpos(d * c->S + k, s, 3 * c->S)] = 1.0;
         }
-    return soa;
+    return dev;
 }
-size_t alg_count(const opx_ctx* c) { return (size_t)c->n * 3 * c->AD; }
-size_t rho_count(const opx_ctx* c) { return (size_t)c->n * c->AD; }
+size_t alg_count(const opx_ctx* c) { return (size_t)opx::padded_cells((uint64_t)c->n) * 3 * c->AD; }
+size_t rho_count(const opx_ctx* c) { return (size_t)opx::padded_cells((uint64_t)c->n) * c->AD; }
 
 // SU(3) Hermitian storage <-> Gell-Mann coefficients
 // (math/SU3AlgebraElement.java:93-129: get(j) and the incremental set(j))
@@ -186,14 +186,14 @@ int copy_from_device(opx_ctx* ctx, const void* src, std::vector<double>& dst)
     return OPX_OK;
 }
 
-int to_device(opx_ctx* ctx, void* dst, const std::vector<double>& soa)
+int to_device(opx_ctx* ctx, void* dst, const std::vector<double>& vals)
 {
-    return ctx->prec == OPX_FP64 ? copy_to_device<double>(ctx, dst, soa) : copy_to_device<float>(ctx, dst, soa);
+    return ctx->prec == OPX_FP64 ? copy_to_device<double>(ctx, dst, vals) : copy_to_device<float>(ctx, dst, vals);
 }
 
-int from_device(opx_ctx* ctx, const void* src, std::vector<double>& soa)
+int from_device(opx_ctx* ctx, const void* src, std::vector<double>& vals)
 {
-    return ctx->prec == OPX_FP64 ? copy_from_device<double>(ctx, src, soa) : copy_from_device<float>(ctx, src, soa);
+    return ctx->prec == OPX_FP64 ? copy_from_device<double>(ctx, src, vals) : copy_from_device<float>(ctx, src, vals);
 }
 
 void* field_ptr(opx_ctx* c, int field)
@@ -266,7 +266,7 @@ hipError_t launch_update(opx_ctx* c, const opx::Lattice& L, const void* U, void*
                                         c->stream);
     if (c->kernel == OPX_KERNEL_PLAQ) {
         if (!c->P) {
-            hipError_t e = hipMalloc(&c->P, (size_t)c->n * 3 * c->AD * c->tsz);
+            hipError_t e = hipMalloc(&c->P, alg_count(c) * c->tsz);
             if (e != hipSuccess) return e;
         }
         return opx::launch_plaq_update(c->nc, c->prec, L, c->active, U, c->P, Unext, E, c->J, c->j_nonzero, c->stream);
@@ -398,9 +398,9 @@ int opx_create(int num_colors, int num_dims, const int* num_cells, const double*
     if (rc == OPX_OK) rc = hip_check(ctx, hipMalloc(&ctx->counters, 4 * sizeof(unsigned long long)), "alloc counters");
     if (rc == OPX_OK) {
         // identities (physics/grid/Cell.java:63-68) and zero fields
-        const std::vector<double> soa = identity_links(ctx);
-        rc = to_device(ctx, ctx->U, soa);
-        if (rc == OPX_OK) rc = to_device(ctx, ctx->Unext, soa);
+        const std::vector<double> ident = identity_links(ctx);
+        rc = to_device(ctx, ctx->U, ident);
+        if (rc == OPX_OK) rc = to_device(ctx, ctx->Unext, ident);
     }
     if (rc == OPX_OK) rc = hip_check(ctx, hipMemsetAsync(ctx->E, 0, alg_count(ctx) * ctx->tsz, ctx->stream), "zero E");
     if (rc == OPX_OK) rc = hip_check(ctx, hipMemsetAsync(ctx->J, 0, alg_count(ctx) * ctx->tsz, ctx->stream), "zero J");
@@ -477,18 +477,19 @@ int opx_upload_field(opx_ctx* ctx, int field, const double* host, int64_t n_valu
     void* dst = field_ptr(ctx, field);
     if (!dst) return fail(ctx, OPX_E_ARG, "upload: unknown field");
     const int64_t n = ctx->n;
-    std::vector<double> soa;
+    std::vector<double> dev;   // device layout (opx::fpos), padding zero
     if (field == OPX_FIELD_U || field == OPX_FIELD_UNEXT) {
         if (n_values != n * 3 * ctx->G) return fail(ctx, OPX_E_ARG, "upload: link buffer size mismatch");
-        soa.resize(link_count(ctx));
+        dev.assign(link_count(ctx), 0.0);
         for (int64_t s = 0; s < n; s++)
             for (int d = 0; d < 3; d++)
                 for (int k = 0; k < ctx->S; k++)
-                    soa[((size_t)d * ctx->S + k) * n + s] = host[(s * 3 + d) * ctx->G + stored_slot(ctx->S, k)];
+                    dev[opx::fpos(d * ctx->S + k, s, 3 * ctx->S)] = host[(s * 3 + d) * ctx->G + stored_slot(ctx->S, k)];
     } else {
         const int ndir = field == OPX_FIELD_RHO ? 1 : 3;
         if (n_values != n * ndir * ctx->AS) return fail(ctx, OPX_E_ARG, "upload: algebra buffer size mismatch");
-        soa.resize((size_t)n * ndir * ctx->AD);
+        const uint32_t comps = ndir * ctx->AD;
+        dev.assign((size_t)opx::padded_cells((uint64_t)n) * comps, 0.0);
         for (int64_t s = 0; s < n; s++)
             for (int d = 0; d < ndir; d++) {
                 const double* v = host + (s * ndir + d) * ctx->AS;
@@ -498,12 +499,12 @@ int opx_upload_field(opx_ctx* ctx, int field, const double* host, int64_t n_valu
                 } else {
                     su3_storage_to_coeffs(v, c);
                 }
-                for (int k = 0; k < ctx->AD; k++) soa[((size_t)d * ctx->AD + k) * n + s] = c[k];
+                for (int k = 0; k < ctx->AD; k++) dev[opx::fpos(d * ctx->AD + k, s, comps)] = c[k];
             }
         if (field == OPX_FIELD_J) extend_current_planes(ctx, 0, ctx->dims[0]);
         if (field == OPX_FIELD_RHO) ctx->rho_pending = false;
     }
-    return to_device(ctx, dst, soa);
+    return to_device(ctx, dst, dev);
 }
 
 int opx_download_field(opx_ctx* ctx, int field, double* host, int64_t n_values)
@@ -519,26 +520,27 @@ int opx_download_field(opx_ctx* ctx, int field, double* host, int64_t n_values)
     }
     if (field == OPX_FIELD_U || field == OPX_FIELD_UNEXT) {
         if (n_values != n * 3 * ctx->G) return fail(ctx, OPX_E_ARG, "download: link buffer size mismatch");
-        std::vector<double> soa(link_count(ctx));
-        int rc = from_device(ctx, src, soa);
+        std::vector<double> dev(link_count(ctx));
+        int rc = from_device(ctx, src, dev);
         if (rc != OPX_OK) return rc;
         for (int64_t s = 0; s < n; s++)
             for (int d = 0; d < 3; d++) {
                 double* u = host + (s * 3 + d) * ctx->G;
-                for (int k = 0; k < ctx->S; k++) u[stored_slot(ctx->S, k)] = soa[((size_t)d * ctx->S + k) * n + s];
+                for (int k = 0; k < ctx->S; k++) u[stored_slot(ctx->S, k)] = dev[opx::fpos(d * ctx->S + k, s, 3 * ctx->S)];
                 if (ctx->S != ctx->G) su3_rebuild_third_row(u);
             }
         return OPX_OK;
     }
     const int ndir = field == OPX_FIELD_RHO ? 1 : 3;
     if (n_values != n * ndir * ctx->AS) return fail(ctx, OPX_E_ARG, "download: algebra buffer size mismatch");
-    std::vector<double> soa((size_t)n * ndir * ctx->AD);
-    int rc = from_device(ctx, src, soa);
+    const uint32_t comps = ndir * ctx->AD;
+    std::vector<double> dev((size_t)opx::padded_cells((uint64_t)n) * comps);
+    int rc = from_device(ctx, src, dev);
     if (rc != OPX_OK) return rc;
     for (int64_t s = 0; s < n; s++)
         for (int d = 0; d < ndir; d++) {
             double c[8];
-            for (int k = 0; k < ctx->AD; k++) c[k] = soa[((size_t)d * ctx->AD + k) * n + s];
+            for (int k = 0; k < ctx->AD; k++) c[k] = dev[opx::fpos(d * ctx->AD + k, s, comps)];
             double* v = host + (s * ndir + d) * ctx->AS;
             if (ctx->nc == 2) {
                 v[0] = c[0]; v[1] = c[1]; v[2] = c[2];
@@ -579,11 +581,12 @@ int opx_set_efield_from_coefficients(opx_ctx* ctx, const double* coeffs, int64_t
     if (!coeffs) return fail(ctx, OPX_E_ARG, "coefficients: null buffer");
     const int64_t n = ctx->n;
     if (n_values != n * 3 * ctx->AD) return fail(ctx, OPX_E_ARG, "coefficients: size mismatch");
-    std::vector<double> soa((size_t)n_values);
+    std::vector<double> dev(alg_count(ctx), 0.0);
     for (int64_t s = 0; s < n; s++)
         for (int d = 0; d < 3; d++)
-            for (int k = 0; k < ctx->AD; k++) soa[((size_t)d * ctx->AD + k) * n + s] = coeffs[(s * 3 + d) * ctx->AD + k];
-    return to_device(ctx, ctx->E, soa);
+            for (int k = 0; k < ctx->AD; k++)
+                dev[opx::fpos(d * ctx->AD + k, s, 3 * ctx->AD)] = coeffs[(s * 3 + d) * ctx->AD + k];
+    return to_device(ctx, ctx->E, dev);
 }
 
 int opx_set_active_region(opx_ctx* ctx, const int* lo, const int* hi)
@@ -1549,8 +1552,9 @@ int opx_gauss_project(opx_ctx* ctx, double tol, int max_iter, int* iterations, d
     int rc = materialize_rho(ctx);
     if (rc != OPX_OK) return rc;
     opx::Lattice L = lattice(ctx, ctx->at);
-    const uint64_t ms = (uint64_t)ctx->n * ctx->AD;   // per-cell algebra values
-    const uint64_t ml = 3 * ms;                        // per-link algebra values
+    // the solver's double work arrays are [component][cell]: ms per-cell, ml per-link algebra values
+    const uint64_t ms = (uint64_t)ctx->n * ctx->AD;
+    const uint64_t ml = 3 * ms;
     double *Ed = nullptr, *F = nullptr, *lam = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *scr = nullptr;
     double* bufs_sz[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     const size_t sizes[7] = {ml, ml, ms, ms, ms, ms, opx::gauss_dot_scratch()};
@@ -1569,7 +1573,7 @@ int opx_gauss_project(opx_ctx* ctx, double tol, int max_iter, int* iterations, d
     };
     double rr = 0, rr0 = 0;
     int it = 0;
-    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_to_double(ctx->prec, ml, ctx->E, Ed, ctx->stream), "gauss E");
+    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_to_double(ctx->prec, ml, (uint32_t)ctx->n, ctx->E, Ed, ctx->stream), "gauss E");
     // r = G = D^T E - rho ; lambda = 0 ; p = r
     if (rc == OPX_OK)
         rc = hip_check(ctx, opx::gauss_apply_Dt(ctx->nc, ctx->prec, L, ctx->U, Ed, ctx->rho, r, ctx->stream), "gauss G");
@@ -1597,8 +1601,8 @@ int opx_gauss_project(opx_ctx* ctx, double tol, int max_iter, int* iterations, d
     }
     // E' = E - D lambda, then the Gauss residual of the stored field
     if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_apply_D(ctx->nc, ctx->prec, L, ctx->U, lam, F, ctx->stream), "gauss D");
-    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_sub_into(ctx->prec, ml, Ed, F, ctx->E, ctx->stream), "gauss update");
-    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_to_double(ctx->prec, ml, ctx->E, Ed, ctx->stream), "gauss E");
+    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_sub_into(ctx->prec, ml, (uint32_t)ctx->n, Ed, F, ctx->E, ctx->stream), "gauss update");
+    if (rc == OPX_OK) rc = hip_check(ctx, opx::gauss_to_double(ctx->prec, ml, (uint32_t)ctx->n, ctx->E, Ed, ctx->stream), "gauss E");
     if (rc == OPX_OK)
         rc = hip_check(ctx, opx::gauss_apply_Dt(ctx->nc, ctx->prec, L, ctx->U, Ed, ctx->rho, r, ctx->stream), "gauss G");
     double g2 = 0;
@@ -1872,9 +1876,11 @@ int opx_batch_create(int num_colors, int count, const int* dims, const double* s
         E.item_start = items;
         E.pad = 0;
         b->ev.push_back(E);
-        lo += (uint64_t)n * 3 * b->S;
-        ao += (uint64_t)n * 3 * b->AD;
-        if ((uint64_t)n * 3 * b->S * b->tsz >= (1ull << 32)) b->small_events = false;
+        // each event's fields in the device block layout (opx::fpos), padded to whole 64-cell blocks
+        const uint64_t np = opx::padded_cells((uint64_t)n);
+        lo += np * 3 * b->S;
+        ao += np * 3 * b->AD;
+        if (np * 3 * b->S * b->tsz >= (1ull << 32)) b->small_events = false;
         const uint64_t it = 3 * (((uint64_t)n + 255) / 256);
         if ((uint64_t)items + it >= (1ull << 32)) rc = bfail(b, OPX_E_UNSUPPORTED, "batch: too many cells");
         items += (uint32_t)it;
@@ -1933,9 +1939,9 @@ int opx_batch_create(int num_colors, int count, const int* dims, const double* s
         // identity links in both buffers (physics/grid/Cell.java:63-68)
         for (int e = 0; e < count && rc == OPX_OK; e++) {
             opx_ctx* v = b->views[e];
-            const std::vector<double> soa = identity_links(v);
-            rc = to_device(v, v->U, soa);
-            if (rc == OPX_OK) rc = to_device(v, v->Unext, soa);
+            const std::vector<double> ident = identity_links(v);
+            rc = to_device(v, v->U, ident);
+            if (rc == OPX_OK) rc = to_device(v, v->Unext, ident);
             if (rc != OPX_OK) b->err = v->err;
         }
     }

@@ -7,7 +7,8 @@
 //     (LightConeNGPSuperParticleCreator.java:104-307).
 //
 // Arithmetic is double throughout; the context's links, E and rho are read and written in its
-// own precision. Work arrays are component-major [component][cell] (complex ones as double2),
+// own precision and the device block layout (fpos). Work arrays are component-major
+// [component][cell] (complex ones as double2),
 // group-element arrays [real][cell]. The discrete Fourier transforms run one x-axis at a time:
 // a workgroup stages a few lines in LDS and runs radix-2 butterflies for power-of-two lengths,
 // or the direct sum against a twiddle table for other lengths.
@@ -55,30 +56,53 @@ __device__ __forceinline__ bool in_active(const Box& b, const uint32_t x[3])
     return true;
 }
 
-// a context link (precision T, S stored reals) in double
+// a context link (precision T, S stored reals, device block layout) in double
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, double>::M ld_link_d(const T* __restrict__ U, uint32_t n, uint32_t site,
-                                                                  int dir)
+__device__ __forceinline__ typename Group<NC, double>::M ld_link_d(const T* __restrict__ U, uint32_t site, int dir)
 {
     using GR = Group<NC, T>;
     using GD = Group<NC, double>;
     typename GD::M m = GD::zero();
-    const T* p = U + (size_t)dir * GR::S * n + site;
+    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[(size_t)k * n];
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[k * 64];
     if constexpr (GR::S != GR::G) su3_third_row(m);
     return m;
 }
 
 template <int NC, typename T>
-__device__ __forceinline__ void st_link_d(T* __restrict__ U, uint32_t n, uint32_t site, int dir,
+__device__ __forceinline__ void st_link_d(T* __restrict__ U, uint32_t site, int dir,
                                           const typename Group<NC, double>::M& m)
 {
     using GR = Group<NC, T>;
     using GD = Group<NC, double>;
-    T* p = U + (size_t)dir * GR::S * n + site;
+    T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) p[(size_t)k * n] = (T)GD::gget(m, GR::slot(k));
+    for (int k = 0; k < GR::S; k++) p[k * 64] = (T)GD::gget(m, GR::slot(k));
+}
+
+// algebra coefficients of direction dir of a context field in the device block layout (E: comps
+// 3 AD; the charge density: AD)
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, double>::A ld_field_alg(const T* __restrict__ X, uint32_t site, int dir,
+                                                                     uint32_t comps)
+{
+    using GD = Group<NC, double>;
+    typename GD::A a;
+    const T* p = fcell(X, site, comps) + dir * GD::AD * 64;
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[k * 64];
+    return a;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void st_field_alg(T* __restrict__ X, uint32_t site, int dir, uint32_t comps,
+                                             const typename Group<NC, double>::A& a)
+{
+    using GD = Group<NC, double>;
+    T* p = fcell(X, site, comps) + dir * GD::AD * 64;
+#pragma unroll
+    for (int k = 0; k < GD::AD; k++) p[k * 64] = (T)aget(a, k);
 }
 
 // group element of a double [real][cell] array
@@ -101,7 +125,7 @@ __device__ __forceinline__ void st_g(double* __restrict__ V, uint32_t n, uint32_
     for (int k = 0; k < GD::G; k++) V[(size_t)k * n + site] = GD::gget(m, k);
 }
 
-// algebra coefficients of a [component][cell] array (precision T), optionally offset by dir blocks
+// algebra coefficients of a [component][cell] work array
 template <int NC, typename T>
 __device__ __forceinline__ typename Group<NC, double>::A ld_coef(const T* __restrict__ X, uint32_t n, uint32_t site,
                                                                 int dir = 0)
@@ -114,15 +138,6 @@ __device__ __forceinline__ typename Group<NC, double>::A ld_coef(const T* __rest
     return a;
 }
 
-template <int NC, typename T>
-__device__ __forceinline__ void st_coef(T* __restrict__ X, uint32_t n, uint32_t site, int dir,
-                                        const typename Group<NC, double>::A& a)
-{
-    using GD = Group<NC, double>;
-    T* p = X + (size_t)dir * GD::AD * n + site;
-#pragma unroll
-    for (int k = 0; k < GD::AD; k++) p[(size_t)k * n] = (T)aget(a, k);
-}
 
 // ------------------------------------------------------------------ discrete Fourier transforms
 // tw[t] = (cos 2 pi t / n, sin 2 pi t / n), t < n
@@ -421,18 +436,18 @@ __global__ __launch_bounds__(kB) void k_cgc_links(Lattice L, const double* __res
     const double f = -1.0 / L.at;
 #pragma unroll 1
     for (int d = 0; d < 3; d++) {
-        typename GD::M u = ld_link_d<NC, T>(U, L.n, c, d), un = ld_link_d<NC, T>(Unext, L.n, c, d);
+        typename GD::M u = ld_link_d<NC, T>(U, c, d), un = ld_link_d<NC, T>(Unext, c, d);
         if (d > 0) {
             const uint32_t xd = nb(L, c, d, 1);
             u = mul_adj(mul(ld_g<NC>(Vm, L.n, c), u), ld_g<NC>(Vm, L.n, xd));
             un = mul_adj(mul(ld_g<NC>(Vp, L.n, c), un), ld_g<NC>(Vp, L.n, xd));
-            st_link_d<NC, T>(U, L.n, c, d, u);
-            st_link_d<NC, T>(Unext, L.n, c, d, un);
+            st_link_d<NC, T>(U, c, d, u);
+            st_link_d<NC, T>(Unext, c, d, un);
         }
         typename GD::A e = proj(mul_adj(un, u));
 #pragma unroll
         for (int k = 0; k < GD::AD; k++) aref(e, k) *= f;
-        st_coef<NC, T>(E, L.n, c, d, e);
+        st_field_alg<NC, T>(E, c, d, 3 * GD::AD, e);
     }
 }
 
@@ -455,7 +470,7 @@ __global__ __launch_bounds__(kB) void k_cgc_gauss(Lattice L, Box active, const d
     A g = GD::azero();
     if (in_active(active, x)) {
         auto copy_u = [&](const double* V, const T* grid, uint32_t s, int d) -> M {
-            if (d == 0) return ld_link_d<NC, T>(grid, L.n, s, 0);
+            if (d == 0) return ld_link_d<NC, T>(grid, s, 0);
             return mul_adj(ld_g<NC>(V, L.n, s), ld_g<NC>(V, L.n, nb(L, s, d, 1)));
         };
         auto copy_e = [&](uint32_t s, int d, const M& u) -> A {
@@ -475,7 +490,7 @@ __global__ __launch_bounds__(kB) void k_cgc_gauss(Lattice L, Box active, const d
             for (int k = 0; k < GD::AD; k++) aref(g, k) += (aget(ex, k) - aget(t, k)) * f;
         }
         if (rho) {
-            const A r = ld_coef<NC, T>(rho, L.n, c);
+            const A r = ld_field_alg<NC, T>(rho, c, 0, GD::AD);
 #pragma unroll
             for (int k = 0; k < GD::AD; k++) aref(g, k) -= aget(r, k);
         }

@@ -8,7 +8,9 @@
 // transpose of D). The projected field E' = E - D lambda with D^T D lambda = G
 // satisfies G' = 0 and is the closest such field to E. D^T D is the covariant
 // lattice Laplacian; it is solved by conjugate gradients on the host driving the
-// kernels below. Every reduction is a fixed-order two-level sum.
+// kernels below. Every reduction is a fixed-order two-level sum. The context's fields (U, Unext,
+// E, rho) are in the device block layout (fpos); the double work arrays of the solver (phi, F,
+// G, R, the CG vectors) are [component][cell] with n cells each.
 #include "opx_group.h"
 #include "opx_kernels.h"
 
@@ -40,13 +42,14 @@ __device__ __forceinline__ void coords3(const Lattice& L, uint32_t site, uint32_
 }
 
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, double>::M link_d(const T* __restrict__ U, uint32_t n, uint32_t site, int d)
+__device__ __forceinline__ typename Group<NC, double>::M link_d(const T* __restrict__ U, uint32_t site, int d)
 {
     using GD = Group<NC, double>;
     using GR = Group<NC, T>;
     typename GD::M m;
+    const T* p = fcell(U, site, 3 * GR::S) + d * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)U[((size_t)d * GR::S + k) * n + site];
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[k * 64];
     if constexpr (GR::S != GR::G) su3_third_row(m);   // rebuilt in double
     return m;
 }
@@ -75,7 +78,7 @@ __global__ __launch_bounds__(kB) void k_apply_D(Lattice L, const T* __restrict__
     coords3(L, site, x);
     const double w = 1.0 / (L.g * L.as[d] * L.as[d]);
     const uint32_t xp = nb_fwd(L, site, x[d], d);
-    const auto u = link_d<NC, T>(U, L.n, site, d);
+    const auto u = link_d<NC, T>(U, site, d);
     const auto t = act(alg_at<NC>(phi, L.n, xp, 0), u);
     const auto p = alg_at<NC>(phi, L.n, site, 0);
 #pragma unroll
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(kB) void k_apply_Dt(Lattice L, const T* __restrict_
     for (int d = 0; d < 3; d++) {
         const double w = 1.0 / (L.g * L.as[d] * L.as[d]);
         const uint32_t xm = nb_bwd(L, site, x[d], d);
-        const auto u = link_d<NC, T>(U, L.n, xm, d);
+        const auto u = link_d<NC, T>(U, xm, d);
         const auto t = act(alg_at<NC>(F, L.n, xm, d), adj(u));
         const auto f = alg_at<NC>(F, L.n, site, d);
 #pragma unroll
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(kB) void k_apply_Dt(Lattice L, const T* __restrict_
     }
     if (rho) {
 #pragma unroll
-        for (int k = 0; k < GD::AD; k++) aref(g, k) -= (double)rho[(size_t)k * L.n + site];
+        for (int k = 0; k < GD::AD; k++) aref(g, k) -= (double)rho[fpos(k, site, GD::AD)];
     }
 #pragma unroll
     for (int k = 0; k < GD::AD; k++) out[(size_t)k * L.n + site] = aget(g, k);
@@ -131,8 +134,8 @@ __global__ __launch_bounds__(kB) void k_restore_violation(Lattice L, const T* __
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const uint32_t xm = nb_bwd(L, site, x[j], j);
-        const auto c1 = proj(mul_adj(link_d<NC, T>(Unext, L.n, site, j), link_d<NC, T>(U, L.n, site, j)));
-        const auto c2 = proj(adj_mul(link_d<NC, T>(U, L.n, xm, j), link_d<NC, T>(Unext, L.n, xm, j)));
+        const auto c1 = proj(mul_adj(link_d<NC, T>(Unext, site, j), link_d<NC, T>(U, site, j)));
+        const auto c2 = proj(adj_mul(link_d<NC, T>(U, xm, j), link_d<NC, T>(Unext, xm, j)));
 #pragma unroll
         for (int k = 0; k < GD::AD; k++) aref(g, k) += aget(c1, k) - aget(c2, k);
     }
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kB) void k_restore_violation(Lattice L, const T* __
     for (int k = 0; k < GD::AD; k++) {
         const double v = aget(g, k) * factor;
         G[(size_t)k * L.n + site] = v;
-        R[(size_t)k * L.n + site] = v - (rho ? (double)rho[(size_t)k * L.n + site] : 0.0);
+        R[(size_t)k * L.n + site] = v - (rho ? (double)rho[fpos(k, site, GD::AD)] : 0.0);
     }
 }
 
@@ -157,13 +160,13 @@ __global__ __launch_bounds__(kB) void k_restore_correct(Lattice L, const T* __re
     uint32_t x[3];
     coords3(L, site, x);
     const uint32_t xp = nb_fwd(L, site, x[d], d);
-    const auto t = act(alg_at<NC>(G, L.n, xp, 0), link_d<NC, T>(U, L.n, site, d));
+    const auto t = act(alg_at<NC>(G, L.n, xp, 0), link_d<NC, T>(U, site, d));
     const auto gi = alg_at<NC>(G, L.n, site, 0);
-    T* e = E + (size_t)d * GD::AD * L.n + site;
+    T* e = fcell(E, site, 3 * GD::AD) + d * GD::AD * 64;
 #pragma unroll
     for (int k = 0; k < GD::AD; k++) {
-        const double v = (double)e[(size_t)k * L.n] + (aget(t, k) - aget(gi, k)) * f2;
-        e[(size_t)k * L.n] = (T)v;
+        const double v = (double)e[k * 64] + (aget(t, k) - aget(gi, k)) * f2;
+        e[k * 64] = (T)v;
     }
 }
 
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(kB) void k_trace_flags(uint32_t n, const T* __restr
     bool bad = false;
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-        const auto u = link_d<NC, T>(t ? Unext : U, n, site, d);
+        const auto u = link_d<NC, T>(t ? Unext : U, site, d);
         double tr;
         if constexpr (NC == 2)
             tr = 2.0 * u.e0;
@@ -192,19 +195,25 @@ __global__ __launch_bounds__(kB) void k_trace_flags(uint32_t n, const T* __restr
     if (bad) atomicAdd(count, 1ull);
 }
 
+// field a (block layout, m / n components per cell) -> [component][cell] double copy b
 template <typename T>
-__global__ __launch_bounds__(kB) void k_to_double(uint64_t m, const T* __restrict__ a, double* __restrict__ b)
+__global__ __launch_bounds__(kB) void k_to_double(uint64_t m, uint32_t n, const T* __restrict__ a, double* __restrict__ b)
 {
     const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
-    if (i < m) b[i] = (double)a[i];
+    if (i >= m) return;
+    const uint32_t c = (uint32_t)(i / n), s = (uint32_t)(i - (uint64_t)c * n);
+    b[i] = (double)a[fpos(c, s, (uint32_t)(m / n))];
 }
 
+// E (block layout) = Ed - F ([component][cell])
 template <typename T>
-__global__ __launch_bounds__(kB) void k_sub_into(uint64_t m, const double* __restrict__ Ed, const double* __restrict__ F,
-                                                 T* __restrict__ E)
+__global__ __launch_bounds__(kB) void k_sub_into(uint64_t m, uint32_t n, const double* __restrict__ Ed,
+                                                 const double* __restrict__ F, T* __restrict__ E)
 {
     const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
-    if (i < m) E[i] = (T)(Ed[i] - F[i]);
+    if (i >= m) return;
+    const uint32_t c = (uint32_t)(i / n), s = (uint32_t)(i - (uint64_t)c * n);
+    E[fpos(c, s, (uint32_t)(m / n))] = (T)(Ed[i] - F[i]);
 }
 
 __device__ __forceinline__ void two_sum_merge(double& s1, double& c1, double s2, double c2)
@@ -304,21 +313,21 @@ hipError_t gauss_apply_Dt(int nc, int prec, const Lattice& L, const void* U, con
     return hipGetLastError();
 }
 
-hipError_t gauss_to_double(int prec, uint64_t m, const void* a, double* b, hipStream_t st)
+hipError_t gauss_to_double(int prec, uint64_t m, uint32_t n, const void* a, double* b, hipStream_t st)
 {
     if (prec == 0)
-        hipLaunchKernelGGL(k_to_double<double>, g1(m), dim3(kB), 0, st, m, (const double*)a, b);
+        hipLaunchKernelGGL(k_to_double<double>, g1(m), dim3(kB), 0, st, m, n, (const double*)a, b);
     else
-        hipLaunchKernelGGL(k_to_double<float>, g1(m), dim3(kB), 0, st, m, (const float*)a, b);
+        hipLaunchKernelGGL(k_to_double<float>, g1(m), dim3(kB), 0, st, m, n, (const float*)a, b);
     return hipGetLastError();
 }
 
-hipError_t gauss_sub_into(int prec, uint64_t m, const double* Ed, const double* F, void* E, hipStream_t st)
+hipError_t gauss_sub_into(int prec, uint64_t m, uint32_t n, const double* Ed, const double* F, void* E, hipStream_t st)
 {
     if (prec == 0)
-        hipLaunchKernelGGL(k_sub_into<double>, g1(m), dim3(kB), 0, st, m, Ed, F, (double*)E);
+        hipLaunchKernelGGL(k_sub_into<double>, g1(m), dim3(kB), 0, st, m, n, Ed, F, (double*)E);
     else
-        hipLaunchKernelGGL(k_sub_into<float>, g1(m), dim3(kB), 0, st, m, Ed, F, (float*)E);
+        hipLaunchKernelGGL(k_sub_into<float>, g1(m), dim3(kB), 0, st, m, n, Ed, F, (float*)E);
     return hipGetLastError();
 }
 

@@ -28,60 +28,61 @@ OPX_HD double fatan2(double y, double x) { return atan2(y, x); }
 OPX_HD float fpow(float x, float y) { return powf(x, y); }
 OPX_HD double fpow(double x, double y) { return pow(x, y); }
 
+// fields in the device block layout (fpos); the cell count argument is not needed to address them
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, T>::M ld_link(const T* __restrict__ U, uint32_t n, uint32_t site, int dir)
+__device__ __forceinline__ typename Group<NC, T>::M ld_link(const T* __restrict__ U, uint32_t, uint32_t site, int dir)
 {
     using GR = Group<NC, T>;
     typename GR::M m;
-    const T* p = U + (size_t)dir * GR::S * n + site;
+    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[(size_t)k * n];
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[k * 64];
     GR::complete(m);
     return m;
 }
 
 // the stored reals only (two rows of an FP32 SU(3) link), row 2 left zero
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, T>::M ld_link_r01(const T* __restrict__ U, uint32_t n, uint32_t site,
+__device__ __forceinline__ typename Group<NC, T>::M ld_link_r01(const T* __restrict__ U, uint32_t, uint32_t site,
                                                                 int dir)
 {
     using GR = Group<NC, T>;
     typename GR::M m = GR::zero();
-    const T* p = U + (size_t)dir * GR::S * n + site;
+    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[(size_t)k * n];
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[k * 64];
     return m;
 }
 
 template <int NC, typename T>
-__device__ __forceinline__ void st_link(T* __restrict__ U, uint32_t n, uint32_t site, int dir,
+__device__ __forceinline__ void st_link(T* __restrict__ U, uint32_t, uint32_t site, int dir,
                                         const typename Group<NC, T>::M& m)
 {
     using GR = Group<NC, T>;
-    T* p = U + (size_t)dir * GR::S * n + site;
+    T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) p[(size_t)k * n] = GR::gget(m, GR::slot(k));
+    for (int k = 0; k < GR::S; k++) p[k * 64] = GR::gget(m, GR::slot(k));
 }
 
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, T>::A ld_alg(const T* __restrict__ X, uint32_t n, uint32_t site, int dir)
+__device__ __forceinline__ typename Group<NC, T>::A ld_alg(const T* __restrict__ X, uint32_t, uint32_t site, int dir)
 {
     using GR = Group<NC, T>;
     typename GR::A a;
-    const T* p = X + (size_t)dir * GR::AD * n + site;
+    const T* p = fcell(X, site, 3 * GR::AD) + dir * GR::AD * 64;
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[(size_t)k * n];
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[k * 64];
     return a;
 }
 
 template <int NC, typename T>
-__device__ __forceinline__ void st_alg(T* __restrict__ X, uint32_t n, uint32_t site, int dir,
+__device__ __forceinline__ void st_alg(T* __restrict__ X, uint32_t, uint32_t site, int dir,
                                        const typename Group<NC, T>::A& a)
 {
     using GR = Group<NC, T>;
-    T* p = X + (size_t)dir * GR::AD * n + site;
+    T* p = fcell(X, site, 3 * GR::AD) + dir * GR::AD * 64;
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) p[(size_t)k * n] = aget(a, k);
+    for (int k = 0; k < GR::AD; k++) p[k * 64] = aget(a, k);
 }
 
 // ------------------------------------------------------------------ logarithm and power
@@ -433,11 +434,16 @@ __global__ __launch_bounds__(kIB) void k_implicit_end(Lattice L, Box active, con
     uint32_t x[3];
     site_coords(L, site, x);
     if (!in_region(active, x)) return;
-    const size_t n = L.n;
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) E[(d * GR::AD + k) * n + site] = iE[(d * GR::AD + k) * n + site];
+    for (int k = 0; k < GR::AD; k++) {
+        const uint64_t i = fpos(d * GR::AD + k, site, 3 * GR::AD);
+        E[i] = iE[i];
+    }
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) Unext[(d * GR::S + k) * n + site] = iUn[(d * GR::S + k) * n + site];
+    for (int k = 0; k < GR::S; k++) {
+        const uint64_t i = fpos(d * GR::S + k, site, 3 * GR::S);
+        Unext[i] = iUn[i];
+    }
 }
 
 // single-element power for tests of the device arithmetic, in double as the damping evaluates it

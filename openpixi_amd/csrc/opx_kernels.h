@@ -6,6 +6,34 @@
 
 namespace opx {
 
+// Device layout of a cell field (links, E, J, rho and the scratch fields built like them): the
+// cells are cut into blocks of kCellBlock = 64 consecutive cells (one wavefront), and a block
+// holds every component of its cells, component-major: component c of cell s of a field with
+// `comps` components per cell sits at element ((s / 64) * comps + c) * 64 + s % 64. A wavefront
+// reading one component of 64 consecutive cells reads one contiguous run, and the components
+// of a cell lie within comps * 64 elements of each other (one array per component, 2^k cells
+// apart, sends every component of a cell to the same memory channels). A field of n cells holds
+// padded_cells(n) * comps elements.
+constexpr uint32_t kCellBlock = 64;
+__host__ __device__ __forceinline__ uint64_t fpos(uint32_t c, uint64_t s, uint32_t comps)
+{
+    return ((s >> 6) * comps + c) * 64 + (s & 63);
+}
+__host__ __device__ __forceinline__ uint64_t padded_cells(uint64_t n) { return (n + 63) & ~(uint64_t)63; }
+// byte offsets of element (c, s) for buffer instructions: the per-cell part and the per-component
+// part (uniform over a wavefront, kept in a scalar register)
+__host__ __device__ __forceinline__ uint32_t fvoff(uint32_t s, uint32_t comps, uint32_t sz)
+{
+    return ((s >> 6) * comps * 64 + (s & 63)) * sz;
+}
+__host__ __device__ __forceinline__ uint32_t fsoff(uint32_t c, uint32_t sz) { return c * 64 * sz; }
+// element (0, s) of a field: component c of cell s is fcell(f, s, comps)[c * 64]
+template <typename T>
+__host__ __device__ __forceinline__ T* fcell(T* f, uint32_t s, uint32_t comps)
+{
+    return f + (uint64_t)(s >> 6) * (comps * 64u) + (s & 63u);
+}
+
 // Inclusive box of cells; all = 1 means every cell.
 struct Box {
     int lo[3];
@@ -97,8 +125,9 @@ hipError_t launch_stream_update(int nc, int prec, const Lattice& L, const Box& a
 hipError_t gauss_apply_D(int nc, int prec, const Lattice& L, const void* U, const double* phi, double* F, hipStream_t st);
 hipError_t gauss_apply_Dt(int nc, int prec, const Lattice& L, const void* U, const double* F, const void* rho,
                           double* out, hipStream_t st);
-hipError_t gauss_to_double(int prec, uint64_t m, const void* a, double* b, hipStream_t st);
-hipError_t gauss_sub_into(int prec, uint64_t m, const double* Ed, const double* F, void* E, hipStream_t st);
+// field (block layout, m / n components per cell) <-> [component][cell] double work array
+hipError_t gauss_to_double(int prec, uint64_t m, uint32_t n, const void* a, double* b, hipStream_t st);
+hipError_t gauss_sub_into(int prec, uint64_t m, uint32_t n, const double* Ed, const double* F, void* E, hipStream_t st);
 size_t gauss_dot_scratch();
 hipError_t gauss_dot(uint64_t m, const double* a, const double* b, double* scratch, double* out, hipStream_t st);
 hipError_t gauss_axpy(uint64_t m, double alpha, const double* x, double* y, hipStream_t st);

@@ -1,10 +1,10 @@
 // opx_kernels.hip -- gfx950 kernels of the temporal-gauge Yang-Mills leapfrog.
 //
-// Device layout: structure of arrays. A link field is [dir][component][cell]
-// (component k of link (cell, dir) at ((dir*G)+k)*n + cell), an algebra field is
-// [dir][coefficient][cell], the charge density [coefficient][cell]. Consecutive
-// lanes of a wavefront touch consecutive cells, so every load and store is a
-// fully coalesced 256-byte (fp32) or 512-byte (fp64) wavefront access.
+// Device layout: 64-cell blocks (fpos in opx_kernels.h). A link field has 3 S components
+// per cell (component k of link (cell, dir) is dir*S + k), an algebra field 3 AD (coefficient k
+// of direction dir is dir*AD + k), the charge density AD. Consecutive lanes of a wavefront touch
+// consecutive cells of one block, so every load and store is a fully coalesced 256-byte (fp32)
+// or 512-byte (fp64) wavefront access.
 #include <mutex>
 #include <type_traits>
 
@@ -65,47 +65,49 @@ __device__ __forceinline__ uint32_t bwd(const Lattice& L, uint32_t site, uint32_
     return (c == 0) ? site + (n - 1) * s : site - s;
 }
 
+// link (site, dir) of a link field / coefficients of direction dir of an algebra field (3 AD
+// components per cell; dir 0 with comps = AD for the charge density)
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, T>::M load_link(const T* __restrict__ U, uint32_t n, uint32_t site, int dir)
+__device__ __forceinline__ typename Group<NC, T>::M load_link(const T* __restrict__ U, uint32_t site, int dir)
 {
     using GR = Group<NC, T>;
     typename GR::M m;
-    const T* p = U + (size_t)dir * GR::S * n + site;
+    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[(size_t)k * n];
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[k * 64];
     GR::complete(m);
     return m;
 }
 
 template <int NC, typename T>
-__device__ __forceinline__ void store_link(T* __restrict__ U, uint32_t n, uint32_t site, int dir,
-                                           const typename Group<NC, T>::M& m)
+__device__ __forceinline__ void store_link(T* __restrict__ U, uint32_t site, int dir, const typename Group<NC, T>::M& m)
 {
     using GR = Group<NC, T>;
-    T* p = U + (size_t)dir * GR::S * n + site;
+    T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) p[(size_t)k * n] = GR::gget(m, GR::slot(k));
+    for (int k = 0; k < GR::S; k++) p[k * 64] = GR::gget(m, GR::slot(k));
 }
 
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, T>::A load_alg(const T* __restrict__ X, uint32_t n, uint32_t site, int dir)
+__device__ __forceinline__ typename Group<NC, T>::A load_alg(const T* __restrict__ X, uint32_t site, int dir,
+                                                             uint32_t comps = 3 * Group<NC, T>::AD)
 {
     using GR = Group<NC, T>;
     typename GR::A a;
-    const T* p = X + (size_t)dir * GR::AD * n + site;
+    const T* p = fcell(X, site, comps) + dir * GR::AD * 64;
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[(size_t)k * n];
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[k * 64];
     return a;
 }
 
 template <int NC, typename T>
-__device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t n, uint32_t site, int dir,
-                                          typename Group<NC, T>::A a)
+__device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t site, int dir, typename Group<NC, T>::A a,
+                                          uint32_t comps = 3 * Group<NC, T>::AD)
 {
     using GR = Group<NC, T>;
-    T* p = X + (size_t)dir * GR::AD * n + site;
+    T* p = fcell(X, site, comps) + dir * GR::AD * 64;
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) p[(size_t)k * n] = aref(a, k);
+    for (int k = 0; k < GR::AD; k++) p[k * 64] = aref(a, k);
 }
 
 // Non-temporal cache policy on the per-link kernel's once-touched data (E read and
@@ -114,26 +116,28 @@ __device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t n, uint32_
 #define OPX_STREAMING 1
 #endif
 
-// Field accessors of the per-link kernel: component k of cell s of a field with
-// n cells. FieldB uses buffer instructions (component offset in an SGPR); FieldG
-// plain global pointers (fields of 4 GiB or more).
+// Field accessors of the fused kernels: component k of cell s of a field of n cells with
+// `comps` components per cell. FieldB uses buffer instructions (the component's offset in an
+// SGPR, the cell's in a VGPR); FieldG plain global pointers (fields of 4 GiB or more).
 template <typename T>
 struct FieldB {
     Buf<T> b;
-    uint32_t n;
-    __device__ __forceinline__ FieldB(const T* p, uint32_t n_, uint32_t comps) : b(p, comps * n_ * (uint32_t)sizeof(T)), n(n_) {}
-    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return b.ld(s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T)); }
-    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { b.st(v, s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T)); }
-    __device__ __forceinline__ T ld_once(uint32_t k, uint32_t s) const { return b.ld(s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T), OPX_STREAMING); }
-    __device__ __forceinline__ void st_once(uint32_t k, uint32_t s, T v) const { b.st(v, s * (uint32_t)sizeof(T), k * n * (uint32_t)sizeof(T), OPX_STREAMING); }
+    uint32_t bstride;   // bytes per 64-cell block
+    __device__ __forceinline__ FieldB(const T* p, uint32_t n_, uint32_t comps_)
+        : b(p, comps_ * (uint32_t)padded_cells(n_) * (uint32_t)sizeof(T)), bstride(comps_ * 64u * (uint32_t)sizeof(T)) {}
+    __device__ __forceinline__ uint32_t vo(uint32_t s) const { return (s >> 6) * bstride + (s & 63u) * (uint32_t)sizeof(T); }
+    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return b.ld(vo(s), fsoff(k, sizeof(T))); }
+    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { b.st(v, vo(s), fsoff(k, sizeof(T))); }
+    __device__ __forceinline__ T ld_once(uint32_t k, uint32_t s) const { return b.ld(vo(s), fsoff(k, sizeof(T)), OPX_STREAMING); }
+    __device__ __forceinline__ void st_once(uint32_t k, uint32_t s, T v) const { b.st(v, vo(s), fsoff(k, sizeof(T)), OPX_STREAMING); }
 };
 template <typename T>
 struct FieldG {
     T* p;
-    uint32_t n;
-    __device__ __forceinline__ FieldG(const T* p_, uint32_t n_, uint32_t) : p(const_cast<T*>(p_)), n(n_) {}
-    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return p[(size_t)k * n + s]; }
-    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { p[(size_t)k * n + s] = v; }
+    uint32_t comps;
+    __device__ __forceinline__ FieldG(const T* p_, uint32_t, uint32_t comps_) : p(const_cast<T*>(p_)), comps(comps_) {}
+    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return fcell(p, s, comps)[k * 64]; }
+    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { fcell(p, s, comps)[k * 64] = v; }
     __device__ __forceinline__ T ld_once(uint32_t k, uint32_t s) const { return ld(k, s); }
     __device__ __forceinline__ void st_once(uint32_t k, uint32_t s, T v) const { st(k, s, v); }
 };
@@ -578,8 +582,8 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
         const int d = (int)(local % 3);
         const uint32_t site = (local / 3) * kBlock + threadIdx.x;
         if (site >= E.L.n) continue;
-        const FieldG<T> U(Ubase + E.link_offset, E.L.n, 0), Un(Unbase + E.link_offset, E.L.n, 0);
-        const FieldG<T> Ef(Ebase + E.alg_offset, E.L.n, 0);
+        const FieldG<T> U(Ubase + E.link_offset, E.L.n, 3 * Group<NC, T>::S), Un(Unbase + E.link_offset, E.L.n, 3 * Group<NC, T>::S);
+        const FieldG<T> Ef(Ebase + E.alg_offset, E.L.n, 3 * Group<NC, T>::AD);
         link_update<NC, T, false>(E.L, U, Un, Ef, Ef, site, d);
     }
 }
@@ -629,9 +633,9 @@ __global__ __launch_bounds__(kBlock) void k_step_links(Lattice L, Box active, co
     if (site >= L.n) return;
     const Coord c = coords(L, site);
     if (!in_box(active, c.x0, c.x1, c.x2)) return;
-    const auto e = load_alg<NC, T>(E, L.n, site, d);
-    const auto u = load_link<NC, T>(U, L.n, site, d);
-    store_link<NC, T>(Unext, L.n, site, d, mul(exp_link(e, -(T)L.at), u));
+    const auto e = load_alg<NC, T>(E, site, d);
+    const auto u = load_link<NC, T>(U, site, d);
+    store_link<NC, T>(Unext, site, d, mul(exp_link(e, -(T)L.at), u));
 }
 
 // ---------------------------------------------------------------- kernel 3
@@ -644,14 +648,14 @@ __global__ __launch_bounds__(kBlock) void k_reunitarize(uint32_t n, T* __restric
     const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
     const int d = blockIdx.y;
     if (site >= n) return;
-    typename GR::M u = load_link<NC, T>(U, n, site, d);
+    typename GR::M u = load_link<NC, T>(U, site, d);
     typename GD::M ud;
 #pragma unroll
     for (int k = 0; k < GR::G; k++) GD::gref(ud, k) = (double)GR::gget(u, k);
     ud = reunit(ud);
 #pragma unroll
     for (int k = 0; k < GR::G; k++) GR::gref(u, k) = (T)GD::gget(ud, k);
-    store_link<NC, T>(U, n, site, d, u);
+    store_link<NC, T>(U, site, d, u);
 }
 
 // reprojection of every link of a batch (same static schedule as k_batch_update)
@@ -679,14 +683,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_reunitarize(const BatchEvent* 
         const uint32_t site = (local / 3) * kBlock + threadIdx.x;
         if (site >= E.L.n) continue;
         T* U = Ubase + E.link_offset;
-        typename GR::M u = load_link<NC, T>(U, E.L.n, site, d);
+        typename GR::M u = load_link<NC, T>(U, site, d);
         typename GD::M ud;
 #pragma unroll
         for (int k = 0; k < GR::G; k++) GD::gref(ud, k) = (double)GR::gget(u, k);
         ud = reunit(ud);
 #pragma unroll
         for (int k = 0; k < GR::G; k++) GR::gref(u, k) = (T)GD::gget(ud, k);
-        store_link<NC, T>(U, E.L.n, site, d, u);
+        store_link<NC, T>(U, site, d, u);
     }
 }
 
@@ -708,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_links_from_coeffs(uint32_t n, const 
     typename GR::M o;
 #pragma unroll
     for (int k = 0; k < GR::G; k++) GR::gref(o, k) = (T)GD::gget(u, k);
-    store_link<NC, T>(U, n, site, d, o);
+    store_link<NC, T>(U, site, d, o);
 }
 
 // ---------------------------------------------------------------- measurements
@@ -738,15 +742,14 @@ constexpr int kMeasureBlocks = 4096;
 constexpr int kNQ = 8;
 
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, double>::M load_link_d(const T* __restrict__ U, uint32_t n, uint32_t site,
-                                                                     int dir)
+__device__ __forceinline__ typename Group<NC, double>::M load_link_d(const T* __restrict__ U, uint32_t site, int dir)
 {
     using GD = Group<NC, double>;
     using GR = Group<NC, T>;
     typename GD::M m;
-    const T* p = U + (size_t)dir * GR::S * n + site;
+    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[(size_t)k * n];
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[k * 64];
     if constexpr (GR::S != GR::G) su3_third_row(m);   // rebuilt in double
     return m;
 }
@@ -756,23 +759,23 @@ template <int NC, typename T>
 __device__ __forceinline__ typename Group<NC, double>::M plaquette_d(const Lattice& L, const T* __restrict__ U,
                                                                      uint32_t site, const Coord& c, int j, int k)
 {
-    const uint32_t n = L.n;
     uint32_t xpj = fwd(L, site, coord_of(c, j), j);
     uint32_t xpk = fwd(L, site, coord_of(c, k), k);
-    return mul_adj(mul_adj(mul(load_link_d<NC, T>(U, n, site, j), load_link_d<NC, T>(U, n, xpj, k)),
-                           load_link_d<NC, T>(U, n, xpk, j)),
-                   load_link_d<NC, T>(U, n, site, k));
+    return mul_adj(mul_adj(mul(load_link_d<NC, T>(U, site, j), load_link_d<NC, T>(U, xpj, k)),
+                           load_link_d<NC, T>(U, xpk, j)),
+                   load_link_d<NC, T>(U, site, k));
 }
 
+// comps: 3 AD for E (the default), AD for the charge density
 template <int NC, typename T>
-__device__ __forceinline__ typename Group<NC, double>::A load_alg_d(const T* __restrict__ X, uint32_t n, uint32_t site,
-                                                                    int dir)
+__device__ __forceinline__ typename Group<NC, double>::A load_alg_d(const T* __restrict__ X, uint32_t site, int dir,
+                                                                    uint32_t comps = 3 * Group<NC, T>::AD)
 {
     using GD = Group<NC, double>;
     typename GD::A a;
-    const T* p = X + (size_t)dir * GD::AD * n + site;
+    const T* p = fcell(X, site, comps) + dir * GD::AD * 64;
 #pragma unroll
-    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[(size_t)k * n];
+    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[k * 64];
     return a;
 }
 
@@ -797,16 +800,16 @@ __device__ __forceinline__ double gauss2(const Lattice& L, const T* __restrict__
 #pragma unroll 1
     for (int i = 0; i < 3; i++) {
         uint32_t xm = bwd(L, site, coord_of(c, i), i);
-        auto em = load_alg_d<NC, T>(E, L.n, xm, i);
-        auto u = load_link_d<NC, T>(U, L.n, xm, i);
+        auto em = load_alg_d<NC, T>(E, xm, i);
+        auto u = load_link_d<NC, T>(U, xm, i);
         auto t = act(em, adj(u));
-        auto ex = load_alg_d<NC, T>(E, L.n, site, i);
+        auto ex = load_alg_d<NC, T>(E, site, i);
         double f = 1.0 / (L.as[i] * (L.g * L.as[i]));
 #pragma unroll
         for (int k = 0; k < GD::AD; k++) aref(g, k) += (aref(ex, k) - aref(t, k)) * f;
     }
     if (rho) {
-        auto r = load_alg_d<NC, T>(rho, L.n, site, 0);
+        auto r = load_alg_d<NC, T>(rho, site, 0, Group<NC, T>::AD);
 #pragma unroll
         for (int k = 0; k < GD::AD; k++) aref(g, k) -= aref(r, k);
     }
@@ -827,7 +830,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(Lattice L, Box eval, const T
         const Coord c = coords(L, site);
         if (!in_box(eval, c.x0, c.x1, c.x2)) continue;
 #pragma unroll
-        for (int i = 0; i < 3; i++) acc[i].add(square(load_alg_d<NC, T>(E, L.n, site, i)) * uf[i]);
+        for (int i = 0; i < 3; i++) acc[i].add(square(load_alg_d<NC, T>(E, site, i)) * uf[i]);
         // one direction at a time: the plaquettes of both time levels are the largest live state
 #pragma unroll 1
         for (int i = 0; i < 3; i++) {
@@ -840,7 +843,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(Lattice L, Box eval, const T
                 acc[5].add(v);
         }
         acc[6].add(gauss2<NC, T>(L, U, E, rho, site, c));
-        if (rho) acc[7].add(square(load_alg_d<NC, T>(rho, L.n, site, 0)));
+        if (rho) acc[7].add(square(load_alg_d<NC, T>(rho, site, 0, Group<NC, T>::AD)));
     }
     // wavefront reduction (64 lanes), then across the 4 waves of the block
     __shared__ double sh[kBlock / 64][kNQ][2];
@@ -918,7 +921,7 @@ __global__ __launch_bounds__(kBlock) void k_projected_energy(Lattice L, Box eval
         Coord c{x[0], x[1], x[2]};
         if (!in_box(eval, c.x0, c.x1, c.x2)) continue;
         for (int j = 0; j < 3; j++) {
-            double el = 0.5 * square(load_alg_d<NC, T>(E, L.n, site, j)) * uf[j];
+            double el = 0.5 * square(load_alg_d<NC, T>(E, site, j)) * uf[j];
             double mag = 0.25 * (b2<NC, T>(L, U, site, c, j) + b2<NC, T>(L, Unext, site, c, j)) * uf[j];
             if (j == dir) {
                 a[2].add(el);
@@ -971,8 +974,8 @@ __device__ __forceinline__ uint32_t site_of(const Lattice& L, const Coord& c) { 
 template <typename T>
 __device__ __forceinline__ Su2<double> link_or(const Lattice& L, const T* __restrict__ U, const Coord& c, int d, int o)
 {
-    if (o > 0) return load_link_d<2, T>(U, L.n, site_of(L, c), d);
-    return adj(load_link_d<2, T>(U, L.n, site_of(L, shifted(L, c, d, -1)), d));
+    if (o > 0) return load_link_d<2, T>(U, site_of(L, c), d);
+    return adj(load_link_d<2, T>(U, site_of(L, shifted(L, c, d, -1)), d));
 }
 
 // Grid.getPlaquette(index, d1, d2, o1, o2, t) (Grid.java:609-642)
@@ -1031,13 +1034,13 @@ __global__ __launch_bounds__(kBlock) void k_projected_poynting(Lattice L, int di
             const double ff = ufF[i] / 8.0;
             for (int k = 0; k < 3; k++) F.v[k] *= ff;
             const Coord cm = shifted(L, c, i, -1);
-            const Alg2<double> e2 = act(load_alg_d<2, T>(E, L.n, site_of(L, cm), i), link_or<T>(L, U, c, i, -1));
-            const Alg2<double> e1 = load_alg_d<2, T>(E, L.n, site, i);
+            const Alg2<double> e2 = act(load_alg_d<2, T>(E, site_of(L, cm), i), link_or<T>(L, U, c, i, -1));
+            const Alg2<double> e1 = load_alg_d<2, T>(E, site, i);
             Alg2<double> e;
             for (int k = 0; k < 3; k++) e.v[k] = (e1.v[k] + e2.v[k]) * (0.5 * ufEB[i]);
             s_avg += dot(e, F);
         }
-        Alg2<double> E1 = load_alg_d<2, T>(E, L.n, site, d1), E2 = load_alg_d<2, T>(E, L.n, site, d2);
+        Alg2<double> E1 = load_alg_d<2, T>(E, site, d1), E2 = load_alg_d<2, T>(E, site, d2);
         for (int k = 0; k < 3; k++) {
             E1.v[k] *= ufEB[d1];
             E2.v[k] *= ufEB[d2];
@@ -1092,7 +1095,7 @@ __global__ __launch_bounds__(kBlock) void k_health(uint32_t n, const T* __restri
     const uint32_t site = blockIdx.x * kBlock + threadIdx.x;
     const int d = blockIdx.y;
     if (site >= n) return;
-    auto u = load_link_d<NC, T>(U, n, site, d);
+    auto u = load_link_d<NC, T>(U, site, d);
     bool finite = true;
 #pragma unroll
     for (int k = 0; k < GD::G; k++) finite = finite && isfinite(GD::gget(u, k));
@@ -1168,19 +1171,17 @@ __global__ __launch_bounds__(kBlock) void k_sp_to_grid(Lattice L, SuperParticles
     typename GR::A Q;
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) aref(Q, m) = q[(size_t)m * sp.total + qi];
-    const auto U = load_link<NC, T>(Unext, L.n, index, 0);
+    const auto U = load_link<NC, T>(Unext, index, 0);
     const T f = (T)(L.as[0] / L.at);
     if (o > 0) {
-        T* Jc = J + index + quickfix;
 #pragma unroll
-        for (int m = 0; m < GR::AD; m++) atomicAdd(Jc + (size_t)m * L.n, aref(Q, m) * f);
+        for (int m = 0; m < GR::AD; m++) atomicAdd(J + fpos(m, index + quickfix, 3 * GR::AD), aref(Q, m) * f);
         Q = act(Q, adj(U));
     } else {
         Q = act(Q, U);
-        T* Jc = J + index;
         const T fm = (T)(-L.as[0] / L.at);
 #pragma unroll
-        for (int m = 0; m < GR::AD; m++) atomicAdd(Jc + (size_t)m * L.n, aref(Q, m) * fm);
+        for (int m = 0; m < GR::AD; m++) atomicAdd(J + fpos(m, index, 3 * GR::AD), aref(Q, m) * fm);
     }
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) q[(size_t)m * sp.total + qi] = aref(Q, m);
@@ -1207,7 +1208,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, int64_t 
     const uint32_t cell = cs.first[lo] + (i - cs.cells_prefix[lo]);
     typename GR::A a;
 #pragma unroll
-    for (int m = 0; m < GR::AD; m++) aref(a, m) = rho[(size_t)m * L.n + cell];
+    for (int m = 0; m < GR::AD; m++) aref(a, m) = rho[fpos(m, cell, GR::AD)];
     const uint32_t e1 = cs.list_prefix[lo + 1];
     for (uint32_t e = cs.list_prefix[lo]; e < e1; e++) {
         const size_t qi = (size_t)(cs.qbase[e] + (int64_t)cell);
@@ -1215,19 +1216,18 @@ __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, int64_t 
         for (int m = 0; m < GR::AD; m++) aref(a, m) += q[(size_t)m * q_total + qi];
     }
 #pragma unroll
-    for (int m = 0; m < GR::AD; m++) rho[(size_t)m * L.n + cell] = aref(a, m);
+    for (int m = 0; m < GR::AD; m++) rho[fpos(m, cell, GR::AD)] = aref(a, m);
 }
 
 dim3 grid_links(uint32_t n) { return dim3((n + kBlock - 1) / kBlock, 3, 1); }
 
-// component blockIdx.y of cells [first, first + count) of a [component][n] field set to zero
+// component blockIdx.y of cells [first, first + count) of a field of gridDim.y components set to zero
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_zero_cells(T* __restrict__ f, uint32_t n, uint32_t first,
-                                                       uint32_t count)
+__global__ __launch_bounds__(kBlock) void k_zero_cells(T* __restrict__ f, uint32_t first, uint32_t count)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= count) return;
-    f[(size_t)blockIdx.y * n + first + i] = T(0);
+    f[fpos(blockIdx.y, first + i, gridDim.y)] = T(0);
 }
 
 }  // namespace
@@ -1250,7 +1250,7 @@ hipError_t launch_combined_update(int nc, int prec, const Lattice& L, const Box&
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const dim3 g1(3 * order.blocks_per_tile_plane * L.d0 * tiles);
-        const bool buf = !L.force_global && (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
+        const bool buf = !L.force_global && padded_cells(L.n) * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
         if (buf && has_j)
             hipLaunchKernelGGL((k_combined_update<NC, T, true, true>), g1, dim3(kBlock), 0, st, L, order, active,
                                (const T*)U, (T*)Unext, (T*)E, (const T*)J);
@@ -1295,7 +1295,7 @@ hipError_t launch_cell_update(int nc, int prec, const Lattice& L, const Box& act
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const dim3 g1(order.blocks_per_tile_plane * L.d0 * tiles);
-        const bool buf = !L.force_global && (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
+        const bool buf = !L.force_global && padded_cells(L.n) * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
         if (reunit)
             launch_cell_variant<NC, T, true>(L, active, order, g1, buf, U, Unext, E, J, has_j, st);
         else
@@ -1313,7 +1313,7 @@ hipError_t launch_plaq_update(int nc, int prec, const Lattice& L, const Box& act
         order.blocks_per_tile_plane = (order.rows * L.d2 + kBlock - 1) / kBlock;
         const uint32_t tiles = (L.d1 + order.rows - 1) / order.rows;
         const uint32_t cell_blocks = order.blocks_per_tile_plane * L.d0 * tiles;
-        const bool buf = !L.force_global && (uint64_t)L.n * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
+        const bool buf = !L.force_global && padded_cells(L.n) * 3 * Group<NC, T>::S * sizeof(T) < (1ull << 32);
         const T* Jt = has_j ? (const T*)J : (const T*)E;
         if (buf) {
             hipLaunchKernelGGL((k_plaquettes<NC, T, true>), dim3(cell_blocks), dim3(kBlock), 0, st, L, order,
@@ -1471,9 +1471,10 @@ hipError_t launch_zero_cells(int prec, void* f, uint32_t n, uint32_t comps, uint
     if (count == 0 || comps == 0) return hipSuccess;
     const dim3 g((count + kBlock - 1) / kBlock, comps);
     if (prec == 0)
-        hipLaunchKernelGGL((k_zero_cells<double>), g, dim3(kBlock), 0, st, (double*)f, n, first, count);
+        hipLaunchKernelGGL((k_zero_cells<double>), g, dim3(kBlock), 0, st, (double*)f, first, count);
     else
-        hipLaunchKernelGGL((k_zero_cells<float>), g, dim3(kBlock), 0, st, (float*)f, n, first, count);
+        hipLaunchKernelGGL((k_zero_cells<float>), g, dim3(kBlock), 0, st, (float*)f, first, count);
+    (void)n;
     return hipGetLastError();
 }
 

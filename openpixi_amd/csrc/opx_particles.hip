@@ -92,9 +92,10 @@ __global__ __launch_bounds__(kPBlock) void k_p_rho_gather(Lattice L, uint32_t co
     if (i > 0 && keys[i - 1] == cell) return;
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) {
-        T v = rho[(size_t)m * L.n + cell];
+        const uint64_t r = fpos(m, cell, GR::AD);
+        T v = rho[r];
         for (uint32_t j = i; j < count && keys[j] == cell; j++) v += Q0[(size_t)m * count + vals[j]];
-        rho[(size_t)m * L.n + cell] = v;
+        rho[r] = v;
     }
 }
 
@@ -125,9 +126,9 @@ __global__ __launch_bounds__(kPBlock) void k_p_push(Lattice L, uint32_t count, c
         const bool forward = vel[(size_t)d * count + k] > 0;
         const uint32_t cell = cell_of(L, forward ? no : nn);
         typename GR::M U;
-        const T* p = Unext + (size_t)d * GR::S * L.n + cell;
+        const T* p = fcell(Unext, cell, 3 * GR::S) + d * GR::S * 64;
 #pragma unroll
-        for (int c = 0; c < GR::S; c++) GR::gref(U, GR::slot(c)) = p[(size_t)c * L.n];
+        for (int c = 0; c < GR::S; c++) GR::gref(U, GR::slot(c)) = p[c * 64];
         GR::complete(U);
         if (!forward) U = adj(U);
         store_charge<NC, T>(Q1, count, k, act(q0, adj(U)));
@@ -176,10 +177,10 @@ __global__ __launch_bounds__(kPBlock) void k_p_j_gather(Lattice L, uint32_t coun
     const int d = (int)(key / L.n);
     const uint32_t cell = key - (uint32_t)d * L.n;
     const T fp = (T)(L.as[d] / L.at), fm = (T)(-L.as[d] / L.at);
-    T* Jc = J + (size_t)d * GR::AD * L.n + cell;
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) {
-        T v = Jc[(size_t)m * L.n];
+        const uint64_t r = fpos(d * GR::AD + m, cell, 3 * GR::AD);
+        T v = J[r];
         for (uint32_t j = i; j < count && keys[j] == key; j++) {
             const uint32_t k = vals[j];
             const bool forward = vel[(size_t)d * count + k] > 0;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(kPBlock) void k_p_j_gather(Lattice L, uint32_t coun
             const T c = q * (forward ? fp : fm);
             v += c;
         }
-        Jc[(size_t)m * L.n] = v;
+        J[r] = v;
     }
 }
 

@@ -43,7 +43,7 @@ __device__ __forceinline__ void plane_link(const Lattice& L, const T (*sl)[NS], 
     using M = typename GR::M;
     using A = typename GR::A;
     constexpr int S = GR::S, AD = GR::AD;
-    const uint32_t n = L.n, sz = sizeof(T);
+    const uint32_t sz = sizeof(T);
     const uint32_t pp = p + 1 == L.d0 ? 0 : p + 1, pm = p == 0 ? L.d0 - 1 : p - 1;
     const uint32_t x1p = x1 + 1 == L.d1 ? 0 : x1 + 1, x1m = x1 == 0 ? L.d1 - 1 : x1 - 1;
     const uint32_t x2p = x2 + 1 == L.d2 ? 0 : x2 + 1, x2m = x2 == 0 ? L.d2 - 1 : x2 - 1;
@@ -61,7 +61,7 @@ __device__ __forceinline__ void plane_link(const Lattice& L, const T (*sl)[NS], 
             const uint32_t y2 = O2 > 0 ? x2p : (O2 < 0 ? x2m : x2);
             const uint32_t site = (O0 > 0 ? pp : pm) * L.s0 + y1 * L.s1 + y2;
 #pragma unroll
-            for (int k = 0; k < S; k++) GR::gref(m, GR::slot(k)) = bu.ld(site * sz, (uint32_t)(DIR * S + k) * n * sz);
+            for (int k = 0; k < S; k++) GR::gref(m, GR::slot(k)) = bu.ld(fvoff(site, 3 * S, sz), fsoff(DIR * S + k, sz));
         }
         GR::complete(m);
         return m;
@@ -98,19 +98,19 @@ __device__ __forceinline__ void plane_link(const Lattice& L, const T (*sl)[NS], 
     const T at = (T)L.at;
     A e;
 #pragma unroll
-    for (int k = 0; k < AD; k++) aref(e, k) = be.ld(site * sz, (uint32_t)(D * AD + k) * n * sz, 1);
+    for (int k = 0; k < AD; k++) aref(e, k) = be.ld(fvoff(site, 3 * AD, sz), fsoff(D * AD + k, sz), 1);
 #pragma unroll
     for (int k = 0; k < AD; k++) aref(e, k) += aget(pr, k) * at;
     if (HAS_J) {
         const T uf = (T)L.uf[D];
 #pragma unroll
-        for (int k = 0; k < AD; k++) aref(e, k) += Jp[(size_t)(D * AD + k) * n + site] * uf;
+        for (int k = 0; k < AD; k++) aref(e, k) += Jp[fpos(D * AD + k, site, 3 * AD)] * uf;
     }
 #pragma unroll
-    for (int k = 0; k < AD; k++) be.st(aget(e, k), site * sz, (uint32_t)(D * AD + k) * n * sz, 1);
+    for (int k = 0; k < AD; k++) be.st(aget(e, k), fvoff(site, 3 * AD, sz), fsoff(D * AD + k, sz), 1);
     const M un = mul(exp_link(e, -at), Ud);
 #pragma unroll
-    for (int k = 0; k < S; k++) bun.st(GR::gget(un, GR::slot(k)), site * sz, (uint32_t)(D * S + k) * n * sz, 1);
+    for (int k = 0; k < S; k++) bun.st(GR::gget(un, GR::slot(k)), fvoff(site, 3 * S, sz), fsoff(D * S + k, sz), 1);
 }
 
 template <int NC, typename T, int TY, int TX, bool HAS_J>
@@ -123,8 +123,8 @@ __global__ __launch_bounds__(3 * TY * TX) void k_plane_update(Lattice L, Box act
     constexpr int PW = TX + 2, NS = (TY + 2) * (TX + 2), CELLS = TY * TX, NTHREADS = 3 * CELLS;
     static_assert(CELLS % 64 == 0, "a wave updates links of one direction");
     __shared__ T sl[3 * S][NS];   // [dir * S + stored value][position]
-    const uint32_t n = L.n, sz = sizeof(T);
-    const Buf<T> bu(Up, 3u * S * n * sz), bun(Unp, 3u * S * n * sz), be(Ep, 3u * AD * n * sz);
+    const uint32_t np = (uint32_t)padded_cells(L.n), sz = sizeof(T);
+    const Buf<T> bu(Up, 3u * S * np * sz), bun(Unp, 3u * S * np * sz), be(Ep, 3u * AD * np * sz);
     // XCD k takes a contiguous range of (tile, plane) work, planes fastest
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(3 * TY * TX) void k_plane_update(Lattice L, Box act
         const int hy = i / PW, hx = i - hy * PW;
         const uint32_t site = p * L.s0 + pwrap(row0 - 1 + hy, L.d1) * L.s1 + pwrap(col0 - 1 + hx, L.d2);
 #pragma unroll
-        for (int k = 0; k < 3 * S; k++) sl[k][i] = bu.ld(site * sz, (uint32_t)k * n * sz);
+        for (int k = 0; k < 3 * S; k++) sl[k][i] = bu.ld(fvoff(site, 3 * S, sz), fsoff(k, sz));
     }
     __syncthreads();
     // direction of this wave's links (uniform: CELLS is a multiple of the wave size)
@@ -183,7 +183,7 @@ bool plane_supported(int nc, int prec, const Lattice& L, int shape)
 {
     if (shape < 0 || shape >= kPlaneShapes) return false;
     // buffer offsets are 32-bit
-    const uint64_t bytes = (uint64_t)L.n * 3 * (nc == 2 ? 4 : (prec == 0 ? 18 : 12)) * (prec == 0 ? 8 : 4);
+    const uint64_t bytes = padded_cells(L.n) * 3 * (nc == 2 ? 4 : (prec == 0 ? 18 : 12)) * (prec == 0 ? 8 : 4);
     return bytes < (1ull << 32);
 }
 

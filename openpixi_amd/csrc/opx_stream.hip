@@ -69,11 +69,11 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
     constexpr int SMEM = LINK_WORDS > CONTRIB_WORDS ? LINK_WORDS : CONTRIB_WORDS;
     __shared__ T smem[SMEM];
 
-    const uint32_t n = L.n;
+    const uint32_t np = (uint32_t)padded_cells(L.n);
     const uint32_t sz = sizeof(T);
-    const Buf<T> bu(Up, 3u * GR::S * n * sz);
-    const Buf<T> bun(Unp, 3u * GR::S * n * sz);
-    const Buf<T> be(Ep, 3u * AD * n * sz);
+    const Buf<T> bu(Up, 3u * GR::S * np * sz);
+    const Buf<T> bun(Unp, 3u * GR::S * np * sz);
+    const Buf<T> be(Ep, 3u * AD * np * sz);
 
     // XCD-aware block order: XCD k (= blockIdx % 8) gets a contiguous range of tiles
     const uint32_t nb = gridDim.x, b = blockIdx.x;
@@ -113,11 +113,11 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
     const T at = (T)L.at;
 
     auto load_links = [&](uint32_t site, M* u) {
-        const uint32_t vo = site * sz;
+        const uint32_t vo = fvoff(site, 3 * GR::S, sz);
 #pragma unroll
         for (int l = 0; l < 3; l++) {
 #pragma unroll
-            for (int k = 0; k < GR::S; k++) GR::gref(u[l], GR::slot(k)) = bu.ld(vo, (uint32_t)(l * GR::S + k) * n * sz);
+            for (int k = 0; k < GR::S; k++) GR::gref(u[l], GR::slot(k)) = bu.ld(vo, fsoff(l * GR::S + k, sz));
             GR::complete(u[l]);
         }
     };
@@ -165,13 +165,13 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
 #pragma unroll
             for (int d = 0; d < 3; d++)
 #pragma unroll
-                for (int k = 0; k < AD; k++) aref(e[d], k) = be.ld(site * sz, (uint32_t)(d * AD + k) * n * sz);
+                for (int k = 0; k < AD; k++) aref(e[d], k) = be.ld(fvoff(site, 3 * AD, sz), fsoff(d * AD + k, sz));
             if (HAS_J) {
-                const Buf<T> bj(Jp, 3u * AD * n * sz);
+                const Buf<T> bj(Jp, 3u * AD * np * sz);
 #pragma unroll
                 for (int d = 0; d < 3; d++)
 #pragma unroll
-                    for (int k = 0; k < AD; k++) aref(jv[d], k) = bj.ld(site * sz, (uint32_t)(d * AD + k) * n * sz);
+                    for (int k = 0; k < AD; k++) aref(jv[d], k) = bj.ld(fvoff(site, 3 * AD, sz), fsoff(d * AD + k, sz));
             }
         }
         // transported contributions
@@ -216,11 +216,11 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
                 axpy(e[d], f[d], at);
                 if (HAS_J) axpy(e[d], jv[d], (T)L.uf[d]);
 #pragma unroll
-                for (int k = 0; k < AD; k++) be.st(aget(e[d], k), site * sz, (uint32_t)(d * AD + k) * n * sz);
+                for (int k = 0; k < AD; k++) be.st(aget(e[d], k), fvoff(site, 3 * AD, sz), fsoff(d * AD + k, sz));
                 const M un = mul(exp_link(e[d], -at), ucur[d]);
 #pragma unroll
                 for (int k = 0; k < GR::S; k++)
-                    bun.st(GR::gget(un, GR::slot(k)), site * sz, (uint32_t)(d * GR::S + k) * n * sz);
+                    bun.st(GR::gget(un, GR::slot(k)), fvoff(site, 3 * GR::S, sz), fsoff(d * GR::S + k, sz));
             }
         }
         carry1 = act_adj(a01, ucur[0]);
@@ -279,7 +279,7 @@ bool stream_supported(int nc, int prec, const Lattice& L, int tile)
 {
     const uint64_t G = nc == 2 ? 4 : (prec == 0 ? 18 : 12);   // reals stored per link
     const uint64_t sz = prec == 0 ? 8 : 4;
-    if ((uint64_t)L.n * 3 * G * sz >= (1ull << 32)) return false;  // one buffer descriptor per field
+    if (padded_cells(L.n) * 3 * G * sz >= (1ull << 32)) return false;  // one buffer descriptor per field
     if (tile < 0 || tile >= kStreamTiles) return false;
     int ty, tx;
     stream_tile_shape(tile, &ty, &tx);

@@ -42,11 +42,12 @@ __device__ __forceinline__ bool in_box_tile(const Box& b, uint32_t x0, uint32_t 
            (int)x2 >= b.lo[2] && (int)x2 <= b.hi[2];
 }
 
+// SU(2) link field (12 components per cell) in the device block layout (fpos)
 template <typename T>
-__device__ __forceinline__ Q2<T> ld_link2(const T* __restrict__ U, uint32_t n, uint32_t site, int d)
+__device__ __forceinline__ Q2<T> ld_link2(const T* __restrict__ U, uint32_t site, int d)
 {
-    const T* p = U + (size_t)d * 4 * n + site;
-    return Q2<T>{p[0], p[(size_t)n], p[2 * (size_t)n], p[3 * (size_t)n]};
+    const T* p = fcell(U, site, 12) + 4 * d * 64;
+    return Q2<T>{p[0], p[64], p[128], p[192]};
 }
 
 // One block's march over planes [pb, pe) of the tile at (row0, col0) of lattice L.
@@ -60,14 +61,13 @@ __device__ __forceinline__ void tile_march(const Lattice& L, const Box& active, 
     constexpr int PW = TX + 2, NS = (TY + 2) * (TX + 2), NTHREADS = TY * TX;
     const int tid = threadIdx.x;
     const int ty = tid / TX, tx = tid - ty * TX;
-    const uint32_t n = L.n;
     auto load_plane = [&](uint32_t plane, int slot) {
         for (int i = tid; i < NS; i += NTHREADS) {
             const int hy = i / PW, hx = i - hy * PW;
             const uint32_t x1 = wrap((int)row0 - 1 + hy, L.d1), x2 = wrap((int)col0 - 1 + hx, L.d2);
             const uint32_t site = plane * L.s0 + x1 * L.s1 + x2;
 #pragma unroll
-            for (int d = 0; d < 3; d++) sl[slot][d][i] = ld_link2(U, n, site, d);
+            for (int d = 0; d < 3; d++) sl[slot][d][i] = ld_link2(U, site, d);
         }
     };
     const uint32_t x1 = row0 + (uint32_t)ty, x2 = col0 + (uint32_t)tx;
@@ -105,24 +105,23 @@ __device__ __forceinline__ void tile_march(const Lattice& L, const Box& active, 
                 }
                 const Q2<T> Ud = at_off(0, 0, 0, d);
                 const A pr = proj(mul(Ud, S));
-                T* pe_ = E + (size_t)d * 3 * n + site;
-                A e{{pe_[0], pe_[(size_t)n], pe_[2 * (size_t)n]}};
+                T* pe_ = fcell(E, site, 9) + 3 * d * 64;
+                A e{{pe_[0], pe_[64], pe_[128]}};
 #pragma unroll
                 for (int k = 0; k < 3; k++) e.v[k] += pr.v[k] * at;
                 if (HAS_J) {
-                    const T* pj = J + (size_t)d * 3 * n + site;
                     const T uf = (T)L.uf[d];
 #pragma unroll
-                    for (int k = 0; k < 3; k++) e.v[k] += pj[(size_t)k * n] * uf;
+                    for (int k = 0; k < 3; k++) e.v[k] += fcell(J, site, 9)[(3 * d + k) * 64] * uf;
                 }
 #pragma unroll
-                for (int k = 0; k < 3; k++) st_once<NT>(e.v[k], pe_ + (size_t)k * n);
+                for (int k = 0; k < 3; k++) st_once<NT>(e.v[k], pe_ + k * 64);
                 const Q2<T> un = mul(exp_link(e, -at), Ud);
-                T* pu = Un + (size_t)d * 4 * n + site;
+                T* pu = fcell(Un, site, 12) + 4 * d * 64;
                 st_once<NT>(un.e0, pu);
-                st_once<NT>(un.e1, pu + (size_t)n);
-                st_once<NT>(un.e2, pu + 2 * (size_t)n);
-                st_once<NT>(un.e3, pu + 3 * (size_t)n);
+                st_once<NT>(un.e1, pu + 64);
+                st_once<NT>(un.e2, pu + 128);
+                st_once<NT>(un.e3, pu + 192);
             }
         }
         __syncthreads();   // plane p-1 is read before it is overwritten
