@@ -154,8 +154,10 @@ int opx_reset_charge(opx_ctx *ctx);
  * OPX_OPT_TILE selects the STREAM tile shape (0: 16x16, 1: 8x32, 2: 16x32,
  * 3: 8x64, 4: 8x16 threads over (x1,x2)); OPX_OPT_CHUNKS the number of x0
  * chunks (0 = derived from the CU count). opx_autotune times every candidate
- * on scratch copies of E and Unext (the lattice state is not changed), keeps
- * the fastest and returns its time per launch in *best_ms. */
+ * on scratch copies of E and Unext (the lattice state is not changed) over reps
+ * launches, re-times the three fastest in four interleaved rounds of max(reps, 8)
+ * launches with no host synchronization between them, keeps the one with the
+ * fastest round and returns its time per launch in *best_ms. */
 /* OPX_OPT_GLOBAL_POINTERS 1: the per-link, cell and plaquette forms read their fields through
  * plain global pointers, as they do by themselves for fields of 4 GiB or more (0, the default:
  * buffer instructions below 4 GiB); for testing that path on small lattices. */
