@@ -690,12 +690,17 @@ int opx_store_fields(opx_ctx* ctx)
 int opx_reset_current(opx_ctx* ctx)
 {
     if (!ctx) return OPX_E_ARG;
-    // J is zero outside its slabs: only they are cleared
+    // J is zero outside its slabs: only they are cleared, all in one launch
     const uint32_t s0 = (uint32_t)ctx->dims[1] * (uint32_t)ctx->dims[2];
-    for (const auto& r : ctx->j_slabs)
-        OPX_TRY(opx::launch_zero_cells(ctx->prec, ctx->J, (uint32_t)ctx->n, 3 * ctx->AD, (uint32_t)r.first * s0,
-                                       (uint32_t)(r.second - r.first) * s0, ctx->stream),
-                "reset current");
+    opx::CellRanges cr{};
+    cr.start[0] = 0;
+    for (const auto& r : ctx->j_slabs) {
+        if (cr.count == opx::kMaxCurrentSlabs) break;   // the slab list never holds more
+        cr.first[cr.count] = (uint32_t)r.first * s0;
+        cr.start[cr.count + 1] = cr.start[cr.count] + (uint32_t)(r.second - r.first) * s0;
+        cr.count++;
+    }
+    OPX_TRY(opx::launch_zero_cell_ranges(ctx->prec, ctx->J, 3 * ctx->AD, cr, ctx->stream), "reset current");
     ctx->j_slabs.clear();
     ctx->j_nonzero = false;
     return OPX_OK;

@@ -255,9 +255,14 @@ hipError_t launch_projected_energy(int nc, int prec, const Lattice& L, const Box
 constexpr int kNoAllocMaxSlices = 4096;
 constexpr int kNoAllocMaxDevices = 64;
 hipError_t noalloc_scratch(int device, double** out);
-// zero components [0, comps) of cells [first, first + count) of a [component][n] field
-hipError_t launch_zero_cells(int prec, void* f, uint32_t n, uint32_t comps, uint32_t first, uint32_t count,
-                             hipStream_t st);
+// zero every component (comps per cell) of the cells of up to kMaxCurrentSlabs ranges
+// [first[r], first[r] + count[r]) of a field, in one launch
+struct CellRanges {
+    uint32_t first[kMaxCurrentSlabs];
+    uint32_t start[kMaxCurrentSlabs + 1];   // prefix sums of the counts
+    int count;
+};
+hipError_t launch_zero_cell_ranges(int prec, void* f, uint32_t comps, const CellRanges& r, hipStream_t st);
 hipError_t launch_health(int nc, int prec, uint32_t n, const void* U, double tol,
                          unsigned long long* counters, hipStream_t st);
 hipError_t launch_health_alg(int nc, int prec, uint32_t n_values, const void* X,

@@ -1221,13 +1221,17 @@ __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, int64_t 
 
 dim3 grid_links(uint32_t n) { return dim3((n + kBlock - 1) / kBlock, 3, 1); }
 
-// component blockIdx.y of cells [first, first + count) of a field of gridDim.y components set to zero
+// component blockIdx.y (of gridDim.y per cell) of the cells of the ranges set to zero; thread i takes
+// the i-th cell of the ranges laid end to end
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_zero_cells(T* __restrict__ f, uint32_t first, uint32_t count)
+__global__ __launch_bounds__(kBlock) void k_zero_cells(T* __restrict__ f, CellRanges r)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= count) return;
-    f[fpos(blockIdx.y, first + i, gridDim.y)] = T(0);
+    if (i >= r.start[r.count]) return;
+    int k = 0;
+#pragma unroll
+    for (int q = 1; q < kMaxCurrentSlabs; q++) k += (q < r.count) & (i >= r.start[q]);
+    f[fpos(blockIdx.y, r.first[k] + (i - r.start[k]), gridDim.y)] = T(0);
 }
 
 }  // namespace
@@ -1465,16 +1469,16 @@ hipError_t launch_sp_to_grid(int nc, int prec, const Lattice& L, const SuperPart
     return hipGetLastError();
 }
 
-hipError_t launch_zero_cells(int prec, void* f, uint32_t n, uint32_t comps, uint32_t first, uint32_t count,
-                             hipStream_t st)
+hipError_t launch_zero_cell_ranges(int prec, void* f, uint32_t comps, const CellRanges& r, hipStream_t st)
 {
-    if (count == 0 || comps == 0) return hipSuccess;
-    const dim3 g((count + kBlock - 1) / kBlock, comps);
+    if (r.count <= 0 || comps == 0) return hipSuccess;
+    const uint32_t total = r.start[r.count];
+    if (total == 0) return hipSuccess;
+    const dim3 g((total + kBlock - 1) / kBlock, comps);
     if (prec == 0)
-        hipLaunchKernelGGL((k_zero_cells<double>), g, dim3(kBlock), 0, st, (double*)f, first, count);
+        hipLaunchKernelGGL((k_zero_cells<double>), g, dim3(kBlock), 0, st, (double*)f, r);
     else
-        hipLaunchKernelGGL((k_zero_cells<float>), g, dim3(kBlock), 0, st, (float*)f, first, count);
-    (void)n;
+        hipLaunchKernelGGL((k_zero_cells<float>), g, dim3(kBlock), 0, st, (float*)f, r);
     return hipGetLastError();
 }
 
