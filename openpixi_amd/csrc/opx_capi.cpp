@@ -54,6 +54,11 @@ opx::Lattice lattice(const opx_ctx* c, double at)
         L.area[i] = 1.0 / std::pow(c->as[i], 2);
         L.as[i] = c->as[i];
     }
+    for (int d = 0; d < 3; d++) {
+        const int j1 = d == 0 ? 1 : 0, j2 = d == 2 ? 1 : 2;
+        L.cell_ratio[d] = L.area[j2] / L.area[j1];
+        L.cell_scale[d] = at * L.area[j1];
+    }
     L.g = c->g;
     L.force_global = c->force_global ? 1u : 0u;
     L.j_slabs = (uint32_t)c->j_slabs.size();

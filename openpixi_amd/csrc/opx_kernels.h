@@ -49,6 +49,10 @@ struct Lattice {
     double at;         // time step argument of step/stepLinks
     double uf[3];      // -g a_i a_t (FastTYMSolver.step unitFactor)
     double area[3];    // 1/a_i^2 (Grid.getStapleSum areaFactor)
+    // the cell form's staple-sum factors of direction d, whose planes are (d, j1), (d, j2), j1 < j2:
+    // cell_ratio[d] = area[j2] / area[j1], cell_scale[d] = at area[j1]
+    double cell_ratio[3];
+    double cell_scale[3];
     double as[3];      // lattice spacings
     double g;          // coupling
     uint32_t force_global = 0;   // 1: the fused kernels read through plain pointers below 4 GiB too

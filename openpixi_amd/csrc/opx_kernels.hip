@@ -359,31 +359,32 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
             return mul_adj(fload_link<NC, T>(U, xp[a], b), fload_link<NC, T>(U, xp[b], a));
         }
     };
-    // staple of link a in plane (a, b): P U_b(x)^+ + (U_a(x-b) U_b(x+a-b))^+ U_b(x-b)
-    auto staple_a = [&](int a, int b, const M& P) -> M {
+    // staple of link a in plane (a, b): P U_b(x)^+ + (U_a(x-b) U_b(x+a-b))^+ U_b(x-b); ub() gives U_b(x)
+    auto staple_a = [&](int a, int b, const M& P, auto ub) -> M {
         const uint32_t xpa_mb = bwd(L, xp[a], coord_of(c, b), b);
         M u, l;
         if constexpr (TWO_ROWS) {
-            u = mul_adj_r01(P, fload_link<NC, T>(U, site, b));
+            u = mul_adj_r01(P, ub());
             su3_third_row(u);
             M y = mul_r01(fload_link_r01<NC, T>(U, xm[b], a), fload_link<NC, T>(U, xpa_mb, b));
             su3_third_row(y);
             l = adj_mul_r01(y, fload_link<NC, T>(U, xm[b], b));
             su3_third_row(l);
         } else {
-            u = mul_adj(P, fload_link<NC, T>(U, site, b));
+            u = mul_adj(P, ub());
             l = adj_mul(mul(fload_link<NC, T>(U, xm[b], a), fload_link<NC, T>(U, xpa_mb, b)),
                         fload_link<NC, T>(U, xm[b], b));
         }
         acc(u, l);
         return u;
     };
-    // staple of link b in plane (a, b): (U_a(x) P)^+ + (U_b(x-a) U_a(x+b-a))^+ U_a(x-a)
-    auto staple_b = [&](int a, int b, const M& P) -> M {
+    // staple of link b in plane (a, b): (U_a(x) P)^+ + (U_b(x-a) U_a(x+b-a))^+ U_a(x-a); ua() gives
+    // U_a(x), of which the two stored rows are read in the two-row form
+    auto staple_b = [&](int a, int b, const M& P, auto ua) -> M {
         const uint32_t xpb_ma = bwd(L, xp[b], coord_of(c, a), a);
         M u, l;
         if constexpr (TWO_ROWS) {
-            M t = mul_r01(fload_link_r01<NC, T>(U, site, a), P);
+            M t = mul_r01(ua(), P);
             su3_third_row(t);
             u = adj(t);
             M y = mul_r01(fload_link_r01<NC, T>(U, xm[a], b), fload_link<NC, T>(U, xpb_ma, a));
@@ -391,7 +392,7 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
             l = adj_mul_r01(y, fload_link<NC, T>(U, xm[a], a));
             su3_third_row(l);
         } else {
-            u = adj(mul(fload_link<NC, T>(U, site, a), P));
+            u = adj(mul(ua(), P));
             l = adj_mul(mul(fload_link<NC, T>(U, xm[a], b), fload_link<NC, T>(U, xpb_ma, a)),
                         fload_link<NC, T>(U, xm[a], a));
         }
@@ -411,11 +412,12 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
         }
         return e;
     };
-    auto finish = [&](int d, const M& S, A e) {
-        const M Ud = fload_link<NC, T>(U, site, d);
+    // E(x) += fs proj(U_d(x) S) and the new link; ud() gives U_d(x)
+    auto finish = [&](int d, const M& S, A e, auto ud, T fs) {
+        const M Ud = ud();
         const A pr = proj(mul(Ud, S));
 #pragma unroll
-        for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * at;
+        for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * fs;
         fstore_alg_once<NC, T>(E, site, d, e);
         M nu;
         if constexpr (TWO_ROWS)
@@ -425,27 +427,69 @@ __device__ __forceinline__ void cell_update(const Lattice& L, const Fld& U, cons
         if constexpr (REUNIT) nu = reunit_link<NC, T>(nu);
         fstore_link_once<NC, T>(Unext, site, d, nu);
     };
-    M S0 = GR::zero(), S1 = GR::zero(), S2 = GR::zero();
+    // In float the staple sum of direction d over its planes (d, j1), (d, j2), j1 < j2, is kept as
+    // S_d = s_j1 + (a_j2^-2 / a_j1^-2) s_j2 and E gets at a_j1^-2 proj(U_d S_d) (Lattice::cell_ratio
+    // and cell_scale): one scaled accumulation per direction instead of two. In float each own link
+    // is also read once per use group: U_0 serves finish(0) and then staple_b(0, 2), U_1 finish(1)
+    // and then staple_b(1, 2), U_2 staple_a(1, 2) and finish(2) (a read after a store of the cell
+    // is not merged with an earlier one). The double form sums a_j^-2 (upper + lower) plane by plane
+    // and reads an own link at each use: with the float arrangement its launch takes 2 % longer.
     // E of all three directions (and the current term) is read before the first store to E: a load
-    // of E cannot move above a store to it, and read late its latency is exposed at every update
+    // of E cannot move above a store to it, and read late its latency is exposed at every update.
+    constexpr bool FLT = sizeof(T) == 4;
     const A e0 = e_start(0), e1 = e_start(1), e2 = e_start(2);
+    auto own = [&](int d) { return [&, d]() -> M { return fload_link<NC, T>(U, site, d); }; };
+    auto held = [](const M& m) { return [&m]() -> M { return m; }; };
+    auto first = [&](M& S, const M& s, int j) {
+        if constexpr (FLT) {
+            S = s;
+        } else {
+            S = GR::zero();
+            acc_scaled(S, s, (T)L.area[j]);
+        }
+    };
+    auto second = [&](M& S, const M& s, int d, int j) { acc_scaled(S, s, FLT ? (T)L.cell_ratio[d] : (T)L.area[j]); };
+    auto scale = [&](int d) -> T { return FLT ? (T)L.cell_scale[d] : at; };
+    M S0, S1, S2;
     {
         const M P = upper(0, 1);
-        acc_scaled(S0, staple_a(0, 1, P), (T)L.area[1]);
-        acc_scaled(S1, staple_b(0, 1, P), (T)L.area[0]);
+        first(S0, staple_a(0, 1, P, own(1)), 1);
+        if constexpr (TWO_ROWS)
+            first(S1, staple_b(0, 1, P, [&]() -> M { return fload_link_r01<NC, T>(U, site, 0); }), 0);
+        else
+            first(S1, staple_b(0, 1, P, own(0)), 0);
     }
-    {
-        const M P = upper(0, 2);
-        acc_scaled(S0, staple_a(0, 2, P), (T)L.area[2]);
-        finish(0, S0, e0);
-        acc_scaled(S2, staple_b(0, 2, P), (T)L.area[0]);
-    }
-    {
-        const M P = upper(1, 2);
-        acc_scaled(S1, staple_a(1, 2, P), (T)L.area[2]);
-        finish(1, S1, e1);
-        acc_scaled(S2, staple_b(1, 2, P), (T)L.area[1]);
-        finish(2, S2, e2);
+    if constexpr (FLT) {
+        {
+            const M P = upper(0, 2);
+            second(S0, staple_a(0, 2, P, own(2)), 0, 2);
+            const M U0 = own(0)();
+            finish(0, S0, e0, held(U0), scale(0));
+            first(S2, staple_b(0, 2, P, held(U0)), 0);
+        }
+        {
+            const M P = upper(1, 2);
+            const M U2 = own(2)();
+            second(S1, staple_a(1, 2, P, held(U2)), 1, 2);
+            const M U1 = own(1)();
+            finish(1, S1, e1, held(U1), scale(1));
+            second(S2, staple_b(1, 2, P, held(U1)), 2, 1);
+            finish(2, S2, e2, held(U2), scale(2));
+        }
+    } else {
+        {
+            const M P = upper(0, 2);
+            second(S0, staple_a(0, 2, P, own(2)), 0, 2);
+            finish(0, S0, e0, own(0), scale(0));
+            first(S2, staple_b(0, 2, P, own(0)), 0);
+        }
+        {
+            const M P = upper(1, 2);
+            second(S1, staple_a(1, 2, P, own(2)), 1, 2);
+            finish(1, S1, e1, own(1), scale(1));
+            second(S2, staple_b(1, 2, P, own(1)), 2, 1);
+            finish(2, S2, e2, own(2), scale(2));
+        }
     }
 }
 
