@@ -21,6 +21,9 @@ SOURCES = ["opx_kernels.hip", "opx_stream.hip", "opx_gauss.hip", "opx_adjoint.hi
            "opx_adjoint_capi.cpp", "opx_cgc_capi.cpp"]
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 are generated.
+# -amdgpu-disable-unclustered-high-rp-reschedule: the machine scheduler keeps its first schedule of
+# kernels at high register pressure instead of re-scheduling them unclustered; the double cell form
+# (one wave per SIMD) keeps more loads in flight and runs 4 % faster, the other forms are unchanged.
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -28,6 +31,7 @@ HIP_FLAGS = [
     "-fPIC",
     "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-disable-unclustered-high-rp-reschedule",
     "-Wall",
     "-Wno-unused-function",
 ]

@@ -208,6 +208,42 @@ def test_fp32_form_tracks_fp64(gpu, nc, kernel):
     assert abs(m_d[0:6].sum() - e_o) <= 1e-5 * abs(e_o)
 
 
+@pytest.mark.parametrize("kernel", [("cell", 0, None), ("cell", 1, None), ("link", None, None)],
+                         ids=["cell32rows", "cell16rows", "link"])
+@pytest.mark.parametrize("nc", [2, 3])
+def test_fp32_anisotropic_spacing_with_current(gpu, nc, kernel):
+    """Uneven spacings (the float cell form keeps each staple sum as s_j1 + (a_j1/a_j2)^2 s_j2 and
+    puts at / a_j1^2 into the E update) and a current, against the FP64 oracle."""
+    dims = (6, 40, 9)
+    n = int(np.prod(dims))
+    spacing = (1.0, 0.8, 1.3)
+    at, steps = 0.1, 20
+    print(f"fp32 anisotropic nc={nc} kernel={kernel}", flush=True)
+    U, E = make_state(nc, dims, 0.2, 0.5, seed=19)
+    rng = np.random.default_rng(19)
+    if nc == 3:
+        from openpixi_amd.inputs import su3_coeffs_to_storage
+
+        J = su3_coeffs_to_storage(rng.normal(0, 0.1, size=(n, 3, 8)))
+    else:
+        J = rng.normal(0, 0.1, size=(n, 3, O.asize(nc)))
+    og = oracle_grid(nc, dims, U, E, at, spacing, J=J)
+    dg = device_grid(nc, dims, U, E, at, spacing, precision="fp32", J=J)
+    dg.setKernel(*kernel)
+    from openpixi_amd import _lib
+
+    L = _lib.load()
+    og.step_links(at)
+    FastTYMSolver().stepLinks(dg, at)
+    for _ in range(steps):
+        og.store_fields()
+        og.step(at)
+    dg._check(L.opx_leapfrog(dg.handle, steps, 10))
+    assert rel_err(dg.getU(), og.U) < 2e-5
+    assert rel_err(dg.getE(), og.E) < 2e-5
+    assert rel_err(dg.getUnext(), og.Unext) < 2e-5
+
+
 @pytest.mark.parametrize("dims,active,with_j", [((4, 3, 64), None, False), ((3, 2, 128), None, True),
                                                 ((5, 3, 64), ((1, 1, 0), (3, 2, 63)), True),
                                                 ((3, 2, 128), ((0, 0, 1), (2, 1, 100)), False)],
