@@ -18,6 +18,7 @@ relative drift from the FP64 form for every observable.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import platform
@@ -171,6 +172,19 @@ def make_grid(precision, dims, seed=0, nc=NC):
     return g
 
 
+def ready_events(stream, count):
+    """`count` timing events, each recorded once on `stream` here. The first record of a new event
+    creates it, which can hold the host for milliseconds; between the synchronization that opens a
+    timed region and its first launch that leaves the GPU idle, and after an idle millisecond the
+    GPU's clock dips over its next tens of milliseconds of load (up to 20 % slower launches)."""
+    import torch
+
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(count)]
+    for e in evs:
+        e.record(stream)
+    return evs
+
+
 def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True):
     import torch
     from openpixi_amd import FieldMeasurements, _lib
@@ -183,10 +197,15 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
     # initial half step U(-dt/2) -> U(+dt/2), as Simulation.initialize
     g._check(L.opx_step_links(g.handle, DT))
     m0 = FieldMeasurements().measure(g)
+    ev0, ev1 = ready_events(stream, 2)
     # the autotuner times its candidates on copies of the fields; it runs last before the warm-up
     # steps, so that the GPU is under load from then until the timed region
     tune_ms = g.autotune(reps=3) if autotune else None
     kcfg = g.kernelConfig()
+    fused = kcfg["kernel"] == "cell"
+    per_launch = not fused and reunit != 0
+    step_flat = ready_events(stream, 2 * steps) if per_launch else []
+    step_ev = list(zip(step_flat[0::2], step_flat[1::2]))
     g._check(L.opx_leapfrog(g.handle, warmup, reunit))
     torch.cuda.synchronize()
     # The timed region: `steps` calls of opx_leapfrog(1 step), a HIP event pair on the kernels'
@@ -196,12 +215,8 @@ def time_form(precision, dims, steps, warmup, kernel_reps, stream, autotune=True
     # two launches holds the second back until the first has finished and its timestamp is written
     # (about 10 us per step in a kernel trace), so per-launch events are recorded only for a form
     # that runs the reprojection as a pass of its own, whose steps then leave out of the average.
-    fused = kcfg["kernel"] == "cell"
-    per_launch = not fused and reunit != 0
-    step_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(steps if per_launch else 0)]
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    # Its events exist before the warm-up (ready_events), so nothing but the launches follows the
+    # synchronization.
     t0 = time.perf_counter()
     ev0.record(stream)
     for i in range(steps):
@@ -296,11 +311,10 @@ def time_cpic(steps, stream, autotune=True):
     g.interpolateChargedensity(0)
     cg_iters, gb, ga = g.projectGauss(tol=1e-8, max_iter=5000)
     g.simulationInitialize()
+    e0, e1 = ready_events(stream, 2)
     tune_ms = g.autotune(reps=3) if autotune else None
     m0 = FieldMeasurements().measure(g)
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
     done = (steps // REUNIT_EVERY) * REUNIT_EVERY
     # every 10th Simulation step reprojects the new links inside its update (OPX_OPT_REUNIT_EVERY)
     g.setReunitarizeEvery(REUNIT_EVERY)
@@ -382,10 +396,9 @@ def time_batch(count, steps, warmup, stream, autotune=True):
         b, tune_ms = make_batch(precision, sides, stream, autotune)
         reunit = REUNIT_EVERY if precision == "fp32" else 0
         m0 = b.measure()
+        e0, e1 = ready_events(stream, 2)
         b.leapfrog(warmup, reunit, graph=True)
         torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         b.leapfrog(steps, reunit, graph=True)
         e1.record(stream)
@@ -519,10 +532,9 @@ def time_implicit(steps, iterations, damping, stream):
         reunit = REUNIT_EVERY if precision == "fp32" else 0
         g._check(L.opx_step_links(g.handle, DT))
         g._check(L.opx_set_field_solver(g.handle, _lib.SOLVER_IMPLICIT, iterations, damping))
+        ev0, ev1 = ready_events(stream, 2)
         g._check(L.opx_leapfrog(g.handle, 2, reunit))
         torch.cuda.synchronize()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         g._check(L.opx_leapfrog(g.handle, steps, reunit))
         ev1.record(stream)
@@ -711,6 +723,10 @@ def main():
         probe(dims, forms, parse_kernels(args.probe_kernels), nc=args.probe_colors)
         return
     stream = torch.cuda.Stream()
+    # no cyclic garbage collection during the run: a collection pause between a timed region's
+    # opening synchronization and its first launch would leave the GPU idle there (see
+    # ready_events); the bench's large arrays are freed by reference counting
+    gc.disable()
     results = {}
     for f in forms:
         results[f] = time_form(f, dims, args.steps, args.warmup, args.kernel_reps, stream, not args.no_autotune)
