@@ -1,9 +1,9 @@
 // opx_buf.h -- raw buffer loads/stores with the component offset in an SGPR.
 //
-// A field of n cells is one buffer resource; element (component k, cell s) is
-// read with voffset = s * sizeof(T) (per lane) and soffset = k * n * sizeof(T)
-// (uniform, held in a scalar register), so no per-load vector address
-// arithmetic is needed. Usable while the field is smaller than 4 GiB.
+// A field is one buffer resource in the 64-cell block layout; element (component k, cell s) is
+// read with voffset = the byte offset of cell s's block plus (s % 64) * sizeof(T) (per lane) and
+// soffset = k * 64 * sizeof(T) (uniform, held in a scalar register), so the components of a cell
+// share one vector address. Usable while the field is smaller than 4 GiB.
 #pragma once
 
 #include <hip/hip_runtime.h>
