@@ -85,6 +85,37 @@ __device__ __forceinline__ void st_alg(T* __restrict__ X, uint32_t, uint32_t sit
     for (int k = 0; k < GR::AD; k++) p[k * 64] = aget(a, k);
 }
 
+// the same accesses with the non-temporal hint, for data a launch touches once (E, J, iE, iUn):
+// the streamed fields then leave the neighbour links of the three link fields in L2
+template <int NC, typename T>
+__device__ __forceinline__ typename Group<NC, T>::A ld_alg_once(const T* __restrict__ X, uint32_t site, int dir)
+{
+    using GR = Group<NC, T>;
+    typename GR::A a;
+    const T* p = fcell(X, site, 3 * GR::AD) + dir * GR::AD * 64;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = __builtin_nontemporal_load(p + k * 64);
+    return a;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void st_alg_once(T* __restrict__ X, uint32_t site, int dir, const typename Group<NC, T>::A& a)
+{
+    using GR = Group<NC, T>;
+    T* p = fcell(X, site, 3 * GR::AD) + dir * GR::AD * 64;
+#pragma unroll
+    for (int k = 0; k < GR::AD; k++) __builtin_nontemporal_store(aget(a, k), p + k * 64);
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void st_link_once(T* __restrict__ U, uint32_t site, int dir, const typename Group<NC, T>::M& m)
+{
+    using GR = Group<NC, T>;
+    T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+#pragma unroll
+    for (int k = 0; k < GR::S; k++) __builtin_nontemporal_store(GR::gget(m, GR::slot(k)), p + k * 64);
+}
+
 // ------------------------------------------------------------------ logarithm and power
 // SU2GroupElement.getAlgebraElement (math/SU2GroupElement.java:234-251):
 // v = 2 asin|e| / |e| e, or 2 e when |e| < 1e-15
@@ -370,7 +401,7 @@ __global__ __launch_bounds__(kIB, 2) void k_implicit_iteration(Lattice L, LinkOr
     auto field = [&](int t) { return t < 0 ? Uold : (t == 0 ? U : iU); };
     auto link = [&](int t, uint32_t s, int dir) -> M { return ld_link<NC, T>(field(t), n, s, dir); };
     const T at = (T)L.at;
-    A e = ld_alg<NC, T>(E, n, site, d);
+    A e = ld_alg_once<NC, T>(E, site, d);
     const uint32_t xpd = step_site(L, x, d, 1);
     uint32_t xp_d[3];
     site_coords(L, xpd, xp_d);
@@ -409,16 +440,16 @@ __global__ __launch_bounds__(kIB, 2) void k_implicit_iteration(Lattice L, LinkOr
         for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(pr, k) * (at * T(0.5));
     }
     if (HAS_J) {
-        const A jj = ld_alg<NC, T>(J, n, site, d);
+        const A jj = ld_alg_once<NC, T>(J, site, d);
         const T uf = (T)L.uf[d];
 #pragma unroll
         for (int k = 0; k < GR::AD; k++) aref(e, k) += aget(jj, k) * uf;
     }
-    st_alg<NC, T>(iE, n, site, d, e);
+    st_alg_once<NC, T>(iE, site, d, e);
     const M V = mul(exp_link(e, -at), ld_link<NC, T>(U, n, site, d));
     // Unext = U^damping Unext^(1 - damping) on the implicit links
     const M out = damped_link(ld_link<NC, T>(iU, n, site, d), V, damping);
-    st_link<NC, T>(iUn, n, site, d, out);
+    st_link_once<NC, T>(iUn, site, d, out);
 }
 
 // ImplicitEnd: E and Unext of the active links from the implicit copies
