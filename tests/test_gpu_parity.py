@@ -439,6 +439,38 @@ def test_su3_128cube_fp32_cell_against_fp64_link(gpu):
         assert abs(m32[k] - m64[k]) <= 1e-7 * abs(m64[k]), (k, m32[k], m64[k])
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5)], ids=["fp64", "fp32"])
+def test_su3_128cube_cell_form_fields_against_per_link_form(gpu, precision, tol):
+    """The benchmark's kernel at its own launch shape: SU(3) 128^3 (the bench's seeded inputs,
+    Gauss-projected E, the initial half step), 20 leapfrog steps of the cell form with 16-row
+    tiles (eight tiles along x1, the autotuner's pick on MI355X) against the per-link form, every
+    entry of E and Unext compared (relative to the largest): 1e-12 in FP64, 1e-5 in FP32 (FP64
+    reunitarization every 10 steps in both)."""
+    from openpixi_amd import Grid, _lib, inputs
+
+    dims, nc, at, steps = (128, 128, 128), 3, 0.25, 20
+    n = int(np.prod(dims))
+    A = inputs.random_links_coefficients(n, nc, 0.1, 0)
+    Ec = inputs.random_efield_coefficients(n, nc, 0.05, 0)
+    L = _lib.load()
+    fields = []
+    for kernel in (("cell", 1, None), ("link", None, None)):
+        print(f"128^3 fields {precision} {kernel}", flush=True)
+        g = Grid(nc, dims, 1.0, at, 1.0, precision)
+        g.setLinksFromCoefficients(A)
+        g.setEFromCoefficients(Ec)
+        g.projectGauss(tol=1e-10, max_iter=4000)
+        g.setKernel(*kernel)
+        FastTYMSolver().stepLinks(g, at)
+        g._check(L.opx_leapfrog(g.handle, steps, 10 if precision == "fp32" else 0))
+        assert g.kernelConfig()["kernel"] == kernel[0]
+        fields.append((g.getE(), g.getUnext()))
+        g.close()
+    (e_cell, u_cell), (e_link, u_link) = fields
+    assert rel_err(e_cell, e_link) < tol
+    assert rel_err(u_cell, u_link) < tol
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("nc", [2, 3])
 def test_fields_from_coefficients(gpu, nc, precision):
