@@ -62,38 +62,38 @@ __device__ __forceinline__ Site step(const Lattice& L, Site c, int d, int o)
 }
 
 // link fields (U, Ubar: 12 components per cell) and algebra fields (E, Ebar: 9) in the device
-// block layout (fpos); the cell count argument is not needed to address them
+// layout (fpos); the cell count argument is not needed to address them
 template <typename T>
 __device__ __forceinline__ Q<T> ld_q(const T* __restrict__ U, uint32_t, uint32_t s, int d)
 {
-    const T* p = fcell(U, s, 12) + 4 * d * 64;
-    return Q<T>{p[0], p[64], p[128], p[192]};
+    const T* p = fcell(U, s, 12);
+    return Q<T>{p[fcomp<T>(4 * d, 12)], p[fcomp<T>(4 * d + 1, 12)], p[fcomp<T>(4 * d + 2, 12)], p[fcomp<T>(4 * d + 3, 12)]};
 }
 
 template <typename T>
 __device__ __forceinline__ void st_q(T* __restrict__ U, uint32_t, uint32_t s, int d, const Q<T>& q)
 {
-    T* p = fcell(U, s, 12) + 4 * d * 64;
-    p[0] = q.e0;
-    p[64] = q.e1;
-    p[128] = q.e2;
-    p[192] = q.e3;
+    T* p = fcell(U, s, 12);
+    p[fcomp<T>(4 * d, 12)] = q.e0;
+    p[fcomp<T>(4 * d + 1, 12)] = q.e1;
+    p[fcomp<T>(4 * d + 2, 12)] = q.e2;
+    p[fcomp<T>(4 * d + 3, 12)] = q.e3;
 }
 
 template <typename T>
 __device__ __forceinline__ V3<T> ld_v(const T* __restrict__ X, uint32_t, uint32_t s, int d)
 {
-    const T* p = fcell(X, s, 9) + 3 * d * 64;
-    return V3<T>{{p[0], p[64], p[128]}};
+    const T* p = fcell(X, s, 9);
+    return V3<T>{{p[fcomp<T>(3 * d, 9)], p[fcomp<T>(3 * d + 1, 9)], p[fcomp<T>(3 * d + 2, 9)]}};
 }
 
 template <typename T>
 __device__ __forceinline__ void st_v(T* __restrict__ X, uint32_t, uint32_t s, int d, const V3<T>& v)
 {
-    T* p = fcell(X, s, 9) + 3 * d * 64;
-    p[0] = v.v[0];
-    p[64] = v.v[1];
-    p[128] = v.v[2];
+    T* p = fcell(X, s, 9);
+    p[fcomp<T>(3 * d, 9)] = v.v[0];
+    p[fcomp<T>(3 * d + 1, 9)] = v.v[1];
+    p[fcomp<T>(3 * d + 2, 9)] = v.v[2];
 }
 
 template <typename T>

@@ -189,7 +189,7 @@ extern "C" int opx_adjoint_gradient(opx_ctx* ctx, int steps, int reunit_every, i
     for (uint32_t c = 0; c < n; c++)
         for (int d = 0; d < 3; d++)
             for (int k = 0; k < 3; k++) {
-                const size_t i = opx::fpos(d * 3 + k, c, 9);
+                const size_t i = opx::fpos(d * 3 + k, c, 9, ctx->prec == OPX_FP64 ? 8u : 4u);
                 grad[((size_t)c * 3 + d) * 3 + k] =
                     ctx->prec == OPX_FP64 ? ((const double*)h.data())[i] : (double)((const float*)h.data())[i];
             }

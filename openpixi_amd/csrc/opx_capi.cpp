@@ -128,7 +128,7 @@ std::vector<double> identity_links(const opx_ctx* c)
             const int e = stored_slot(c->S, k);
             const bool diag = c->nc == 2 ? e == 0 : (e == 0 || e == 4 || e == 8);
             if (!diag) continue;
-            for (int64_t s = 0; s < c->n; s++) dev[opx::fpos(d * c->S + k, s, 3 * c->S)] = 1.0;
+            for (int64_t s = 0; s < c->n; s++) dev[opx::fpos(d * c->S + k, s, 3 * c->S, c->tsz)] = 1.0;
         }
     return dev;
 }
@@ -489,7 +489,7 @@ int opx_upload_field(opx_ctx* ctx, int field, const double* host, int64_t n_valu
         for (int64_t s = 0; s < n; s++)
             for (int d = 0; d < 3; d++)
                 for (int k = 0; k < ctx->S; k++)
-                    dev[opx::fpos(d * ctx->S + k, s, 3 * ctx->S)] = host[(s * 3 + d) * ctx->G + stored_slot(ctx->S, k)];
+                    dev[opx::fpos(d * ctx->S + k, s, 3 * ctx->S, ctx->tsz)] = host[(s * 3 + d) * ctx->G + stored_slot(ctx->S, k)];
     } else {
         const int ndir = field == OPX_FIELD_RHO ? 1 : 3;
         if (n_values != n * ndir * ctx->AS) return fail(ctx, OPX_E_ARG, "upload: algebra buffer size mismatch");
@@ -504,7 +504,7 @@ int opx_upload_field(opx_ctx* ctx, int field, const double* host, int64_t n_valu
                 } else {
                     su3_storage_to_coeffs(v, c);
                 }
-                for (int k = 0; k < ctx->AD; k++) dev[opx::fpos(d * ctx->AD + k, s, comps)] = c[k];
+                for (int k = 0; k < ctx->AD; k++) dev[opx::fpos(d * ctx->AD + k, s, comps, ctx->tsz)] = c[k];
             }
         if (field == OPX_FIELD_J) extend_current_planes(ctx, 0, ctx->dims[0]);
         if (field == OPX_FIELD_RHO) ctx->rho_pending = false;
@@ -531,7 +531,7 @@ int opx_download_field(opx_ctx* ctx, int field, double* host, int64_t n_values)
         for (int64_t s = 0; s < n; s++)
             for (int d = 0; d < 3; d++) {
                 double* u = host + (s * 3 + d) * ctx->G;
-                for (int k = 0; k < ctx->S; k++) u[stored_slot(ctx->S, k)] = dev[opx::fpos(d * ctx->S + k, s, 3 * ctx->S)];
+                for (int k = 0; k < ctx->S; k++) u[stored_slot(ctx->S, k)] = dev[opx::fpos(d * ctx->S + k, s, 3 * ctx->S, ctx->tsz)];
                 if (ctx->S != ctx->G) su3_rebuild_third_row(u);
             }
         return OPX_OK;
@@ -545,7 +545,7 @@ int opx_download_field(opx_ctx* ctx, int field, double* host, int64_t n_values)
     for (int64_t s = 0; s < n; s++)
         for (int d = 0; d < ndir; d++) {
             double c[8];
-            for (int k = 0; k < ctx->AD; k++) c[k] = dev[opx::fpos(d * ctx->AD + k, s, comps)];
+            for (int k = 0; k < ctx->AD; k++) c[k] = dev[opx::fpos(d * ctx->AD + k, s, comps, ctx->tsz)];
             double* v = host + (s * ndir + d) * ctx->AS;
             if (ctx->nc == 2) {
                 v[0] = c[0]; v[1] = c[1]; v[2] = c[2];
@@ -590,7 +590,7 @@ int opx_set_efield_from_coefficients(opx_ctx* ctx, const double* coeffs, int64_t
     for (int64_t s = 0; s < n; s++)
         for (int d = 0; d < 3; d++)
             for (int k = 0; k < ctx->AD; k++)
-                dev[opx::fpos(d * ctx->AD + k, s, 3 * ctx->AD)] = coeffs[(s * 3 + d) * ctx->AD + k];
+                dev[opx::fpos(d * ctx->AD + k, s, 3 * ctx->AD, ctx->tsz)] = coeffs[(s * 3 + d) * ctx->AD + k];
     return to_device(ctx, ctx->E, dev);
 }
 

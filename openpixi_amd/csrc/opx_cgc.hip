@@ -63,9 +63,9 @@ __device__ __forceinline__ typename Group<NC, double>::M ld_link_d(const T* __re
     using GR = Group<NC, T>;
     using GD = Group<NC, double>;
     typename GD::M m = GD::zero();
-    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    const T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[k * 64];
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[fcomp<T>(dir * GR::S + k, 3 * GR::S)];
     if constexpr (GR::S != GR::G) su3_third_row(m);
     return m;
 }
@@ -76,9 +76,9 @@ __device__ __forceinline__ void st_link_d(T* __restrict__ U, uint32_t site, int 
 {
     using GR = Group<NC, T>;
     using GD = Group<NC, double>;
-    T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) p[k * 64] = (T)GD::gget(m, GR::slot(k));
+    for (int k = 0; k < GR::S; k++) p[fcomp<T>(dir * GR::S + k, 3 * GR::S)] = (T)GD::gget(m, GR::slot(k));
 }
 
 // algebra coefficients of direction dir of a context field in the device block layout (E: comps
@@ -89,9 +89,9 @@ __device__ __forceinline__ typename Group<NC, double>::A ld_field_alg(const T* _
 {
     using GD = Group<NC, double>;
     typename GD::A a;
-    const T* p = fcell(X, site, comps) + dir * GD::AD * 64;
+    const T* p = fcell(X, site, comps);
 #pragma unroll
-    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[k * 64];
+    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[fcomp<T>(dir * GD::AD + k, comps)];
     return a;
 }
 
@@ -100,9 +100,9 @@ __device__ __forceinline__ void st_field_alg(T* __restrict__ X, uint32_t site, i
                                              const typename Group<NC, double>::A& a)
 {
     using GD = Group<NC, double>;
-    T* p = fcell(X, site, comps) + dir * GD::AD * 64;
+    T* p = fcell(X, site, comps);
 #pragma unroll
-    for (int k = 0; k < GD::AD; k++) p[k * 64] = (T)aget(a, k);
+    for (int k = 0; k < GD::AD; k++) p[fcomp<T>(dir * GD::AD + k, comps)] = (T)aget(a, k);
 }
 
 // group element of a double [real][cell] array

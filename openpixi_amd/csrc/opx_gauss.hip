@@ -47,9 +47,9 @@ __device__ __forceinline__ typename Group<NC, double>::M link_d(const T* __restr
     using GD = Group<NC, double>;
     using GR = Group<NC, T>;
     typename GD::M m;
-    const T* p = fcell(U, site, 3 * GR::S) + d * GR::S * 64;
+    const T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[k * 64];
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[fcomp<T>(d * GR::S + k, 3 * GR::S)];
     if constexpr (GR::S != GR::G) su3_third_row(m);   // rebuilt in double
     return m;
 }
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kB) void k_apply_Dt(Lattice L, const T* __restrict_
     }
     if (rho) {
 #pragma unroll
-        for (int k = 0; k < GD::AD; k++) aref(g, k) -= (double)rho[fpos(k, site, GD::AD)];
+        for (int k = 0; k < GD::AD; k++) aref(g, k) -= (double)rho[fposT<T>(k, site, GD::AD)];
     }
 #pragma unroll
     for (int k = 0; k < GD::AD; k++) out[(size_t)k * L.n + site] = aget(g, k);
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kB) void k_restore_violation(Lattice L, const T* __
     for (int k = 0; k < GD::AD; k++) {
         const double v = aget(g, k) * factor;
         G[(size_t)k * L.n + site] = v;
-        R[(size_t)k * L.n + site] = v - (rho ? (double)rho[fpos(k, site, GD::AD)] : 0.0);
+        R[(size_t)k * L.n + site] = v - (rho ? (double)rho[fposT<T>(k, site, GD::AD)] : 0.0);
     }
 }
 
@@ -162,11 +162,11 @@ __global__ __launch_bounds__(kB) void k_restore_correct(Lattice L, const T* __re
     const uint32_t xp = nb_fwd(L, site, x[d], d);
     const auto t = act(alg_at<NC>(G, L.n, xp, 0), link_d<NC, T>(U, site, d));
     const auto gi = alg_at<NC>(G, L.n, site, 0);
-    T* e = fcell(E, site, 3 * GD::AD) + d * GD::AD * 64;
+    T* e = fcell(E, site, 3 * GD::AD);
 #pragma unroll
     for (int k = 0; k < GD::AD; k++) {
-        const double v = (double)e[k * 64] + (aget(t, k) - aget(gi, k)) * f2;
-        e[k * 64] = (T)v;
+        const double v = (double)e[fcomp<T>(d * GD::AD + k, 3 * GD::AD)] + (aget(t, k) - aget(gi, k)) * f2;
+        e[fcomp<T>(d * GD::AD + k, 3 * GD::AD)] = (T)v;
     }
 }
 
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kB) void k_to_double(uint64_t m, uint32_t n, const 
     const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
     if (i >= m) return;
     const uint32_t c = (uint32_t)(i / n), s = (uint32_t)(i - (uint64_t)c * n);
-    b[i] = (double)a[fpos(c, s, (uint32_t)(m / n))];
+    b[i] = (double)a[fposT<T>(c, s, (uint32_t)(m / n))];
 }
 
 // E (block layout) = Ed - F ([component][cell])
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kB) void k_sub_into(uint64_t m, uint32_t n, const d
     const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
     if (i >= m) return;
     const uint32_t c = (uint32_t)(i / n), s = (uint32_t)(i - (uint64_t)c * n);
-    E[fpos(c, s, (uint32_t)(m / n))] = (T)(Ed[i] - F[i]);
+    E[fposT<T>(c, s, (uint32_t)(m / n))] = (T)(Ed[i] - F[i]);
 }
 
 __device__ __forceinline__ void two_sum_merge(double& s1, double& c1, double s2, double c2)

@@ -28,15 +28,15 @@ OPX_HD double fatan2(double y, double x) { return atan2(y, x); }
 OPX_HD float fpow(float x, float y) { return powf(x, y); }
 OPX_HD double fpow(double x, double y) { return pow(x, y); }
 
-// fields in the device block layout (fpos); the cell count argument is not needed to address them
+// fields in the device layout (fpos); the cell count argument is not needed to address them
 template <int NC, typename T>
 __device__ __forceinline__ typename Group<NC, T>::M ld_link(const T* __restrict__ U, uint32_t, uint32_t site, int dir)
 {
     using GR = Group<NC, T>;
     typename GR::M m;
-    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    const T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[k * 64];
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[fcomp<T>(dir * GR::S + k, 3 * GR::S)];
     GR::complete(m);
     return m;
 }
@@ -48,9 +48,9 @@ __device__ __forceinline__ typename Group<NC, T>::M ld_link_r01(const T* __restr
 {
     using GR = Group<NC, T>;
     typename GR::M m = GR::zero();
-    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    const T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[k * 64];
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[fcomp<T>(dir * GR::S + k, 3 * GR::S)];
     return m;
 }
 
@@ -59,9 +59,9 @@ __device__ __forceinline__ void st_link(T* __restrict__ U, uint32_t, uint32_t si
                                         const typename Group<NC, T>::M& m)
 {
     using GR = Group<NC, T>;
-    T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) p[k * 64] = GR::gget(m, GR::slot(k));
+    for (int k = 0; k < GR::S; k++) p[fcomp<T>(dir * GR::S + k, 3 * GR::S)] = GR::gget(m, GR::slot(k));
 }
 
 template <int NC, typename T>
@@ -69,9 +69,9 @@ __device__ __forceinline__ typename Group<NC, T>::A ld_alg(const T* __restrict__
 {
     using GR = Group<NC, T>;
     typename GR::A a;
-    const T* p = fcell(X, site, 3 * GR::AD) + dir * GR::AD * 64;
+    const T* p = fcell(X, site, 3 * GR::AD);
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[k * 64];
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[fcomp<T>(dir * GR::AD + k, 3 * GR::AD)];
     return a;
 }
 
@@ -80,9 +80,9 @@ __device__ __forceinline__ void st_alg(T* __restrict__ X, uint32_t, uint32_t sit
                                        const typename Group<NC, T>::A& a)
 {
     using GR = Group<NC, T>;
-    T* p = fcell(X, site, 3 * GR::AD) + dir * GR::AD * 64;
+    T* p = fcell(X, site, 3 * GR::AD);
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) p[k * 64] = aget(a, k);
+    for (int k = 0; k < GR::AD; k++) p[fcomp<T>(dir * GR::AD + k, 3 * GR::AD)] = aget(a, k);
 }
 
 // the same accesses with the non-temporal hint, for data a launch touches once (E, J, iE, iUn):
@@ -92,9 +92,9 @@ __device__ __forceinline__ typename Group<NC, T>::A ld_alg_once(const T* __restr
 {
     using GR = Group<NC, T>;
     typename GR::A a;
-    const T* p = fcell(X, site, 3 * GR::AD) + dir * GR::AD * 64;
+    const T* p = fcell(X, site, 3 * GR::AD);
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) aref(a, k) = __builtin_nontemporal_load(p + k * 64);
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = __builtin_nontemporal_load(p + fcomp<T>(dir * GR::AD + k, 3 * GR::AD));
     return a;
 }
 
@@ -102,18 +102,18 @@ template <int NC, typename T>
 __device__ __forceinline__ void st_alg_once(T* __restrict__ X, uint32_t site, int dir, const typename Group<NC, T>::A& a)
 {
     using GR = Group<NC, T>;
-    T* p = fcell(X, site, 3 * GR::AD) + dir * GR::AD * 64;
+    T* p = fcell(X, site, 3 * GR::AD);
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) __builtin_nontemporal_store(aget(a, k), p + k * 64);
+    for (int k = 0; k < GR::AD; k++) __builtin_nontemporal_store(aget(a, k), p + fcomp<T>(dir * GR::AD + k, 3 * GR::AD));
 }
 
 template <int NC, typename T>
 __device__ __forceinline__ void st_link_once(T* __restrict__ U, uint32_t site, int dir, const typename Group<NC, T>::M& m)
 {
     using GR = Group<NC, T>;
-    T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) __builtin_nontemporal_store(GR::gget(m, GR::slot(k)), p + k * 64);
+    for (int k = 0; k < GR::S; k++) __builtin_nontemporal_store(GR::gget(m, GR::slot(k)), p + fcomp<T>(dir * GR::S + k, 3 * GR::S));
 }
 
 // ------------------------------------------------------------------ logarithm and power
@@ -467,12 +467,12 @@ __global__ __launch_bounds__(kIB) void k_implicit_end(Lattice L, Box active, con
     if (!in_region(active, x)) return;
 #pragma unroll
     for (int k = 0; k < GR::AD; k++) {
-        const uint64_t i = fpos(d * GR::AD + k, site, 3 * GR::AD);
+        const uint64_t i = fposT<T>(d * GR::AD + k, site, 3 * GR::AD);
         E[i] = iE[i];
     }
 #pragma unroll
     for (int k = 0; k < GR::S; k++) {
-        const uint64_t i = fpos(d * GR::S + k, site, 3 * GR::S);
+        const uint64_t i = fposT<T>(d * GR::S + k, site, 3 * GR::S);
         Unext[i] = iUn[i];
     }
 }

@@ -8,30 +8,61 @@ namespace opx {
 
 // Device layout of a cell field (links, E, J, rho and the scratch fields built like them): the
 // cells are cut into blocks of kCellBlock = 64 consecutive cells (one wavefront), and a block
-// holds every component of its cells, component-major: component c of cell s of a field with
-// `comps` components per cell sits at element ((s / 64) * comps + c) * 64 + s % 64. A wavefront
-// reading one component of 64 consecutive cells reads one contiguous run, and the components
-// of a cell lie within comps * 64 elements of each other (one array per component, 2^k cells
-// apart, sends every component of a cell to the same memory channels). A field of n cells holds
-// padded_cells(n) * comps elements.
+// holds every component of its cells. Within a block the components come in groups of V: group g
+// holds components gV .. gV + V - 1 of the block's 64 cells, cell by cell (V values of a cell
+// adjacent), so component c of cell s of a field with `comps` components per cell sits at element
+//   ((s / 64) * comps + (c / V) * V) * 64 + (s % 64) * V + c % V.
+// V = 2 for the double fields of SU(3) (links 54, algebra 24, charge density 8 components per
+// cell): a lane reads a pair of its cell's components in one 16-byte access, a wavefront one
+// contiguous 1 KiB run; per-direction counts (18 link reals, 8 coefficients) are even, so each
+// direction starts a pair. Every other field keeps V = 1 (one component of 64 cells contiguous):
+// four floats per 16-byte access made the SU(3) float cell form no faster and the SU(2) float
+// cell form 7 % slower, and pairs made the SU(2) double cell form 6.5 % slower; the SU(2) algebra
+// fields (3 per direction) could not be paired anyway. The components of a cell lie within comps * 64 elements of each
+// other (one array per component, 2^k cells apart, sends every component of a cell to the same
+// memory channels). A field of n cells holds padded_cells(n) * comps elements.
 constexpr uint32_t kCellBlock = 64;
-__host__ __device__ __forceinline__ uint64_t fpos(uint32_t c, uint64_t s, uint32_t comps)
+__host__ __device__ constexpr uint32_t fvec(uint32_t comps, uint32_t tsz)
 {
-    return ((s >> 6) * comps + c) * 64 + (s & 63);
+    return (tsz == 8 && comps % 2 == 0 && comps != 12) ? 2u : 1u;
+}
+// the group width of the link and algebra fields of SU(NC) in T (as fvec: SU(3) double fields only)
+template <int NC, typename T>
+constexpr uint32_t kGroup = (NC == 3 && sizeof(T) == 8) ? 2u : 1u;
+// element index of component c of cell s (tsz: bytes per real of the field)
+__host__ __device__ __forceinline__ uint64_t fpos(uint32_t c, uint64_t s, uint32_t comps, uint32_t tsz)
+{
+    const uint32_t v = fvec(comps, tsz);
+    return ((s >> 6) * comps + (c / v) * v) * 64 + (s & 63) * v + c % v;
+}
+template <typename T>
+__host__ __device__ __forceinline__ uint64_t fposT(uint32_t c, uint64_t s, uint32_t comps)
+{
+    return fpos(c, s, comps, (uint32_t)sizeof(T));
 }
 __host__ __device__ __forceinline__ uint64_t padded_cells(uint64_t n) { return (n + 63) & ~(uint64_t)63; }
 // byte offsets of element (c, s) for buffer instructions: the per-cell part and the per-component
 // part (uniform over a wavefront, kept in a scalar register)
 __host__ __device__ __forceinline__ uint32_t fvoff(uint32_t s, uint32_t comps, uint32_t sz)
 {
-    return ((s >> 6) * comps * 64 + (s & 63)) * sz;
+    return ((s >> 6) * comps * 64 + (s & 63) * fvec(comps, sz)) * sz;
 }
-__host__ __device__ __forceinline__ uint32_t fsoff(uint32_t c, uint32_t sz) { return c * 64 * sz; }
-// element (0, s) of a field: component c of cell s is fcell(f, s, comps)[c * 64]
+__host__ __device__ __forceinline__ uint32_t fsoff(uint32_t c, uint32_t comps, uint32_t sz)
+{
+    const uint32_t v = fvec(comps, sz);
+    return ((c / v) * v * 64 + c % v) * sz;
+}
+// the first element of cell s (component 0); component c is fcell(f, s, comps)[fcomp<T>(c, comps)]
 template <typename T>
 __host__ __device__ __forceinline__ T* fcell(T* f, uint32_t s, uint32_t comps)
 {
-    return f + (uint64_t)(s >> 6) * (comps * 64u) + (s & 63u);
+    return f + (uint64_t)(s >> 6) * (comps * 64u) + (s & 63u) * fvec(comps, (uint32_t)sizeof(T));
+}
+template <typename T>
+__host__ __device__ constexpr uint32_t fcomp(uint32_t c, uint32_t comps)
+{
+    return (c / fvec(comps, (uint32_t)sizeof(T))) * fvec(comps, (uint32_t)sizeof(T)) * 64 +
+           c % fvec(comps, (uint32_t)sizeof(T));
 }
 
 // Inclusive box of cells; all = 1 means every cell.

@@ -1,10 +1,10 @@
 // opx_kernels.hip -- gfx950 kernels of the temporal-gauge Yang-Mills leapfrog.
 //
-// Device layout: 64-cell blocks (fpos in opx_kernels.h). A link field has 3 S components
-// per cell (component k of link (cell, dir) is dir*S + k), an algebra field 3 AD (coefficient k
-// of direction dir is dir*AD + k), the charge density AD. Consecutive lanes of a wavefront touch
-// consecutive cells of one block, so every load and store is a fully coalesced 256-byte (fp32)
-// or 512-byte (fp64) wavefront access.
+// Device layout: 64-cell blocks, double components in pairs per cell (fpos in opx_kernels.h). A
+// link field has 3 S components per cell (component k of link (cell, dir) is dir*S + k), an
+// algebra field 3 AD (coefficient k of direction dir is dir*AD + k), the charge density AD.
+// Consecutive lanes of a wavefront touch consecutive cells of one block, so every access of a
+// component (or pair) is one contiguous wavefront access: 256 B (float), 1 KiB (double pair).
 #include <mutex>
 #include <type_traits>
 
@@ -72,9 +72,9 @@ __device__ __forceinline__ typename Group<NC, T>::M load_link(const T* __restric
 {
     using GR = Group<NC, T>;
     typename GR::M m;
-    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    const T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[k * 64];
+    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = p[fcomp<T>(dir * GR::S + k, 3 * GR::S)];
     GR::complete(m);
     return m;
 }
@@ -83,9 +83,9 @@ template <int NC, typename T>
 __device__ __forceinline__ void store_link(T* __restrict__ U, uint32_t site, int dir, const typename Group<NC, T>::M& m)
 {
     using GR = Group<NC, T>;
-    T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) p[k * 64] = GR::gget(m, GR::slot(k));
+    for (int k = 0; k < GR::S; k++) p[fcomp<T>(dir * GR::S + k, 3 * GR::S)] = GR::gget(m, GR::slot(k));
 }
 
 template <int NC, typename T>
@@ -94,9 +94,9 @@ __device__ __forceinline__ typename Group<NC, T>::A load_alg(const T* __restrict
 {
     using GR = Group<NC, T>;
     typename GR::A a;
-    const T* p = fcell(X, site, comps) + dir * GR::AD * 64;
+    const T* p = fcell(X, site, comps);
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[k * 64];
+    for (int k = 0; k < GR::AD; k++) aref(a, k) = p[fcomp<T>(dir * GR::AD + k, comps)];
     return a;
 }
 
@@ -105,9 +105,9 @@ __device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t site, int 
                                           uint32_t comps = 3 * Group<NC, T>::AD)
 {
     using GR = Group<NC, T>;
-    T* p = fcell(X, site, comps) + dir * GR::AD * 64;
+    T* p = fcell(X, site, comps);
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) p[k * 64] = aref(a, k);
+    for (int k = 0; k < GR::AD; k++) p[fcomp<T>(dir * GR::AD + k, comps)] = aref(a, k);
 }
 
 // Non-temporal cache policy on the per-link kernel's once-touched data (E read and
@@ -118,37 +118,89 @@ __device__ __forceinline__ void store_alg(T* __restrict__ X, uint32_t site, int 
 
 // Field accessors of the fused kernels: component k of cell s of a field of n cells with
 // `comps` components per cell. FieldB uses buffer instructions (the component's offset in an
-// SGPR, the cell's in a VGPR); FieldG plain global pointers (fields of 4 GiB or more).
-template <typename T>
+// SGPR, the cell's in a VGPR); FieldG plain global pointers (fields of 4 GiB or more). FieldB's G
+// is the field's group width (fvec; kGroup<NC, T> for the link and algebra fields of the fused
+// kernels): with vw() > 1, ldv/stv move the G components of group k0 / G of a cell at once, as
+// one 16-byte load and two 8-byte stores. Stores are kept to 8 bytes: 16-byte
+// stores whose last data register was rewritten by the next instruction stored a wrong value in
+// some lanes on MI355X.
+template <typename T, uint32_t G = 1>
 struct FieldB {
     Buf<T> b;
     uint32_t bstride;   // bytes per 64-cell block
     __device__ __forceinline__ FieldB(const T* p, uint32_t n_, uint32_t comps_)
         : b(p, comps_ * (uint32_t)padded_cells(n_) * (uint32_t)sizeof(T)), bstride(comps_ * 64u * (uint32_t)sizeof(T)) {}
-    __device__ __forceinline__ uint32_t vo(uint32_t s) const { return (s >> 6) * bstride + (s & 63u) * (uint32_t)sizeof(T); }
-    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return b.ld(vo(s), fsoff(k, sizeof(T))); }
-    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { b.st(v, vo(s), fsoff(k, sizeof(T))); }
-    __device__ __forceinline__ T ld_once(uint32_t k, uint32_t s) const { return b.ld(vo(s), fsoff(k, sizeof(T)), OPX_STREAMING); }
-    __device__ __forceinline__ void st_once(uint32_t k, uint32_t s, T v) const { b.st(v, vo(s), fsoff(k, sizeof(T)), OPX_STREAMING); }
+    __device__ __forceinline__ uint32_t vw() const { return G; }
+    __device__ __forceinline__ uint32_t vo(uint32_t s) const { return (s >> 6) * bstride + (s & 63u) * G * (uint32_t)sizeof(T); }
+    __device__ __forceinline__ uint32_t so(uint32_t k) const { return ((k / G) * G * 64u + k % G) * (uint32_t)sizeof(T); }
+    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return b.ld(vo(s), so(k)); }
+    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T x) const { b.st(x, vo(s), so(k)); }
+    __device__ __forceinline__ T ld_once(uint32_t k, uint32_t s) const { return b.ld(vo(s), so(k), OPX_STREAMING); }
+    __device__ __forceinline__ void st_once(uint32_t k, uint32_t s, T x) const { b.st(x, vo(s), so(k), OPX_STREAMING); }
+    __device__ __forceinline__ void ldv(uint32_t k0, uint32_t s, T* out, int once) const { b.ld16(out, vo(s), so(k0), once); }
+    __device__ __forceinline__ void stv(uint32_t k0, uint32_t s, const T* in, int once) const { b.st16(in, vo(s), so(k0), once); }
 };
 template <typename T>
 struct FieldG {
     T* p;
     uint32_t comps;
     __device__ __forceinline__ FieldG(const T* p_, uint32_t, uint32_t comps_) : p(const_cast<T*>(p_)), comps(comps_) {}
-    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return fcell(p, s, comps)[k * 64]; }
-    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { fcell(p, s, comps)[k * 64] = v; }
+    __device__ __forceinline__ uint32_t vw() const { return 1; }
+    __device__ __forceinline__ T ld(uint32_t k, uint32_t s) const { return fcell(p, s, comps)[fcomp<T>(k, comps)]; }
+    __device__ __forceinline__ void st(uint32_t k, uint32_t s, T v) const { fcell(p, s, comps)[fcomp<T>(k, comps)] = v; }
     __device__ __forceinline__ T ld_once(uint32_t k, uint32_t s) const { return ld(k, s); }
     __device__ __forceinline__ void st_once(uint32_t k, uint32_t s, T v) const { st(k, s, v); }
+    __device__ __forceinline__ void ldv(uint32_t, uint32_t, T*, int) const {}
+    __device__ __forceinline__ void stv(uint32_t, uint32_t, const T*, int) const {}
 };
 
+// N consecutive components from `base` of cell `site` (base and N multiples of the group width)
+template <int N, typename T, typename F>
+__device__ __forceinline__ void fld_n(const F& f, uint32_t base, uint32_t site, T* out, int once)
+{
+    const uint32_t w = f.vw();
+    if (w > 1) {
+#pragma unroll
+        for (int k = 0; k < N; k += (int)(16 / sizeof(T))) f.ldv(base + k, site, out + k, once);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) out[k] = once ? f.ld_once(base + k, site) : f.ld(base + k, site);
+    }
+}
+template <int N, typename T, typename F>
+__device__ __forceinline__ void fst_n(const F& f, uint32_t base, uint32_t site, const T* in, int once)
+{
+    const uint32_t w = f.vw();
+    if (w > 1) {
+#pragma unroll
+        for (int k = 0; k < N; k += (int)(16 / sizeof(T))) f.stv(base + k, site, in + k, once);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            if (once)
+                f.st_once(base + k, site, in[k]);
+            else
+                f.st(base + k, site, in[k]);
+        }
+    }
+}
+
+// Each accessor moves the components one by one when the field's group width is 1 and group by
+// group (ldv / stv) otherwise.
 template <int NC, typename T, typename F>
 __device__ __forceinline__ typename Group<NC, T>::M fload_link(const F& f, uint32_t site, int dir)
 {
     using GR = Group<NC, T>;
     typename GR::M m;
+    if (f.vw() > 1) {
+        T v[GR::S];
+        fld_n<GR::S, T>(f, (uint32_t)(dir * GR::S), site, v, 0);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = f.ld((uint32_t)(dir * GR::S + k), site);
+        for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = f.ld((uint32_t)(dir * GR::S + k), site);
+    }
     GR::complete(m);
     return m;
 }
@@ -158,24 +210,45 @@ __device__ __forceinline__ typename Group<NC, T>::M fload_link_r01(const F& f, u
 {
     using GR = Group<NC, T>;
     typename GR::M m = GR::zero();
+    if (f.vw() > 1) {
+        T v[GR::S];
+        fld_n<GR::S, T>(f, (uint32_t)(dir * GR::S), site, v, 0);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = f.ld((uint32_t)(dir * GR::S + k), site);
+        for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < GR::S; k++) GR::gref(m, GR::slot(k)) = f.ld((uint32_t)(dir * GR::S + k), site);
+    }
     return m;
 }
 template <int NC, typename T, typename F>
 __device__ __forceinline__ void fstore_link(const F& f, uint32_t site, int dir, const typename Group<NC, T>::M& m)
 {
     using GR = Group<NC, T>;
+    if (f.vw() > 1) {
+        T v[GR::S];
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) f.st((uint32_t)(dir * GR::S + k), site, GR::gget(m, GR::slot(k)));
+        for (int k = 0; k < GR::S; k++) v[k] = GR::gget(m, GR::slot(k));
+        fst_n<GR::S, T>(f, (uint32_t)(dir * GR::S), site, v, 0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < GR::S; k++) f.st((uint32_t)(dir * GR::S + k), site, GR::gget(m, GR::slot(k)));
+    }
 }
 template <int NC, typename T, typename F>
 __device__ __forceinline__ typename Group<NC, T>::A fload_alg(const F& f, uint32_t site, int dir)
 {
     using GR = Group<NC, T>;
     typename GR::A a;
+    if (f.vw() > 1) {
+        T v[GR::AD];
+        fld_n<GR::AD, T>(f, (uint32_t)(dir * GR::AD), site, v, 0);
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) aref(a, k) = f.ld((uint32_t)(dir * GR::AD + k), site);
+        for (int k = 0; k < GR::AD; k++) aref(a, k) = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < GR::AD; k++) aref(a, k) = f.ld((uint32_t)(dir * GR::AD + k), site);
+    }
     return a;
 }
 template <int NC, typename T, typename F>
@@ -183,30 +256,58 @@ __device__ __forceinline__ typename Group<NC, T>::A fload_alg_once(const F& f, u
 {
     using GR = Group<NC, T>;
     typename GR::A a;
+    if (f.vw() > 1) {
+        T v[GR::AD];
+        fld_n<GR::AD, T>(f, (uint32_t)(dir * GR::AD), site, v, 1);
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) aref(a, k) = f.ld_once((uint32_t)(dir * GR::AD + k), site);
+        for (int k = 0; k < GR::AD; k++) aref(a, k) = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < GR::AD; k++) aref(a, k) = f.ld_once((uint32_t)(dir * GR::AD + k), site);
+    }
     return a;
 }
 template <int NC, typename T, typename F>
 __device__ __forceinline__ void fstore_alg_once(const F& f, uint32_t site, int dir, typename Group<NC, T>::A a)
 {
     using GR = Group<NC, T>;
+    if (f.vw() > 1) {
+        T v[GR::AD];
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) f.st_once((uint32_t)(dir * GR::AD + k), site, aref(a, k));
+        for (int k = 0; k < GR::AD; k++) v[k] = aref(a, k);
+        fst_n<GR::AD, T>(f, (uint32_t)(dir * GR::AD), site, v, 1);
+    } else {
+#pragma unroll
+        for (int k = 0; k < GR::AD; k++) f.st_once((uint32_t)(dir * GR::AD + k), site, aref(a, k));
+    }
 }
 template <int NC, typename T, typename F>
 __device__ __forceinline__ void fstore_link_once(const F& f, uint32_t site, int dir, const typename Group<NC, T>::M& m)
 {
     using GR = Group<NC, T>;
+    if (f.vw() > 1) {
+        T v[GR::S];
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) f.st_once((uint32_t)(dir * GR::S + k), site, GR::gget(m, GR::slot(k)));
+        for (int k = 0; k < GR::S; k++) v[k] = GR::gget(m, GR::slot(k));
+        fst_n<GR::S, T>(f, (uint32_t)(dir * GR::S), site, v, 1);
+    } else {
+#pragma unroll
+        for (int k = 0; k < GR::S; k++) f.st_once((uint32_t)(dir * GR::S + k), site, GR::gget(m, GR::slot(k)));
+    }
 }
 template <int NC, typename T, typename F>
 __device__ __forceinline__ void fstore_alg(const F& f, uint32_t site, int dir, typename Group<NC, T>::A a)
 {
     using GR = Group<NC, T>;
+    if (f.vw() > 1) {
+        T v[GR::AD];
 #pragma unroll
-    for (int k = 0; k < GR::AD; k++) f.st((uint32_t)(dir * GR::AD + k), site, aref(a, k));
+        for (int k = 0; k < GR::AD; k++) v[k] = aref(a, k);
+        fst_n<GR::AD, T>(f, (uint32_t)(dir * GR::AD), site, v, 0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < GR::AD; k++) f.st((uint32_t)(dir * GR::AD + k), site, aref(a, k));
+    }
 }
 
 // ---------------------------------------------------------------- kernel 1
@@ -285,7 +386,7 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_LINK_WAVES : OPX_LINK
                                                             T* __restrict__ Unextp, T* __restrict__ Ep,
                                                             const T* __restrict__ Jp)
 {
-    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+    using Fld = typename std::conditional<BUF, FieldB<T, kGroup<NC, T>>, FieldG<T>>::type;
     const Fld U(Up, L.n, 3 * Group<NC, T>::S), Unext(Unextp, L.n, 3 * Group<NC, T>::S);
     const Fld E(Ep, L.n, 3 * Group<NC, T>::AD), J(Jp, L.n, 3 * Group<NC, T>::AD);
     // Block order (opx_order.h) with the direction fastest: the three links of a cell
@@ -512,7 +613,7 @@ __global__ __launch_bounds__(kBlock, (NC == 3 && sizeof(T) == 8) ? OPX_CELL_WAVE
                                                                         const T* __restrict__ Up, T* __restrict__ Unextp,
                                                                         T* __restrict__ Ep, const T* __restrict__ Jp)
 {
-    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+    using Fld = typename std::conditional<BUF, FieldB<T, kGroup<NC, T>>, FieldG<T>>::type;
     const Fld U(Up, L.n, 3 * Group<NC, T>::S), Unext(Unextp, L.n, 3 * Group<NC, T>::S);
     const Fld E(Ep, L.n, 3 * Group<NC, T>::AD), J(Jp, L.n, 3 * Group<NC, T>::AD);
     uint32_t site;
@@ -533,7 +634,7 @@ __global__ __launch_bounds__(kBlock, OPX_PLAQ_WAVES) void k_plaquettes(Lattice L
                                                        T* __restrict__ Pp)
 {
     using GR = Group<NC, T>;
-    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+    using Fld = typename std::conditional<BUF, FieldB<T, kGroup<NC, T>>, FieldG<T>>::type;
     const Fld U(Up, L.n, 3 * GR::S), P(Pp, L.n, 3 * GR::AD);
     uint32_t site;
     if (!tile_site(L, order, xcd_logical(blockIdx.x, gridDim.x), site)) return;
@@ -560,7 +661,7 @@ __global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? OPX_PLAQ_UPDATE_WAVES : O
 {
     using GR = Group<NC, T>;
     using A = typename GR::A;
-    using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+    using Fld = typename std::conditional<BUF, FieldB<T, kGroup<NC, T>>, FieldG<T>>::type;
     const Fld U(Up, L.n, 3 * GR::S), Unext(Unextp, L.n, 3 * GR::S);
     const Fld P(Pp, L.n, 3 * GR::AD), E(Ep, L.n, 3 * GR::AD), J(Jp, L.n, 3 * GR::AD);
     const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
@@ -659,7 +760,7 @@ __global__ __launch_bounds__(kBlock, OPX_CELL_WAVES) void k_batch_cell(const Bat
         if (site >= E.L.n) continue;
         // the event's fields, by buffer instructions when every event is below 4 GiB
         using GR = Group<NC, T>;
-        using Fld = typename std::conditional<BUF, FieldB<T>, FieldG<T>>::type;
+        using Fld = typename std::conditional<BUF, FieldB<T, kGroup<NC, T>>, FieldG<T>>::type;
         const Fld U(Ubase + E.link_offset, E.L.n, 3 * GR::S), Un(Unbase + E.link_offset, E.L.n, 3 * GR::S);
         const Fld Ef(Ebase + E.alg_offset, E.L.n, 3 * GR::AD);
         cell_update<NC, T, false, REUNIT>(E.L, U, Un, Ef, Ef, site);
@@ -791,9 +892,9 @@ __device__ __forceinline__ typename Group<NC, double>::M load_link_d(const T* __
     using GD = Group<NC, double>;
     using GR = Group<NC, T>;
     typename GD::M m;
-    const T* p = fcell(U, site, 3 * GR::S) + dir * GR::S * 64;
+    const T* p = fcell(U, site, 3 * GR::S);
 #pragma unroll
-    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[k * 64];
+    for (int k = 0; k < GR::S; k++) GD::gref(m, GR::slot(k)) = (double)p[fcomp<T>(dir * GR::S + k, 3 * GR::S)];
     if constexpr (GR::S != GR::G) su3_third_row(m);   // rebuilt in double
     return m;
 }
@@ -817,9 +918,9 @@ __device__ __forceinline__ typename Group<NC, double>::A load_alg_d(const T* __r
 {
     using GD = Group<NC, double>;
     typename GD::A a;
-    const T* p = fcell(X, site, comps) + dir * GD::AD * 64;
+    const T* p = fcell(X, site, comps);
 #pragma unroll
-    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[k * 64];
+    for (int k = 0; k < GD::AD; k++) aref(a, k) = (double)p[fcomp<T>(dir * GD::AD + k, comps)];
     return a;
 }
 
@@ -1219,13 +1320,13 @@ __global__ __launch_bounds__(kBlock) void k_sp_to_grid(Lattice L, SuperParticles
     const T f = (T)(L.as[0] / L.at);
     if (o > 0) {
 #pragma unroll
-        for (int m = 0; m < GR::AD; m++) atomicAdd(J + fpos(m, index + quickfix, 3 * GR::AD), aref(Q, m) * f);
+        for (int m = 0; m < GR::AD; m++) atomicAdd(J + fposT<T>(m, index + quickfix, 3 * GR::AD), aref(Q, m) * f);
         Q = act(Q, adj(U));
     } else {
         Q = act(Q, U);
         const T fm = (T)(-L.as[0] / L.at);
 #pragma unroll
-        for (int m = 0; m < GR::AD; m++) atomicAdd(J + fpos(m, index, 3 * GR::AD), aref(Q, m) * fm);
+        for (int m = 0; m < GR::AD; m++) atomicAdd(J + fposT<T>(m, index, 3 * GR::AD), aref(Q, m) * fm);
     }
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) q[(size_t)m * sp.total + qi] = aref(Q, m);
@@ -1252,7 +1353,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, int64_t 
     const uint32_t cell = cs.first[lo] + (i - cs.cells_prefix[lo]);
     typename GR::A a;
 #pragma unroll
-    for (int m = 0; m < GR::AD; m++) aref(a, m) = rho[fpos(m, cell, GR::AD)];
+    for (int m = 0; m < GR::AD; m++) aref(a, m) = rho[fposT<T>(m, cell, GR::AD)];
     const uint32_t e1 = cs.list_prefix[lo + 1];
     for (uint32_t e = cs.list_prefix[lo]; e < e1; e++) {
         const size_t qi = (size_t)(cs.qbase[e] + (int64_t)cell);
@@ -1260,7 +1361,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_chargedensity(Lattice L, int64_t 
         for (int m = 0; m < GR::AD; m++) aref(a, m) += q[(size_t)m * q_total + qi];
     }
 #pragma unroll
-    for (int m = 0; m < GR::AD; m++) rho[fpos(m, cell, GR::AD)] = aref(a, m);
+    for (int m = 0; m < GR::AD; m++) rho[fposT<T>(m, cell, GR::AD)] = aref(a, m);
 }
 
 dim3 grid_links(uint32_t n) { return dim3((n + kBlock - 1) / kBlock, 3, 1); }
@@ -1275,7 +1376,7 @@ __global__ __launch_bounds__(kBlock) void k_zero_cells(T* __restrict__ f, CellRa
     int k = 0;
 #pragma unroll
     for (int q = 1; q < kMaxCurrentSlabs; q++) k += (q < r.count) & (i >= r.start[q]);
-    f[fpos(blockIdx.y, r.first[k] + (i - r.start[k]), gridDim.y)] = T(0);
+    f[fposT<T>(blockIdx.y, r.first[k] + (i - r.start[k]), gridDim.y)] = T(0);
 }
 
 }  // namespace

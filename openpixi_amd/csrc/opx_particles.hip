@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kPBlock) void k_p_rho_gather(Lattice L, uint32_t co
     if (i > 0 && keys[i - 1] == cell) return;
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) {
-        const uint64_t r = fpos(m, cell, GR::AD);
+        const uint64_t r = fposT<T>(m, cell, GR::AD);
         T v = rho[r];
         for (uint32_t j = i; j < count && keys[j] == cell; j++) v += Q0[(size_t)m * count + vals[j]];
         rho[r] = v;
@@ -126,9 +126,9 @@ __global__ __launch_bounds__(kPBlock) void k_p_push(Lattice L, uint32_t count, c
         const bool forward = vel[(size_t)d * count + k] > 0;
         const uint32_t cell = cell_of(L, forward ? no : nn);
         typename GR::M U;
-        const T* p = fcell(Unext, cell, 3 * GR::S) + d * GR::S * 64;
+        const T* p = fcell(Unext, cell, 3 * GR::S);
 #pragma unroll
-        for (int c = 0; c < GR::S; c++) GR::gref(U, GR::slot(c)) = p[c * 64];
+        for (int c = 0; c < GR::S; c++) GR::gref(U, GR::slot(c)) = p[fcomp<T>(d * GR::S + c, 3 * GR::S)];
         GR::complete(U);
         if (!forward) U = adj(U);
         store_charge<NC, T>(Q1, count, k, act(q0, adj(U)));
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kPBlock) void k_p_j_gather(Lattice L, uint32_t coun
     const T fp = (T)(L.as[d] / L.at), fm = (T)(-L.as[d] / L.at);
 #pragma unroll
     for (int m = 0; m < GR::AD; m++) {
-        const uint64_t r = fpos(d * GR::AD + m, cell, 3 * GR::AD);
+        const uint64_t r = fposT<T>(d * GR::AD + m, cell, 3 * GR::AD);
         T v = J[r];
         for (uint32_t j = i; j < count && keys[j] == key; j++) {
             const uint32_t k = vals[j];

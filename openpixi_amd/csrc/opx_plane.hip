@@ -61,7 +61,7 @@ __device__ __forceinline__ void plane_link(const Lattice& L, const T (*sl)[NS], 
             const uint32_t y2 = O2 > 0 ? x2p : (O2 < 0 ? x2m : x2);
             const uint32_t site = (O0 > 0 ? pp : pm) * L.s0 + y1 * L.s1 + y2;
 #pragma unroll
-            for (int k = 0; k < S; k++) GR::gref(m, GR::slot(k)) = bu.ld(fvoff(site, 3 * S, sz), fsoff(DIR * S + k, sz));
+            for (int k = 0; k < S; k++) GR::gref(m, GR::slot(k)) = bu.ld(fvoff(site, 3 * S, sz), fsoff(DIR * S + k, 3 * S, sz));
         }
         GR::complete(m);
         return m;
@@ -98,19 +98,19 @@ __device__ __forceinline__ void plane_link(const Lattice& L, const T (*sl)[NS], 
     const T at = (T)L.at;
     A e;
 #pragma unroll
-    for (int k = 0; k < AD; k++) aref(e, k) = be.ld(fvoff(site, 3 * AD, sz), fsoff(D * AD + k, sz), 1);
+    for (int k = 0; k < AD; k++) aref(e, k) = be.ld(fvoff(site, 3 * AD, sz), fsoff(D * AD + k, 3 * AD, sz), 1);
 #pragma unroll
     for (int k = 0; k < AD; k++) aref(e, k) += aget(pr, k) * at;
     if (HAS_J) {
         const T uf = (T)L.uf[D];
 #pragma unroll
-        for (int k = 0; k < AD; k++) aref(e, k) += Jp[fpos(D * AD + k, site, 3 * AD)] * uf;
+        for (int k = 0; k < AD; k++) aref(e, k) += Jp[fposT<T>(D * AD + k, site, 3 * AD)] * uf;
     }
 #pragma unroll
-    for (int k = 0; k < AD; k++) be.st(aget(e, k), fvoff(site, 3 * AD, sz), fsoff(D * AD + k, sz), 1);
+    for (int k = 0; k < AD; k++) be.st(aget(e, k), fvoff(site, 3 * AD, sz), fsoff(D * AD + k, 3 * AD, sz), 1);
     const M un = mul(exp_link(e, -at), Ud);
 #pragma unroll
-    for (int k = 0; k < S; k++) bun.st(GR::gget(un, GR::slot(k)), fvoff(site, 3 * S, sz), fsoff(D * S + k, sz), 1);
+    for (int k = 0; k < S; k++) bun.st(GR::gget(un, GR::slot(k)), fvoff(site, 3 * S, sz), fsoff(D * S + k, 3 * S, sz), 1);
 }
 
 template <int NC, typename T, int TY, int TX, bool HAS_J>
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(3 * TY * TX) void k_plane_update(Lattice L, Box act
         const int hy = i / PW, hx = i - hy * PW;
         const uint32_t site = p * L.s0 + pwrap(row0 - 1 + hy, L.d1) * L.s1 + pwrap(col0 - 1 + hx, L.d2);
 #pragma unroll
-        for (int k = 0; k < 3 * S; k++) sl[k][i] = bu.ld(fvoff(site, 3 * S, sz), fsoff(k, sz));
+        for (int k = 0; k < 3 * S; k++) sl[k][i] = bu.ld(fvoff(site, 3 * S, sz), fsoff(k, 3 * S, sz));
     }
     __syncthreads();
     // direction of this wave's links (uniform: CELLS is a multiple of the wave size)

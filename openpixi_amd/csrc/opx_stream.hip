@@ -117,7 +117,7 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
 #pragma unroll
         for (int l = 0; l < 3; l++) {
 #pragma unroll
-            for (int k = 0; k < GR::S; k++) GR::gref(u[l], GR::slot(k)) = bu.ld(vo, fsoff(l * GR::S + k, sz));
+            for (int k = 0; k < GR::S; k++) GR::gref(u[l], GR::slot(k)) = bu.ld(vo, fsoff(l * GR::S + k, 3 * GR::S, sz));
             GR::complete(u[l]);
         }
     };
@@ -165,13 +165,13 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
 #pragma unroll
             for (int d = 0; d < 3; d++)
 #pragma unroll
-                for (int k = 0; k < AD; k++) aref(e[d], k) = be.ld(fvoff(site, 3 * AD, sz), fsoff(d * AD + k, sz));
+                for (int k = 0; k < AD; k++) aref(e[d], k) = be.ld(fvoff(site, 3 * AD, sz), fsoff(d * AD + k, 3 * AD, sz));
             if (HAS_J) {
                 const Buf<T> bj(Jp, 3u * AD * np * sz);
 #pragma unroll
                 for (int d = 0; d < 3; d++)
 #pragma unroll
-                    for (int k = 0; k < AD; k++) aref(jv[d], k) = bj.ld(fvoff(site, 3 * AD, sz), fsoff(d * AD + k, sz));
+                    for (int k = 0; k < AD; k++) aref(jv[d], k) = bj.ld(fvoff(site, 3 * AD, sz), fsoff(d * AD + k, 3 * AD, sz));
             }
         }
         // transported contributions
@@ -216,11 +216,11 @@ __global__ __launch_bounds__(TY* TX, (sizeof(T) == 4 ? 2 : 1)) void k_stream_upd
                 axpy(e[d], f[d], at);
                 if (HAS_J) axpy(e[d], jv[d], (T)L.uf[d]);
 #pragma unroll
-                for (int k = 0; k < AD; k++) be.st(aget(e[d], k), fvoff(site, 3 * AD, sz), fsoff(d * AD + k, sz));
+                for (int k = 0; k < AD; k++) be.st(aget(e[d], k), fvoff(site, 3 * AD, sz), fsoff(d * AD + k, 3 * AD, sz));
                 const M un = mul(exp_link(e[d], -at), ucur[d]);
 #pragma unroll
                 for (int k = 0; k < GR::S; k++)
-                    bun.st(GR::gget(un, GR::slot(k)), fvoff(site, 3 * GR::S, sz), fsoff(d * GR::S + k, sz));
+                    bun.st(GR::gget(un, GR::slot(k)), fvoff(site, 3 * GR::S, sz), fsoff(d * GR::S + k, 3 * GR::S, sz));
             }
         }
         carry1 = act_adj(a01, ucur[0]);

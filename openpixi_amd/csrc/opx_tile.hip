@@ -42,12 +42,12 @@ __device__ __forceinline__ bool in_box_tile(const Box& b, uint32_t x0, uint32_t 
            (int)x2 >= b.lo[2] && (int)x2 <= b.hi[2];
 }
 
-// SU(2) link field (12 components per cell) in the device block layout (fpos)
+// SU(2) link field (12 components per cell) in the device layout (fpos)
 template <typename T>
 __device__ __forceinline__ Q2<T> ld_link2(const T* __restrict__ U, uint32_t site, int d)
 {
-    const T* p = fcell(U, site, 12) + 4 * d * 64;
-    return Q2<T>{p[0], p[64], p[128], p[192]};
+    const T* p = fcell(U, site, 12);
+    return Q2<T>{p[fcomp<T>(4 * d, 12)], p[fcomp<T>(4 * d + 1, 12)], p[fcomp<T>(4 * d + 2, 12)], p[fcomp<T>(4 * d + 3, 12)]};
 }
 
 // One block's march over planes [pb, pe) of the tile at (row0, col0) of lattice L.
@@ -105,23 +105,23 @@ __device__ __forceinline__ void tile_march(const Lattice& L, const Box& active, 
                 }
                 const Q2<T> Ud = at_off(0, 0, 0, d);
                 const A pr = proj(mul(Ud, S));
-                T* pe_ = fcell(E, site, 9) + 3 * d * 64;
-                A e{{pe_[0], pe_[64], pe_[128]}};
+                T* pe_ = fcell(E, site, 9);
+                A e{{pe_[fcomp<T>(3 * d, 9)], pe_[fcomp<T>(3 * d + 1, 9)], pe_[fcomp<T>(3 * d + 2, 9)]}};
 #pragma unroll
                 for (int k = 0; k < 3; k++) e.v[k] += pr.v[k] * at;
                 if (HAS_J) {
                     const T uf = (T)L.uf[d];
 #pragma unroll
-                    for (int k = 0; k < 3; k++) e.v[k] += fcell(J, site, 9)[(3 * d + k) * 64] * uf;
+                    for (int k = 0; k < 3; k++) e.v[k] += fcell(J, site, 9)[fcomp<T>(3 * d + k, 9)] * uf;
                 }
 #pragma unroll
-                for (int k = 0; k < 3; k++) st_once<NT>(e.v[k], pe_ + k * 64);
+                for (int k = 0; k < 3; k++) st_once<NT>(e.v[k], pe_ + fcomp<T>(3 * d + k, 9));
                 const Q2<T> un = mul(exp_link(e, -at), Ud);
-                T* pu = fcell(Un, site, 12) + 4 * d * 64;
-                st_once<NT>(un.e0, pu);
-                st_once<NT>(un.e1, pu + 64);
-                st_once<NT>(un.e2, pu + 128);
-                st_once<NT>(un.e3, pu + 192);
+                T* pu = fcell(Un, site, 12);
+                st_once<NT>(un.e0, pu + fcomp<T>(4 * d, 12));
+                st_once<NT>(un.e1, pu + fcomp<T>(4 * d + 1, 12));
+                st_once<NT>(un.e2, pu + fcomp<T>(4 * d + 2, 12));
+                st_once<NT>(un.e3, pu + fcomp<T>(4 * d + 3, 12));
             }
         }
         __syncthreads();   // plane p-1 is read before it is overwritten
