@@ -46,14 +46,14 @@ BYTES_PER_LINK = {"fp32": 48 + 32 + 32 + 48, "fp64": 144 + 64 + 64 + 144}
 # run: the JSON line carries them as roofline.traffic with roofline.traffic_source naming the
 # profile summary. Keyed by (precision, kernel form, tile, lattice): (FETCH_SIZE KiB, WRITE_SIZE KiB, file).
 _PROFILE = "profiles/k_cell_update_su3_128cube.md"
-PROFILED_TRAFFIC = {("fp32", "cell", 0, DIMS): (266488.0, 491520.0, _PROFILE),
-                    ("fp32", "cell", 1, DIMS): (266426.0, 491520.0, _PROFILE),
-                    ("fp32", "cell", 2, DIMS): (281192.0, 491520.0, _PROFILE),
-                    ("fp32", "link", 0, DIMS): (263228.0, 491520.0, _PROFILE),
-                    ("fp64", "cell", 0, DIMS): (1457701.0, 1278135.0, _PROFILE),
-                    ("fp64", "cell", 1, DIMS): (1150413.0, 1278405.0, _PROFILE),
-                    ("fp64", "cell", 2, DIMS): (1067008.0, 1277958.0, _PROFILE),
-                    ("fp64", "link", 0, DIMS): (717969.0, 1277952.0, _PROFILE)}
+PROFILED_TRAFFIC = {("fp32", "cell", 0, DIMS): (265956.0, 491521.0, _PROFILE),
+                    ("fp32", "cell", 1, DIMS): (266075.0, 491520.0, _PROFILE),
+                    ("fp32", "cell", 2, DIMS): (280926.0, 491520.0, _PROFILE),
+                    ("fp32", "link", 0, DIMS): (263227.0, 491520.0, _PROFILE),
+                    ("fp64", "cell", 0, DIMS): (1476429.0, 1363132.0, _PROFILE),
+                    ("fp64", "cell", 1, DIMS): (1169774.0, 1355513.0, _PROFILE),
+                    ("fp64", "cell", 2, DIMS): (1079910.0, 1348559.0, _PROFILE),
+                    ("fp64", "link", 0, DIMS): (700078.0, 1298745.0, _PROFILE)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 # VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, same probe runs): with the launch
 # time this run measures, the share of the launch the VALU needs to issue them at the 2.4 GHz peak
@@ -62,9 +62,9 @@ PROFILED_VALU = {("fp32", "cell", 0, DIMS): (121012224.0, _PROFILE),
                  ("fp32", "cell", 1, DIMS): (121012224.0, _PROFILE),
                  ("fp32", "cell", 2, DIMS): (121012224.0, _PROFILE),
                  ("fp32", "link", 0, DIMS): (144932352.0, _PROFILE),
-                 ("fp64", "cell", 0, DIMS): (164368179.0, _PROFILE),
-                 ("fp64", "cell", 1, DIMS): (164657515.0, _PROFILE),
-                 ("fp64", "cell", 2, DIMS): (164658764.0, _PROFILE),
+                 ("fp64", "cell", 0, DIMS): (166137634.0, _PROFILE),
+                 ("fp64", "cell", 1, DIMS): (166426987.0, _PROFILE),
+                 ("fp64", "cell", 2, DIMS): (166428236.0, _PROFILE),
                  ("fp64", "link", 0, DIMS): (163118117.0, _PROFILE)}
 SIMDS, PEAK_CLOCK_GHZ = 1024, 2.4
 

@@ -18,9 +18,10 @@ namespace opx {
 // direction starts a pair. Every other field keeps V = 1 (one component of 64 cells contiguous):
 // four floats per 16-byte access made the SU(3) float cell form no faster and the SU(2) float
 // cell form 7 % slower, and pairs made the SU(2) double cell form 6.5 % slower; the SU(2) algebra
-// fields (3 per direction) could not be paired anyway. The components of a cell lie within comps * 64 elements of each
-// other (one array per component, 2^k cells apart, sends every component of a cell to the same
-// memory channels). A field of n cells holds padded_cells(n) * comps elements.
+// fields (3 per direction) could not be paired anyway. The components of a cell lie within
+// comps * 64 elements of each other (one array per component, 2^k cells apart, sends every
+// component of a cell to the same memory channels). A field of n cells holds
+// padded_cells(n) * comps elements.
 constexpr uint32_t kCellBlock = 64;
 __host__ __device__ constexpr uint32_t fvec(uint32_t comps, uint32_t tsz)
 {
