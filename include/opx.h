@@ -17,8 +17,12 @@
  *                                 of math/SU3AlgebraElement.java:3-11)
  *       charge  [cell][A]
  *     cell = ((x0*N1)+x1)*N2+x2 (physics/grid/Grid.java:757-770).
- *     The device keeps structure-of-arrays [dir][component][cell] in the
- *     context's precision; SU(3) algebra is kept as 8 Gell-Mann coefficients.
+ *     The device keeps each field in blocks of 64 consecutive cells, a block
+ *     holding every component (dir-major, then element) of its cells, one
+ *     component of the 64 cells after another; FP64 SU(3) fields hold the
+ *     components in pairs (components 2k and 2k+1 of a cell side by side).
+ *     Fields are in the context's precision; SU(3) algebra is kept as 8
+ *     Gell-Mann coefficients.
  *     OPX_FP32 SU(3) contexts store the first two rows of each link (12 reals)
  *     and rebuild the third as conj(row0 x row1), on the device and on download.
  *   - One host thread per context; calls are ordered on the context's HIP
